@@ -1,0 +1,138 @@
+/*
+ * shark_gpu.h — C-ABI of libshark_gpu.so, the gfx950 HIP implementation of SHARK's
+ * Godunov step for isothermal gas + N pressureless dust fluids.
+ *
+ * The reference has no FFI: its hot path is the parameterless Fortran subroutine
+ * `solve(verbose,outputing)` (src/solver/solve.f90:10-181) acting on module-global arrays
+ * allocated in allocate_init (src/commons/allocate_init.f90:3-282).  This header is the
+ * boundary a Fortran driver binds with ISO_C_BINDING (shark_amd/fortran/shark_gpu_mod.f90):
+ * every entry point takes plain pointers and sizes; arrays are the reference's own layout,
+ * `real(dp) u_prim(1:ncells,1:nvar)` column-major (one contiguous plane of ncells doubles
+ * per variable), cells x-fastest, icell = 1+(ix-1)+(iy-1)*nx_max (src/grid/grid.f90:12),
+ * variable order rho, rho*vx, rho*vy, rho*vz, E, rhod(1..N), rhod*vdx(1..N),
+ * rhod*vdy(1..N), rhod*vdz(1..N) (src/commons/allocate_init.f90:225-262).
+ *
+ * Error convention: 0 on success, a positive SHARK_E* code otherwise; the message is
+ * available from shark_gpu_last_error() (the reference prints and stops,
+ * src/commons/read_params.f90:18-33).  Calls are not thread-safe; one context owns one
+ * HIP stream and its device buffers.
+ */
+#ifndef SHARK_GPU_H
+#define SHARK_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHARK_CFG_VERSION 1
+#define SHARK_MAX_DUST 64
+
+enum {
+  SHARK_OK = 0,
+  SHARK_EINVAL = 1,   /* bad configuration or argument */
+  SHARK_EUNSUP = 2,   /* configuration outside the implemented scope */
+  SHARK_EHIP = 3,     /* HIP runtime error */
+  SHARK_ESTATE = 4,   /* call out of order (e.g. step before upload) */
+  SHARK_ENOMEM = 5
+};
+
+/* geometry (reference GEOM macro, bin/Makefile:14) */
+enum { SHARK_GEOM_CARTESIAN = 0, SHARK_GEOM_POLAR = 2 };
+/* gas Riemann solver (SOLVER macro, src/solver/godunov.f90:783-791) */
+enum { SHARK_SOLVER_LLF = 0, SHARK_SOLVER_HLL = 1, SHARK_SOLVER_HLLC = 2 };
+/* dust Riemann solver (SOLVERDUST macro, src/solver/godunov.f90:797-813) */
+enum { SHARK_DUST_HUANG_BAI = 0, SHARK_DUST_LLF = 1, SHARK_DUST_HLL = 2, SHARK_DUST_HLLC = 3 };
+/* per-step setup hooks, resolved at link time in the reference (bin/Makefile:67) */
+enum { SHARK_BC_PERIODIC = 0,   /* src/grid/boundary_types.f90:149-186 (DUSTYWAVE) */
+       SHARK_BC_SHEAR_SI = 1,   /* src/setups/SI/boundaries.f90:10-85 */
+       SHARK_BC_DISK = 2 };     /* src/setups/disk_polar/boundaries.f90:10-62 */
+enum { SHARK_TSTOP_DUSTYWAVE = 0,  /* src/setups/DUSTYWAVE/setup.f90:217 */
+       SHARK_TSTOP_SI = 1,         /* src/setups/SI/setup.f90:392 */
+       SHARK_TSTOP_EPSTEIN_DISK = 2 }; /* src/setups/disk_polar/setup.f90:436 */
+enum { SHARK_FORCE_NONE = 0,
+       SHARK_FORCE_SI = 1,          /* src/setups/SI/setup.f90:335-368 */
+       SHARK_FORCE_DISK = 2 };      /* src/setups/disk_polar/setup.f90:360-403 */
+enum { SHARK_INLOOP_NONE = 0,
+       SHARK_INLOOP_SI = 1,         /* src/setups/SI/setup.f90:299-332 */
+       SHARK_INLOOP_DISK = 2 };     /* src/setups/disk_polar/setup.f90:206-284 */
+
+/* Configuration: the reference's cpp macro set (bin/Makefile:5-36) plus the run-time
+ * namelist scalars the step reads (&GRID_PARAMS src/commons/read_params.f90:10,
+ * &DUST_PARAMS src/dust/dust_commons.f90, &SETUP_PARAMS of the setup). */
+typedef struct shark_cfg {
+  int32_t version;          /* SHARK_CFG_VERSION */
+  int32_t nx, ny, ndust;    /* active cells per direction (ny=1 means 1D), dust species */
+  int32_t nghost;           /* must be 2 (bin/Makefile:8) */
+  int32_t geom, solver, solverdust, slope_type;
+  int32_t iso_cs;           /* must be 1: isothermal gas (adiabatic path not built) */
+  int32_t force_kick, fargo, drag, dust_back_reaction, growth, viscosity;
+  int32_t bc_kind, tstop_kind, force_kind, inloop_kind;
+  int32_t disk_bc_mode;     /* 0: reference indexing (boundaries.f90:45-46), 1: per-row copy */
+  int32_t pad0;
+  double gamma, cfl, smallr, smallc, smallp, dust_ratio_min;
+  /* DUSTYWAVE */
+  double K[SHARK_MAX_DUST];
+  /* SI shearing box */
+  double q_shear, omega_shear, box_l, rad0, eta_stream, rho_init, hoverr;
+  double rhograin;          /* grain material density in code units (rhograin/unit_d) */
+  /* disk_polar */
+  double mstar, r_relax, n_rel, smooth_r;
+  double reserved[16];
+} shark_cfg;
+
+typedef struct shark_ctx shark_ctx;
+
+/* Allocates device state for the configuration. */
+int shark_gpu_create(const shark_cfg* cfg, shark_ctx** ctx);
+
+/* Grid arrays as built by gridinit (src/grid/grid.f90:104-240): dx(ncells,ndim),
+ * surf(ncells,ndim), vol(ncells), radii_c(ncells), position(ncells,ndim). */
+int shark_gpu_set_geometry(shark_ctx* ctx, const double* dx, const double* surf,
+                           const double* vol, const double* radii_c, const double* position);
+
+/* State after setup: u_prim(ncells,nvar), cs(ncells), sdust(ncells,ndust) (may be NULL when
+ * the stopping-time law does not read it), uprim_condinit(ncells,nvar) (disk setup only,
+ * else NULL), fargo_velocity(ncells) (when fargo, else NULL). Copies; pointers are not kept. */
+int shark_gpu_upload(shark_ctx* ctx, const double* u_prim, const double* cs,
+                     const double* sdust, const double* uprim_condinit,
+                     const double* fargo_velocity);
+
+/* One call == one `solve` + `time = time + dt` of the reference time loop
+ * (src/shark.f90:102-107).  dt_out (may be NULL) receives the step's dt; passing NULL lets
+ * consecutive steps run without a host synchronisation. */
+int shark_gpu_step(shark_ctx* ctx, double* dt_out);
+
+/* Runs up to nsteps steps, stopping (on the device, without host round trips) once
+ * time >= tend as flag_continue does (e.g. src/setups/DUSTYWAVE/setup.f90:136).
+ * Writes the number of steps taken and the final time. */
+int shark_gpu_run(shark_ctx* ctx, int64_t nsteps, double tend, int64_t* steps_done,
+                  double* time_out);
+
+/* Copies u_prim (ncells,nvar) and, if q != NULL, the primitive variables computed by
+ * ctoprim (src/solver/solve.f90:195-253) from it. Either pointer may be NULL. */
+int shark_gpu_download(shark_ctx* ctx, double* u_prim, double* q);
+
+/* Current simulated time and the dt of the last step. */
+int shark_gpu_get_time(shark_ctx* ctx, double* time, double* dt_last);
+int shark_gpu_set_time(shark_ctx* ctx, double time);
+
+/* Per-kernel average device time (ms) of the last shark_gpu_run, measured with HIP events
+ * on the context's stream. names: "bc", "hydro", "post", "dt". Returns SHARK_EINVAL for an
+ * unknown name. */
+int shark_gpu_kernel_time(shark_ctx* ctx, const char* name, double* avg_ms, int64_t* count);
+int shark_gpu_set_timing(shark_ctx* ctx, int enable);
+
+/* Algorithmic HBM bytes per step of the plain step (one read + one write of u_prim plus cs,
+ * plus a second read + write pass when FARGO is on). */
+int64_t shark_gpu_algorithmic_bytes(const shark_cfg* cfg);
+
+int shark_gpu_destroy(shark_ctx* ctx);
+const char* shark_gpu_last_error(void);
+int shark_gpu_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,127 @@
+! Dump harness for the reference build (test infrastructure, never shipped).
+! Calls the reference's own read_params (src/commons/read_params.f90:3), setup
+! (src/setups/<SETUP>/setup.f90:1) and solve (src/solver/solve.f90:10) exactly as
+! time_loop does (src/shark.f90:71-108), for a fixed number of steps, and writes:
+!   <out>/ic.bin     : u_prim after setup (ncells*nvar, column-major FP64)
+!   <out>/aux.bin    : cs, dx(:,1..ndim), surf(:,1..ndim), vol, radii_c, position(:,1..ndim),
+!                      active_cell, active_cell_predictor, [sdust], [fargo_velocity], [uprim_condinit]
+!   <out>/final.bin  : u_prim after nsteps
+!   <out>/dt.bin     : dt of every step
+!   <out>/meta.txt   : sizes and scalars the restatement needs
+! usage: shark_ref <namelist> <nsteps> <outdir>
+program ref_harness
+  use parameters
+  use commons
+  use units
+#ifdef REF_DISK
+  use setup_parameters, only: uprim_condinit, mstar, r_relax, n_rel, smooth_r, hoverr
+#endif
+#ifdef REF_SI
+  use setup_parameters, only: box_l, rho_init, omega_shear, q_shear, hoverr, eta_stream, rad0
+#endif
+  implicit none
+  integer :: nsteps, istep, ilun
+  character(len=256) :: arg, outdir
+  real(dp), allocatable :: dts(:)
+
+  call read_params
+  call setup
+  call getarg(2, arg)
+  read(arg, *) nsteps
+  call getarg(3, outdir)
+  ! nsteps < 0: run until time >= tend, as flag_continue does (e.g. setups/DUSTYWAVE/setup.f90:136)
+  allocate(dts(max(abs(nsteps),1)*1 + 200000))
+
+  ilun = 31
+  open(ilun, file=trim(outdir)//'/ic.bin', form='unformatted', access='stream', status='replace')
+  write(ilun) u_prim
+  close(ilun)
+
+  open(ilun, file=trim(outdir)//'/aux.bin', form='unformatted', access='stream', status='replace')
+  write(ilun) cs
+  write(ilun) dx
+  write(ilun) surf
+  write(ilun) vol
+  write(ilun) radii_c
+  write(ilun) position
+  write(ilun) active_cell
+  write(ilun) active_cell_predictor
+#if NDUST>0
+  write(ilun) sdust
+#endif
+  if (fargo) write(ilun) fargo_velocity
+#ifdef REF_DISK
+  write(ilun) uprim_condinit
+#endif
+  close(ilun)
+
+  time = 0.0d0
+  if (nsteps >= 0) then
+     do istep = 1, nsteps
+        call solve(.false., .false.)
+        time = time + dt
+        dts(istep) = dt
+     end do
+  else
+     istep = 0
+     do while (time < tend)
+        istep = istep + 1
+        call solve(.false., .false.)
+        time = time + dt
+        dts(istep) = dt
+     end do
+     nsteps = istep
+  end if
+
+  open(ilun, file=trim(outdir)//'/final.bin', form='unformatted', access='stream', status='replace')
+  write(ilun) u_prim
+  close(ilun)
+  open(ilun, file=trim(outdir)//'/dt.bin', form='unformatted', access='stream', status='replace')
+  write(ilun) dts(1:nsteps)
+  close(ilun)
+
+  open(ilun, file=trim(outdir)//'/meta.txt', status='replace')
+  write(ilun, '(A,I12)') 'ncells ', ncells
+  write(ilun, '(A,I12)') 'nvar ', nvar
+  write(ilun, '(A,I12)') 'nx ', nx
+  write(ilun, '(A,I12)') 'ny ', ny
+  write(ilun, '(A,I12)') 'ndim ', ndim
+  write(ilun, '(A,I12)') 'ndust ', ndust
+  write(ilun, '(A,I12)') 'nsteps ', nsteps
+  write(ilun, '(A,I12)') 'slope_type ', slope_type
+  write(ilun, '(A,I12)') 'iso_cs ', iso_cs
+  write(ilun, '(A,L2)') 'force_kick ', force_kick
+  write(ilun, '(A,L2)') 'fargo ', fargo
+#if NDUST>0
+  write(ilun, '(A,L2)') 'drag ', drag
+  write(ilun, '(A,L2)') 'dust_back_reaction ', dust_back_reaction
+  write(ilun, '(A,ES25.17)') 'dust_ratio_min ', dust_ratio_min
+  write(ilun, '(A,ES25.17)') 'rhograin ', rhograin
+#endif
+  write(ilun, '(A,ES25.17)') 'gamma ', gamma
+  write(ilun, '(A,ES25.17)') 'CFL ', CFL
+  write(ilun, '(A,ES25.17)') 'smallr ', smallr
+  write(ilun, '(A,ES25.17)') 'smallc ', smallc
+  write(ilun, '(A,ES25.17)') 'smallp ', smallp
+  write(ilun, '(A,ES25.17)') 'time ', time
+  write(ilun, '(A,ES25.17)') 'unit_d ', unit_d
+  write(ilun, '(A,ES25.17)') 'unit_l ', unit_l
+  write(ilun, '(A,ES25.17)') 'unit_t ', unit_t
+#ifdef REF_DISK
+  write(ilun, '(A,ES25.17)') 'mstar ', mstar
+  write(ilun, '(A,ES25.17)') 'r_relax ', r_relax
+  write(ilun, '(A,ES25.17)') 'n_rel ', n_rel
+  write(ilun, '(A,ES25.17)') 'smooth_r ', smooth_r
+  write(ilun, '(A,ES25.17)') 'hoverr ', hoverr
+#endif
+#ifdef REF_SI
+  write(ilun, '(A,ES25.17)') 'box_l ', box_l
+  write(ilun, '(A,ES25.17)') 'rho_init ', rho_init
+  write(ilun, '(A,ES25.17)') 'omega_shear ', omega_shear
+  write(ilun, '(A,ES25.17)') 'q_shear ', q_shear
+  write(ilun, '(A,ES25.17)') 'hoverr ', hoverr
+  write(ilun, '(A,ES25.17)') 'eta_stream ', eta_stream
+  write(ilun, '(A,ES25.17)') 'rad0 ', rad0
+#endif
+  close(ilun)
+end program ref_harness

@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Generate reference fixtures tests/golden/ref_<case>.npz from the flang-built reference.
+
+Runs in the development container only (needs /root/reference and flang).  For each case
+it builds the reference Fortran step with oracle/ref/build_ref.sh (the reference's own
+sources, our harness main oracle/ref/ref_harness.f90), runs it single-threaded on the
+namelist below for a fixed number of steps, and stores:
+  ic (nvar,ncells)      u_prim after the reference setup
+  final (nvar,ncells)   u_prim after `nsteps` calls of solve (src/solver/solve.f90:10)
+  dt (nsteps,)          dt of each step
+  cs, dx, surf, vol, radii_c, position, active, active_pred, [sdust], [condinit]
+  meta                  JSON: macro set, namelist scalars, sizes
+The fixtures are data (inputs and expected outputs); no reference source is stored.
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.abspath(os.path.join(HERE, "..", ".."))
+BUILD = os.path.join(REPO, "oracle", "ref", "build_ref.sh")
+
+WAVE = """&GRID_PARAMS
+freq_out=100000
+CFL = {cfl}
+tend = {tend}
+gamma = 1.6666667d0
+slope_type = {slope}
+/
+&SETUP_PARAMS
+box_l = 1.0d0
+cs_0 = 1.0d0
+rho_0 = 1.0d0
+kx_wave = 1.0d0
+vx_0 = {vx0}
+delta_rho = {drho}
+delta_rho_d = 0.0d0
+K(1) = {K}
+dust2gas_ratio(1) = 1.0d0
+/
+&DUST_PARAMS
+drag=.true.
+/
+&GRAVITY_PARAMS
+/
+&IONISATION_PARAMS
+charging=.false.
+/
+"""
+
+SI = """&GRID_PARAMS
+freq_out=100000
+CFL=0.5
+tend=1.0d10
+gamma=1.666667d0
+force_kick=.true.
+slope_type={slope}
+/
+&SETUP_PARAMS
+box_l       = 2d0
+box_l_y     = 2d0
+rho_init    = 1.0d0
+omega_shear = 1.0d0
+hoverr      = 0.1
+Stokes_species(1) = 0.1d0
+Stokes_species(2) = 1.0d0
+dust2gas_species(1) = 0.5d0
+dust2gas_species(2) = 1.0d0
+/
+&DUST_PARAMS
+dust_back_reaction = .true.
+drag               = .true.
+/
+&GRAVITY_PARAMS
+/
+&IONISATION_PARAMS
+charging=.false.
+/
+"""
+
+DISK = """&GRID_PARAMS
+freq_out=100000
+CFL  = 0.5
+tend = 1.0d10
+gamma=1.666667d0
+force_kick =.true.
+slope_type = {slope}
+/
+&SETUP_PARAMS
+box_l        = 2.1d0
+R_out_disk   = 1d30
+smooth_r     = 0.4d0
+sigma_R0     = 1700.0d0
+hoverr       = 0.05d0
+r_relax      = {r_relax}
+/
+&DUST_PARAMS
+scut     = 1.0d0
+dust2gas = 0.01d0
+dust_back_reaction=.true.
+drag=.true.
+/
+&GRAVITY_PARAMS
+/
+&IONISATION_PARAMS
+charging=.false.
+/
+"""
+
+# name: (setup, nx, ny, ndust, geom, gridspace, solver, solverdust, namelist, nsteps)
+CASES = {
+    "dustywave_k50": ("DUSTYWAVE", 128, 1, 1, 0, 0, 1, 0,
+                      WAVE.format(cfl=0.8, tend=4.5, slope=2, vx0="1.0d-4", drho="1.0d-4", K="50d0"), -1),
+    "dustywave_c1": ("DUSTYWAVE", 512, 1, 1, 0, 0, 1, 0,
+                     WAVE.format(cfl=0.5, tend="1d10", slope=1, vx0="1.0d-4", drho="0.0d0", K="10d0"), 1000),
+    "dustywave_hllc_s3": ("DUSTYWAVE", 64, 1, 1, 0, 0, 2, 0,
+                          WAVE.format(cfl=0.5, tend="1d10", slope=3, vx0="3.0d-1", drho="1.0d-1", K="100d0"), 60),
+    "dustywave_llf_dllf": ("DUSTYWAVE", 64, 1, 1, 0, 0, 0, 1,
+                           WAVE.format(cfl=0.5, tend="1d10", slope=2, vx0="3.0d-1", drho="1.0d-1", K="5d0"), 60),
+    "dustywave_dhll": ("DUSTYWAVE", 64, 1, 1, 0, 0, 1, 2,
+                       WAVE.format(cfl=0.5, tend="1d10", slope=1, vx0="3.0d-1", drho="1.0d-1", K="5d0"), 60),
+    "dustywave_dhllc": ("DUSTYWAVE", 64, 1, 1, 0, 0, 1, 3,
+                        WAVE.format(cfl=0.5, tend="1d10", slope=3, vx0="3.0d-1", drho="1.0d-1", K="5d0"), 60),
+    "si_2d": ("SI", 24, 24, 2, 0, 0, 1, 0, SI.format(slope=1), 25),
+    "si_2d_s3_hllc": ("SI", 24, 24, 2, 0, 0, 2, 0, SI.format(slope=3), 25),
+    "disk_2d": ("disk_polar", 24, 32, 2, 2, 0, 1, 0, DISK.format(slope=1, r_relax="0.3d0"), 25),
+    "disk_2d_relax_s2": ("disk_polar", 24, 32, 2, 2, 0, 1, 0, DISK.format(slope=2, r_relax="3.0d0"), 25),
+}
+
+
+def _meta(path):
+    m = {}
+    for line in open(path):
+        k, v = line.split(None, 1)
+        v = v.strip()
+        if v in ("T", "F"):
+            m[k] = v == "T"
+        else:
+            try:
+                m[k] = int(v)
+            except ValueError:
+                m[k] = float(v.replace("D", "E"))
+    return m
+
+
+def run_case(name, spec, force_build=False):
+    setup, nx, ny, nd, geom, gs, solver, sdust, nml, nsteps = spec
+    bname = f"{setup}_{nx}x{ny}_n{nd}_g{geom}{gs}_s{solver}{sdust}"
+    exe = os.path.join(REPO, "oracle", "_ref", bname, "shark_ref")
+    if force_build or not os.path.exists(exe):
+        subprocess.run([BUILD, setup, str(nx), str(ny), str(nd), str(geom), str(gs), str(solver),
+                        str(sdust)], check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL)
+    with tempfile.TemporaryDirectory() as td:
+        nmlp = os.path.join(td, "run.nml")
+        open(nmlp, "w").write(nml)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([exe, nmlp, str(nsteps), td], check=True, cwd=td, env=env,
+                       stdout=subprocess.DEVNULL)
+        m = _meta(os.path.join(td, "meta.txt"))
+        nc, nv, ndim = m["ncells"], m["nvar"], m["ndim"]
+        ic = np.fromfile(os.path.join(td, "ic.bin")).reshape(nv, nc)
+        final = np.fromfile(os.path.join(td, "final.bin")).reshape(nv, nc)
+        dts = np.fromfile(os.path.join(td, "dt.bin"))
+        aux = np.fromfile(os.path.join(td, "aux.bin"))
+    off = 0
+
+    def take(n, shape=None):
+        nonlocal off
+        a = aux[off:off + n]
+        off += n
+        return a.reshape(shape) if shape else a
+
+    arrs = dict(ic=ic, final=final, dt=dts)
+    arrs["cs"] = take(nc)
+    arrs["dx"] = take(nc * ndim, (ndim, nc))
+    arrs["surf"] = take(nc * ndim, (ndim, nc))
+    arrs["vol"] = take(nc)
+    arrs["radii_c"] = take(nc)
+    arrs["position"] = take(nc * ndim, (ndim, nc))
+    arrs["active"] = take(nc)
+    arrs["active_pred"] = take(nc)
+    if nd > 0:
+        arrs["sdust"] = take(nc * nd, (nd, nc))
+    if m.get("fargo"):
+        arrs["fargo_velocity"] = take(nc)
+    if setup == "disk_polar":
+        arrs["condinit"] = take(nc * nv, (nv, nc))
+    assert off == aux.size, (name, off, aux.size)
+    m.update(setup=setup, geom=geom, gridspace=gs, solver=solver, solverdust=sdust,
+             namelist=nml, case=name)
+    np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), meta=json.dumps(m), **arrs)
+    return m
+
+
+def main(argv):
+    names = argv[1:] or list(CASES)
+    for n in names:
+        m = run_case(n, CASES[n])
+        print(n, "steps", m["nsteps"], "ncells", m["ncells"], "nvar", m["nvar"])
+
+
+if __name__ == "__main__":
+    main(sys.argv)

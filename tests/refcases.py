@@ -1,0 +1,69 @@
+"""Load the reference fixtures (tests/golden/ref_*.npz) as (cfg, geometry, state)."""
+import json
+import os
+import re
+
+import numpy as np
+
+from shark_amd import cfg as C
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+REF_CASES = sorted(f[4:-4] for f in os.listdir(GOLDEN) if f.startswith("ref_") and f.endswith(".npz"))
+
+
+def _K(nml):
+    return [float(v.replace("d", "e").replace("D", "e"))
+            for v in re.findall(r"K\(\d+\)\s*=\s*([0-9.eEdD+-]+)", nml)]
+
+
+def load_ref(name):
+    z = np.load(os.path.join(GOLDEN, f"ref_{name}.npz"))
+    m = json.loads(str(z["meta"]))
+    kw = dict(nx=m["nx"], ny=m["ny"], ndust=m["ndust"], geom=m["geom"], solver=m["solver"],
+              solverdust=m["solverdust"], slope_type=m["slope_type"], iso_cs=1,
+              force_kick=int(m["force_kick"]), fargo=int(m["fargo"]), drag=int(m.get("drag", 1)),
+              dust_back_reaction=int(m.get("dust_back_reaction", 1)), gamma=m["gamma"],
+              cfl=m["CFL"], smallr=m["smallr"], smallc=m["smallc"], smallp=m["smallp"],
+              dust_ratio_min=m.get("dust_ratio_min", 1e-11))
+    setup = m["setup"]
+    if setup == "DUSTYWAVE":
+        kw.update(bc_kind=C.BC_PERIODIC, tstop_kind=C.TSTOP_DUSTYWAVE, force_kind=C.FORCE_NONE,
+                  inloop_kind=C.INLOOP_NONE, K=_K(m["namelist"]))
+    elif setup == "SI":
+        kw.update(bc_kind=C.BC_SHEAR_SI, tstop_kind=C.TSTOP_SI, force_kind=C.FORCE_SI,
+                  inloop_kind=C.INLOOP_SI, q_shear=m["q_shear"], omega_shear=m["omega_shear"],
+                  box_l=m["box_l"], rad0=m["rad0"], eta_stream=m["eta_stream"],
+                  rho_init=m["rho_init"], hoverr=m["hoverr"], rhograin=m["rhograin"] / m["unit_d"])
+    elif setup == "disk_polar":
+        kw.update(bc_kind=C.BC_DISK, tstop_kind=C.TSTOP_EPSTEIN_DISK, force_kind=C.FORCE_DISK,
+                  inloop_kind=C.INLOOP_DISK, mstar=m["mstar"], r_relax=m["r_relax"],
+                  n_rel=m["n_rel"], smooth_r=m["smooth_r"], hoverr=m["hoverr"],
+                  rhograin=m["rhograin"] / m["unit_d"], disk_bc_mode=0)
+    cfg = C.make_cfg(**kw)
+    geom = dict(dx=z["dx"], surf=z["surf"], vol=z["vol"], radii_c=z["radii_c"], position=z["position"])
+    state = dict(u=z["ic"], cs=z["cs"], sdust=z["sdust"] if "sdust" in z else None,
+                 condinit=z["condinit"] if "condinit" in z else None,
+                 fargo_velocity=z["fargo_velocity"] if "fargo_velocity" in z else None)
+    return cfg, geom, state, z["final"], z["dt"], m
+
+
+def active_mask(cfg):
+    s = C.sizes_of(cfg)
+    if s.ndim == 1:
+        a = np.zeros(s.nx_max, bool)
+        a[2:2 + s.nx] = True
+        return a
+    a = np.zeros((s.ny_max, s.nx_max), bool)
+    a[2:2 + s.ny, 2:2 + s.nx] = True
+    return a.reshape(-1)
+
+
+def rel_l1(a, b, mask=None):
+    """Per-field relative L1 over active cells: sum|a-b| / max(sum|b|, tiny); absolute L1
+    for fields that are identically zero in the reference."""
+    if mask is not None:
+        a = a[:, mask]
+        b = b[:, mask]
+    num = np.abs(a - b).sum(axis=1)
+    den = np.abs(b).sum(axis=1)
+    return np.where(den > 0, num / np.where(den > 0, den, 1.0), num)
