@@ -1,0 +1,429 @@
+// shark_gpu.hip — C-ABI of libshark_gpu.so (include/shark_gpu.h): device state, upload and
+// download, and the step driver (one hipGraph per step, replayed; or per-kernel launches
+// bracketed by HIP events when timing is on).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/shark_gpu.h"
+#include "growth.h"
+#include "physics.h"
+#include "step_kernels.h"
+
+using shark::DevScalars;
+using shark::StepArgs;
+
+namespace shark {
+bool nd_supported(int nd);
+}
+
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess)                                                           \
+      return fail(SHARK_EHIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_),   \
+                  __FILE__, __LINE__);                                              \
+  } while (0)
+
+enum { KT_BC = 0, KT_HYDRO, KT_POST, KT_DT, KT_N };
+static const char* kKernelNames[KT_N] = {"bc", "hydro", "post", "dt"};
+
+struct shark_ctx {
+  shark_cfg cfg;
+  StepArgs a;
+  hipStream_t stream = nullptr;
+  double* dU = nullptr;
+  double* dB = nullptr;
+  double* dcs = nullptr;
+  double* dgeom = nullptr;   // 7 per-column arrays
+  double* dsdust = nullptr;
+  double* dcond = nullptr;
+  double* dfargo = nullptr;
+  DevScalars* dsc = nullptr;
+  DevScalars* hsc = nullptr;  // pinned mirror
+  bool geom_set = false, uploaded = false;
+  hipGraphExec_t gexec = nullptr;
+  bool timing = false;
+  double kt_ms[KT_N] = {0, 0, 0, 0};
+  int64_t kt_n[KT_N] = {0, 0, 0, 0};
+  shark_growth* growth = nullptr;
+};
+
+static int nvar_of(const shark_cfg* c) { return 5 + 4 * c->ndust; }
+
+extern "C" const char* shark_gpu_last_error(void) { return g_err; }
+extern "C" int shark_gpu_version(void) { return SHARK_CFG_VERSION; }
+
+extern "C" int64_t shark_gpu_algorithmic_bytes(const shark_cfg* c) {
+  const int64_t nv = nvar_of(c);
+  const int64_t cells = (int64_t)c->nx * (c->ny > 1 ? c->ny : 1);
+  return cells * (8 * (2 * nv + 1) + (c->fargo ? 16 * nv : 0));
+}
+
+static int validate(const shark_cfg* c) {
+  if (!c) return fail(SHARK_EINVAL, "null cfg");
+  if (c->version != SHARK_CFG_VERSION) return fail(SHARK_EINVAL, "cfg version %d, expected %d", c->version, SHARK_CFG_VERSION);
+  if (c->nx < 4 || c->ny < 1) return fail(SHARK_EINVAL, "nx=%d ny=%d: need nx>=4, ny>=1", c->nx, c->ny);
+  if (c->ny > 1 && c->ny < 4) return fail(SHARK_EINVAL, "2D runs need ny>=4 (got %d)", c->ny);
+  if (c->nghost != 2) return fail(SHARK_EUNSUP, "nghost=%d: only NGHOST=2 is built", c->nghost);
+  if (c->ndust < 0 || c->ndust > SHARK_MAX_DUST) return fail(SHARK_EINVAL, "ndust=%d out of range", c->ndust);
+  if (!shark::nd_supported(c->ndust))
+    return fail(SHARK_EUNSUP, "ndust=%d: hydro step is instantiated for ndust in {0,1,2,3,4,8,16}", c->ndust);
+  if (c->iso_cs != 1) return fail(SHARK_EUNSUP, "iso_cs=%d: only the isothermal gas path is built", c->iso_cs);
+  if (c->geom != SHARK_GEOM_CARTESIAN && c->geom != SHARK_GEOM_POLAR)
+    return fail(SHARK_EUNSUP, "geom=%d: GEOM 0 and 2 are built", c->geom);
+  if (c->geom == SHARK_GEOM_POLAR && c->ny < 4) return fail(SHARK_EUNSUP, "polar geometry needs a 2D grid");
+  if (c->solver < 0 || c->solver > 2) return fail(SHARK_EINVAL, "solver=%d", c->solver);
+  if (c->solverdust < 0 || c->solverdust > 3) return fail(SHARK_EINVAL, "solverdust=%d", c->solverdust);
+  if (c->slope_type < 1 || c->slope_type > 3) return fail(SHARK_EINVAL, "unknown slope limiter %d", c->slope_type);
+  if (c->viscosity) return fail(SHARK_EUNSUP, "viscosity source terms are not built");
+  if (c->fargo && c->ny < 4) return fail(SHARK_EUNSUP, "FARGO needs a 2D grid");
+  if (c->force_kick && !c->drag && c->ndust > 0)
+    return fail(SHARK_EUNSUP, "explicit drag in the force (drag=.false. with force_kick) is not built");
+  if (c->bc_kind < 0 || c->bc_kind > 2 || c->tstop_kind < 0 || c->tstop_kind > 2 ||
+      c->force_kind < 0 || c->force_kind > 2 || c->inloop_kind < 0 || c->inloop_kind > 2)
+    return fail(SHARK_EINVAL, "unknown setup hook kind");
+  if (c->bc_kind == SHARK_BC_DISK && c->ny < 4) return fail(SHARK_EUNSUP, "disk boundaries need a 2D grid");
+  if (c->cfl <= 0) return fail(SHARK_EINVAL, "CFL must be positive");
+  return SHARK_OK;
+}
+
+extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
+  if (!out) return fail(SHARK_EINVAL, "null ctx pointer");
+  *out = nullptr;
+  int rc = validate(cfg);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(SHARK_EHIP, "no HIP device visible");
+  shark_ctx* c = new shark_ctx();
+  c->cfg = *cfg;
+  StepArgs& a = c->a;
+  memset(&a, 0, sizeof(a));
+  a.nx = cfg->nx;
+  a.ny = cfg->ny > 1 ? cfg->ny : 1;
+  a.nd = cfg->ndust;
+  a.ndim = cfg->ny > 1 ? 2 : 1;
+  a.nxm = cfg->nx + 4;
+  a.nym = a.ndim == 2 ? cfg->ny + 4 : 1;
+  a.nc = a.nxm * a.nym;
+  a.nvar = nvar_of(cfg);
+  a.geom = cfg->geom; a.solver = cfg->solver; a.solverdust = cfg->solverdust; a.slope = cfg->slope_type;
+  a.force_kick = cfg->force_kick; a.force_kind = cfg->force_kick ? cfg->force_kind : 0;
+  a.tstop_kind = cfg->tstop_kind; a.inloop_kind = cfg->inloop_kind; a.bc_kind = cfg->bc_kind;
+  a.drag = cfg->drag; a.back_reaction = cfg->dust_back_reaction; a.fargo = cfg->fargo;
+  a.disk_bc_mode = cfg->disk_bc_mode;
+  a.gamma = cfg->gamma; a.cfl = cfg->cfl; a.smallr = cfg->smallr; a.dust_ratio_min = cfg->dust_ratio_min;
+  a.q_shear = cfg->q_shear; a.omega_shear = cfg->omega_shear; a.box_l = cfg->box_l; a.rad0 = cfg->rad0;
+  a.eta_stream = cfg->eta_stream; a.rho_init = cfg->rho_init; a.hoverr = cfg->hoverr;
+  a.rhograin = cfg->rhograin; a.mstar = cfg->mstar; a.r_relax = cfg->r_relax; a.n_rel = cfg->n_rel;
+  a.smooth_r = cfg->smooth_r;
+  for (int k = 0; k < SHARK_MAX_DUST; ++k) a.K[k] = cfg->K[k];
+  a.strip_h = (int)(cfg->reserved[0] > 0 ? cfg->reserved[0] : 32);
+  a.ngroups = (a.nx + 59) / 60;
+  a.nstrips = a.ndim == 2 ? (a.ny + a.strip_h - 1) / a.strip_h : 1;
+
+  const size_t nbytes = (size_t)a.nc * a.nvar * sizeof(double);
+  hipError_t e = hipSuccess;
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipMalloc(&c->dU, nbytes)) != hipSuccess || (e = hipMalloc(&c->dB, nbytes)) != hipSuccess ||
+      (e = hipMalloc(&c->dcs, (size_t)a.nc * 8)) != hipSuccess ||
+      (e = hipMalloc(&c->dgeom, (size_t)7 * a.nxm * 8)) != hipSuccess ||
+      (e = hipMalloc(&c->dsc, sizeof(DevScalars))) != hipSuccess ||
+      (e = hipHostMalloc(&c->hsc, sizeof(DevScalars))) != hipSuccess) {
+    rc = fail(e == hipErrorOutOfMemory ? SHARK_ENOMEM : SHARK_EHIP, "allocation failed: %s", hipGetErrorString(e));
+    shark_gpu_destroy(c);
+    return rc;
+  }
+  hipMemsetAsync(c->dU, 0, nbytes, c->stream);
+  hipMemsetAsync(c->dB, 0, nbytes, c->stream);
+  memset(c->hsc, 0, sizeof(DevScalars));
+  c->hsc->dtbits = 0;
+  hipMemcpyAsync(c->dsc, c->hsc, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream);
+  a.U = c->dU; a.B = c->dB; a.cs = c->dcs; a.sc = c->dsc;
+  const double* gp = c->dgeom;
+  a.g.dx1 = gp; a.g.dx2 = gp + a.nxm; a.g.surf1 = gp + 2 * a.nxm; a.g.surf2 = gp + 3 * a.nxm;
+  a.g.vol = gp + 4 * a.nxm; a.g.rc = gp + 5 * a.nxm; a.g.pos1 = gp + 6 * a.nxm;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    rc = fail(SHARK_EHIP, "stream sync failed");
+    shark_gpu_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return SHARK_OK;
+}
+
+extern "C" int shark_gpu_set_geometry(shark_ctx* c, const double* dx, const double* surf,
+                                      const double* vol, const double* radii_c, const double* position) {
+  if (!c) return fail(SHARK_EINVAL, "null ctx");
+  if (!dx || !surf || !vol || !radii_c) return fail(SHARK_EINVAL, "null geometry array");
+  const StepArgs& a = c->a;
+  const int nxm = a.nxm, nym = a.nym, nc = a.nc;
+  std::vector<double> col((size_t)7 * nxm, 0.0);
+  auto at = [&](const double* arr, int plane, int ix, int iy) {
+    return arr[(size_t)plane * nc + (ix - 1) + (size_t)(iy - 1) * nxm];
+  };
+  for (int ix = 1; ix <= nxm; ++ix) {
+    col[0 * nxm + ix - 1] = at(dx, 0, ix, 1);
+    col[1 * nxm + ix - 1] = a.ndim == 2 ? at(dx, 1, ix, 1) : 0.0;
+    col[2 * nxm + ix - 1] = at(surf, 0, ix, 1);
+    col[3 * nxm + ix - 1] = a.ndim == 2 ? at(surf, 1, ix, 1) : 0.0;
+    col[4 * nxm + ix - 1] = vol[ix - 1];
+    col[5 * nxm + ix - 1] = radii_c[ix - 1];
+    col[6 * nxm + ix - 1] = position ? position[ix - 1] : 0.0;
+    // the kernels read one value per column: the grid must be separable
+    for (int iy = 2; iy <= nym; ++iy) {
+      const size_t ci = (ix - 1) + (size_t)(iy - 1) * nxm;
+      bool ok = dx[ci] == col[ix - 1] && surf[ci] == col[2 * nxm + ix - 1] && vol[ci] == col[4 * nxm + ix - 1] &&
+                radii_c[ci] == col[5 * nxm + ix - 1];
+      if (a.ndim == 2) ok = ok && dx[nc + ci] == col[nxm + ix - 1] && surf[nc + ci] == col[3 * nxm + ix - 1];
+      // position(:,1) is read only by the shearing-box force and in-loop hooks
+      const bool need_pos = c->cfg.force_kind == SHARK_FORCE_SI || c->cfg.inloop_kind == SHARK_INLOOP_SI;
+      if (position && need_pos) ok = ok && position[ci] == col[6 * nxm + ix - 1];
+      if (!ok) return fail(SHARK_EUNSUP, "geometry varies along y at column %d: only separable grids are built", ix);
+    }
+  }
+  HIPCHK(hipMemcpyAsync(c->dgeom, col.data(), col.size() * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->geom_set = true;
+  return SHARK_OK;
+}
+
+static void drop_graph(shark_ctx* c) {
+  if (c->gexec) {
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+}
+
+extern "C" int shark_gpu_upload(shark_ctx* c, const double* u, const double* cs, const double* sdust,
+                                const double* condinit, const double* fargo_velocity) {
+  if (!c) return fail(SHARK_EINVAL, "null ctx");
+  if (!c->geom_set) return fail(SHARK_ESTATE, "shark_gpu_set_geometry must precede upload");
+  if (!u || !cs) return fail(SHARK_EINVAL, "u_prim and cs are required");
+  StepArgs& a = c->a;
+  const shark_cfg& cf = c->cfg;
+  const size_t nc = a.nc;
+  drop_graph(c);
+  HIPCHK(hipMemcpyAsync(c->dU, u, nc * a.nvar * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dcs, cs, nc * 8, hipMemcpyHostToDevice, c->stream));
+  // grain sizes: one value per species when uniform over the grid (every reference setup)
+  a.sdust_uniform = 1;
+  a.sdust = nullptr;
+  if (a.nd > 0 && cf.tstop_kind != SHARK_TSTOP_DUSTYWAVE) {
+    if (!sdust) return fail(SHARK_EINVAL, "sdust is required by this stopping-time law");
+    for (int k = 0; k < a.nd; ++k) {
+      const double s0 = sdust[(size_t)k * nc + (a.ndim == 2 ? 2 + 2 * (size_t)a.nxm : 2)];
+      a.sdust_k[k] = s0;
+      for (int iy = (a.ndim == 2 ? 3 : 1); iy <= (a.ndim == 2 ? a.ny + 2 : 1); ++iy)
+        for (int ix = 3; ix <= a.nx + 2; ++ix)
+          if (sdust[(size_t)k * nc + (ix - 1) + (size_t)(iy - 1) * a.nxm] != s0) a.sdust_uniform = 0;
+    }
+    if (!a.sdust_uniform) {
+      if (!c->dsdust) HIPCHK(hipMalloc(&c->dsdust, nc * a.nd * 8));
+      HIPCHK(hipMemcpyAsync(c->dsdust, sdust, nc * a.nd * 8, hipMemcpyHostToDevice, c->stream));
+      a.sdust = c->dsdust;
+    }
+  }
+  if (cf.inloop_kind == SHARK_INLOOP_DISK) {
+    if (!condinit) return fail(SHARK_EINVAL, "the disk in-loop hook needs uprim_condinit");
+    if (!c->dcond) HIPCHK(hipMalloc(&c->dcond, nc * a.nvar * 8));
+    HIPCHK(hipMemcpyAsync(c->dcond, condinit, nc * a.nvar * 8, hipMemcpyHostToDevice, c->stream));
+    a.condinit = c->dcond;
+  }
+  if (cf.fargo) {
+    if (!fargo_velocity) return fail(SHARK_EINVAL, "fargo needs fargo_velocity");
+    std::vector<double> vk(a.nxm);
+    for (int ix = 1; ix <= a.nxm; ++ix) {
+      vk[ix - 1] = fargo_velocity[ix - 1 + 2 * (size_t)a.nxm];
+      for (int iy = 3; iy <= a.ny + 2; ++iy)
+        if (fargo_velocity[(ix - 1) + (size_t)(iy - 1) * a.nxm] != vk[ix - 1] && ix >= 3 && ix <= a.nx + 2)
+          return fail(SHARK_EUNSUP, "fargo_velocity varies along y at column %d", ix);
+    }
+    if (!c->dfargo) HIPCHK(hipMalloc(&c->dfargo, a.nxm * 8));
+    HIPCHK(hipMemcpyAsync(c->dfargo, vk.data(), a.nxm * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    a.fargo_vk = c->dfargo;
+  }
+  // scalars: time 0, dt from the uploaded state (the first courant of the run)
+  memset(c->hsc, 0, sizeof(DevScalars));
+  c->hsc->dtbits = shark::dbits_host(2e44);
+  c->hsc->active = 1;
+  c->hsc->tend = INFINITY;
+  HIPCHK(hipMemcpyAsync(c->dsc, c->hsc, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream));
+  shark::launch_courant_init(a, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->uploaded = true;
+  return SHARK_OK;
+}
+
+static void enqueue_step(shark_ctx* c) {
+  shark::launch_bc(c->a, c->stream);
+  shark::launch_hydro(c->a, c->stream);
+  shark::launch_post(c->a, c->stream);
+  shark::launch_finalize(c->a, c->stream);
+}
+
+static int build_graph(shark_ctx* c) {
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_step(c);
+  HIPCHK(hipStreamEndCapture(c->stream, &g));
+  hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c->gexec = nullptr;
+    return fail(SHARK_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  return SHARK_OK;
+}
+
+static int run_steps(shark_ctx* c, int64_t n, double tend) {
+  shark::launch_arm(c->a, tend, c->stream);
+  HIPCHK(hipGetLastError());
+  if (c->timing) {
+    std::vector<hipEvent_t> ev((size_t)n * (KT_N + 1));
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    for (int64_t i = 0; i < n; ++i) {
+      hipEvent_t* e = &ev[(size_t)i * (KT_N + 1)];
+      HIPCHK(hipEventRecord(e[0], c->stream));
+      shark::launch_bc(c->a, c->stream);
+      HIPCHK(hipEventRecord(e[1], c->stream));
+      shark::launch_hydro(c->a, c->stream);
+      HIPCHK(hipEventRecord(e[2], c->stream));
+      shark::launch_post(c->a, c->stream);
+      HIPCHK(hipEventRecord(e[3], c->stream));
+      shark::launch_finalize(c->a, c->stream);
+      HIPCHK(hipEventRecord(e[4], c->stream));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < n; ++i) {
+      hipEvent_t* e = &ev[(size_t)i * (KT_N + 1)];
+      for (int k = 0; k < KT_N; ++k) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+        c->kt_ms[k] += ms;
+        c->kt_n[k] += 1;
+      }
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    return SHARK_OK;
+  }
+  if (!c->gexec) {
+    int rc = build_graph(c);
+    if (rc) return rc;
+  }
+  for (int64_t i = 0; i < n; ++i) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  return SHARK_OK;
+}
+
+extern "C" int shark_gpu_run(shark_ctx* c, int64_t nsteps, double tend, int64_t* steps_done, double* time_out) {
+  if (!c) return fail(SHARK_EINVAL, "null ctx");
+  if (!c->uploaded) return fail(SHARK_ESTATE, "upload before stepping");
+  if (nsteps < 0) return fail(SHARK_EINVAL, "nsteps < 0");
+  HIPCHK(hipMemcpyAsync(c->hsc, c->dsc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int64_t before = c->hsc->nstep;
+  int rc = run_steps(c, nsteps, tend);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(c->hsc, c->dsc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (steps_done) *steps_done = c->hsc->nstep - before;
+  if (time_out) *time_out = c->hsc->time;
+  return SHARK_OK;
+}
+
+extern "C" int shark_gpu_step(shark_ctx* c, double* dt_out) {
+  if (!c) return fail(SHARK_EINVAL, "null ctx");
+  if (!c->uploaded) return fail(SHARK_ESTATE, "upload before stepping");
+  int rc = run_steps(c, 1, INFINITY);
+  if (rc) return rc;
+  if (dt_out) {
+    HIPCHK(hipMemcpyAsync(c->hsc, c->dsc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *dt_out = c->hsc->dt_last;
+  }
+  return SHARK_OK;
+}
+
+extern "C" int shark_gpu_download(shark_ctx* c, double* u, double* q) {
+  if (!c) return fail(SHARK_EINVAL, "null ctx");
+  const size_t n = (size_t)c->a.nc * c->a.nvar;
+  if (u) HIPCHK(hipMemcpyAsync(u, c->dU, n * 8, hipMemcpyDeviceToHost, c->stream));
+  if (q) {
+    shark::launch_ctoprim(c->a, c->dB, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(q, c->dB, n * 8, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SHARK_OK;
+}
+
+extern "C" int shark_gpu_get_time(shark_ctx* c, double* time, double* dt_last) {
+  if (!c) return fail(SHARK_EINVAL, "null ctx");
+  HIPCHK(hipMemcpyAsync(c->hsc, c->dsc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (time) *time = c->hsc->time;
+  if (dt_last) *dt_last = c->hsc->dt_last;
+  return SHARK_OK;
+}
+
+extern "C" int shark_gpu_set_time(shark_ctx* c, double time) {
+  if (!c) return fail(SHARK_EINVAL, "null ctx");
+  HIPCHK(hipMemcpyAsync(c->hsc, c->dsc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->hsc->time = time;
+  HIPCHK(hipMemcpyAsync(c->dsc, c->hsc, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SHARK_OK;
+}
+
+extern "C" int shark_gpu_set_timing(shark_ctx* c, int enable) {
+  if (!c) return fail(SHARK_EINVAL, "null ctx");
+  c->timing = enable != 0;
+  for (int k = 0; k < KT_N; ++k) { c->kt_ms[k] = 0; c->kt_n[k] = 0; }
+  return SHARK_OK;
+}
+
+extern "C" int shark_gpu_kernel_time(shark_ctx* c, const char* name, double* avg_ms, int64_t* count) {
+  if (!c || !name) return fail(SHARK_EINVAL, "null argument");
+  for (int k = 0; k < KT_N; ++k)
+    if (strcmp(name, kKernelNames[k]) == 0) {
+      if (avg_ms) *avg_ms = c->kt_n[k] ? c->kt_ms[k] / c->kt_n[k] : 0.0;
+      if (count) *count = c->kt_n[k];
+      return SHARK_OK;
+    }
+  return fail(SHARK_EINVAL, "unknown kernel name '%s'", name);
+}
+
+extern "C" int shark_gpu_destroy(shark_ctx* c) {
+  if (!c) return SHARK_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  if (c->growth) shark_growth_free(c->growth);
+  hipFree(c->dU); hipFree(c->dB); hipFree(c->dcs); hipFree(c->dgeom); hipFree(c->dsc);
+  if (c->dsdust) hipFree(c->dsdust);
+  if (c->dcond) hipFree(c->dcond);
+  if (c->dfargo) hipFree(c->dfargo);
+  if (c->hsc) hipHostFree(c->hsc);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return SHARK_OK;
+}

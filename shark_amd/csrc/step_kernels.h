@@ -1,0 +1,62 @@
+// step_kernels.h — argument block and launchers of the per-step kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace shark {
+
+// Scalars that live on the device so consecutive steps need no host round trip.
+struct DevScalars {
+  double dt;            // dt of the step being taken (computed at the end of the previous one)
+  double time;          // simulated time before the step
+  double tend;          // run stops once time >= tend (flag_continue)
+  double dt_last;       // dt of the last completed step
+  unsigned long long dtbits;  // running min of next-step dt candidates (ordered bit pattern)
+  long long nstep;      // completed steps
+  int active;           // 0 once time >= tend: every kernel returns immediately
+  int pad;
+};
+
+// Geometry of a separable grid (every reference grid is: gridinit builds dx, surf, vol and
+// radii_c from ix alone, src/grid/grid.f90:404-455,480-496): one entry per column ix.
+struct ColGeom {
+  const double* dx1;    // dx(:,1)
+  const double* dx2;    // dx(:,2)
+  const double* surf1;  // surf(:,1)
+  const double* surf2;  // surf(:,2)
+  const double* vol;
+  const double* rc;     // radii_c
+  const double* pos1;   // position(:,1)
+};
+
+struct StepArgs {
+  double* U;            // state, (nvar, ncells)
+  double* B;            // post-hydro state of the current step, (nvar, ncells)
+  const double* cs;     // (ncells)
+  ColGeom g;
+  const double* sdust;  // (ndust, ncells) or nullptr when sdust_uniform
+  const double* condinit;   // (nvar, ncells), disk in-loop hook
+  const double* fargo_vk;   // fargo_velocity per column
+  DevScalars* sc;
+  int nx, ny, nd, nxm, nym, nc, nvar, ndim;
+  int geom, solver, solverdust, slope, force_kick, force_kind, tstop_kind, inloop_kind;
+  int bc_kind, drag, back_reaction, fargo, disk_bc_mode, sdust_uniform;
+  int strip_h;          // rows per hydro work item
+  int ngroups, nstrips; // hydro decomposition
+  double gamma, cfl, smallr, dust_ratio_min;
+  double q_shear, omega_shear, box_l, rad0, eta_stream, rho_init, hoverr, rhograin;
+  double mstar, r_relax, n_rel, smooth_r;
+  double K[64];
+  double sdust_k[64];   // per-species grain size when uniform over cells
+};
+
+void launch_bc(const StepArgs& a, hipStream_t s);
+void launch_hydro(const StepArgs& a, hipStream_t s);
+void launch_post(const StepArgs& a, hipStream_t s);
+void launch_courant_init(const StepArgs& a, hipStream_t s);
+void launch_finalize(const StepArgs& a, hipStream_t s);
+void launch_set_dt(const StepArgs& a, hipStream_t s);
+void launch_arm(const StepArgs& a, double tend, hipStream_t s);
+void launch_ctoprim(const StepArgs& a, double* q, hipStream_t s);
+
+}  // namespace shark

@@ -1,0 +1,714 @@
+// step_kernels.hip — the per-step kernels of SHARK's Godunov step on gfx950.
+//
+// One step (src/solver/solve.f90:10-181) runs as four launches on one stream:
+//   k_bc       ghost fill of the state (apply_boundaries)
+//   k_hydro    per fluid: ctoprim + force + kick + predictor + Riemann fluxes + update +
+//              polar sources, fused; a wavefront owns 60 columns x a strip of rows and
+//              marches the strip row by row with a 3-row register window; x neighbours are
+//              exchanged with cross-lane shuffles.  qm/qp/flux/dq never reach HBM.
+//   k_post     per cell, all fluids in registers: FARGO remap, implicit drag + floor,
+//              in-loop hook, and the next step's Courant candidate (dt fused into the
+//              epilogue of the previous step; the min over cells is an integer atomic min
+//              on the ordered bit pattern, exact and order independent)
+//   k_finalize time += dt; dt <- min candidate
+// All arithmetic keeps the reference's evaluation order; see physics.h.
+#include <cfloat>
+
+#include "physics.h"
+#include "step_kernels.h"
+
+namespace shark {
+
+#define DEV __device__ __forceinline__
+
+DEV int cell_of(const StepArgs& a, int ix, int iy) {
+  return (a.ndim == 1) ? (ix - 1) : (ix - 1) + (iy - 1) * a.nxm;
+}
+DEV bool is_active(const StepArgs& a, int ix, int iy) {
+  return ix >= 3 && ix <= a.nx + 2 && (a.ndim == 1 || (iy >= 3 && iy <= a.ny + 2));
+}
+
+// External accelerations of update_force_setup for an active cell (SI: setup.f90:346-353,
+// disk_polar: setup.f90:366-377).  qa/qb: gas (vx, vz) or dust (vdx, vdz) velocities.
+DEV void cell_force(const StepArgs& a, int ix, double qx, double qz, bool gas, double& f1,
+                    double& f2, double& f3) {
+  f1 = 0.0; f2 = 0.0; f3 = 0.0;
+  if (a.force_kind == 1) {
+    const double Om = a.omega_shear;
+    double t = ((2.0 * a.q_shear) * (Om * Om)) * (a.g.pos1[ix - 1] - kHalf * a.box_l);
+    if (gas) f1 = (t - (2.0 * Om) * qz) - ((2.0 * a.rad0) * Om) * a.eta_stream;
+    else f1 = t - (2.0 * Om) * qz;
+    f3 = (2.0 * Om) * qx;
+  } else if (a.force_kind == 2) {
+    const double r = a.g.rc[ix - 1];
+    f1 = -(a.mstar / (r * r));
+  }
+}
+
+// ------------------------------------------------------------------ ghost fill
+// apply_boundaries: periodic (src/grid/boundary_types.f90:149-186), SI shearing box
+// (src/setups/SI/boundaries.f90:10-85), disk (src/setups/disk_polar/boundaries.f90:10-62).
+// The reference fills y-ghost rows of active columns first, then x-ghost columns of every
+// row; each thread here resolves its ghost cell through those (at most two) copies.
+__global__ void k_bc(StepArgs a) {
+  if (!a.sc->active) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n1 = (a.ndim == 2) ? 4 * a.nx : 0;
+  const int n2 = 4 * a.nym;
+  if (t >= n1 + n2) return;
+  const size_t nc = a.nc;
+  auto ysrc = [&](int iy) { return iy <= 2 ? a.ny + iy : iy - a.ny; };
+  auto is_grow = [&](int iy) { return a.ndim == 2 && (iy <= 2 || iy >= a.ny + 3); };
+  if (t < n1) {
+    const int r = t / a.nx, ix = 3 + t % a.nx;
+    const int iy = (r == 0) ? 1 : (r == 1) ? 2 : (r == 2) ? a.nym : a.nym - 1;
+    const int dst = cell_of(a, ix, iy), src = cell_of(a, ix, ysrc(iy));
+    for (int v = 0; v < a.nvar; ++v) a.U[v * nc + dst] = a.U[v * nc + src];
+    return;
+  }
+  const int t2 = t - n1;
+  const int c = t2 / a.nym, iy = 1 + t2 % a.nym;
+  const int ix = (c == 0) ? 1 : (c == 1) ? 2 : (c == 2) ? a.nxm : a.nxm - 1;
+  const bool left = ix <= 2;
+  int sx, sy = iy;
+  if (a.bc_kind == 2 && a.disk_bc_mode == 0) {
+    // boundaries.f90:45-46: first_active/last_active used as flat cell indices
+    const int flat = left ? 3 : a.nx + 2;
+    if (a.ndim == 2) { sx = (flat - 1) % a.nxm + 1; sy = (flat - 1) / a.nxm + 1; }
+    else { sx = flat; sy = 1; }
+  } else if (a.bc_kind == 2) {
+    sx = left ? 3 : a.nx + 2;
+  } else {
+    sx = left ? a.nx + ix : ix - a.nx;
+  }
+  if (is_grow(sy)) sy = ysrc(sy);  // the source itself is a y-ghost filled first
+  const int dst = cell_of(a, ix, iy), src = cell_of(a, sx, sy);
+  for (int v = 0; v < a.nvar; ++v) a.U[v * nc + dst] = a.U[v * nc + src];
+  if (a.bc_kind == 1) {
+    const double s = left ? -1.0 : 1.0;
+    // boundaries.f90:66-77
+    const double sh = ((a.U[0 * nc + src] * a.q_shear) * a.omega_shear) * a.box_l;
+    a.U[3 * nc + dst] = left ? a.U[3 * nc + src] - sh : a.U[3 * nc + src] + sh;
+    for (int k = 0; k < a.nd; ++k) {
+      const int ir = 5 + k, iz = 5 + 3 * a.nd + k;
+      const double shd = ((a.U[ir * nc + src] * a.q_shear) * a.omega_shear) * a.box_l;
+      a.U[iz * nc + dst] = left ? a.U[iz * nc + src] - shd : a.U[iz * nc + src] + shd;
+    }
+    (void)s;
+  } else if (a.bc_kind == 2) {
+    // boundaries.f90:51-57 inflow clamps
+    double& vx = a.U[1 * nc + dst];
+    vx = left ? fmaxf_(vx, 0.0) : fminf_(vx, 0.0);
+    for (int k = 0; k < a.nd; ++k) {
+      double& vd = a.U[(5 + a.nd + k) * nc + dst];
+      vd = left ? fmaxf_(vd, 0.0) : fminf_(vd, 0.0);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ hydro
+// Per-lane state of one loaded row of one fluid.
+struct RowF {
+  double u[5];   // conservative (gas: rho, mx, my, mz, E; dust: rhod, mdx, mdy, mdz)
+  double q[4];   // primitive rho, vx, vy, vz (ctoprim, solve.f90:211-232)
+  double cs;
+};
+struct FaceSet {
+  double m[4], p[4];  // qm (state at the right face) and qp (left face) of rho, vx, vy, vz
+};
+
+DEV double shfl_up1(double x) { return __shfl_up(x, 1); }
+DEV double shfl_dn1(double x) { return __shfl_down(x, 1); }
+
+template <int NDIM, int GEOM, bool GAS>
+__device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int lane) {
+  constexpr int NU = GAS ? 5 : 4;
+  const size_t nc = a.nc;
+  int p[5];
+  if (GAS) { p[0] = 0; p[1] = 1; p[2] = 2; p[3] = 3; p[4] = 4; }
+  else {
+    const int k = fluid - 1;
+    p[0] = 5 + k; p[1] = 5 + a.nd + k; p[2] = 5 + 2 * a.nd + k; p[3] = 5 + 3 * a.nd + k; p[4] = p[3];
+  }
+  const bool need_cs = GAS || a.solverdust == 3;
+  const int ix = 1 + 60 * g + lane;
+  const int ixc = ix < a.nxm ? ix : a.nxm;
+  const int ixl = ixc > 1 ? ixc - 1 : 1;
+  const int ixr = ixc < a.nxm ? ixc + 1 : a.nxm;
+  const bool own_col = lane >= 2 && lane <= 61 && ix <= a.nx + 2;
+  const double dt = a.sc->dt;
+  const int st = a.slope;
+  const double dx1 = a.g.dx1[ixc - 1], dx1m = a.g.dx1[ixl - 1];
+  const double rp = (GEOM == 2) ? a.g.rc[ixc - 1] : 1.0;
+  const double dx2 = (NDIM == 2) ? a.g.dx2[ixc - 1] : 0.0;
+  const double surf1 = a.g.surf1[ixc - 1], surf1r = a.g.surf1[ixr - 1];
+  const double surf2 = (NDIM == 2) ? a.g.surf2[ixc - 1] : 0.0;
+  const double vol = a.g.vol[ixc - 1];
+  const bool col_active = ix >= 3 && ix <= a.nx + 2;
+
+  int y0 = 1, y1 = 1;
+  if (NDIM == 2) {
+    y0 = 3 + strip * a.strip_h;
+    y1 = y0 + a.strip_h - 1;
+    if (y1 > a.ny + 2) y1 = a.ny + 2;
+  }
+
+  auto load_row = [&](int j, RowF& r) {
+    const int cell = (NDIM == 1) ? ixc - 1 : (ixc - 1) + (j - 1) * a.nxm;
+#pragma unroll
+    for (int c = 0; c < NU; ++c) r.u[c] = a.U[p[c] * nc + cell];
+    if (GAS) {
+      r.q[0] = fmaxf_(r.u[0], a.smallr);
+      r.q[1] = r.u[1] / r.u[0];
+      r.q[2] = r.u[2] / r.u[0];
+      r.q[3] = r.u[3] / r.u[0];
+    } else {
+      r.q[0] = r.u[0];
+      r.q[1] = r.u[1] / r.u[0];
+      r.q[2] = r.u[2] / r.u[0];
+      r.q[3] = r.u[3] / r.u[0];
+    }
+    r.cs = need_cs ? a.cs[cell] : 0.0;
+  };
+
+  // face states of a row (predictor, godunov.f90:89-611); dqy = 0 in 1D
+  auto faces = [&](const RowF& r, const double dqx[4], const double dqy[4], int iy, FaceSet& fx,
+                   FaceSet& fy) {
+    double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+    if (a.force_kick && col_active && (NDIM == 1 || (iy >= 3 && iy <= a.ny + 2)))
+      cell_force(a, ix, r.q[1], r.q[3], GAS, f1, f2, f3);
+    const double r_rho = r.q[0];
+    double u = r.q[1], v = r.q[2], w = r.q[3];
+    const double drx = dqx[0], dux = dqx[1], dvx = dqx[2], dwx = dqx[3];
+    const double dry = dqy[0], duy = dqy[1], dvy = dqy[2], dwy = dqy[3];
+    double sr0, su0, sv0, sw0;
+    if (a.force_kick) {
+      u = u + (f1 * kHalf) * dt;
+      v = v + (f2 * kHalf) * dt;
+      w = w + (f3 * kHalf) * dt;
+    }
+    if (GAS) {
+      const double cs2 = r.cs * r.cs;
+      const double dpx = dqx[0] * cs2;
+      const double dpy = (NDIM == 2) ? dqy[0] * cs2 : 0.0;
+      if (GEOM == 2) {
+        sr0 = ((-(u * drx)) - v * dry - (dux + dvy) * r_rho) - (r_rho * u) / rp;
+        su0 = ((-(u * dux)) - v * duy - dpx / r_rho) + (v * v) / rp;
+        sv0 = ((-(u * dvx)) - v * dvy - dpy / r_rho) - (u * v) / rp;
+        sw0 = (-(u * dwx)) - v * dwy;
+      } else {
+        sr0 = (-(u * drx)) - v * dry - (dux + dvy) * r_rho;
+        su0 = (-(u * dux)) - v * duy - dpx / r_rho;
+        sv0 = (-(u * dvx)) - v * dvy - dpy / r_rho;
+        sw0 = (-(u * dwx)) - v * dwy;
+      }
+    } else {
+      sr0 = (-(u * drx)) - v * dry - (dux + dvy) * r_rho;
+      su0 = (-(u * dux)) - v * duy;
+      sv0 = (-(u * dvx)) - v * dvy;
+      sw0 = (-(u * dwx)) - v * dwy;
+      if (GEOM == 2) {
+        sr0 = sr0 - (r_rho * u) / rp;
+        su0 = su0 + (v * v) / rp;
+        sv0 = sv0 - (u * v) / rp;
+      }
+    }
+    const double hdt = kHalf * dt;
+    const double b0 = r_rho + hdt * sr0, b1 = u + hdt * su0, b2 = v + hdt * sv0, b3 = w + hdt * sw0;
+    fx.m[0] = b0 + kHalf * drx * dx1; fx.p[0] = b0 - kHalf * drx * dx1;
+    fx.m[1] = b1 + kHalf * dux * dx1; fx.p[1] = b1 - kHalf * dux * dx1;
+    fx.m[2] = b2 + kHalf * dvx * dx1; fx.p[2] = b2 - kHalf * dvx * dx1;
+    fx.m[3] = b3 + kHalf * dwx * dx1; fx.p[3] = b3 - kHalf * dwx * dx1;
+    if (NDIM == 2) {
+      const double dyl = rp * dx2;
+      fy.m[0] = b0 + kHalf * dry * dyl; fy.p[0] = b0 - kHalf * dry * dyl;
+      fy.m[1] = b1 + kHalf * duy * dyl; fy.p[1] = b1 - kHalf * duy * dyl;
+      fy.m[2] = b2 + kHalf * dvy * dyl; fy.p[2] = b2 - kHalf * dvy * dyl;
+      fy.m[3] = b3 + kHalf * dwy * dyl; fy.p[3] = b3 - kHalf * dwy * dyl;
+    }
+  };
+
+  // Riemann flux between left state L(q) and right state R(q); dir 0: x (normal vx),
+  // dir 1: y (normal vy). Returns conservative components in plane order.
+  auto riemann = [&](const double L[4], double csl, const double R[4], double csr, int dir,
+                     double F[5]) {
+    Face fl, fr;
+    fl.r = L[0]; fr.r = R[0];
+    fl.un = dir == 0 ? L[1] : L[2]; fl.ut = dir == 0 ? L[2] : L[1]; fl.w = L[3];
+    fr.un = dir == 0 ? R[1] : R[2]; fr.ut = dir == 0 ? R[2] : R[1]; fr.w = R[3];
+    Flux5 f;
+    if (GAS) {
+      fl.p = (csl * csl) * L[0];   // isothermal face pressure, godunov.f90:606-611
+      fr.p = (csr * csr) * R[0];
+      f = gas_flux(a.solver, a.gamma, fl, fr, csl, csr);
+    } else {
+      fl.p = fr.p = 0.0;
+      f = dust_flux(a.solverdust, fl, fr, csl, csr);
+    }
+    F[0] = f.r;
+    F[1] = dir == 0 ? f.un : f.ut;
+    F[2] = dir == 0 ? f.ut : f.un;
+    F[3] = f.w;
+    F[4] = f.e;
+  };
+
+  // ---- march
+  RowF R0, R1, R2;
+  FaceSet fx1, fy1;
+  double sLy_prev[4] = {0, 0, 0, 0};
+  double fym_prev[4] = {0, 0, 0, 0};
+  double cs_prev = 0.0;
+  double Fx_own_prev[5] = {0, 0, 0, 0, 0}, Fx_rt_prev[5] = {0, 0, 0, 0, 0};
+  double Fy_bot[5] = {0, 0, 0, 0, 0};
+
+  const int jstart = (NDIM == 2) ? y0 - 2 : 1;
+  const int jend = (NDIM == 2) ? y1 + 2 : 1;
+  for (int j = jstart; j <= jend; ++j) {
+    R0 = R1;
+    R1 = R2;
+    load_row(j, R2);
+    // in 1D the single row is row "R1" of the loop body below
+    const RowF& Rm = (NDIM == 2) ? R1 : R2;
+    const int m = (NDIM == 2) ? j - 1 : 1;   // row whose face states are built now
+    double sLy_new[4] = {0, 0, 0, 0};
+    if (NDIM == 2 && j >= y0 - 1) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sLy_new[c] = (2.0 * (R2.q[c] - R1.q[c])) / (dx2 + dx2);
+    }
+    const bool have_faces = (NDIM == 1) || (j >= y0);
+    double Fx_own[5] = {0, 0, 0, 0, 0}, Fx_rt[5] = {0, 0, 0, 0, 0};
+    double Fy_top[5] = {0, 0, 0, 0, 0};
+    if (have_faces) {
+      double dqx[4], dqy[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double ql = shfl_up1(Rm.q[c]);
+        const double sL = (2.0 * (Rm.q[c] - ql)) / (dx1 + dx1m);
+        const double sR = shfl_dn1(sL);
+        dqx[c] = slope_limit(st, sL, sR);
+        dqy[c] = (NDIM == 2) ? slope_limit(st, sLy_prev[c], sLy_new[c]) / rp : 0.0;
+      }
+      faces(Rm, dqx, dqy, m, fx1, fy1);
+      // x fluxes of row m, needed when m is an owned row
+      if (NDIM == 1 || (m >= y0 && m <= y1)) {
+        double L[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) L[c] = shfl_up1(fx1.m[c]);
+        const double csl = shfl_up1(Rm.cs);
+        riemann(L, csl, fx1.p, Rm.cs, 0, Fx_own);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) Fx_rt[c] = shfl_dn1(Fx_own[c]);
+      }
+      // y flux through the face between rows m-1 and m
+      if (NDIM == 2 && m - 1 >= y0 - 1 && m - 1 <= y1) riemann(fym_prev, cs_prev, fy1.p, Rm.cs, 1, Fy_top);
+    }
+    // finalize row m-1 (2D) or the single row (1D)
+    const int fr = (NDIM == 2) ? m - 1 : 1;
+    const RowF& Rf = (NDIM == 2) ? R0 : R2;
+    const double* Fxo = (NDIM == 2) ? Fx_own_prev : Fx_own;
+    const double* Fxr = (NDIM == 2) ? Fx_rt_prev : Fx_rt;
+    if (own_col && ((NDIM == 1) || (have_faces && fr >= y0 && fr <= y1))) {
+      const int cell = (NDIM == 1) ? ixc - 1 : (ixc - 1) + (fr - 1) * a.nxm;
+      double uk[5];
+#pragma unroll
+      for (int c = 0; c < NU; ++c) uk[c] = Rf.u[c];
+      if (a.force_kick) {  // kick(1.0d0), force_kicks.f90:12-39
+        double f1, f2, f3;
+        cell_force(a, ix, Rf.q[1], Rf.q[3], GAS, f1, f2, f3);
+        const double rho = Rf.u[0];
+        if (GAS) {
+          const double eold = ((kHalf * (uk[1] * uk[1])) / rho + (kHalf * (uk[2] * uk[2])) / rho) +
+                              (kHalf * (uk[3] * uk[3])) / rho;
+          uk[1] = uk[1] + ((rho * 1.0) * dt) * f1;
+          uk[2] = uk[2] + ((rho * 1.0) * dt) * f2;
+          uk[3] = uk[3] + ((rho * 1.0) * dt) * f3;
+          const double enew = ((kHalf * (uk[1] * uk[1])) / rho + (kHalf * (uk[2] * uk[2])) / rho) +
+                              (kHalf * (uk[3] * uk[3])) / rho;
+          uk[4] = uk[4] + (enew - eold);
+        } else {
+          uk[1] = uk[1] + ((rho * 1.0) * dt) * f1;
+          uk[2] = uk[2] + ((rho * 1.0) * dt) * f2;
+          uk[3] = uk[3] + ((rho * 1.0) * dt) * f3;
+        }
+      }
+      // sources (source_terms.f90:37-47), from the start-of-step primitives
+      double S[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      if (GEOM == 2) {
+        const double r = rp;
+        if (GAS) {
+          const double P = Rf.u[0] * (Rf.cs * Rf.cs);
+          S[1] = 0.0 + dt * (Rf.q[0] * (Rf.q[2] * Rf.q[2]) + P) / r;
+          S[2] = 0.0 - dt * (Rf.q[0] * Rf.q[2] * Rf.q[1] / r);
+        } else {
+          S[1] = 0.0 + dt * (Rf.q[0] * (Rf.q[2] * Rf.q[2]) / r);
+          S[2] = 0.0 - dt * (Rf.q[0] * Rf.q[2] * Rf.q[1] / r);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NU; ++c) {
+        double t = uk[c] + (Fxo[c] * surf1 - Fxr[c] * surf1r) / vol * dt;
+        if (NDIM == 2) t = t + (Fy_bot[c] * surf2 - Fy_top[c] * surf2) / vol * dt;
+        t = t + S[c];
+        a.B[p[c] * nc + cell] = t;
+      }
+    }
+    // rotate
+    if (NDIM == 2) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) { sLy_prev[c] = sLy_new[c]; fym_prev[c] = fy1.m[c]; }
+      cs_prev = R1.cs;
+#pragma unroll
+      for (int c = 0; c < 5; ++c) { Fx_own_prev[c] = Fx_own[c]; Fx_rt_prev[c] = Fx_rt[c]; Fy_bot[c] = Fy_top[c]; }
+    }
+  }
+}
+
+template <int NDIM, int GEOM>
+__global__ __launch_bounds__(256) void k_hydro(StepArgs a) {
+  if (!a.sc->active) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long item = (long)blockIdx.x * 4 + wave;
+  const long G = a.ngroups, S = a.nstrips, F = 1 + a.nd;
+  if (item >= G * S * F) return;
+  const int g = (int)(item % G);
+  const long t = item / G;
+  const int strip = (int)(t % S);
+  const int fluid = (int)(t / S);
+  if (fluid == 0) hydro_fluid<NDIM, GEOM, true>(a, 0, g, strip, lane);
+  else hydro_fluid<NDIM, GEOM, false>(a, fluid, g, strip, lane);
+}
+
+// ------------------------------------------------------------------ post
+// Courant candidate of one active cell (courant.f90:68-158), from its conservative state.
+template <int ND>
+DEV double courant_cell(const StepArgs& a, const double* un, int ix, int cell) {
+  double dxx;
+  if (a.ndim == 1) dxx = a.g.dx1[ix - 1];
+  else if (a.geom == 2) dxx = fminf_(a.g.dx1[ix - 1], a.g.rc[ix - 1] * a.g.dx2[ix - 1]);
+  else dxx = fminf_(a.g.dx1[ix - 1], a.g.dx2[ix - 1]);
+  const double rho = un[0];
+  const double vx = un[1] / rho, vy = un[2] / rho, vz = un[3] / rho;
+  const double cs = a.cs[cell];
+  double vv = (fabs(vx) + fabs(vy)) + fabs(vz);
+  double vmax = cs + vv;
+  double vdx[ND > 0 ? ND : 1], vdz[ND > 0 ? ND : 1];
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    const double rd = un[5 + k];
+    vdx[k] = un[5 + ND + k] / rd;
+    const double vdy = un[5 + 2 * ND + k] / rd;
+    vdz[k] = un[5 + 3 * ND + k] / rd;
+    vv = (fabs(vdx[k]) + fabs(vdy)) + fabs(vdz[k]);
+    vmax = fmaxf_(vmax, 0.0 + vv);
+  }
+  double dt = (a.cfl * dxx) / fabs(vmax);
+  if (a.force_kick) {
+    double f1, f2, f3;
+    cell_force(a, ix, vx, vz, true, f1, f2, f3);
+    double fm = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+      cell_force(a, ix, vdx[k], vdz[k], false, f1, f2, f3);
+      fm = fmaxf_(fm, sqrt((f1 * f1 + f2 * f2) + f3 * f3));
+    }
+    if (vv != 0.0) {
+      const double fratio = fmaxf_((fm * dxx) / (vv * vv), 1e-3);
+      dt = fminf_(dt, (((a.cfl * dxx) / vv) * (sqrt(1.0 + (2.0 * a.cfl) * fratio) - 1.0)) / fratio);
+    }
+  }
+  return dt;
+}
+
+DEV unsigned long long wave_min_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    unsigned long long w = __shfl_xor(v, o);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
+DEV void block_atomic_min(DevScalars* sc, double d) {
+  __shared__ unsigned long long red[4];
+  unsigned long long v = wave_min_u64(dbits(d));
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = red[w] < m ? red[w] : m;
+    atomicMin(&sc->dtbits, m);
+  }
+}
+
+template <int ND>
+DEV double tstop_of(const StepArgs& a, int k, int cell, int ix, double rho_q, double rhod_q) {
+  if (a.tstop_kind == 0) return ((rhod_q * rho_q) / a.K[k]) / rho_q;  // DUSTYWAVE setup.f90:217
+  const double s = a.sdust_uniform ? a.sdust_k[k] : a.sdust[(size_t)k * a.nc + cell];
+  if (a.tstop_kind == 1)  // SI setup.f90:392
+    return ((a.rhograin * s) / a.rho_init) / ((a.rad0 * a.omega_shear) * a.hoverr);
+  const double H = a.hoverr * a.g.rc[ix - 1];  // disk_polar setup.f90:436
+  return (sqrt((kPi * a.gamma) / 8.0) * a.rhograin) * s /
+         ((rho_q / (sqrt(2.0 * kPi) * H)) * a.cs[cell]);
+}
+
+template <int ND, bool FARGO, int NDIM>
+__global__ __launch_bounds__(256) void k_post(StepArgs a) {
+  if (!a.sc->active) return;
+  constexpr int NV = 5 + 4 * ND;
+  int ix, iy;
+  if (NDIM == 2) {
+    ix = 3 + blockIdx.x * 64 + (threadIdx.x & 63);
+    iy = 3 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  } else {
+    ix = 3 + blockIdx.x * 256 + threadIdx.x;
+    iy = 1;
+  }
+  const bool valid = ix <= a.nx + 2 && (NDIM == 1 || iy <= a.ny + 2);
+  double dtc = 2e44;
+  if (valid) {
+    const size_t nc = a.nc;
+    const int cell = cell_of(a, ix, iy);
+    const double dt = a.sc->dt;
+    double un[NV];
+    if (FARGO) {
+      // fargo_scheme.f90:44-74 (one = 1.0d0, FP64 eps, periodic slopes; rows wrap mod ny)
+      const double rp = (a.geom == 2) ? a.g.rc[ix - 1] : 1.0;
+      const double vk = a.fargo_vk[ix - 1];
+      const double dy = a.g.dx2[ix - 1] * rp;
+      const double sh = fabs(vk * dt);
+      const int nshft = (int)(sh / dy);
+      const double eps = fmod(sh, dy) / dy;
+      const int ny = a.ny;
+      int j0 = (vk > 0.0) ? iy - nshft - 1 : iy + nshft;
+      j0 = ((j0 - 3) % ny + ny) % ny + 3;
+      const int j1 = (j0 - 3 + 1) % ny + 3;
+      const int jm = (j0 - 3 + ny - 1) % ny + 3;
+      const int jp = (j1 - 3 + 1) % ny + 3;
+      const int cm = cell_of(a, ix, jm), c0 = cell_of(a, ix, j0), c1 = cell_of(a, ix, j1),
+                cp = cell_of(a, ix, jp);
+      const double h2 = (kHalf - eps) * (kHalf - eps);
+      const double w0 = (vk > 0.0) ? eps : (1.0 - eps), w1 = (vk > 0.0) ? (1.0 - eps) : eps;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const double* Bv = a.B + v * nc;
+        const double um = Bv[cm], u0 = Bv[c0], u1 = Bv[c1], up = Bv[cp];
+        const double d0 = slope_limit(a.slope, u0 - um, u1 - u0);
+        const double d1 = slope_limit(a.slope, u1 - u0, up - u1);
+        double x = u0 * w0 + u1 * w1;
+        x = x + kHalf * d0 * (0.25 - h2) + kHalf * d1 * (h2 - 0.25);
+        un[v] = x;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) un[v] = a.B[v * nc + cell];
+    }
+    if (ND > 0 && a.drag) {
+      // dust_drag(1.0d0), dust_drag.f90:21-73, with the stopping times of the step
+      // (compute_tstop from the start-of-step primitives)
+      const double rho_q = fmaxf_(a.U[cell], a.smallr);
+      double ak[ND > 0 ? ND : 1];
+      double rhon = un[0], pnx = un[1], pny = un[2], pnz = un[3];
+#pragma unroll
+      for (int k = 0; k < ND; ++k) {
+        const double ts = tstop_of<ND>(a, k, cell, ix, rho_q, a.U[(5 + k) * nc + cell]);
+        ak[k] = (1.0 * dt) / ts;
+        const double f = ak[k] / (1.0 + ak[k]);
+        pnx = pnx + f * un[5 + ND + k];
+        pny = pny + f * un[5 + 2 * ND + k];
+        pnz = pnz + f * un[5 + 3 * ND + k];
+        rhon = rhon + f * un[5 + k];
+      }
+#pragma unroll
+      for (int k = 0; k < ND; ++k) {
+        const double f = ak[k] / (1.0 + ak[k]);
+        const double rd = un[5 + k];
+        un[5 + ND + k] = un[5 + ND + k] / (1.0 + ak[k]) + f * pnx / rhon * rd;
+        un[5 + 2 * ND + k] = un[5 + 2 * ND + k] / (1.0 + ak[k]) + f * pny / rhon * rd;
+        un[5 + 3 * ND + k] = un[5 + 3 * ND + k] / (1.0 + ak[k]) + f * pnz / rhon * rd;
+      }
+      if (a.back_reaction) {
+        un[1] = pnx / rhon * un[0];
+        un[2] = pny / rhon * un[0];
+        un[3] = pnz / rhon * un[0];
+      }
+#pragma unroll
+      for (int k = 0; k < ND; ++k) un[5 + k] = fmaxf_(un[0] * a.dust_ratio_min, un[5 + k]);
+    }
+    if (a.inloop_kind == 2) {
+      // disk_polar setup_inloop, setup.f90:226-281
+      const double r = a.g.rc[ix - 1];
+      if (r < a.r_relax) {
+        const double t_rel = a.n_rel * 2.0 * kPi / (sqrt(a.mstar / (r * r * r))) -
+                             a.n_rel * 2.0 * kPi / (sqrt(a.mstar / (a.smooth_r * a.smooth_r * a.smooth_r)));
+        const double e = exp(-dt / t_rel);
+        const double* ci = a.condinit;
+#pragma unroll
+        for (int f = 0; f <= ND; ++f) {
+          const int ir = f == 0 ? 0 : 5 + f - 1;
+          const int iu = f == 0 ? 1 : 5 + ND + f - 1;
+          const int iv = f == 0 ? 2 : 5 + 2 * ND + f - 1;
+          const double rho_i = ci[ir * nc + cell], vx_i = ci[iu * nc + cell] / rho_i,
+                       vy_i = ci[iv * nc + cell] / rho_i;
+          const double rho_o = un[ir], vx_o = un[iu] / rho_o, vy_o = un[iv] / rho_o;
+          const double rho_n = (rho_i) * (1.0 - e) + rho_o * e;
+          const double vx_n = (vx_i) * (1.0 - e) + vx_o * e;
+          const double vy_n = (vy_i) * (1.0 - e) + vy_o * e;
+          un[ir] = rho_n; un[iu] = rho_n * vx_n; un[iv] = rho_n * vy_n;
+        }
+      }
+      if (ix == 3) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) un[v] = a.condinit[v * nc + cell];
+      }
+    } else if (a.inloop_kind == 1 && !a.force_kick) {
+      // SI Crank-Nicolson rotation, SI/setup.f90:309-331
+      const double Om = a.omega_shear, O = Om * dt;
+      const double x0 = a.g.pos1[ix - 1] - kHalf * a.box_l;
+      double AA = (2.0 * a.q_shear) * Om * x0 - (2.0 * a.rad0) * a.eta_stream;
+      double u = un[3] / un[0], v = un[1] / un[0], d = un[0];
+      un[3] = d * (u * (1. - O * O) + 2. * v * O + AA * (O * O)) / (1. + O * O);
+      un[1] = d * (v * (1. - O * O) - 2. * u * O + AA * O) / (1. + O * O);
+      AA = (2.0 * a.q_shear) * Om * x0;
+#pragma unroll
+      for (int k = 0; k < ND; ++k) {
+        const int ir = 5 + k, ix_ = 5 + ND + k, iz = 5 + 3 * ND + k;
+        u = un[iz] / un[ir]; v = un[ix_] / un[ir]; d = un[ir];
+        un[iz] = d * (u * (1. - O * O) + 2. * v * O + AA * (O * O)) / (1. + O * O);
+        un[ix_] = d * (v * (1. - O * O) - 2. * u * O + AA * O) / (1. + O * O);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) a.U[v * nc + cell] = un[v];
+    dtc = courant_cell<ND>(a, un, ix, cell);
+  }
+  block_atomic_min(a.sc, dtc);
+}
+
+template <int ND, int NDIM>
+__global__ __launch_bounds__(256) void k_courant(StepArgs a) {
+  constexpr int NV = 5 + 4 * ND;
+  int ix, iy;
+  if (NDIM == 2) {
+    ix = 3 + blockIdx.x * 64 + (threadIdx.x & 63);
+    iy = 3 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  } else {
+    ix = 3 + blockIdx.x * 256 + threadIdx.x;
+    iy = 1;
+  }
+  double dtc = 2e44;
+  if (ix <= a.nx + 2 && (NDIM == 1 || iy <= a.ny + 2)) {
+    const int cell = cell_of(a, ix, iy);
+    double un[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) un[v] = a.U[(size_t)v * a.nc + cell];
+    dtc = courant_cell<ND>(a, un, ix, cell);
+  }
+  block_atomic_min(a.sc, dtc);
+}
+
+__global__ void k_set_dt(DevScalars* sc) {
+  sc->dt = __longlong_as_double((long long)sc->dtbits);
+  sc->dtbits = dbits(2e44);
+}
+
+__global__ void k_finalize(DevScalars* sc) {
+  if (!sc->active) return;
+  sc->time = sc->time + sc->dt;
+  sc->dt_last = sc->dt;
+  sc->nstep = sc->nstep + 1;
+  sc->dt = __longlong_as_double((long long)sc->dtbits);
+  sc->dtbits = dbits(2e44);
+  sc->active = sc->time < sc->tend ? 1 : 0;
+}
+
+__global__ void k_arm(DevScalars* sc, double tend) {
+  sc->tend = tend;
+  sc->active = sc->time < tend ? 1 : 0;
+}
+
+// ctoprim over every cell (solve.f90:206-248, isothermal)
+__global__ void k_ctoprim(StepArgs a, double* q) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.nc) return;
+  const size_t nc = a.nc;
+  const double rho = a.U[c];
+  q[c] = fmaxf_(rho, a.smallr);
+  q[nc + c] = a.U[nc + c] / rho;
+  q[2 * nc + c] = a.U[2 * nc + c] / rho;
+  q[3 * nc + c] = a.U[3 * nc + c] / rho;
+  q[4 * nc + c] = rho * (a.cs[c] * a.cs[c]);
+  for (int k = 0; k < a.nd; ++k) {
+    const double rd = a.U[(5 + k) * nc + c];
+    q[(5 + k) * nc + c] = rd;
+    q[(5 + a.nd + k) * nc + c] = a.U[(5 + a.nd + k) * nc + c] / rd;
+    q[(5 + 2 * a.nd + k) * nc + c] = a.U[(5 + 2 * a.nd + k) * nc + c] / rd;
+    q[(5 + 3 * a.nd + k) * nc + c] = a.U[(5 + 3 * a.nd + k) * nc + c] / rd;
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+void launch_bc(const StepArgs& a, hipStream_t s) {
+  const int n = (a.ndim == 2 ? 4 * a.nx : 0) + 4 * a.nym;
+  hipLaunchKernelGGL(k_bc, dim3((n + 255) / 256), dim3(256), 0, s, a);
+}
+
+void launch_hydro(const StepArgs& a, hipStream_t s) {
+  const long items = (long)a.ngroups * a.nstrips * (1 + a.nd);
+  const dim3 grid((unsigned)((items + 3) / 4));
+  if (a.ndim == 1) hipLaunchKernelGGL((k_hydro<1, 0>), grid, dim3(256), 0, s, a);
+  else if (a.geom == 2) hipLaunchKernelGGL((k_hydro<2, 2>), grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_hydro<2, 0>), grid, dim3(256), 0, s, a);
+}
+
+template <int ND>
+static void post_nd(const StepArgs& a, hipStream_t s) {
+  if (a.ndim == 1) {
+    dim3 grid((a.nx + 255) / 256);
+    hipLaunchKernelGGL((k_post<ND, false, 1>), grid, dim3(256), 0, s, a);
+  } else {
+    dim3 grid((a.nx + 63) / 64, (a.ny + 3) / 4);
+    if (a.fargo) hipLaunchKernelGGL((k_post<ND, true, 2>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_post<ND, false, 2>), grid, dim3(256), 0, s, a);
+  }
+}
+
+template <int ND>
+static void courant_nd(const StepArgs& a, hipStream_t s) {
+  if (a.ndim == 1) hipLaunchKernelGGL((k_courant<ND, 1>), dim3((a.nx + 255) / 256), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_courant<ND, 2>), dim3((a.nx + 63) / 64, (a.ny + 3) / 4), dim3(256), 0, s, a);
+}
+
+#define SHARK_ND_SWITCH(FN)                          \
+  switch (a.nd) {                                    \
+    case 0: FN<0>(a, s); break;                      \
+    case 1: FN<1>(a, s); break;                      \
+    case 2: FN<2>(a, s); break;                      \
+    case 3: FN<3>(a, s); break;                      \
+    case 4: FN<4>(a, s); break;                      \
+    case 8: FN<8>(a, s); break;                      \
+    case 16: FN<16>(a, s); break;                    \
+    default: break;                                  \
+  }
+
+void launch_post(const StepArgs& a, hipStream_t s) { SHARK_ND_SWITCH(post_nd) }
+void launch_courant_init(const StepArgs& a, hipStream_t s) {
+  SHARK_ND_SWITCH(courant_nd)
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, s, a.sc);
+}
+void launch_set_dt(const StepArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, s, a.sc);
+}
+void launch_finalize(const StepArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1), 0, s, a.sc);
+}
+void launch_arm(const StepArgs& a, double tend, hipStream_t s) {
+  hipLaunchKernelGGL(k_arm, dim3(1), dim3(1), 0, s, a.sc, tend);
+}
+void launch_ctoprim(const StepArgs& a, double* q, hipStream_t s) {
+  hipLaunchKernelGGL(k_ctoprim, dim3((a.nc + 255) / 256), dim3(256), 0, s, a, q);
+}
+
+bool nd_supported(int nd) { return nd == 0 || nd == 1 || nd == 2 || nd == 3 || nd == 4 || nd == 8 || nd == 16; }
+
+}  // namespace shark

@@ -1,0 +1,152 @@
+"""ctypes binding of libshark_gpu.so (include/shark_gpu.h).
+
+``Step`` mirrors the reference's usage of `solve` (src/solver/solve.f90:10): build it from a
+configuration, the grid arrays and the state after setup, then call ``step()`` once per time
+step (or ``run(n, tend)`` for n steps without host round trips) and ``download()`` at output
+times.  There is no CPU fallback: if the HIP library is missing or no GPU is visible the
+constructor raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .cfg import ShARKCfg
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libshark_gpu.so")
+_P = C.POINTER(C.c_double)
+
+# every entry point of include/shark_gpu.h
+EXPORTS = [
+    "shark_gpu_create", "shark_gpu_set_geometry", "shark_gpu_upload", "shark_gpu_step",
+    "shark_gpu_run", "shark_gpu_download", "shark_gpu_get_time", "shark_gpu_set_time",
+    "shark_gpu_kernel_time", "shark_gpu_set_timing", "shark_gpu_algorithmic_bytes",
+    "shark_gpu_destroy", "shark_gpu_last_error", "shark_gpu_version",
+    "shark_growth_create", "shark_growth_run", "shark_growth_download", "shark_growth_upload",
+    "shark_growth_destroy", "shark_growth_kernel_time",
+    "shark_mlp_create", "shark_mlp_set_params", "shark_mlp_get_params", "shark_mlp_forward",
+    "shark_mlp_train_step", "shark_mlp_destroy", "shark_mlp_grads",
+]
+
+
+class SharkGPUError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("arrays passed to libshark_gpu must be contiguous float64")
+    return a.ctypes.data_as(_P)
+
+
+def load():
+    """Load libshark_gpu.so (built in-tree by __graft_entry__.build())."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise SharkGPUError(f"{LIB_PATH} is missing; build it with `python -c "
+                            "'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(LIB_PATH)
+    lib.shark_gpu_last_error.restype = C.c_char_p
+    lib.shark_gpu_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+    lib.shark_gpu_set_geometry.argtypes = [C.c_void_p] + [_P] * 5
+    lib.shark_gpu_upload.argtypes = [C.c_void_p] + [_P] * 5
+    lib.shark_gpu_step.argtypes = [C.c_void_p, _P]
+    lib.shark_gpu_run.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.POINTER(C.c_int64), _P]
+    lib.shark_gpu_download.argtypes = [C.c_void_p, _P, _P]
+    lib.shark_gpu_get_time.argtypes = [C.c_void_p, _P, _P]
+    lib.shark_gpu_set_time.argtypes = [C.c_void_p, C.c_double]
+    lib.shark_gpu_kernel_time.argtypes = [C.c_void_p, C.c_char_p, _P, C.POINTER(C.c_int64)]
+    lib.shark_gpu_set_timing.argtypes = [C.c_void_p, C.c_int]
+    lib.shark_gpu_algorithmic_bytes.restype = C.c_int64
+    lib.shark_gpu_algorithmic_bytes.argtypes = [C.c_void_p]
+    lib.shark_gpu_destroy.argtypes = [C.c_void_p]
+    _LIB = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        raise SharkGPUError(f"libshark_gpu error {rc}: {load().shark_gpu_last_error().decode()}")
+
+
+class Step:
+    """Device-resident SHARK step for one configuration (one HIP stream, one context)."""
+
+    def __init__(self, cfg: ShARKCfg, geom: dict, state: dict):
+        self.L = load()
+        self.cfg = cfg
+        h = C.c_void_p()
+        check(self.L.shark_gpu_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.nvar = 5 + 4 * cfg.ndust
+        nxm = cfg.nx + 4
+        nym = cfg.ny + 4 if cfg.ny > 1 else 1
+        self.ncells = nxm * nym
+        g = {k: (None if v is None else np.ascontiguousarray(v, dtype=np.float64))
+             for k, v in geom.items()}
+        check(self.L.shark_gpu_set_geometry(self.h, _ptr(g["dx"]), _ptr(g["surf"]),
+                                            _ptr(g["vol"]), _ptr(g["radii_c"]),
+                                            _ptr(g.get("position"))))
+        self.upload(state)
+
+    def upload(self, state):
+        s = {k: (None if v is None else np.ascontiguousarray(v, dtype=np.float64))
+             for k, v in state.items()}
+        check(self.L.shark_gpu_upload(self.h, _ptr(s["u"]), _ptr(s["cs"]), _ptr(s.get("sdust")),
+                                      _ptr(s.get("condinit")), _ptr(s.get("fargo_velocity"))))
+
+    def step(self):
+        dt = C.c_double()
+        check(self.L.shark_gpu_step(self.h, C.byref(dt)))
+        return dt.value
+
+    def run(self, nsteps, tend=float("inf")):
+        n = C.c_int64()
+        t = C.c_double()
+        check(self.L.shark_gpu_run(self.h, int(nsteps), float(tend), C.byref(n), C.byref(t)))
+        return n.value, t.value
+
+    def download(self, prims=False):
+        u = np.empty(self.nvar * self.ncells)
+        q = np.empty(self.nvar * self.ncells) if prims else None
+        check(self.L.shark_gpu_download(self.h, _ptr(u), _ptr(q)))
+        u = u.reshape(self.nvar, self.ncells)
+        return (u, q.reshape(self.nvar, self.ncells)) if prims else u
+
+    @property
+    def time(self):
+        t = C.c_double()
+        d = C.c_double()
+        check(self.L.shark_gpu_get_time(self.h, C.byref(t), C.byref(d)))
+        return t.value
+
+    def set_timing(self, on=True):
+        check(self.L.shark_gpu_set_timing(self.h, 1 if on else 0))
+
+    def kernel_time(self, name):
+        ms = C.c_double()
+        n = C.c_int64()
+        check(self.L.shark_gpu_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def algorithmic_bytes(self):
+        return self.L.shark_gpu_algorithmic_bytes(C.byref(self.cfg))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.shark_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,245 @@
+"""Initial states for the workloads, built the way the reference's setups build them.
+
+Each function returns ``(cfg, geom, state)``: the ``shark_cfg`` struct, the grid arrays of
+gridinit (src/grid/grid.f90:404-496) and the state after setup (u_prim after primtoc,
+src/solver/solve.f90:265-302, plus cs, sdust, uprim_condinit, fargo_velocity).
+
+* ``dustywave``  — src/setups/DUSTYWAVE/setup.f90:1-79 (1D periodic sound wave + drag)
+* ``si_box``     — src/setups/SI/setup.f90:1-163: multi-species NSH equilibrium in a
+                   shearing box, here with seeded Gaussian velocity noise on every fluid
+* ``disk``       — src/setups/disk_polar/setup.f90:1-84 on the linear polar grid
+                   (gridinit_disk_log with GRIDSPACE=0), with per-species grain sizes and
+                   seeded density noise
+Noise comes from numpy's Philox generator keyed by the seed, so every run of the same
+workload starts from the same state.
+"""
+import math
+
+import numpy as np
+
+from . import cfg as C
+
+PI = 3.141592653589793238462643383279  # src/commons/phys_const.f90:14
+AU = 1.5e13
+MSUN = 2e33
+GRAV = 6.67e-8
+
+
+def _f32(x):
+    """A Fortran default-REAL literal or float() conversion promoted to double."""
+    return float(np.float32(x))
+
+
+# --------------------------------------------------------------------------- grids
+def grid_cartesian_1d(nx, box_l):
+    """gridinit, GEOM=0, ndim=1 (grid.f90:408-414): positions accumulate dx over active cells."""
+    nxm = nx + 4
+    dx = box_l / _f32(nx)
+    g = dict(dx=np.full((1, nxm), dx), surf=np.ones((1, nxm)), vol=np.full(nxm, dx),
+             radii_c=np.ones(nxm), position=np.zeros((1, nxm)))
+    pos = 0.0
+    for ix in range(1, nxm + 1):
+        if 3 <= ix <= nx + 2:
+            pos = pos + dx
+            g["position"][0, ix - 1] = pos
+    return g
+
+
+def _ix_iy(nx, ny):
+    nxm, nym = nx + 4, ny + 4
+    ix = np.tile(np.arange(1, nxm + 1, dtype=np.float64), nym)
+    iy = np.repeat(np.arange(1, nym + 1, dtype=np.float64), nxm)
+    return ix, iy
+
+
+def grid_cartesian_2d(nx, ny, lx, ly):
+    """gridinit, GEOM=0, ndim=2 (grid.f90:416-428)."""
+    ix, iy = _ix_iy(nx, ny)
+    d1 = lx / _f32(nx)
+    d2 = ly / _f32(ny)
+    nc = ix.size
+    dx = np.stack([np.full(nc, d1), np.full(nc, d2)])
+    pos = np.stack([(((ix - 3) + 0.5) * lx) / _f32(nx), (((iy - 3) + 0.5) * ly) / _f32(ny)])
+    return dict(dx=dx, surf=dx.copy(), vol=np.full(nc, d1 * d2), radii_c=np.ones(nc), position=pos)
+
+
+def grid_polar(nx, ny, box_l, inner_r):
+    """gridinit_disk_log, GEOM=2, GRIDSPACE=0 (grid.f90:480-495)."""
+    ix, iy = _ix_iy(nx, ny)
+    nc = ix.size
+    d1 = box_l / _f32(nx)
+    d2 = (2.0 * PI) / _f32(ny)
+    rc = inner_r + (((ix - 3) + 0.5) * box_l) / _f32(nx)
+    phi = (((iy - 3) + 0.5) * (2.0 * PI)) / _f32(ny)
+    dx = np.stack([np.full(nc, d1), np.full(nc, d2)])
+    surf = np.stack([(rc - 0.5 * d1) * d2, np.full(nc, d1)])
+    vol = (rc * d1) * d2
+    pos = np.stack([rc * np.cos(phi), rc * np.sin(phi)])
+    return dict(dx=dx, surf=surf, vol=vol, radii_c=rc, position=pos)
+
+
+def primtoc(q, nd, gamma):
+    """primtoc, src/solver/solve.f90:265-302."""
+    u = np.empty_like(q)
+    u[0] = q[0]
+    u[1] = q[0] * q[1]
+    u[2] = q[0] * q[2]
+    u[3] = q[0] * q[3]
+    u[4] = q[4] / (gamma - 1.0) + 0.5 * q[0] * q[1] ** 2 + 0.5 * q[0] * q[2] ** 2 + 0.5 * q[0] * q[3] ** 2
+    for k in range(nd):
+        r, x, y, z = 5 + k, 5 + nd + k, 5 + 2 * nd + k, 5 + 3 * nd + k
+        u[r] = q[r]
+        u[x] = q[r] * q[x]
+        u[y] = q[r] * q[y]
+        u[z] = q[r] * q[z]
+    return u
+
+
+# --------------------------------------------------------------------------- setups
+def dustywave(nx=512, K=10.0, dust2gas=1.0, vx0=1e-4, drho=0.0, cs0=1.0, rho0=1.0, box_l=1.0,
+              kx=1.0, cfl=0.5, slope_type=1, solver=C.SOLVER_HLL, solverdust=C.DUST_HUANG_BAI,
+              gamma=1.6666667):
+    """1D dusty wave (DUSTYWAVE setup.f90:12-78); defaults are the 512-cell, K=10 case."""
+    cfg = C.make_cfg(nx=nx, ny=1, ndust=1, cfl=cfl, slope_type=slope_type, solver=solver,
+                     solverdust=solverdust, gamma=gamma, bc_kind=C.BC_PERIODIC,
+                     tstop_kind=C.TSTOP_DUSTYWAVE, force_kind=C.FORCE_NONE,
+                     inloop_kind=C.INLOOP_NONE, K=[K])
+    g = grid_cartesian_1d(nx, box_l)
+    x = g["position"][0]
+    s = np.sin((((2.0 * PI) * x) / box_l) * kx)
+    nv = 9
+    q = np.zeros((nv, nx + 4))
+    q[0] = rho0 * (1.0 + drho * s)
+    q[1] = vx0 * s
+    cs = np.full(nx + 4, cs0)
+    q[4] = cs ** 2 * q[0]
+    q[5] = dust2gas * q[0]
+    q[6] = vx0 * s
+    u = primtoc(q, 1, gamma)
+    return cfg, g, dict(u=u, cs=cs, sdust=None, condinit=None, fargo_velocity=None)
+
+
+def si_box(n=1024, nd=8, st_min=1e-3, st_max=1.0, eps_tot=0.01, hoverr=0.1, noise=1e-3,
+           box_l=1.0, seed=20261202, cfl=0.5, slope_type=1, gamma=1.666667):
+    """Shearing box (SI setup.f90:20-163): NSH equilibrium of gas + nd species with Stokes
+    numbers log-spaced in [st_min, st_max], dust-to-gas eps_tot split as an MRN a^-3.5 size
+    distribution (bin mass ~ a^0.5), pressure-gradient drift eta = 0.05 H/R (0.05 cs), and
+    Gaussian velocity noise of `noise` cs on every velocity component of every fluid."""
+    rad0, om, qsh, rho_init = 1.0, 1.0, 1.5, 1.0
+    eta = _f32(0.05) * hoverr                    # setup.f90:24 (0.05 is a REAL(4) literal)
+    L = box_l * eta * rad0
+    vk = rad0 * om
+    cs0 = hoverr * vk
+    theta_dust = 1e-11
+    rhograin = rho_init / theta_dust
+    st = np.logspace(math.log10(st_min), math.log10(st_max), nd) if nd > 1 else np.array([st_min])
+    w = st ** 0.5
+    eps = eps_tot * w / w.sum()
+    A = 0.0
+    B = 1.0
+    for k in range(nd):
+        A = A + eps[k] * st[k] / (1.0 + st[k] ** 2)
+        B = B + eps[k] / (1.0 + st[k] ** 2)
+    vx_n = 2.0 * eta * vk * A / (A + B)
+    vy_n = -eta * vk * B / (A + B)
+    ux = (vx_n + 2.0 * st * vy_n) / (1.0 + st ** 2)
+    uy = (vy_n - 0.5 * st * vx_n) / (1.0 + st ** 2)
+    cfg = C.make_cfg(nx=n, ny=n, ndust=nd, cfl=cfl, slope_type=slope_type, gamma=gamma,
+                     force_kick=1, bc_kind=C.BC_SHEAR_SI, tstop_kind=C.TSTOP_SI,
+                     force_kind=C.FORCE_SI, inloop_kind=C.INLOOP_SI, q_shear=qsh,
+                     omega_shear=om, box_l=L, rad0=rad0, eta_stream=eta, rho_init=rho_init,
+                     hoverr=hoverr, rhograin=rhograin)
+    g = grid_cartesian_2d(n, n, L, L)
+    nc = g["vol"].size
+    xx = g["position"][0] - 0.5 * L
+    rng = np.random.Generator(np.random.Philox(seed))
+    nv = 5 + 4 * nd
+    q = np.zeros((nv, nc))
+    q[0] = rho_init
+    q[1] = vx_n + noise * cs0 * rng.standard_normal(nc)
+    q[2] = noise * cs0 * rng.standard_normal(nc)
+    q[3] = qsh * om * xx + vy_n + noise * cs0 * rng.standard_normal(nc)
+    q[4] = q[0] * cs0 ** 2
+    sd = np.zeros((nd, nc))
+    for k in range(nd):
+        q[5 + k] = eps[k] * rho_init
+        q[5 + nd + k] = ux[k] + noise * cs0 * rng.standard_normal(nc)
+        q[5 + 2 * nd + k] = noise * cs0 * rng.standard_normal(nc)
+        q[5 + 3 * nd + k] = qsh * om * xx + uy[k] + noise * cs0 * rng.standard_normal(nc)
+        sd[k] = st[k] * rho_init * cs0 / rhograin / om
+    u = primtoc(q, nd, gamma)
+    return cfg, g, dict(u=u, cs=np.full(nc, cs0), sdust=sd, condinit=None, fargo_velocity=None)
+
+
+def disk(nr=1024, nphi=2048, nd=16, st_min=1e-4, st_max=1.0, eps_tot=0.01, hoverr=0.05,
+         r_in=0.4, r_out=2.5, sigma_r0=1700.0, noise=1e-3, fargo=True, frame="reference",
+         seed=20261202, cfl=0.5, slope_type=1, gamma=1.666667, r_relax=0.3, n_rel=8.0):
+    """Polar disk (disk_polar setup.f90:12-84, linear radial grid): Sigma ~ r^-1 (R_out_disk
+    -> infinity), H/R = hoverr, Keplerian rotation about Mstar = 1, nd species whose Epstein
+    Stokes number at r = 1 is log-spaced in [st_min, st_max], dust-to-gas eps_tot split equally,
+    Gaussian density noise of relative amplitude `noise` on the gas (dust follows).
+
+    frame="reference": velocities as the reference setup stores them with fargo on — gas with
+    the full azimuthal velocity, dust with Omega*r + vfarg, FARGO shift velocity
+    vfarg = -Omega*r (setup.f90:35-37,45,64).
+    frame="residual": every fluid stores its velocity relative to Omega*r and the FARGO shift is
+    +Omega*r.
+    r_relax <= r_in keeps the in-loop relaxation of setup.f90:228 off (its default 3.0 would
+    relax every cell toward the initial state)."""
+    unit_l = AU
+    unit_dcol = MSUN / AU ** 2
+    unit_d = MSUN / AU ** 3
+    mstar = 1.0
+    box_l = r_out - r_in
+    g = grid_polar(nr, nphi, box_l, r_in)
+    rr = g["radii_c"]
+    nc = rr.size
+    H = hoverr * rr
+    om = np.sqrt(mstar / rr ** 3)
+    cs0 = om * H
+    sig0 = sigma_r0 / unit_dcol
+    rng = np.random.Generator(np.random.Philox(seed))
+    rho = sig0 * (1.0 / rr) * (1.0 + noise * rng.standard_normal(nc))
+    vkep = om * rr
+    vgas = om * rr * np.sqrt(1.0 - hoverr ** 2 * 2.0)
+    rhograin = 2.7 / unit_d
+    # grain size giving Stokes number st at r=1 (Epstein law, setup.f90:60 / :436)
+    st = np.logspace(math.log10(st_min), math.log10(st_max), nd) if nd > 1 else np.array([st_min])
+    rho1 = sig0 / (math.sqrt(2.0 * PI) * hoverr)
+    sizes = st / (math.sqrt(PI * gamma / 8.0) * rhograin / (rho1 * hoverr))
+    eta_drift = np.sqrt(1.0 - hoverr ** 2 * 2.0) - 1.0
+    nv = 5 + 4 * nd
+    q = np.zeros((nv, nc))
+    q[0] = rho
+    if frame == "reference":
+        vfarg = -vkep
+        q[2] = vgas
+    elif frame == "residual":
+        vfarg = vkep
+        q[2] = vgas - vkep
+    else:
+        raise ValueError(frame)
+    q[4] = rho * cs0 ** 2
+    sd = np.zeros((nd, nc))
+    for k in range(nd):
+        ts = math.sqrt(PI * gamma / 8.0) * rhograin * sizes[k] / (rho / (math.sqrt(2.0 * PI) * H) * cs0)
+        stk = ts * om
+        q[5 + k] = eps_tot / nd * rho
+        q[5 + nd + k] = eta_drift * om * rr / (stk + 1.0 / stk)
+        q[5 + 2 * nd + k] = (vkep + vfarg) if frame == "reference" else 0.0
+        sd[k] = sizes[k]
+    u = primtoc(q, nd, gamma)
+    cfg = C.make_cfg(nx=nr, ny=nphi, ndust=nd, geom=C.GEOM_POLAR, cfl=cfl, slope_type=slope_type,
+                     gamma=gamma, force_kick=1, fargo=1 if fargo else 0, bc_kind=C.BC_DISK,
+                     tstop_kind=C.TSTOP_EPSTEIN_DISK, force_kind=C.FORCE_DISK,
+                     inloop_kind=C.INLOOP_DISK, mstar=mstar, r_relax=r_relax, n_rel=n_rel,
+                     smooth_r=r_in, hoverr=hoverr, rhograin=rhograin, disk_bc_mode=0)
+    fv = vfarg if fargo else None
+    return cfg, g, dict(u=u, cs=cs0.copy(), sdust=sd, condinit=u.copy(), fargo_velocity=fv)
+
+
+WORKLOADS = {
+    "dustywave_512_n1": lambda: dustywave(),
+    "si_1024x1024_n8": lambda: si_box(),
+    "disk_1024x2048_n16_fargo": lambda: disk(),
+}

@@ -1,0 +1,56 @@
+"""The C-ABI library loads without a GPU and exports every entry point include/*.h declares."""
+import ctypes
+import glob
+import os
+import re
+
+from shark_amd import gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    syms = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        syms |= set(re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w]*\s*\**\s*(shark_\w+)\s*\(", text, re.M))
+    return syms
+
+
+def test_header_declares_the_step_boundary():
+    syms = declared_symbols()
+    for s in ["shark_gpu_create", "shark_gpu_set_geometry", "shark_gpu_upload", "shark_gpu_step",
+              "shark_gpu_run", "shark_gpu_download", "shark_gpu_destroy", "shark_gpu_last_error"]:
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(gpu.LIB_PATH)
+    missing = [s for s in sorted(declared_symbols()) if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_algorithmic_bytes_formula():
+    from shark_amd import cfg as C
+    lib = gpu.load()
+    c = C.make_cfg(nx=1024, ny=2048, ndust=16, fargo=1, geom=2)
+    assert lib.shark_gpu_algorithmic_bytes(ctypes.byref(c)) == 2216 * 1024 * 2048
+    assert C.algorithmic_bytes(c) == 2216 * 1024 * 2048
+    c = C.make_cfg(nx=1024, ny=1024, ndust=8)
+    assert lib.shark_gpu_algorithmic_bytes(ctypes.byref(c)) == 600 * 1024 * 1024
+
+
+def test_create_rejects_out_of_scope_configs_without_gpu():
+    """Validation runs before any device call, so error paths are testable on the CPU."""
+    from shark_amd import cfg as C
+    lib = gpu.load()
+    h = ctypes.c_void_p()
+    for kw, frag in [(dict(nx=64, ndust=1, iso_cs=0), b"isothermal"),
+                     (dict(nx=64, ndust=5), b"ndust=5"),
+                     (dict(nx=64, ndust=1, slope_type=7), b"slope limiter"),
+                     (dict(nx=64, ndust=1, viscosity=1), b"viscosity"),
+                     (dict(nx=64, ndust=1, fargo=1), b"FARGO")]:
+        c = C.make_cfg(**kw)
+        assert lib.shark_gpu_create(ctypes.byref(c), ctypes.byref(h)) != 0
+        assert frag in lib.shark_gpu_last_error(), lib.shark_gpu_last_error()
