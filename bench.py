@@ -1,0 +1,147 @@
+#!/usr/bin/env python3
+"""Benchmark of the FP64 SHARK step on one MI355X: zone-updates/s (gas + dust fluids).
+
+Default workload: the 2D polar disk with FARGO, Nr=1024 x Nphi=2048, 16 dust species
+(17 fluids), per-cell 17-fluid implicit drag — the configuration the 60%-of-HBM and 20x-CPU
+targets are stated on.  `--workload si` runs the 1024^2 8-species shearing box, `dustywave`
+the 512-cell 1D wave.
+
+Timed region: W warm-up steps, then exactly K steps with the step kernels bracketed by HIP
+events on the library's stream (per-kernel average durations feed the roofline), host
+synchronised on both sides.  One JSON line is printed.
+
+cpu_baseline: the CPU restatement of the same step (oracle/liboracle.so, OpenMP over the
+host cores) on the same workload for a bounded number of steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "zone-updates/s (gas+dust) on 1 MI355X; growth-step speedup at fixed L1 err"
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    "disk": ("disk_1024x2048_n16_fargo", lambda s: s.disk()),
+    "si": ("si_1024x1024_n8", lambda s: s.si_box()),
+    "dustywave": ("dustywave_512_n1", lambda s: s.dustywave()),
+}
+
+
+def copy_probe(nbytes=1 << 30, reps=10):
+    """A plain device copy of known size, for calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}))
+
+
+def cpu_baseline(cfg, geom, state, zones_per_step, steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = pyoracle.Oracle(cfg, geom, state)
+    o.step()  # first touch of the oracle's work arrays
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step()
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": zones_per_step * steps / dt, "unit": "zone-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{steps} steps of the same workload after 1 untimed step, "
+                      f"{dt:.1f} s, OpenMP over {cores} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="disk")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="time the hipGraph replay path instead "
+                    "of per-kernel event-bracketed launches")
+    ap.add_argument("--copy-probe", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs the step on a single MI355X; "
+              "multi-GPU (radial slab decomposition) is not built.", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
+
+    from shark_amd import gpu, setups
+    from shark_amd.cfg import algorithmic_bytes
+
+    name, make = WORKLOADS[args.workload]
+    cfg, geom, state = make(setups)
+    cells = cfg.nx * (cfg.ny if cfg.ny > 1 else 1)
+    fluids = 1 + cfg.ndust
+    zones_per_step = cells * fluids
+    g = gpu.Step(cfg, geom, state)
+    g.run(args.warmup)
+
+    g.set_timing(not args.graph)
+    t0 = time.perf_counter()
+    n, _ = g.run(args.steps)
+    wall = time.perf_counter() - t0
+    assert n == args.steps, (n, args.steps)
+    kt = {k: g.kernel_time(k)[0] for k in ("bc", "hydro", "post", "dt")}
+
+    nv = 5 + 4 * cfg.ndust
+    bytes_hydro = cells * 8 * (2 * nv + 1)            # read u_prim + cs, write u_prim
+    bytes_post = cells * 16 * nv if cfg.fargo else 0     # FARGO's second full pass
+    step_alg = algorithmic_bytes(cfg)
+    if args.graph or not cfg.fargo:
+        # without FARGO the algorithmic traffic is one pass; rate the whole step against it
+        dom, dom_ms, dom_bytes = "step", wall / args.steps * 1e3, step_alg
+    else:
+        dom = max(("hydro", "post"), key=lambda k: kt[k])
+        dom_ms = kt[dom]
+        dom_bytes = bytes_hydro if dom == "hydro" else bytes_post
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": zones_per_step * args.steps / wall,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": name, "nx": cfg.nx, "ny": cfg.ny, "ndust": cfg.ndust,
+                   "fluids": fluids, "geom": "polar" if cfg.geom == 2 else "cartesian",
+                   "fargo": bool(cfg.fargo), "drag": bool(cfg.drag), "solver": "hll",
+                   "solverdust": "huang_bai", "slope_type": cfg.slope_type},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": dom_bytes, "avg_ms": dom_ms},
+        "kernels_ms": kt,
+        "step_roofline": {"algorithmic_bytes": step_alg,
+                          "achieved_GBs": step_alg / (wall / args.steps) / 1e9,
+                          "frac": step_alg / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},
+    }
+    g.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, geom, state, zones_per_step, args.cpu_steps)
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -120,7 +120,9 @@ struct FaceSet {
 DEV double shfl_up1(double x) { return __shfl_up(x, 1); }
 DEV double shfl_dn1(double x) { return __shfl_down(x, 1); }
 
-template <int NDIM, int GEOM, bool GAS>
+// SLOPE: limiter (1..3); SOLV: compile-time Riemann solver (gas 0..2, dust 0..3) or -1 to
+// read it from the arguments.
+template <int NDIM, int GEOM, bool GAS, int SLOPE, int SOLV>
 __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int lane) {
   constexpr int NU = GAS ? 5 : 4;
   const size_t nc = a.nc;
@@ -137,7 +139,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
   const int ixr = ixc < a.nxm ? ixc + 1 : a.nxm;
   const bool own_col = lane >= 2 && lane <= 61 && ix <= a.nx + 2;
   const double dt = a.sc->dt;
-  const int st = a.slope;
+  constexpr int st = SLOPE;
   const double dx1 = a.g.dx1[ixc - 1], dx1m = a.g.dx1[ixl - 1];
   const double rp = (GEOM == 2) ? a.g.rc[ixc - 1] : 1.0;
   const double dx2 = (NDIM == 2) ? a.g.dx2[ixc - 1] : 0.0;
@@ -153,10 +155,14 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
     if (y1 > a.ny + 2) y1 = a.ny + 2;
   }
 
-  auto load_row = [&](int j, RowF& r) {
+  // raw loads of a row are issued one iteration ahead of their use (prefetch)
+  auto fetch_row = [&](int j, RowF& r) {
     const int cell = (NDIM == 1) ? ixc - 1 : (ixc - 1) + (j - 1) * a.nxm;
 #pragma unroll
     for (int c = 0; c < NU; ++c) r.u[c] = a.U[p[c] * nc + cell];
+    r.cs = need_cs ? a.cs[cell] : 0.0;
+  };
+  auto convert_row = [&](RowF& r) {
     if (GAS) {
       r.q[0] = fmaxf_(r.u[0], a.smallr);
       r.q[1] = r.u[1] / r.u[0];
@@ -168,7 +174,6 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
       r.q[2] = r.u[2] / r.u[0];
       r.q[3] = r.u[3] / r.u[0];
     }
-    r.cs = need_cs ? a.cs[cell] : 0.0;
   };
 
   // face states of a row (predictor, godunov.f90:89-611); dqy = 0 in 1D
@@ -240,10 +245,10 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
     if (GAS) {
       fl.p = (csl * csl) * L[0];   // isothermal face pressure, godunov.f90:606-611
       fr.p = (csr * csr) * R[0];
-      f = gas_flux(a.solver, a.gamma, fl, fr, csl, csr);
+      f = gas_flux(SOLV >= 0 ? SOLV : a.solver, a.gamma, fl, fr, csl, csr);
     } else {
       fl.p = fr.p = 0.0;
-      f = dust_flux(a.solverdust, fl, fr, csl, csr);
+      f = dust_flux(SOLV >= 0 ? SOLV : a.solverdust, fl, fr, csl, csr);
     }
     F[0] = f.r;
     F[1] = dir == 0 ? f.un : f.ut;
@@ -263,10 +268,20 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
 
   const int jstart = (NDIM == 2) ? y0 - 2 : 1;
   const int jend = (NDIM == 2) ? y1 + 2 : 1;
+  // PF rows are in flight ahead of the row being used
+  constexpr int PF = GAS ? 2 : 3;
+  RowF pend[PF];
+#pragma unroll
+  for (int i = 0; i < PF; ++i)
+    if (jstart + i <= jend) fetch_row(jstart + i, pend[i]);
   for (int j = jstart; j <= jend; ++j) {
     R0 = R1;
     R1 = R2;
-    load_row(j, R2);
+    R2 = pend[0];
+#pragma unroll
+    for (int i = 0; i + 1 < PF; ++i) pend[i] = pend[i + 1];
+    if (j + PF <= jend) fetch_row(j + PF, pend[PF - 1]);
+    convert_row(R2);
     // in 1D the single row is row "R1" of the loop body below
     const RowF& Rm = (NDIM == 2) ? R1 : R2;
     const int m = (NDIM == 2) ? j - 1 : 1;   // row whose face states are built now
@@ -363,19 +378,18 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
   }
 }
 
-template <int NDIM, int GEOM>
+template <int NDIM, int GEOM, bool GAS, int SLOPE, int SOLV>
 __global__ __launch_bounds__(256) void k_hydro(StepArgs a) {
   if (!a.sc->active) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long item = (long)blockIdx.x * 4 + wave;
-  const long G = a.ngroups, S = a.nstrips, F = 1 + a.nd;
+  const long G = a.ngroups, S = a.nstrips, F = GAS ? 1 : a.nd;
   if (item >= G * S * F) return;
   const int g = (int)(item % G);
   const long t = item / G;
   const int strip = (int)(t % S);
-  const int fluid = (int)(t / S);
-  if (fluid == 0) hydro_fluid<NDIM, GEOM, true>(a, 0, g, strip, lane);
-  else hydro_fluid<NDIM, GEOM, false>(a, fluid, g, strip, lane);
+  const int fluid = GAS ? 0 : 1 + (int)(t / S);
+  hydro_fluid<NDIM, GEOM, GAS, SLOPE, SOLV>(a, fluid, g, strip, lane);
 }
 
 // ------------------------------------------------------------------ post
@@ -585,6 +599,216 @@ __global__ __launch_bounds__(256) void k_post(StepArgs a) {
   block_atomic_min(a.sc, dtc);
 }
 
+// k_post2: the post pass with all fluids of a row segment of TW cells staged in LDS and the
+// species spread over the NS = 256/TW thread slots of the block.  The drag sums run in
+// species order in one thread per (cell, component), so results equal k_post / the
+// reference bit for bit; loads, divisions and the Courant work are shared by all slots.
+template <int ND, bool FARGO, int NDIM, int TW>
+__global__ __launch_bounds__(256, 3) void k_post2(StepArgs a) {
+  if (!a.sc->active) return;
+  constexpr int NV = 5 + 4 * ND;
+  constexpr int NS = 256 / TW;
+  constexpr int NDL = ND > 0 ? ND : 1;
+  __shared__ double un[NV][TW];
+  __shared__ double akS[NDL][TW];
+  __shared__ double sumS[4][TW];
+  __shared__ double vmaxS[NS][TW];
+  __shared__ double fmS[NS][TW];
+  __shared__ double vvlast[TW];
+  const int c = threadIdx.x % TW, s = threadIdx.x / TW;
+  const int ix = 3 + blockIdx.x * TW + c;
+  const int iy = (NDIM == 2) ? 3 + blockIdx.y : 1;
+  const bool valid = ix <= a.nx + 2;
+  const int ixs = valid ? ix : a.nx + 2;  // clamped column for loads of idle lanes
+  const size_t nc = a.nc;
+  const int cell = cell_of(a, ixs, iy);
+  const double dt = a.sc->dt;
+
+  // ---- phase 1: remap (FARGO) or copy every variable of the segment into LDS
+  if (FARGO) {
+    const double rp = (a.geom == 2) ? a.g.rc[ixs - 1] : 1.0;
+    const double vk = a.fargo_vk[ixs - 1];
+    const double dy = a.g.dx2[ixs - 1] * rp;
+    const double sh = fabs(vk * dt);
+    const int nshft = (int)(sh / dy);
+    const double eps = fmod(sh, dy) / dy;
+    const int ny = a.ny;
+    int j0 = (vk > 0.0) ? iy - nshft - 1 : iy + nshft;
+    j0 = ((j0 - 3) % ny + ny) % ny + 3;
+    const int j1 = (j0 - 3 + 1) % ny + 3;
+    const int jm = (j0 - 3 + ny - 1) % ny + 3;
+    const int jp = (j1 - 3 + 1) % ny + 3;
+    const int cm = cell_of(a, ixs, jm), c0 = cell_of(a, ixs, j0), c1 = cell_of(a, ixs, j1),
+              cp = cell_of(a, ixs, jp);
+    const double h2 = (kHalf - eps) * (kHalf - eps);
+    const double w0 = (vk > 0.0) ? eps : (1.0 - eps), w1 = (vk > 0.0) ? (1.0 - eps) : eps;
+    // all loads of the slot's variables are issued before any is used
+    constexpr int NPS = (NV + NS - 1) / NS;
+    double um[NPS], u0[NPS], u1[NPS], up[NPS];
+#pragma unroll
+    for (int i = 0; i < NPS; ++i) {
+      const int v = s + i * NS;
+      if (v < NV) {
+        const double* Bv = a.B + v * nc;
+        um[i] = Bv[cm]; u0[i] = Bv[c0]; u1[i] = Bv[c1]; up[i] = Bv[cp];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NPS; ++i) {
+      const int v = s + i * NS;
+      if (v < NV) {
+        const double d0 = slope_limit(a.slope, u0[i] - um[i], u1[i] - u0[i]);
+        const double d1 = slope_limit(a.slope, u1[i] - u0[i], up[i] - u1[i]);
+        double x = u0[i] * w0 + u1[i] * w1;
+        x = x + kHalf * d0 * (0.25 - h2) + kHalf * d1 * (h2 - 0.25);
+        un[v][c] = x;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int v = s; v < NV; v += NS) un[v][c] = a.B[v * nc + cell];
+  }
+  const bool do_drag = ND > 0 && a.drag;
+  if (do_drag) {
+    // stopping times of the step (compute_tstop on the start-of-step primitives)
+    const double rho_q = fmaxf_(a.U[cell], a.smallr);
+    for (int k = s; k < ND; k += NS) {
+      const double ts = tstop_of<ND>(a, k, cell, ixs, rho_q, a.U[(5 + k) * nc + cell]);
+      akS[k][c] = (1.0 * dt) / ts;
+    }
+  }
+  __syncthreads();
+  // ---- phase 2: drag sums in species order (dust_drag.f90:24-38)
+  if (do_drag) {
+    if (s < 4) {
+      const int base = (s == 3) ? 5 : 5 + (s + 1) * ND;   // rhod, vdx, vdy, vdz planes
+      double acc = un[s == 3 ? 0 : s + 1][c];
+#pragma unroll 4
+      for (int k = 0; k < ND; ++k) {
+        const double ak = akS[k][c];
+        acc = acc + ak / (1.0 + ak) * un[base + k][c];
+      }
+      sumS[s][c] = acc;
+    }
+    __syncthreads();
+    // ---- phase 3: momentum update and floor (dust_drag.f90:40-73)
+    const double pnx = sumS[0][c], pny = sumS[1][c], pnz = sumS[2][c], rhon = sumS[3][c];
+    for (int k = s; k < ND; k += NS) {
+      const double ak = akS[k][c];
+      const double f = ak / (1.0 + ak);
+      const double rd = un[5 + k][c];
+      un[5 + ND + k][c] = un[5 + ND + k][c] / (1.0 + ak) + f * pnx / rhon * rd;
+      un[5 + 2 * ND + k][c] = un[5 + 2 * ND + k][c] / (1.0 + ak) + f * pny / rhon * rd;
+      un[5 + 3 * ND + k][c] = un[5 + 3 * ND + k][c] / (1.0 + ak) + f * pnz / rhon * rd;
+      un[5 + k][c] = fmaxf_(un[0][c] * a.dust_ratio_min, rd);
+    }
+    if (s == NS - 1 && a.back_reaction) {
+      const double rho = un[0][c];
+      un[1][c] = pnx / rhon * rho;
+      un[2][c] = pny / rhon * rho;
+      un[3][c] = pnz / rhon * rho;
+    }
+    __syncthreads();
+  }
+  // ---- phase 4: in-loop hook, per fluid
+  if (a.inloop_kind == 2) {
+    const double r = a.g.rc[ixs - 1];
+    if (r < a.r_relax) {
+      const double t_rel = a.n_rel * 2.0 * kPi / (sqrt(a.mstar / (r * r * r))) -
+                           a.n_rel * 2.0 * kPi / (sqrt(a.mstar / (a.smooth_r * a.smooth_r * a.smooth_r)));
+      const double e = exp(-dt / t_rel);
+      const double* ci = a.condinit;
+      for (int f = s; f <= ND; f += NS) {
+        const int ir = f == 0 ? 0 : 5 + f - 1;
+        const int iu = f == 0 ? 1 : 5 + ND + f - 1;
+        const int iv = f == 0 ? 2 : 5 + 2 * ND + f - 1;
+        const double rho_i = ci[ir * nc + cell], vx_i = ci[iu * nc + cell] / rho_i,
+                     vy_i = ci[iv * nc + cell] / rho_i;
+        const double rho_o = un[ir][c], vx_o = un[iu][c] / rho_o, vy_o = un[iv][c] / rho_o;
+        const double rho_n = (rho_i) * (1.0 - e) + rho_o * e;
+        const double vx_n = (vx_i) * (1.0 - e) + vx_o * e;
+        const double vy_n = (vy_i) * (1.0 - e) + vy_o * e;
+        un[ir][c] = rho_n; un[iu][c] = rho_n * vx_n; un[iv][c] = rho_n * vy_n;
+      }
+    }
+    __syncthreads();
+    if (ix == 3)
+      for (int v = s; v < NV; v += NS) un[v][c] = a.condinit[v * nc + cell];
+    __syncthreads();
+  } else if (a.inloop_kind == 1 && !a.force_kick) {
+    const double Om = a.omega_shear, O = Om * dt;
+    const double x0 = a.g.pos1[ixs - 1] - kHalf * a.box_l;
+    for (int f = s; f <= ND; f += NS) {
+      const int ir = f == 0 ? 0 : 5 + f - 1;
+      const int ixv = f == 0 ? 1 : 5 + ND + f - 1;
+      const int iz = f == 0 ? 3 : 5 + 3 * ND + f - 1;
+      const double AA = (f == 0) ? (2.0 * a.q_shear) * Om * x0 - (2.0 * a.rad0) * a.eta_stream
+                                 : (2.0 * a.q_shear) * Om * x0;
+      const double u = un[iz][c] / un[ir][c], v = un[ixv][c] / un[ir][c], d = un[ir][c];
+      un[iz][c] = d * (u * (1. - O * O) + 2. * v * O + AA * (O * O)) / (1. + O * O);
+      un[ixv][c] = d * (v * (1. - O * O) - 2. * u * O + AA * O) / (1. + O * O);
+    }
+    __syncthreads();
+  }
+  // ---- write the final state
+  if (valid)
+    for (int v = s; v < NV; v += NS) a.U[v * nc + cell] = un[v][c];
+  // ---- phase 5: next step's Courant candidate (courant.f90:68-158), species split over slots
+  {
+    double f1, f2, f3;
+    double vmax = 0.0, fm = 0.0;
+    bool any = false;
+    if (s == 0) {
+      const double rho = un[0][c];
+      const double vx = un[1][c] / rho, vy = un[2][c] / rho, vz = un[3][c] / rho;
+      const double vv = (fabs(vx) + fabs(vy)) + fabs(vz);
+      vmax = a.cs[cell] + vv;
+      if (ND == 0) vvlast[c] = vv;
+      if (a.force_kick) {
+        cell_force(a, ixs, vx, vz, true, f1, f2, f3);
+        fm = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
+      }
+      any = true;
+    }
+    for (int k = s; k < ND; k += NS) {
+      const double rd = un[5 + k][c];
+      const double vdx = un[5 + ND + k][c] / rd, vdy = un[5 + 2 * ND + k][c] / rd,
+                   vdz = un[5 + 3 * ND + k][c] / rd;
+      const double vv = (fabs(vdx) + fabs(vdy)) + fabs(vdz);
+      vmax = any ? fmaxf_(vmax, 0.0 + vv) : 0.0 + vv;
+      if (a.force_kick) {
+        cell_force(a, ixs, vdx, vdz, false, f1, f2, f3);
+        const double fk = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
+        fm = any ? fmaxf_(fm, fk) : fk;
+      }
+      any = true;
+      if (k == ND - 1) vvlast[c] = vv;
+    }
+    vmaxS[s][c] = any ? vmax : 0.0;
+    fmS[s][c] = any ? fm : 0.0;
+  }
+  __syncthreads();
+  double dtc = 2e44;
+  if (s == 0 && valid) {
+    double vmax = vmaxS[0][c], fm = fmS[0][c];
+    for (int w = 1; w < NS; ++w) {
+      vmax = fmaxf_(vmax, vmaxS[w][c]);
+      fm = fmaxf_(fm, fmS[w][c]);
+    }
+    double dxx;
+    if (a.ndim == 1) dxx = a.g.dx1[ix - 1];
+    else if (a.geom == 2) dxx = fminf_(a.g.dx1[ix - 1], a.g.rc[ix - 1] * a.g.dx2[ix - 1]);
+    else dxx = fminf_(a.g.dx1[ix - 1], a.g.dx2[ix - 1]);
+    dtc = (a.cfl * dxx) / fabs(vmax);
+    const double vv = vvlast[c];
+    if (a.force_kick && vv != 0.0) {
+      const double fratio = fmaxf_((fm * dxx) / (vv * vv), 1e-3);
+      dtc = fminf_(dtc, (((a.cfl * dxx) / vv) * (sqrt(1.0 + (2.0 * a.cfl) * fratio) - 1.0)) / fratio);
+    }
+  }
+  block_atomic_min(a.sc, dtc);
+}
+
 template <int ND, int NDIM>
 __global__ __launch_bounds__(256) void k_courant(StepArgs a) {
   constexpr int NV = 5 + 4 * ND;
@@ -653,23 +877,47 @@ void launch_bc(const StepArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_bc, dim3((n + 255) / 256), dim3(256), 0, s, a);
 }
 
-void launch_hydro(const StepArgs& a, hipStream_t s) {
-  const long items = (long)a.ngroups * a.nstrips * (1 + a.nd);
+template <int NDIM, int GEOM, bool GAS, int SLOPE>
+static void hydro_launch1(const StepArgs& a, hipStream_t s) {
+  const long items = (long)a.ngroups * a.nstrips * (GAS ? 1 : a.nd);
   const dim3 grid((unsigned)((items + 3) / 4));
-  if (a.ndim == 1) hipLaunchKernelGGL((k_hydro<1, 0>), grid, dim3(256), 0, s, a);
-  else if (a.geom == 2) hipLaunchKernelGGL((k_hydro<2, 2>), grid, dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_hydro<2, 0>), grid, dim3(256), 0, s, a);
+  const bool fast = GAS ? a.solver == 1 : a.solverdust == 0;   // HLL gas / Huang-Bai dust
+  if (fast) hipLaunchKernelGGL((k_hydro<NDIM, GEOM, GAS, SLOPE, GAS ? 1 : 0>), grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_hydro<NDIM, GEOM, GAS, SLOPE, -1>), grid, dim3(256), 0, s, a);
+}
+
+template <int NDIM, int GEOM, bool GAS>
+static void hydro_launch_slope(const StepArgs& a, hipStream_t s) {
+  switch (a.slope) {
+    case 1: hydro_launch1<NDIM, GEOM, GAS, 1>(a, s); break;
+    case 2: hydro_launch1<NDIM, GEOM, GAS, 2>(a, s); break;
+    default: hydro_launch1<NDIM, GEOM, GAS, 3>(a, s); break;
+  }
+}
+
+template <bool GAS>
+static void hydro_launch_geom(const StepArgs& a, hipStream_t s) {
+  if (a.ndim == 1) hydro_launch_slope<1, 0, GAS>(a, s);
+  else if (a.geom == 2) hydro_launch_slope<2, 2, GAS>(a, s);
+  else hydro_launch_slope<2, 0, GAS>(a, s);
+}
+
+// gas and dust fluids run as two launches so each gets its own register allocation
+void launch_hydro(const StepArgs& a, hipStream_t s) {
+  hydro_launch_geom<true>(a, s);
+  if (a.nd > 0) hydro_launch_geom<false>(a, s);
 }
 
 template <int ND>
 static void post_nd(const StepArgs& a, hipStream_t s) {
+  constexpr int TW = 64;
   if (a.ndim == 1) {
-    dim3 grid((a.nx + 255) / 256);
-    hipLaunchKernelGGL((k_post<ND, false, 1>), grid, dim3(256), 0, s, a);
+    dim3 grid((a.nx + TW - 1) / TW);
+    hipLaunchKernelGGL((k_post2<ND, false, 1, TW>), grid, dim3(256), 0, s, a);
   } else {
-    dim3 grid((a.nx + 63) / 64, (a.ny + 3) / 4);
-    if (a.fargo) hipLaunchKernelGGL((k_post<ND, true, 2>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_post<ND, false, 2>), grid, dim3(256), 0, s, a);
+    dim3 grid((a.nx + TW - 1) / TW, a.ny);
+    if (a.fargo) hipLaunchKernelGGL((k_post2<ND, true, 2, TW>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_post2<ND, false, 2, TW>), grid, dim3(256), 0, s, a);
   }
 }
 
