@@ -1,0 +1,69 @@
+/*
+ * shark_growth.h — C-ABI of the dust-growth (Smoluchowski coagulation / fragmentation)
+ * step in libshark_gpu.so.
+ *
+ * Replaces `dust_growth(verbose)` (src/dust/dust_growth.f90:3-256), called by `solve`
+ * after the drag (src/solver/solve.f90:112).  The reference's growth runs per cell with an
+ * explicit sub-cycled pair loop over the size bins; its disk branch is empty
+ * (dust_growth.f90:80-82).  This library provides
+ *   mode 0  FP64 direct pair loop (the plain coagulation; reference semantics per cell),
+ *   mode 1  the pair-rate -> bin-gain contraction on MFMA in FP32,
+ *   mode 2  the same contraction with BF16 operands and FP32 accumulation,
+ * both MFMA modes with compensated FP32 accumulation and a per-cell FP64 mass rescale.
+ * The state is the dust densities rhod(ncells, nbins), column-major like u_prim's dust
+ * planes (src/commons/allocate_init.f90:259).
+ */
+#ifndef SHARK_GROWTH_H
+#define SHARK_GROWTH_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHARK_GROWTH_VERSION 1
+#define SHARK_GROWTH_MAX_BINS 64
+
+enum { SHARK_GROWTH_ENV_COLLAPSE = 0, SHARK_GROWTH_ENV_DISK = 1 };
+enum { SHARK_GROWTH_FP64 = 0, SHARK_GROWTH_MFMA_FP32 = 1, SHARK_GROWTH_MFMA_BF16 = 2 };
+
+/* &DUST_PARAMS growth switches and grain properties (src/dust/dust_commons.f90:85-143) */
+typedef struct shark_growth_params {
+  int32_t version;        /* SHARK_GROWTH_VERSION */
+  int32_t nbins;          /* NDUST */
+  int32_t env;            /* SHARK_GROWTH_ENV_* */
+  int32_t kernel_type;    /* 0 physical, 1 constant, 2 additive (dust_growth.f90:150-162) */
+  int32_t fragmentation;  /* fragmentation flag */
+  int32_t frag_mode;      /* 0 energy threshold (dust_growth.f90:142-144), 1 velocity threshold (:145) */
+  int32_t turb, brownian, drift, ambipolar;   /* *_in_growth flags */
+  int32_t decomposed;     /* 1: bin targets independent of the fragmented fraction */
+  int32_t max_substeps;   /* guard on the per-cell sub-step loop (dust_growth.f90:117) */
+  double smin, smax, rhograin, ice_mantle, slope_mono, size_mono, abr, gamma_grains, estar_grains;
+  double turb_coef;       /* turbulent velocity = turb_coef * cs (alpha_turb in the reference) */
+  double vfrag;           /* cm/s */
+  double cfl_growth, eps_threshold, eps_threshold_frag, dust_ratio_min, sticking, clustered;
+  double delta_vambi;
+  double unit_l, unit_d, unit_t, unit_m, unit_v;
+  double mu_gas, gamma;
+  double hoverr, mstar;   /* disk environment */
+  double reserved[8];
+} shark_growth_params;
+
+typedef struct shark_growth shark_growth;
+
+/* Growth state for ncells cells. */
+int shark_growth_create(const shark_growth_params* p, int64_t ncells, shark_growth** g);
+/* rhod (ncells, nbins), gas density (ncells), sound speed (ncells), radius (ncells). */
+int shark_growth_upload(shark_growth* g, const double* rhod, const double* rho_gas,
+                        const double* cs, const double* radius);
+/* One growth call over dt (dust_growth with time_growth from 0 to dt) in the given mode. */
+int shark_growth_run(shark_growth* g, int mode, double dt);
+/* rhod after the last run, and per-cell sub-step counts (either may be NULL). */
+int shark_growth_download(shark_growth* g, double* rhod, int64_t* nsub);
+/* Device time (ms) of the last run. */
+int shark_growth_kernel_time(shark_growth* g, double* ms);
+int shark_growth_destroy(shark_growth* g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -112,3 +112,77 @@ class Oracle:
             self.close()
         except Exception:
             pass
+
+
+# ----------------------------------------------------------------------------- growth
+class GrowthTables(C.Structure):
+    _fields_ = [("nb", C.c_int), ("zeta", C.c_double), ("eta", C.c_double),
+                ("massmin", C.c_double), ("m_mono", C.c_double), ("ebr_mono", C.c_double),
+                ("aplus", C.c_double * 64), ("aminus", C.c_double * 64),
+                ("mminus", C.c_double * 64), ("sdust", C.c_double * 64),
+                ("mdust", C.c_double * 64), ("redis", C.c_double * 4096)]
+
+
+def _glib():
+    L = lib()
+    if not getattr(L, "_growth_bound", False):
+        L.growth_oracle_tables.argtypes = [C.c_void_p, C.c_void_p]
+        L.growth_oracle_collapse.restype = C.c_int64
+        L.growth_oracle_collapse.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                             C.c_int64] + [_P] * 7 + [C.c_void_p, C.c_double, _P,
+                                                                      C.POINTER(C.c_int64)]
+        L.growth_oracle_disk.restype = C.c_int64
+        L.growth_oracle_disk.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, _P, _P, _P,
+                                         C.c_double, _P, C.POINTER(C.c_int64), _P]
+        L._growth_bound = True
+    return L
+
+
+def growth_tables(params):
+    t = GrowthTables()
+    _glib().growth_oracle_tables(C.byref(params), C.byref(t))
+    return t
+
+
+def _c(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def growth_collapse(params, tables, T0, unit_nH, q_rho, u_rho, sdust, mdust, tstop, vd, eta_a,
+                    active, dt, rhod):
+    """Reference collapse branch of dust_growth on ncells cells (arrays (nbins, ncells))."""
+    nc = q_rho.size
+    r = _c(rhod).copy()
+    n = np.zeros(nc, dtype=np.int64)
+    act = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
+    args = [_c(q_rho), _c(u_rho), _c(sdust), _c(mdust), _c(tstop), _c(vd), _c(eta_a)]
+    _glib().growth_oracle_collapse(C.byref(params), C.byref(tables), T0, unit_nH, nc,
+                                   *[_ptr(a) for a in args],
+                                   None if act is None else act.ctypes.data_as(C.c_void_p),
+                                   dt, _ptr(r), n.ctypes.data_as(C.POINTER(C.c_int64)))
+    return r, n
+
+
+def growth_disk(params, tables, sigma, cs, radius, dt, rhod, first_rates=False):
+    """Disk-environment growth over dt for every cell; returns (rhod, substeps[, rates])."""
+    nc = sigma.size
+    r = _c(rhod).copy()
+    n = np.zeros(nc, dtype=np.int64)
+    d1 = np.zeros_like(r) if first_rates else None
+    _glib().growth_oracle_disk(C.byref(params), C.byref(tables), nc, _ptr(_c(sigma)),
+                               _ptr(_c(cs)), _ptr(_c(radius)), dt, _ptr(r),
+                               n.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(d1))
+    return (r, n, d1) if first_rates else (r, n)
+
+
+def growth_disk_env(params, tables, sigma, cs, r):
+    """Per-cell inputs of the disk environment: T, t_L, t_eta, Re and the stopping times."""
+    L = _glib()
+    nb = tables.nb
+    out = [C.c_double() for _ in range(4)]
+    ts = np.zeros(nb)
+    L.growth_oracle_disk_env.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                         C.c_double] + [C.POINTER(C.c_double)] * 4 + [_P]
+    L.growth_oracle_disk_env(C.byref(params), C.byref(tables), sigma, cs, r,
+                             *[C.byref(o) for o in out], _ptr(ts))
+    return dict(T=out[0].value, t_L=out[1].value, t_eta=out[2].value, Re=out[3].value, tstop=ts)

@@ -32,6 +32,14 @@ static int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+namespace shark {
+// error reporting shared with the growth entry points (same last-error buffer)
+int fail_msg(int code, const char* msg) {
+  snprintf(g_err, sizeof(g_err), "%s", msg);
+  return code;
+}
+}  // namespace shark
+
 #define HIPCHK(x)                                                                   \
   do {                                                                              \
     hipError_t e_ = (x);                                                            \
@@ -134,6 +142,7 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   a.rhograin = cfg->rhograin; a.mstar = cfg->mstar; a.r_relax = cfg->r_relax; a.n_rel = cfg->n_rel;
   a.smooth_r = cfg->smooth_r;
   for (int k = 0; k < SHARK_MAX_DUST; ++k) a.K[k] = cfg->K[k];
+  // rows per hydro work item (reserved[0] overrides; the autotuner sweeps it)
   a.strip_h = (int)(cfg->reserved[0] > 0 ? cfg->reserved[0] : 32);
   a.ngroups = (a.nx + 59) / 60;
   a.nstrips = a.ndim == 2 ? (a.ny + a.strip_h - 1) / a.strip_h : 1;

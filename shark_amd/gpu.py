@@ -25,8 +25,6 @@ EXPORTS = [
     "shark_gpu_destroy", "shark_gpu_last_error", "shark_gpu_version",
     "shark_growth_create", "shark_growth_run", "shark_growth_download", "shark_growth_upload",
     "shark_growth_destroy", "shark_growth_kernel_time",
-    "shark_mlp_create", "shark_mlp_set_params", "shark_mlp_get_params", "shark_mlp_forward",
-    "shark_mlp_train_step", "shark_mlp_destroy", "shark_mlp_grads",
 ]
 
 

@@ -1,0 +1,146 @@
+"""Dust growth: the parameter struct of include/shark_growth.h and the GPU binding.
+
+``GrowthParams`` mirrors ``shark_growth_params``; ``Growth`` drives the growth kernels of
+libshark_gpu.so (FP64 direct pair loop, or the MFMA contraction in FP32 / BF16).
+The reference routine is dust_growth (src/dust/dust_growth.f90:3-256).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import gpu
+
+SHARK_GROWTH_VERSION = 1
+MAX_BINS = 64
+ENV_COLLAPSE, ENV_DISK = 0, 1
+FP64, MFMA_FP32, MFMA_BF16 = 0, 1, 2
+
+AU = 1.5e13
+MSUN = 2e33
+GRAV = 6.67e-8
+
+
+class GrowthParams(C.Structure):
+    _fields_ = [
+        ("version", C.c_int32), ("nbins", C.c_int32), ("env", C.c_int32),
+        ("kernel_type", C.c_int32), ("fragmentation", C.c_int32), ("frag_mode", C.c_int32),
+        ("turb", C.c_int32), ("brownian", C.c_int32), ("drift", C.c_int32),
+        ("ambipolar", C.c_int32), ("decomposed", C.c_int32), ("max_substeps", C.c_int32),
+        ("smin", C.c_double), ("smax", C.c_double), ("rhograin", C.c_double),
+        ("ice_mantle", C.c_double), ("slope_mono", C.c_double), ("size_mono", C.c_double),
+        ("abr", C.c_double), ("gamma_grains", C.c_double), ("estar_grains", C.c_double),
+        ("turb_coef", C.c_double), ("vfrag", C.c_double), ("cfl_growth", C.c_double),
+        ("eps_threshold", C.c_double), ("eps_threshold_frag", C.c_double),
+        ("dust_ratio_min", C.c_double), ("sticking", C.c_double), ("clustered", C.c_double),
+        ("delta_vambi", C.c_double), ("unit_l", C.c_double), ("unit_d", C.c_double),
+        ("unit_t", C.c_double), ("unit_m", C.c_double), ("unit_v", C.c_double),
+        ("mu_gas", C.c_double), ("gamma", C.c_double), ("hoverr", C.c_double),
+        ("mstar", C.c_double), ("reserved", C.c_double * 8),
+    ]
+
+
+def make_params(**kw):
+    """Defaults of &DUST_PARAMS (src/dust/dust_commons.f90:85-143), overridden by kw."""
+    p = GrowthParams()
+    p.version = SHARK_GROWTH_VERSION
+    p.nbins = 16
+    p.env = ENV_DISK
+    p.kernel_type = 0
+    p.fragmentation = 0
+    p.frag_mode = 0
+    p.turb = p.brownian = 1
+    p.drift = p.ambipolar = 0
+    p.decomposed = 1
+    p.max_substeps = 1000000
+    p.smin, p.smax, p.rhograin, p.ice_mantle = 1e-7, 1e-2, 2.7, 8.7e-7
+    p.slope_mono, p.size_mono, p.abr = 3.5, 1e-5, 2.8e3
+    p.gamma_grains, p.estar_grains = 25.0, 2.8e11
+    p.turb_coef = 1.5
+    p.vfrag = 100.0
+    p.cfl_growth, p.eps_threshold, p.eps_threshold_frag = 0.1, 1e-10, 1e-10
+    p.dust_ratio_min, p.sticking, p.clustered, p.delta_vambi = 1e-11, 1.0, 1.0, 1.0
+    p.unit_l = p.unit_d = p.unit_t = p.unit_m = p.unit_v = 1.0
+    p.mu_gas, p.gamma = 2.31, 1.66667
+    p.hoverr, p.mstar = 0.05, 1.0
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def disk_params(nbins=64, smin=1e-4, smax=10.0, alpha=1e-3, vfrag=100.0, hoverr=0.05, **kw):
+    """The growth configuration of the 512x512 disk workload: bins log-spaced in size from
+    smin to smax (cm), Brownian + turbulent relative velocities with turbulent speed
+    sqrt(alpha)*cs, velocity-threshold fragmentation at vfrag (cm/s), disk_polar units
+    (src/setups/disk_polar/units.f90)."""
+    unit_l = AU
+    unit_d = MSUN / AU ** 3
+    unit_t = 1.0 / np.sqrt(GRAV * MSUN / unit_l ** 3)
+    unit_m = MSUN
+    unit_v = unit_l / unit_t
+    base = dict(nbins=nbins, env=ENV_DISK, fragmentation=1, frag_mode=1, turb=1, brownian=1,
+                drift=0, ambipolar=0, decomposed=1, smin=smin, smax=smax, rhograin=2.7,
+                ice_mantle=0.0, turb_coef=float(np.sqrt(alpha)), vfrag=vfrag, cfl_growth=0.1,
+                unit_l=unit_l, unit_d=unit_d, unit_t=unit_t, unit_m=unit_m, unit_v=unit_v,
+                hoverr=hoverr, mstar=1.0)
+    base.update(kw)
+    return make_params(**base)
+
+
+_P = C.POINTER(C.c_double)
+
+
+def _lib():
+    lib = gpu.load()
+    if not getattr(lib, "_growth_bound", False):
+        lib.shark_growth_create.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]
+        lib.shark_growth_upload.argtypes = [C.c_void_p, _P, _P, _P, _P]
+        lib.shark_growth_run.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        lib.shark_growth_download.argtypes = [C.c_void_p, _P, C.POINTER(C.c_int64)]
+        lib.shark_growth_kernel_time.argtypes = [C.c_void_p, _P]
+        lib.shark_growth_destroy.argtypes = [C.c_void_p]
+        lib._growth_bound = True
+    return lib
+
+
+class Growth:
+    """Device growth state: rhod (ncells, nbins) plus per-cell gas density, cs and radius."""
+
+    def __init__(self, params, ncells):
+        self.L = _lib()
+        self.p = params
+        self.ncells = int(ncells)
+        h = C.c_void_p()
+        gpu.check(self.L.shark_growth_create(C.byref(params), self.ncells, C.byref(h)))
+        self.h = h
+
+    def upload(self, rhod, rho_gas, cs, radius):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (rhod, rho_gas, cs, radius)]
+        gpu.check(self.L.shark_growth_upload(self.h, *[x.ctypes.data_as(_P) for x in a]))
+
+    def run(self, mode, dt):
+        gpu.check(self.L.shark_growth_run(self.h, int(mode), float(dt)))
+
+    def download(self):
+        r = np.empty(self.p.nbins * self.ncells)
+        n = np.empty(self.ncells, dtype=np.int64)
+        gpu.check(self.L.shark_growth_download(self.h, r.ctypes.data_as(_P),
+                                               n.ctypes.data_as(C.POINTER(C.c_int64))))
+        return r.reshape(self.p.nbins, self.ncells), n
+
+    def kernel_ms(self):
+        ms = C.c_double()
+        gpu.check(self.L.shark_growth_kernel_time(self.h, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.shark_growth_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

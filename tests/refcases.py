@@ -8,7 +8,8 @@ import numpy as np
 from shark_amd import cfg as C
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-REF_CASES = sorted(f[4:-4] for f in os.listdir(GOLDEN) if f.startswith("ref_") and f.endswith(".npz"))
+REF_CASES = sorted(f[4:-4] for f in os.listdir(GOLDEN)
+                   if f.startswith("ref_") and f.endswith(".npz") and not f.startswith("ref_growth_"))
 
 
 def _K(nml):

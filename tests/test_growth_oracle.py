@@ -1,0 +1,185 @@
+"""The growth oracle (oracle/growth_oracle.c) against the reference's own dust_growth.
+
+Fixtures tests/golden/ref_growth_*.npz come from the flang build of the reference
+Collapse_1D setup (tests/golden/make_growth_fixtures.py): inputs after ctoprim and
+compute_tstop, and the dust densities after one dust_growth call
+(src/dust/dust_growth.f90:3-256).  The collapse branch must agree bit for bit.  Beyond the
+reference's branch, the invariants that pin the disk environment and the decomposed bin
+targets are checked here too: per-pair mass conservation of the rates, the constant- and
+additive-kernel closed forms of a bin's loss rate, the midplane Epstein Stokes number, and
+decomposed == reference when no pair fragments.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle
+from shark_amd import growth as G
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = ["all_terms", "no_fragmentation", "turb_brownian_clustered"]
+
+
+def load(name):
+    z = np.load(os.path.join(GOLD, f"ref_growth_{name}.npz"))
+    return {k: z[k] for k in z.files if k != "meta"}, json.loads(str(z["meta"]))
+
+
+def params_of(m, **kw):
+    base = dict(nbins=m["ndust"], env=G.ENV_COLLAPSE, kernel_type=m["kernel_type"],
+                fragmentation=int(m["fragmentation"]), frag_mode=0, turb=int(m["turb"]),
+                brownian=int(m["brownian"]), drift=int(m["drift"]), ambipolar=int(m["ambipolar"]),
+                decomposed=0, smin=m["smin"], smax=m["smax"], rhograin=m["rhograin"],
+                ice_mantle=m["ice_mantle"], slope_mono=m["slope_mono"], size_mono=m["size_mono"],
+                abr=m["abr"], gamma_grains=m["gamma_grains"], estar_grains=m["estar_grains"],
+                turb_coef=m["alpha_turb"], vfrag=m["vfrag"], cfl_growth=m["cfl_growth"],
+                eps_threshold=m["eps_threshold"], eps_threshold_frag=m["eps_threshold_frag"],
+                dust_ratio_min=m["dust_ratio_min"], sticking=m["sticking"], clustered=m["clustered"],
+                delta_vambi=m["delta_vambi"], unit_l=m["unit_l"], unit_d=m["unit_d"],
+                unit_t=m["unit_t"], unit_m=m["unit_m"], unit_v=m["unit_v"], gamma=m["gamma"])
+    base.update(kw)
+    return G.make_params(**base)
+
+
+def run_collapse(name, **kw):
+    a, m = load(name)
+    p = params_of(m, **kw)
+    t = pyoracle.growth_tables(p)
+    nd = m["ndust"]
+    vd = a["q"][5 + nd:5 + 4 * nd]
+    r, n = pyoracle.growth_collapse(p, t, m["T0_cloud"], m["unit_nH"], a["q_rho"], a["u_rho"],
+                                    a["sdust"], a["mdust"], a["tstop"], vd,
+                                    np.zeros(m["ncells"]), a["active"].astype(np.uint8), m["dt"],
+                                    a["rhod_ic"])
+    act = a["active"].astype(bool)
+    return r[:, act], a["rhod_final"][:, act], n[act], t, a, m
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_collapse_branch_bit_exact(name):
+    r, ref, n, _, _, _ = run_collapse(name)
+    assert n.min() >= 1
+    assert np.array_equal(r, ref), np.abs(r / ref - 1).max()
+
+
+def test_tables_match_reference_bins():
+    _, _, _, t, a, m = run_collapse("all_terms")
+    nd = m["ndust"]
+    assert np.array_equal(np.array(t.aplus[:nd]), a["aplus"])
+    assert np.array_equal(np.array(t.aminus[:nd]), a["aminus"])
+    assert np.array_equal(np.array(t.mminus[:nd]), a["mminus"])
+    act = a["active"].astype(bool)
+    assert np.array_equal(np.array(t.mdust[:nd]), a["mdust"][:, act][:, 0])
+    assert np.array_equal(np.array(t.sdust[:nd]), a["sdust"][:, act][:, 0])
+
+
+def test_decomposed_equals_reference_without_fragmentation():
+    r, ref, _, _, _, _ = run_collapse("no_fragmentation", decomposed=1)
+    assert np.array_equal(r, ref)
+
+
+def test_fragmentation_flag_changes_the_answer():
+    """A mis-wired fragmentation switch would make these two fixtures agree."""
+    r1, _, _, _, _, _ = run_collapse("all_terms", fragmentation=0)
+    _, ref, _, _, _, _ = run_collapse("all_terms")
+    assert not np.allclose(r1, ref, rtol=1e-6, atol=0)
+
+
+# ----------------------------------------------------------------------------- disk env
+def disk_cells(nc=24, nb=32, seed=3):
+    rng = np.random.default_rng(seed)
+    p = G.disk_params(nbins=nb, smin=1e-4, smax=10.0, max_substeps=100000)
+    t = pyoracle.growth_tables(p)
+    r = np.linspace(0.4, 2.5, nc)
+    sigma = 1.0e-3 / r * (1 + 1e-3 * rng.standard_normal(nc))
+    cs = 0.05 / np.sqrt(r)
+    md = np.array(t.mdust[:nb])
+    s = np.array(t.sdust[:nb])
+    w = s ** 0.5 * np.ones((nc, 1))      # MRN: mass per log bin ~ s^(4-3.5)
+    rhod = (0.01 * sigma[:, None] * w / w.sum(axis=1, keepdims=True)).T.copy()
+    return p, t, sigma, cs, r, rhod, md
+
+
+def first_rates(p, t, sigma, cs, r, rhod, dt=1e-6):
+    _, _, d1 = pyoracle.growth_disk(p, t, sigma, cs, r, dt, rhod, first_rates=True)
+    return d1
+
+
+@pytest.mark.parametrize("kernel_type,frag", [(0, 0), (0, 1), (1, 0), (2, 0)])
+def test_rates_conserve_mass(kernel_type, frag):
+    """Every pair moves mass between bins: the rates sum to zero up to round-off.  The mass
+    sits in two bins (the rest at 1e-32 of it).  A pair's gain lands mostly back in the bin of
+    its larger grain, so a bin's net rate is the difference of fluxes up to ~1e3 times larger;
+    the bound 1e-11 of sum |rate| leaves room for that cancellation, while a dropped or
+    mis-weighted term breaks the balance by O(1e-3) or more."""
+    p, t, sigma, cs, r, rhod, _ = disk_cells()
+    p.kernel_type, p.fragmentation = kernel_type, frag
+    t = pyoracle.growth_tables(p)
+    rhod[:] = 1e-32 * sigma
+    rhod[8] = 0.004 * sigma
+    rhod[20] = 0.006 * sigma
+    d1 = first_rates(p, t, sigma, cs, r, rhod)
+    scale = np.abs(d1).sum(axis=0)
+    assert (scale > 0).all()
+    assert (np.abs(d1.sum(axis=0)) <= 1e-11 * scale).all()
+    if frag:
+        # fragments of the collisions refill the bins below the smaller grain
+        assert (d1[:8] > 0).all()
+
+
+def test_constant_kernel_self_collision_loss_closed_form():
+    """Constant kernel K=1 with all mass in one bin k: the only non-negligible pair is (k,k),
+    whose collision rate is n_k^2/2 (n = rho/m); it removes two grains of mass m_k per
+    collision, so drhodt_k = -rho_k^2/m_k.  A dropped 1/2 for i==j or a loss booked only once
+    fails this."""
+    nb, k = 16, 5
+    p = G.make_params(nbins=nb, env=G.ENV_DISK, kernel_type=1, fragmentation=0, smin=1e-4,
+                      smax=1e-1, hoverr=0.05, dust_ratio_min=0.0)
+    t = pyoracle.growth_tables(p)
+    m = np.array(t.mdust[:nb])
+    rhod = np.full((nb, 1), 1e-200)
+    rhod[k, 0] = 3.0
+    d1 = first_rates(p, t, np.array([1.0]), np.array([0.05]), np.array([1.0]), rhod)
+    exact = -rhod[k, 0] ** 2 / m[k]
+    assert abs(d1[k, 0] - exact) <= 1e-14 * abs(exact)
+    assert d1[k + 1:, 0].sum() == pytest.approx(-exact, rel=1e-14)
+
+
+def test_additive_kernel_top_bin_loss():
+    """Additive kernel K=m_i+m_j: the top bin's loss rate is
+    -sum_j rho_k rho_j (m_k+m_j)/m_j (self pair halved, booked twice), and every pair's gain
+    is clamped into the top bin, so bin k's rate is that loss plus the gains of all pairs
+    whose mass lands at or above it; the mass rate of the whole cell is zero."""
+    nb = 12
+    p = G.make_params(nbins=nb, env=G.ENV_DISK, kernel_type=2, fragmentation=0, smin=1e-4,
+                      smax=1e-2, hoverr=0.05, dust_ratio_min=0.0)
+    t = pyoracle.growth_tables(p)
+    m = np.array(t.mdust[:nb])
+    rng = np.random.default_rng(7)
+    rhod = rng.uniform(0.5, 1.5, (nb, 1))
+    d1 = first_rates(p, t, np.array([1.0]), np.array([0.05]), np.array([1.0]), rhod)
+    assert abs(d1.sum()) <= 1e-13 * np.abs(d1).sum()
+    k = nb - 1
+    r = rhod[:, 0]
+    loss = -sum(r[k] * r[j] * (m[k] + m[j]) / m[j] for j in range(nb - 1)) - r[k] * r[k] * 2 * m[k] / m[k]
+    # the (k, j) pairs all land in the top bin: their gain is r_k r_j (m_k+m_j)^2/(m_k m_j)
+    gain_top = sum(r[k] * r[j] * (m[k] + m[j]) ** 2 / (m[k] * m[j]) for j in range(nb - 1)) \
+        + 0.5 * r[k] * r[k] * (2 * m[k]) ** 2 / (m[k] * m[k])
+    assert d1[k, 0] >= loss + gain_top * (1 - 1e-13)
+
+
+def test_disk_epstein_stopping_time_is_midplane_stokes_number():
+    """Epstein drag at the midplane: t_s * Omega = (pi/2) sqrt(gamma) rho_grain s / Sigma when
+    cs = H Omega (the textbook St = pi rho_grain s / (2 Sigma) for gamma = 1)."""
+    p, t, sigma, cs, r, rhod, _ = disk_cells(nc=5, nb=8)
+    p.gamma = 1.0
+    om = r ** -1.5
+    cs = p.hoverr * r * om
+    for c in range(5):
+        ts = pyoracle.growth_disk_env(p, t, sigma[c], cs[c], r[c])["tstop"]
+        st = ts * om[c]
+        s = np.array(t.sdust[:8]) * p.unit_l
+        expect = np.pi / 2 * p.rhograin * s / (sigma[c] * p.unit_d * p.unit_l)
+        np.testing.assert_allclose(st, expect, rtol=1e-14)
