@@ -24,7 +24,10 @@ extern "C" {
 #define SHARK_GROWTH_MAX_BINS 64
 
 enum { SHARK_GROWTH_ENV_COLLAPSE = 0, SHARK_GROWTH_ENV_DISK = 1 };
-enum { SHARK_GROWTH_FP64 = 0, SHARK_GROWTH_MFMA_FP32 = 1, SHARK_GROWTH_MFMA_BF16 = 2 };
+enum { SHARK_GROWTH_FP64 = 0, SHARK_GROWTH_MFMA_FP32 = 1, SHARK_GROWTH_MFMA_BF16 = 2,
+       SHARK_GROWTH_MFMA_BF16_PLAIN = 3 };
+/* return codes (0 = success; shark_gpu_last_error() holds the message) */
+enum { SHARK_GROWTH_EINVAL = 1, SHARK_GROWTH_EHIP = 3, SHARK_GROWTH_ENUM = 6 };
 
 /* &DUST_PARAMS growth switches and grain properties (src/dust/dust_commons.f90:85-143) */
 typedef struct shark_growth_params {
@@ -50,12 +53,14 @@ typedef struct shark_growth_params {
 
 typedef struct shark_growth shark_growth;
 
-/* Growth state for ncells cells. */
+/* Growth state for ncells cells.  Rejects (SHARK_GROWTH_EINVAL) the collapse environment, and
+ * fragmentation with decomposed=0, whose target bins depend on the cell. */
 int shark_growth_create(const shark_growth_params* p, int64_t ncells, shark_growth** g);
 /* rhod (ncells, nbins), gas density (ncells), sound speed (ncells), radius (ncells). */
 int shark_growth_upload(shark_growth* g, const double* rhod, const double* rho_gas,
                         const double* cs, const double* radius);
-/* One growth call over dt (dust_growth with time_growth from 0 to dt) in the given mode. */
+/* One growth call over dt (dust_growth with time_growth from 0 to dt) in the given mode.
+ * SHARK_GROWTH_ENUM when a density turns non-finite or a sub-step makes no progress. */
 int shark_growth_run(shark_growth* g, int mode, double dt);
 /* rhod after the last run, and per-cell sub-step counts (either may be NULL). */
 int shark_growth_download(shark_growth* g, double* rhod, int64_t* nsub);

@@ -1,17 +1,941 @@
-// growth.hip — dust-growth entry points (kernels not built yet: every call reports so).
+// growth.hip — dust growth (Smoluchowski coagulation + fragmentation over size bins) on gfx950.
+//
+// Replaces dust_growth (src/dust/dust_growth.f90:3-256) for the disk environment.  Per cell the
+// reference sub-cycles an explicit step of the pair loop over (i, j <= i) bins: a collision rate
+// dndt from the relative velocity dv_ij (:87-114), a fragmented fraction f, losses of both
+// partners, the coagulated mass split between the two bins around m1+m2 (:183-190), fragments
+// redistributed over the bins below (:204-222), and a CFL-limited sub-step (:227-241).
+//
+// Pipeline per growth call, over chunks of cells (the per-pair scratch is bounded):
+//   k_genv     per cell: temperature, eddy turnover time, Reynolds number, Epstein stopping times
+//   k_gpairs   per (pair, cell): dv_ij and from it the collision prefactor A and the fragmented
+//              fraction f (FP64), plus the mass-flux prefactor Ahat = A (m1+m2)/(m1 m2) in FP32
+//   mode 0     k_gfp64: one lane per cell, the pair loop in the reference's order and rounding,
+//              rates accumulated in LDS; bit-exact with the FP64 oracle
+//   modes 1-3  k_gmfma: the rate vector of a cell is one dense contraction
+//                drhodt = rho_d^2 * Gt [B x 2P] . X [2P]  (X = coagulated / fragmented mass flux
+//              of every pair in units of the cell's dust density), run for 64-cell tiles as
+//              32x32 MFMA blocks in FP32 (mode 1), BF16 hi+lo split with three products (mode 2)
+//              or plain BF16 (mode 3), FP32 accumulation with a compensated (Kahan) sum over
+//              K-chunks, then an FP64 update with a per-cell mass rescale.
+// Every kernel launch runs a bounded number of sub-steps and keeps the per-cell time and
+// sub-step count in device memory; the host relaunches until every cell reached dt.
 #include <hip/hip_runtime.h>
+#include <limits.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
 #include "../../include/shark_growth.h"
 #include "growth.h"
 
-struct shark_growth { int dummy; };
-void shark_growth_free(shark_growth* g) { delete g; }
+namespace {
+
+constexpr double kPi = 3.141592653589793238462643383279;
+constexpr double kKB = 1.38e-16, kMH = 1.67e-24;
+constexpr int MAXB = SHARK_GROWTH_MAX_BINS;
+constexpr int KC = 128;       // K values (2 per pair) per MFMA chunk
+constexpr int NCT = 64;       // cells per MFMA workgroup
+constexpr int SUB_PER_LAUNCH = 64;
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------ host tables (bins, pairs)
+struct Tables {
+  int nb;
+  double zeta, eta, massmin, m_mono, ebr_mono;
+  double aplus[MAXB], aminus[MAXB], mminus[MAXB], sdust[MAXB], mdust[MAXB];
+  double redis[MAXB * MAXB];  // redis[j + nb*i]
+};
+
+struct PairC {
+  int i, j, t1, t2;  // partner bins and the two coagulation target bins (0-based)
+  int nfrag, rrow;   // fragments go to bins [0, nfrag); their weights start at redis[rrow]
+  double eps, m1, m2, geom;  // geom = clustered^2 pi (s1+s2)^2
+};
+
+static double powi(double a, int b) {  // x**n for integer n, binary powering
+  int recip = b < 0;
+  if (recip) b = -b;
+  double r = 1.0;
+  for (;;) {
+    if (b & 1) r *= a;
+    b /= 2;
+    if (b == 0) break;
+    a *= a;
+  }
+  return recip ? 1.0 / r : r;
+}
+
+static int ffloor(double x) {  // FLOOR to a default integer; -inf/NaN give INT_MIN
+  double f = floor(x);
+  if (!(f >= -2147483648.0 && f <= 2147483647.0)) return INT_MIN;
+  return (int)f;
+}
+
+// Bin grid of distribution_dust (src/dust/dust_init.f90:22-39) and the growth constants of
+// dust_growth.f90:41-59.  Integral real powers x**3. are products (as the reference's compiler
+// lowers them), the mass grid uses the bare size (the ice mantle is added afterwards, :89-90).
+static void build_tables(const shark_growth_params* p, Tables* t) {
+  const int nb = p->nbins;
+  memset(t, 0, sizeof(*t));
+  t->nb = nb;
+  t->zeta = pow(p->smax / p->smin, 1.0 / nb);
+  t->eta = t->zeta * t->zeta * t->zeta;
+  const double four3 = (double)(4.0f / 3.0f);
+  const double smin3 = p->smin * p->smin * p->smin;
+  double m_min = four3 * kPi * p->rhograin * smin3 / p->unit_m;
+  if (p->kernel_type != 0) m_min = smin3;
+  for (int k = 0; k < nb; ++k) {
+    t->aminus[k] = p->smin * powi(t->zeta, k) / p->unit_l;
+    t->aplus[k] = p->smin * powi(t->zeta, k + 1) / p->unit_l;
+    t->mminus[k] = m_min * powi(t->eta, k);
+    const double s = sqrt(t->aminus[k] * t->aplus[k]);
+    t->mdust[k] = four3 * kPi * p->rhograin / p->unit_d * (s * s * s);
+    t->sdust[k] = (p->kernel_type == 0) ? s + p->ice_mantle / p->unit_l : s;
+  }
+  t->massmin = t->mminus[0];
+  const double a_mu = p->size_mono / 2.0;
+  t->ebr_mono = p->abr * pow(p->gamma_grains, 5.0 / 3.0) * pow(a_mu, 4.0 / 3.0) /
+                pow(p->estar_grains, 2.0 / 3.0) / (p->unit_v * p->unit_v * p->unit_m);
+  t->m_mono = 4.0 / 3.0 * kPi * p->rhograin * (p->size_mono * p->size_mono * p->size_mono) / p->unit_m;
+  const double e = 4.0 - p->slope_mono;
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < nb; ++j)
+      t->redis[j + nb * i] = (j <= i) ? (pow(t->aplus[j], e) - pow(t->aminus[j], e)) /
+                                            (pow(t->aplus[i], e) - pow(t->aminus[0], e))
+                                      : 0.0;
+  for (int i = 0; i < nb; ++i) {
+    double s = 0.0;
+    for (int j = 0; j < nb; ++j) s = s + t->redis[j + nb * i];
+    for (int j = 0; j < nb; ++j) t->redis[j + nb * i] = t->redis[j + nb * i] / s;
+  }
+}
+
+// Pair order (i outer, j <= i inner) as the reference's loop; the coagulation targets are binned
+// by m1+m2 and the fragments by 0.1 (m1+m2) (dust_growth.f90:183-187, :220 with the fragmented
+// fraction factored out), which makes them the same for every cell.
+static void build_pairs(const shark_growth_params* p, const Tables* t, std::vector<PairC>* out) {
+  const int nb = t->nb;
+  const double log_eta = log(t->eta);
+  out->clear();
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j <= i; ++j) {
+      PairC c;
+      const double m1 = t->mdust[i], m2 = t->mdust[j], s1 = t->sdust[i], s2 = t->sdust[j];
+      const double mt = m1 + m2;
+      const int fl = ffloor(log(mt / t->massmin) / log_eta + 1);
+      const int ic1 = std::max(std::min(fl, nb), 1);
+      const int ic2 = std::max(std::min(fl == INT_MIN ? INT_MIN : fl + 1, nb), 1);
+      double eps = 1.0;
+      if (ic1 < ic2) eps = std::min((t->mdust[ic2 - 1] - mt) / (t->mdust[ic2 - 1] - t->mdust[ic1 - 1]), 1.0);
+      const double ff = 1.0;
+      const int g = ffloor(log(0.1 * ff * (m1 + m2) / t->massmin) / log_eta + 1);
+      const int ifrag2 = std::max(std::min(g, ic1), 1);
+      c.i = i;
+      c.j = j;
+      c.t1 = ic1 - 1;
+      c.t2 = ic2 - 1;
+      c.nfrag = ifrag2;
+      c.rrow = nb * (ifrag2 - 1);
+      c.eps = eps;
+      c.m1 = m1;
+      c.m2 = m2;
+      c.geom = p->clustered * p->clustered * kPi * ((s1 + s2) * (s1 + s2));
+      out->push_back(c);
+    }
+}
+
+// ------------------------------------------------------------------ device arguments
+struct GArgs {
+  int nb, npairs, nch, c0;
+  int64_t ld;  // leading dimension (ncells) of rhod
+  int kernel_type, frag, frag_mode, brownian, turb, drift, ambipolar, max_sub, sub_per_launch;
+  int kpad;  // padded K (2 per pair) for the MFMA modes
+  int want64, want32;  // which per-pair prefactors k_gpairs writes
+  double dt, cfl, eps_thr, eps_thr_frag, ratio_min, sticking, vfrag_code, m_mono, ebr_mono;
+  double turb_coef, unit_v, unit_m, unit_d, mu_gas, gamma, hoverr, mstar;
+  double kts;   // sqrt(pi gamma/8) * (rhograin/unit_d): Epstein prefactor
+  double s2pi;  // sqrt(2 pi)
+  const PairC* pairs;
+  const double* redis;  // [nb*nb]
+  const double* mdust;
+  const double* sdust;
+  double* rhod;  // [nb][ld]
+  const double* rho_gas;
+  const double* cs;
+  const double* radius;
+  double* env;  // [4][nch]: T, t_L, t_eta, Re
+  double* ts;   // [nb][nch]
+  double* A;    // [npairs][nch]
+  double* F;    // [npairs][nch]
+  float* Ah;    // [npairs][nch]
+  float* Fh;    // [npairs][nch]
+  double* tg;   // [ld] time reached inside the growth call
+  int* nit;     // [ld] sub-steps taken
+  int* unfinished;
+  int* bad;     // set when a rate or density is not finite
+  const float* Gf32;     // MFMA A operand, FP32 layout
+  const u32x4* Gbhi;     // MFMA A operand, BF16 hi
+  const u32x4* Gblo;     // MFMA A operand, BF16 lo
+};
+
+__device__ __forceinline__ double fmn(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double fmx(double a, double b) { return a > b ? a : b; }
+
+// Disk environment of one cell: T from the local isothermal sound speed, t_L = 1/Omega, the
+// Reynolds number of the midplane density (the Reynolds formula of dust_growth.f90:77), and the
+// Epstein stopping times of compute_tstop (src/setups/disk_polar/setup.f90:436).
+__global__ void k_genv(GArgs a) {
+  const int cc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cc >= a.nch) return;
+  const int64_t c = a.c0 + cc;
+  const double r = a.radius[c], sigma = a.rho_gas[c], cs = a.cs[c];
+  const double H = a.hoverr * r;
+  const double om = sqrt(a.mstar / (r * r * r));
+  const double rho_mid = sigma / (a.s2pi * H);
+  const double csp = cs * a.unit_v;
+  const double T = csp * csp * (a.mu_gas * kMH) / (a.gamma * kKB);
+  const double t_L = 1.0 / om;
+  const double Re = 6.2e7 * sqrt(rho_mid * a.unit_d / (a.mu_gas * kMH) / 1e5) * sqrt(T / 10.0);
+  const double t_eta = t_L / sqrt(Re);
+  a.env[cc] = T;
+  a.env[a.nch + cc] = t_L;
+  a.env[2 * a.nch + cc] = t_eta;
+  a.env[3 * a.nch + cc] = Re;
+  for (int k = 0; k < a.nb; ++k)
+    a.ts[(int64_t)k * a.nch + cc] = a.kts * a.sdust[k] / ((sigma / (a.s2pi * H)) * cs);
+}
+
+// Relative velocity of a pair (dust_growth.f90:87-114): turbulent (Ormel & Cuzzi regimes by
+// the stopping time against t_eta and t_L), Brownian, and the drift/ambipolar terms, which are
+// zero in the disk environment but still enter the quadrature sums as in the reference.
+__device__ double pair_dv(const GArgs& a, double tsi, double tsj, double mi, double mj, double T,
+                          double cs_T, double t_L, double t_eta, double reynolds) {
+  const double x = tsj / tsi;
+  const double fst = (double)3.2f - 1.0 - x + 2.0 / (1. + x) * ((double)(1.0f / 2.6f) + (x * x * x) / ((double)1.6f + x));
+  const double St1 = tsi / t_L, St2 = tsj / t_L;
+  const double rem = 1.0 / sqrt(reynolds);
+  const double v1 = a.turb_coef * cs_T * sqrt((St1 - St2) / (St1 + St2)) *
+                    sqrt(St1 * St1 / (St1 + rem) - St2 * St2 / (St2 + rem));
+  const double v2 = a.turb_coef * cs_T * sqrt(fst * St1);
+  const double v3 = a.turb_coef * cs_T * sqrt(1.0 / (1.0 + St1) + 1.0 / (1.0 + St2));
+  double vt = v2;
+  if (tsi < t_eta) vt = v1;
+  if (tsi > t_L) vt = v3;
+  const double um = a.unit_m;
+  const double inner = (8.0 * kKB * T / kPi) * (mi * um + mj * um) / (mi * mj * (um * um)) /
+                       (a.unit_v * a.unit_v);
+  const double sq = sqrt(inner);
+  const double vb = sqrt(sq * sq);
+  const double vh = 0.0;
+  const double ad = 0.0 / a.unit_v * (0.0 - 0.0);
+  const double va = sqrt(ad * ad);
+  double d = 0.0;
+  if (a.turb) d = vt;
+  if (a.brownian) d = sqrt(d * d + vb * vb);
+  if (a.drift) d = sqrt(d * d + vh * vh);
+  if (a.ambipolar) d = sqrt(d * d + va * va);
+  return d;
+}
+
+// Per (pair, cell): collision prefactor A = clustered^2 pi (s1+s2)^2 dv, fragmented fraction
+// (energy threshold dust_growth.f90:142-144 or velocity threshold :145), and the FP32 copies the
+// contraction reads.
+__global__ void k_gpairs(GArgs a) {
+  const int cc = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (cc >= a.nch) return;
+  const PairC P = a.pairs[p];
+  const int64_t o = (int64_t)p * a.nch + cc;
+  const double m1 = P.m1, m2 = P.m2;
+  double A, f = 0.0, ahat;
+  if (a.kernel_type == 0) {
+    const double T = a.env[cc], t_L = a.env[a.nch + cc], t_eta = a.env[2 * a.nch + cc];
+    const double Re = a.env[3 * a.nch + cc];
+    const double cs_T = a.cs[a.c0 + cc];
+    const double dv = pair_dv(a, a.ts[(int64_t)P.i * a.nch + cc], a.ts[(int64_t)P.j * a.nch + cc],
+                              m1, m2, T, cs_T, t_L, t_eta, Re);
+    if (a.frag) {
+      if (a.frag_mode == 0) {
+        const double Ecol = 0.5 * (m1 * m2) / (m1 + m2) * (dv * dv);
+        const double Ebr = (m1 + m2) / a.m_mono * a.ebr_mono;
+        f = fmx(fmn((Ecol - 0.1 * Ebr) / (4.9 * Ebr), 1.0), 0.0);
+      } else {
+        f = fmx(fmn((fabs(dv) - 0.1 * a.vfrag_code) / a.vfrag_code, 1.0), 0.0);
+      }
+    }
+    A = P.geom * dv;
+    ahat = A * ((m1 + m2) / m1 / m2);
+  } else {
+    A = 0.0;
+    ahat = (a.kernel_type == 1) ? (m1 + m2) / m1 / m2 : (m1 + m2) * (m1 + m2) / m1 / m2;
+  }
+  if (a.want64) {
+    a.A[o] = A;
+    a.F[o] = f;
+  }
+  if (a.want32) {
+    a.Ah[o] = (float)ahat;
+    a.Fh[o] = (float)f;
+  }
+}
+
+// ------------------------------------------------------------------ mode 0: FP64 pair loop
+// One lane per cell, rates in LDS as [bin][lane]; the pair loop is uniform across the wave so
+// the pair constants come through scalar loads.  Statement order and rounding follow the
+// reference (dust_growth.f90:117-241), so the result is bit-exact with the FP64 oracle.
+__global__ __launch_bounds__(64) void k_gfp64(GArgs a) {
+  extern __shared__ double dd[];
+  const int lane = threadIdx.x;
+  const int cc = blockIdx.x * 64 + lane;
+  const bool valid = cc < a.nch;
+  const int64_t c = a.c0 + (valid ? cc : 0);
+  const int nb = a.nb;
+  double* __restrict__ rh = a.rhod + c;
+  const double rg = valid ? a.rho_gas[c] : 1.0;
+  double tgrow = valid ? a.tg[c] : 0.0;
+  int niter = valid ? a.nit[c] : 0;
+  bool act = valid && tgrow < a.dt && !(a.max_sub > 0 && niter >= a.max_sub);
+  const int kt = a.kernel_type;
+  const double st = a.sticking;
+  for (int s = 0; s < a.sub_per_launch; ++s) {
+    if (!__any(act)) break;
+    for (int k = 0; k < nb; ++k) dd[k * 64 + lane] = 0.0;
+    if (act) {
+      niter = niter + 1;
+      int p = 0;
+      for (int i = 0; i < nb; ++i) {
+        const double ri = rh[(int64_t)i * a.ld];
+        for (int j = 0; j <= i; ++j, ++p) {
+          const PairC P = a.pairs[p];
+          const double m1 = P.m1, m2 = P.m2;
+          const double rj = rh[(int64_t)j * a.ld];
+          const int64_t o = (int64_t)p * a.nch + cc;
+          double f = a.frag ? a.F[o] : 0.0, pc = 1.0;
+          double dndt;
+          if (kt == 1) {
+            pc = 1.0; f = 0.0;
+            dndt = ri * rj / m1 / m2;
+          } else if (kt == 2) {
+            pc = 1.0; f = 0.0;
+            dndt = (m1 + m2) * ri * rj / m1 / m2;
+          } else {
+            dndt = a.A[o] * ri * rj / m1 / m2;
+          }
+          if (i == j) dndt = dndt / 2.0;
+          if (kt == 0) {
+            if (a.eps_thr > 0.0 && (ri < a.eps_thr * rg)) pc = 0.0;
+            if (a.eps_thr > 0.0 && (rj < a.eps_thr * rg)) pc = 0.0;
+            if (a.eps_thr_frag > 0.0 && (ri < a.eps_thr_frag * rg)) f = 0.0;
+            if (a.eps_thr_frag > 0.0 && (rj < a.eps_thr_frag * rg)) f = 0.0;
+          }
+          const double eps = P.eps;
+          dd[P.i * 64 + lane] = dd[P.i * 64 + lane] - pc * (1.0 - f) * m1 * dndt * st - pc * f * m1 * dndt;
+          dd[P.j * 64 + lane] = dd[P.j * 64 + lane] - pc * (1.0 - f) * m2 * dndt * st - pc * f * m2 * dndt;
+          if (f < 1.0) {
+            dd[P.t1 * 64 + lane] = dd[P.t1 * 64 + lane] + eps * pc * (1.0 - f) * (m1 + m2) * dndt * st;
+            dd[P.t2 * 64 + lane] = dd[P.t2 * 64 + lane] + (1. - eps) * pc * (1.0 - f) * (m1 + m2) * dndt * st;
+          }
+          if ((a.frag && f > 0.0) && pc > 0.) {
+            const double base = pc * f * (m1 + m2) * dndt;
+            for (int k = 0; k < P.nfrag; ++k)
+              dd[k * 64 + lane] = dd[k * 64 + lane] + base * a.redis[P.rrow + k];
+          }
+        }
+      }
+      double dtg = 1e32;
+      for (int k = 0; k < nb; ++k) {
+        const double d = dd[k * 64 + lane];
+        if (fabs(d) != 0.0) dtg = fmn(dtg, rh[(int64_t)k * a.ld] / fabs(d) * a.cfl);
+      }
+      dtg = fmn(dtg, a.dt - tgrow);
+      for (int k = 0; k < nb; ++k) {
+        const double v = fmx(rh[(int64_t)k * a.ld] + dd[k * 64 + lane] * dtg, rg * a.ratio_min);
+        rh[(int64_t)k * a.ld] = v;
+      }
+      tgrow = tgrow + dtg;
+      if (!(tgrow == tgrow) || !(dtg > 0.0)) {  // NaN time or a sub-step that makes no progress
+        atomicOr(a.bad, 1);
+        act = false;
+      } else {
+        act = tgrow < a.dt && !(a.max_sub > 0 && niter >= a.max_sub);
+      }
+    }
+  }
+  if (valid) {
+    a.tg[c] = tgrow;
+    a.nit[c] = niter;
+    if (act) atomicAdd(a.unfinished, 1);
+  }
+}
+
+// ------------------------------------------------------------------ modes 1-3: MFMA contraction
+__device__ __forceinline__ unsigned short bf16_rne(float x) {
+  unsigned int u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(unsigned short h) { return __uint_as_float((unsigned int)h << 16); }
+
+template <int MODE>
+struct XTile;  // LDS layout of one K-chunk of X for 64 cells
+
+template <>
+struct XTile<1> {
+  float x[KC][NCT];
+};
+template <>
+struct XTile<2> {
+  unsigned short hi[KC / 8][NCT][8];
+  unsigned short lo[KC / 8][NCT][8];
+};
+template <>
+struct XTile<3> {
+  unsigned short hi[KC / 8][NCT][8];
+};
+
+template <int MODE>
+struct GSmem {
+  union {
+    XTile<MODE> xt;
+    double d[MAXB][NCT];  // rates after the contraction
+  } u;
+  float epsf[MAXB][NCT];  // rho_k / rho_d,total in FP32
+  unsigned long long okc[NCT], okf[NCT];
+  double rho2[NCT];       // rho_d,total^2
+  int act[NCT];
+  int any;
+};
+
+template <int MODE>
+__device__ __forceinline__ void put_x(GSmem<MODE>& sm, int kk, int cell, float v) {
+  if constexpr (MODE == 1) {
+    sm.u.xt.x[kk][cell] = v;
+  } else {
+    const unsigned short h = bf16_rne(v);
+    sm.u.xt.hi[kk >> 3][cell][kk & 7] = h;
+    if constexpr (MODE == 2) sm.u.xt.lo[kk >> 3][cell][kk & 7] = bf16_rne(v - bf16_to_f(h));
+  }
+}
+
+// Workgroup: 4 waves over a 64-bin x 64-cell rate tile; wave w owns bins 32*(w&1).. and cells
+// 32*(w>>1)..  The X chunk (KC values x 64 cells) is generated into LDS by all 256 lanes, then
+// each wave multiplies its 32 x KC slice of Gt (pre-laid-out in global memory, read through L2)
+// with it.  Chunk results are added with a Kahan sum.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gmfma(GArgs a) {
+  __shared__ GSmem<MODE> sm;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int rb = w & 1, ch = w >> 1;
+  const int cbase = blockIdx.x * NCT;
+  const int nb = a.nb;
+  const int kt = a.kernel_type;
+  // per-cell state lives with thread `tid` for tid < 64
+  int64_t c = 0;
+  double tgrow = 0.0, rg = 1.0;
+  int niter = 0;
+  bool valid = false;
+  if (tid < NCT) {
+    valid = cbase + tid < a.nch;
+    c = a.c0 + (valid ? cbase + tid : 0);
+    if (valid) {
+      tgrow = a.tg[c];
+      niter = a.nit[c];
+      rg = a.rho_gas[c];
+    }
+  }
+  const int nchunks = a.kpad / KC;
+  for (int s = 0; s < a.sub_per_launch; ++s) {
+    if (tid == 0) sm.any = 0;
+    __syncthreads();
+    if (tid < NCT) {
+      const bool act = valid && tgrow < a.dt && !(a.max_sub > 0 && niter >= a.max_sub);
+      sm.act[tid] = act;
+      if (act) sm.any = 1;  // benign race: every writer stores 1
+      double tot = 0.0;
+      unsigned long long mc = 0, mf = 0;
+      for (int k = 0; k < nb; ++k) {
+        const double r = a.rhod[(int64_t)k * a.ld + c];
+        tot += r;
+        const bool okc = !(kt == 0 && a.eps_thr > 0.0 && r < a.eps_thr * rg);
+        const bool okf = !(kt == 0 && a.eps_thr_frag > 0.0 && r < a.eps_thr_frag * rg);
+        mc |= (unsigned long long)okc << k;
+        mf |= (unsigned long long)okf << k;
+      }
+      const double inv = act ? 1.0 / tot : 0.0;
+      for (int k = 0; k < nb; ++k) sm.epsf[k][tid] = (float)(a.rhod[(int64_t)k * a.ld + c] * inv);
+      sm.okc[tid] = mc;
+      sm.okf[tid] = a.frag ? mf : 0ull;
+      sm.rho2[tid] = tot * tot;
+    }
+    __syncthreads();
+    if (!sm.any) break;
+
+    f32x16 sum, comp;
+    for (int r = 0; r < 16; ++r) { sum[r] = 0.f; comp[r] = 0.f; }
+    for (int q = 0; q < nchunks; ++q) {
+      // ---- X generation: pair pp = w + 4n of the chunk, cell = lane
+      {
+        const int cell = lane;
+        const int cc = cbase + cell;
+        const bool cact = sm.act[cell] != 0;
+        const unsigned long long mc = sm.okc[cell], mf = sm.okf[cell];
+        for (int n = 0; n < KC / 2 / 4; ++n) {
+          const int pp = w + 4 * n;
+          const int p = q * (KC / 2) + pp;
+          float xc = 0.f, xf = 0.f;
+          if (p < a.npairs && cact) {
+            const PairC P = a.pairs[p];
+            const int64_t o = (int64_t)p * a.nch + cc;
+            const float ah = a.Ah[o];
+            float f = a.Fh[o];
+            const bool pc = ((mc >> P.i) & (mc >> P.j) & 1ull) != 0;
+            if (!(((mf >> P.i) & (mf >> P.j) & 1ull) != 0)) f = 0.f;
+            float e = sm.epsf[P.i][cell] * sm.epsf[P.j][cell] * ah;
+            if (P.i == P.j) e *= 0.5f;
+            if (pc) {
+              xc = (1.f - f) * e;
+              xf = f * e;
+            }
+          }
+          put_x<MODE>(sm, 2 * pp, cell, xc);
+          put_x<MODE>(sm, 2 * pp + 1, cell, xf);
+        }
+      }
+      __syncthreads();
+      // ---- contraction of this chunk
+      f32x16 acc;
+      for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+      const int cl = ch * 32 + (lane & 31);
+      if constexpr (MODE == 1) {
+        const float* g = a.Gf32 + ((int64_t)(rb * (a.kpad / 2)) + q * (KC / 2)) * 64 + lane;
+        for (int st = 0; st < KC / 2; ++st) {
+          const float av = g[st * 64];
+          const float bv = sm.u.xt.x[2 * st + (lane >> 5)][cl];
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+        }
+      } else {
+        const int64_t gb = ((int64_t)(rb * (a.kpad / 16)) + q * (KC / 16)) * 64 + lane;
+        for (int st = 0; st < KC / 16; ++st) {
+          const u32x4 ah = a.Gbhi[gb + st * 64];
+          const int kg = 2 * st + (lane >> 5);
+          const u32x4 bh = *reinterpret_cast<const u32x4*>(&sm.u.xt.hi[kg][cl][0]);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah),
+                                                        __builtin_bit_cast(bf16x8, bh), acc, 0, 0, 0);
+          if constexpr (MODE == 2) {
+            const u32x4 al = a.Gblo[gb + st * 64];
+            const u32x4 bl = *reinterpret_cast<const u32x4*>(&sm.u.xt.lo[kg][cl][0]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah),
+                                                          __builtin_bit_cast(bf16x8, bl), acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, al),
+                                                          __builtin_bit_cast(bf16x8, bh), acc, 0, 0, 0);
+          }
+        }
+      }
+      for (int r = 0; r < 16; ++r) {  // Kahan: sum += acc
+        const float y = acc[r] - comp[r];
+        const float t = sum[r] + y;
+        comp[r] = (t - sum[r]) - y;
+        sum[r] = t;
+      }
+      __syncthreads();
+    }
+    // ---- rates to LDS (FP64): lane holds column lane%32, rows 8*(r/4) + 4*(lane/32) + r%4
+    {
+      const int cl = ch * 32 + (lane & 31);
+      for (int r = 0; r < 16; ++r) {
+        const int row = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        sm.u.d[row][cl] = ((double)sum[r] - (double)comp[r]) * sm.rho2[cl];
+      }
+    }
+    __syncthreads();
+    // ---- FP64 update of each cell, CFL-limited sub-step (dust_growth.f90:227-241), mass rescale
+    if (tid < NCT && sm.act[tid]) {
+      niter = niter + 1;
+      double dtg = 1e32, m_old = 0.0, m_new = 0.0;
+      for (int k = 0; k < nb; ++k) {
+        const double d = sm.u.d[k][tid];
+        const double r = a.rhod[(int64_t)k * a.ld + c];
+        m_old += r;
+        if (fabs(d) != 0.0) dtg = fmn(dtg, r / fabs(d) * a.cfl);
+      }
+      dtg = fmn(dtg, a.dt - tgrow);
+      for (int k = 0; k < nb; ++k) {
+        const double v = fmx(a.rhod[(int64_t)k * a.ld + c] + sm.u.d[k][tid] * dtg, rg * a.ratio_min);
+        sm.u.d[k][tid] = v;
+        m_new += v;
+      }
+      const double sc = m_old / m_new;
+      for (int k = 0; k < nb; ++k) a.rhod[(int64_t)k * a.ld + c] = sm.u.d[k][tid] * sc;
+      tgrow = tgrow + dtg;
+      if (!(tgrow == tgrow) || !(dtg > 0.0) || !(sc == sc)) {
+        atomicOr(a.bad, 1);
+        valid = false;
+        sm.act[tid] = 0;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < NCT && valid) {
+    a.tg[c] = tgrow;
+    a.nit[c] = niter;
+    if (tgrow < a.dt && !(a.max_sub > 0 && niter >= a.max_sub)) atomicAdd(a.unfinished, 1);
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host state and C-ABI
+struct shark_growth {
+  shark_growth_params p;
+  Tables t;
+  std::vector<PairC> pairs;
+  int64_t ncells = 0;
+  int nch_max = 0;
+  int kpad = 0;
+  hipStream_t stream = nullptr;
+  PairC* dpairs = nullptr;
+  double *dredis = nullptr, *dmdust = nullptr, *dsdust = nullptr;
+  double *drhod = nullptr, *drho = nullptr, *dcs = nullptr, *drad = nullptr;
+  double *denv = nullptr, *dts = nullptr, *dA = nullptr, *dF = nullptr;
+  float *dAh = nullptr, *dFh = nullptr;
+  double* dtg = nullptr;
+  int* dnit = nullptr;
+  int* dflags = nullptr;  // [0] unfinished, [1] bad
+  int* hflags = nullptr;  // pinned
+  float* dGf32 = nullptr;
+  u32x4 *dGbhi = nullptr, *dGblo = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double last_ms = 0.0;
+  bool uploaded = false;
+};
+
+void shark_growth_free(shark_growth* g) {
+  if (!g) return;
+  void* ptrs[] = {g->dpairs, g->dredis, g->dmdust, g->dsdust, g->drhod, g->drho, g->dcs, g->drad,
+                  g->denv, g->dts, g->dA, g->dF, g->dAh, g->dFh, g->dtg, g->dnit, g->dflags,
+                  g->dGf32, g->dGbhi, g->dGblo};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (g->hflags) (void)hipHostFree(g->hflags);
+  if (g->e0) (void)hipEventDestroy(g->e0);
+  if (g->e1) (void)hipEventDestroy(g->e1);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
+#define GCHK(x)                                                                          \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) {                                                              \
+      char m_[256];                                                                      \
+      snprintf(m_, sizeof(m_), "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
+               __LINE__);                                                                \
+      return shark::fail_msg(SHARK_GROWTH_EHIP, m_);                                      \
+    }                                                                                    \
+  } while (0)
+
+static int gfail(int code, const char* msg) { return shark::fail_msg(code, msg); }
+
+static unsigned short bf16_host(float x) {
+  unsigned int u;
+  memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+static float bf16_host_f(unsigned short h) {
+  unsigned int u = (unsigned int)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+// Gt[k][kk]: kk = 2p (coagulated flux of pair p) or 2p+1 (fragmented flux), per unit mass flux
+// (m1+m2) dndt.  Losses -m1/(m1+m2), -m2/(m1+m2); coagulation gains eps and 1-eps at the two
+// target bins (times the sticking efficiency, which also scales the coagulation losses);
+// fragment gains along the redistribution row of the pair.
+static void build_gt(const shark_growth* g, std::vector<double>* gt) {
+  const int nb = g->t.nb;
+  const int kpad = g->kpad;
+  gt->assign((size_t)MAXB * kpad, 0.0);
+  for (size_t p = 0; p < g->pairs.size(); ++p) {
+    const PairC& P = g->pairs[p];
+    const double mt = P.m1 + P.m2;
+    double* col_c = gt->data() + 2 * p;
+    double* col_f = gt->data() + 2 * p + 1;
+    auto at = [&](double* col, int k) -> double& { return col[(size_t)k * kpad]; };
+    const double st = g->p.sticking;
+    at(col_c, P.i) += -st * P.m1 / mt;
+    at(col_c, P.j) += -st * P.m2 / mt;
+    at(col_c, P.t1) += st * P.eps;
+    at(col_c, P.t2) += st * (1.0 - P.eps);
+    if (g->p.fragmentation && g->p.kernel_type == 0) {
+      at(col_f, P.i) += -P.m1 / mt;
+      at(col_f, P.j) += -P.m2 / mt;
+      for (int k = 0; k < P.nfrag; ++k) at(col_f, k) += g->t.redis[P.rrow + k];
+    }
+  }
+  (void)nb;
+}
 
 extern "C" {
-static int not_built() { return shark::fail_msg(1, "dust growth kernels are not built in this library"); }
-int shark_growth_create(const shark_growth_params*, int64_t, shark_growth** g) { if (g) *g = nullptr; return not_built(); }
-int shark_growth_upload(shark_growth*, const double*, const double*, const double*, const double*) { return not_built(); }
-int shark_growth_run(shark_growth*, int, double) { return not_built(); }
-int shark_growth_download(shark_growth*, double*, int64_t*) { return not_built(); }
-int shark_growth_kernel_time(shark_growth*, double*) { return not_built(); }
-int shark_growth_destroy(shark_growth* g) { shark_growth_free(g); return 0; }
+
+int shark_growth_create(const shark_growth_params* p, int64_t ncells, shark_growth** out) {
+  if (!out) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_create: null output handle");
+  *out = nullptr;
+  if (!p) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_create: null parameters");
+  if (p->version != SHARK_GROWTH_VERSION)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_growth_params.version does not match SHARK_GROWTH_VERSION");
+  if (p->nbins < 2 || p->nbins > MAXB)
+    return gfail(SHARK_GROWTH_EINVAL, "nbins must lie in [2, 64]");
+  if (ncells < 1) return gfail(SHARK_GROWTH_EINVAL, "ncells must be positive");
+  if (p->env != SHARK_GROWTH_ENV_DISK)
+    return gfail(SHARK_GROWTH_EINVAL,
+                 "only the disk environment runs on the GPU (the collapse branch needs the "
+                 "barotropic EOS and ambipolar velocities of its setup)");
+  if (p->kernel_type < 0 || p->kernel_type > 2)
+    return gfail(SHARK_GROWTH_EINVAL, "kernel_type must be 0, 1 or 2");
+  if (p->kernel_type == 0 && p->fragmentation && !p->decomposed)
+    return gfail(SHARK_GROWTH_EINVAL,
+                 "fragmentation needs decomposed=1 on the GPU: otherwise the bin a pair feeds "
+                 "depends on the cell's fragmented fraction");
+  if (p->frag_mode != 0 && p->frag_mode != 1)
+    return gfail(SHARK_GROWTH_EINVAL, "frag_mode must be 0 (energy) or 1 (velocity)");
+  if (p->max_substeps < 0) return gfail(SHARK_GROWTH_EINVAL, "max_substeps must be >= 0");
+  if (!(p->cfl_growth > 0.0)) return gfail(SHARK_GROWTH_EINVAL, "cfl_growth must be positive");
+  shark_growth* g = new shark_growth();
+  g->p = *p;
+  g->ncells = ncells;
+  build_tables(p, &g->t);
+  build_pairs(p, &g->t, &g->pairs);
+  const int np = (int)g->pairs.size();
+  g->kpad = ((2 * np + KC - 1) / KC) * KC;
+  // per-pair scratch (A, F in FP64 and their FP32 copies: 24 B per pair and cell) is bounded
+  // to 4 GiB by processing the cells in chunks
+  const int64_t cap = ((int64_t)4 << 30) / ((int64_t)np * 24);
+  int64_t nch = ncells < cap ? ncells : cap;
+  nch = nch >= NCT ? (nch / NCT) * NCT : nch;
+  if (nch < 1) nch = 1;
+  g->nch_max = (int)nch;
+  int rc = 0;
+#define GC(x)                                     \
+  do {                                            \
+    hipError_t e_ = (x);                          \
+    if (e_ != hipSuccess) {                       \
+      char m_[256];                               \
+      snprintf(m_, sizeof(m_), "%s failed: %s", #x, hipGetErrorString(e_)); \
+      rc = shark::fail_msg(SHARK_GROWTH_EHIP, m_); \
+      shark_growth_free(g);                       \
+      return rc;                                  \
+    }                                             \
+  } while (0)
+  GC(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+  GC(hipEventCreate(&g->e0));
+  GC(hipEventCreate(&g->e1));
+  GC(hipMalloc(&g->dpairs, sizeof(PairC) * np));
+  GC(hipMalloc(&g->dredis, sizeof(double) * MAXB * MAXB));
+  GC(hipMalloc(&g->dmdust, sizeof(double) * MAXB));
+  GC(hipMalloc(&g->dsdust, sizeof(double) * MAXB));
+  GC(hipMalloc(&g->drhod, sizeof(double) * p->nbins * ncells));
+  GC(hipMalloc(&g->drho, sizeof(double) * ncells));
+  GC(hipMalloc(&g->dcs, sizeof(double) * ncells));
+  GC(hipMalloc(&g->drad, sizeof(double) * ncells));
+  GC(hipMalloc(&g->dtg, sizeof(double) * ncells));
+  GC(hipMalloc(&g->dnit, sizeof(int) * ncells));
+  GC(hipMalloc(&g->denv, sizeof(double) * 4 * nch));
+  GC(hipMalloc(&g->dts, sizeof(double) * p->nbins * nch));
+  GC(hipMalloc(&g->dA, sizeof(double) * np * nch));
+  GC(hipMalloc(&g->dF, sizeof(double) * np * nch));
+  GC(hipMalloc(&g->dAh, sizeof(float) * np * nch));
+  GC(hipMalloc(&g->dFh, sizeof(float) * np * nch));
+  GC(hipMalloc(&g->dflags, sizeof(int) * 2));
+  GC(hipHostMalloc(&g->hflags, sizeof(int) * 2, hipHostMallocDefault));
+  GC(hipMemcpy(g->dpairs, g->pairs.data(), sizeof(PairC) * np, hipMemcpyHostToDevice));
+  GC(hipMemcpy(g->dredis, g->t.redis, sizeof(double) * MAXB * MAXB, hipMemcpyHostToDevice));
+  GC(hipMemcpy(g->dmdust, g->t.mdust, sizeof(double) * MAXB, hipMemcpyHostToDevice));
+  GC(hipMemcpy(g->dsdust, g->t.sdust, sizeof(double) * MAXB, hipMemcpyHostToDevice));
+  // contraction operands in the MFMA lane layouts
+  std::vector<double> gt;
+  build_gt(g, &gt);
+  const int kpad = g->kpad;
+  {
+    // FP32, v_mfma_f32_32x32x2f32: lane l of step s supplies Gt[rb*32 + l%32][2s + l/32]
+    std::vector<float> a32((size_t)2 * (kpad / 2) * 64);
+    for (int rb = 0; rb < 2; ++rb)
+      for (int s = 0; s < kpad / 2; ++s)
+        for (int l = 0; l < 64; ++l)
+          a32[((size_t)rb * (kpad / 2) + s) * 64 + l] = (float)gt[(size_t)(rb * 32 + (l & 31)) * kpad + 2 * s + (l >> 5)];
+    GC(hipMalloc(&g->dGf32, sizeof(float) * a32.size()));
+    GC(hipMemcpy(g->dGf32, a32.data(), sizeof(float) * a32.size(), hipMemcpyHostToDevice));
+    // BF16, v_mfma_f32_32x32x16_bf16: lane l of step s supplies Gt[rb*32 + l%32][16s + 8(l/32) + e]
+    const size_t n16 = (size_t)2 * (kpad / 16) * 64;
+    std::vector<unsigned short> hi(n16 * 8), lo(n16 * 8);
+    for (int rb = 0; rb < 2; ++rb)
+      for (int s = 0; s < kpad / 16; ++s)
+        for (int l = 0; l < 64; ++l)
+          for (int e = 0; e < 8; ++e) {
+            const float v = (float)gt[(size_t)(rb * 32 + (l & 31)) * kpad + 16 * s + 8 * (l >> 5) + e];
+            const unsigned short h = bf16_host(v);
+            const size_t o = (((size_t)rb * (kpad / 16) + s) * 64 + l) * 8 + e;
+            hi[o] = h;
+            lo[o] = bf16_host(v - bf16_host_f(h));
+          }
+    GC(hipMalloc(&g->dGbhi, 16 * n16));
+    GC(hipMalloc(&g->dGblo, 16 * n16));
+    GC(hipMemcpy(g->dGbhi, hi.data(), 16 * n16, hipMemcpyHostToDevice));
+    GC(hipMemcpy(g->dGblo, lo.data(), 16 * n16, hipMemcpyHostToDevice));
+  }
+#undef GC
+  *out = g;
+  return 0;
 }
+
+int shark_growth_upload(shark_growth* g, const double* rhod, const double* rho_gas, const double* cs,
+                        const double* radius) {
+  if (!g || !rhod || !rho_gas || !cs || !radius)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_growth_upload: null argument");
+  const int64_t n = g->ncells;
+  GCHK(hipMemcpy(g->drhod, rhod, sizeof(double) * g->p.nbins * n, hipMemcpyHostToDevice));
+  GCHK(hipMemcpy(g->drho, rho_gas, sizeof(double) * n, hipMemcpyHostToDevice));
+  GCHK(hipMemcpy(g->dcs, cs, sizeof(double) * n, hipMemcpyHostToDevice));
+  GCHK(hipMemcpy(g->drad, radius, sizeof(double) * n, hipMemcpyHostToDevice));
+  g->uploaded = true;
+  return 0;
+}
+
+int shark_growth_run(shark_growth* g, int mode, double dt) {
+  if (!g) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: null handle");
+  if (!g->uploaded) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: upload the state first");
+  if (mode < SHARK_GROWTH_FP64 || mode > SHARK_GROWTH_MFMA_BF16_PLAIN)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: unknown mode");
+  if (!(dt >= 0.0)) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: dt must be >= 0");
+  const shark_growth_params& p = g->p;
+  GArgs a;
+  memset(&a, 0, sizeof(a));
+  a.nb = p.nbins;
+  a.npairs = (int)g->pairs.size();
+  a.ld = g->ncells;
+  a.kernel_type = p.kernel_type;
+  a.frag = p.fragmentation ? 1 : 0;
+  a.frag_mode = p.frag_mode;
+  a.brownian = p.brownian;
+  a.turb = p.turb;
+  a.drift = p.drift;
+  a.ambipolar = p.ambipolar;
+  a.max_sub = p.max_substeps;
+  a.sub_per_launch = SUB_PER_LAUNCH;
+  a.kpad = g->kpad;
+  a.want64 = mode == SHARK_GROWTH_FP64;
+  a.want32 = !a.want64;
+  a.dt = dt;
+  a.cfl = p.cfl_growth;
+  a.eps_thr = p.eps_threshold;
+  a.eps_thr_frag = p.eps_threshold_frag;
+  a.ratio_min = p.dust_ratio_min;
+  a.sticking = p.sticking;
+  a.vfrag_code = p.vfrag / p.unit_v;
+  a.m_mono = g->t.m_mono;
+  a.ebr_mono = g->t.ebr_mono;
+  a.turb_coef = p.turb_coef;
+  a.unit_v = p.unit_v;
+  a.unit_m = p.unit_m;
+  a.unit_d = p.unit_d;
+  a.mu_gas = p.mu_gas;
+  a.gamma = p.gamma;
+  a.hoverr = p.hoverr;
+  a.mstar = p.mstar;
+  a.kts = sqrt(kPi * p.gamma / 8.0) * (p.rhograin / p.unit_d);
+  a.s2pi = sqrt(2.0 * kPi);
+  a.pairs = g->dpairs;
+  a.redis = g->dredis;
+  a.mdust = g->dmdust;
+  a.sdust = g->dsdust;
+  a.rhod = g->drhod;
+  a.rho_gas = g->drho;
+  a.cs = g->dcs;
+  a.radius = g->drad;
+  a.env = g->denv;
+  a.ts = g->dts;
+  a.A = g->dA;
+  a.F = g->dF;
+  a.Ah = g->dAh;
+  a.Fh = g->dFh;
+  a.tg = g->dtg;
+  a.nit = g->dnit;
+  a.unfinished = g->dflags;
+  a.bad = g->dflags + 1;
+  a.Gf32 = g->dGf32;
+  a.Gbhi = g->dGbhi;
+  a.Gblo = g->dGblo;
+  hipStream_t s = g->stream;
+  GCHK(hipMemsetAsync(g->dtg, 0, sizeof(double) * g->ncells, s));
+  GCHK(hipMemsetAsync(g->dnit, 0, sizeof(int) * g->ncells, s));
+  GCHK(hipMemsetAsync(g->dflags, 0, sizeof(int) * 2, s));
+  GCHK(hipEventRecord(g->e0, s));
+  for (int64_t c0 = 0; c0 < g->ncells; c0 += g->nch_max) {
+    const int64_t rem = g->ncells - c0;
+    a.c0 = (int)c0;
+    a.nch = (int)(rem < g->nch_max ? rem : g->nch_max);
+    const int nblk = (a.nch + 255) / 256;
+    hipLaunchKernelGGL(k_genv, dim3(nblk), dim3(256), 0, s, a);
+    GCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_gpairs, dim3(nblk, a.npairs), dim3(256), 0, s, a);
+    GCHK(hipGetLastError());
+    // relaunch the bounded sub-step kernel until every cell of the chunk reached dt
+    for (int launch = 0;; ++launch) {
+      GCHK(hipMemsetAsync(g->dflags, 0, sizeof(int), s));
+      if (mode == SHARK_GROWTH_FP64) {
+        hipLaunchKernelGGL(k_gfp64, dim3((a.nch + 63) / 64), dim3(64), sizeof(double) * 64 * a.nb, s, a);
+      } else if (mode == SHARK_GROWTH_MFMA_FP32) {
+        hipLaunchKernelGGL(k_gmfma<1>, dim3((a.nch + NCT - 1) / NCT), dim3(256), 0, s, a);
+      } else if (mode == SHARK_GROWTH_MFMA_BF16) {
+        hipLaunchKernelGGL(k_gmfma<2>, dim3((a.nch + NCT - 1) / NCT), dim3(256), 0, s, a);
+      } else {
+        hipLaunchKernelGGL(k_gmfma<3>, dim3((a.nch + NCT - 1) / NCT), dim3(256), 0, s, a);
+      }
+      GCHK(hipGetLastError());
+      GCHK(hipMemcpyAsync(g->hflags, g->dflags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+      GCHK(hipStreamSynchronize(s));
+      if (g->hflags[1]) return gfail(SHARK_GROWTH_ENUM, "growth produced a non-finite density or a zero sub-step");
+      if (g->hflags[0] == 0) break;
+    }
+  }
+  GCHK(hipEventRecord(g->e1, s));
+  GCHK(hipEventSynchronize(g->e1));
+  float ms = 0.f;
+  GCHK(hipEventElapsedTime(&ms, g->e0, g->e1));
+  g->last_ms = ms;
+  return 0;
+}
+
+int shark_growth_download(shark_growth* g, double* rhod, int64_t* nsub) {
+  if (!g) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_download: null handle");
+  GCHK(hipStreamSynchronize(g->stream));
+  if (rhod) GCHK(hipMemcpy(rhod, g->drhod, sizeof(double) * g->p.nbins * g->ncells, hipMemcpyDeviceToHost));
+  if (nsub) {
+    std::vector<int> n(g->ncells);
+    GCHK(hipMemcpy(n.data(), g->dnit, sizeof(int) * g->ncells, hipMemcpyDeviceToHost));
+    for (int64_t c = 0; c < g->ncells; ++c) nsub[c] = n[c];
+  }
+  return 0;
+}
+
+int shark_growth_kernel_time(shark_growth* g, double* ms) {
+  if (!g || !ms) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_kernel_time: null argument");
+  *ms = g->last_ms;
+  return 0;
+}
+
+int shark_growth_destroy(shark_growth* g) {
+  shark_growth_free(g);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,7 +1,8 @@
 """Dust growth: the parameter struct of include/shark_growth.h and the GPU binding.
 
 ``GrowthParams`` mirrors ``shark_growth_params``; ``Growth`` drives the growth kernels of
-libshark_gpu.so (FP64 direct pair loop, or the MFMA contraction in FP32 / BF16).
+libshark_gpu.so: mode FP64 (pair loop in the reference's order, bit-exact with the FP64
+oracle) or the MFMA contraction in FP32, BF16 hi+lo split, or plain BF16.
 The reference routine is dust_growth (src/dust/dust_growth.f90:3-256).
 """
 import ctypes as C
@@ -13,7 +14,8 @@ from . import gpu
 SHARK_GROWTH_VERSION = 1
 MAX_BINS = 64
 ENV_COLLAPSE, ENV_DISK = 0, 1
-FP64, MFMA_FP32, MFMA_BF16 = 0, 1, 2
+FP64, MFMA_FP32, MFMA_BF16, MFMA_BF16_PLAIN = 0, 1, 2, 3
+MODE_NAMES = {FP64: "fp64", MFMA_FP32: "mfma_fp32", MFMA_BF16: "mfma_bf16x3", MFMA_BF16_PLAIN: "mfma_bf16"}
 
 AU = 1.5e13
 MSUN = 2e33
@@ -86,6 +88,29 @@ def disk_params(nbins=64, smin=1e-4, smax=10.0, alpha=1e-3, vfrag=100.0, hoverr=
                 hoverr=hoverr, mstar=1.0)
     base.update(kw)
     return make_params(**base)
+
+
+def disk_state(params, nr=512, nphi=512, r_in=0.4, r_out=2.5, sigma_r0=1700.0, eps_tot=0.01,
+               noise=1e-3, seed=20261202, bins=None):
+    """Growth inputs of a polar disk with nr x nphi cells (the disk setup's radial grid and
+    Sigma ~ r^-1 normalised to sigma_r0 g/cm^2 at r = 1, locally isothermal cs = (H/R) v_K,
+    seeded Gaussian density noise) and an MRN size distribution (mass per bin ~ s^(4-3.5),
+    src/dust/dust_init.f90:164-197) of total dust-to-gas eps_tot.  `bins` are the bin centre
+    sizes in code units (from the growth tables).  Returns (rhod (nbins, ncells), sigma, cs, r)
+    with cells ordered radius-fastest as the step's cell index."""
+    nb = params.nbins
+    dr = (r_out - r_in) / nr
+    rc = r_in + (np.arange(nr) + 0.5) * dr
+    r = np.tile(rc, nphi)
+    rng = np.random.Generator(np.random.Philox(seed))
+    sig0 = sigma_r0 / (params.unit_m / params.unit_l ** 2)
+    sigma = sig0 / r * (1.0 + noise * rng.standard_normal(r.size))
+    cs = params.hoverr * np.sqrt(params.mstar / r)
+    s = np.asarray(bins, dtype=np.float64)
+    w = s ** (4.0 - 3.5) / s ** 0.0
+    w = w / w.sum()
+    rhod = np.ascontiguousarray((eps_tot * sigma)[None, :] * w[:, None])
+    return rhod, sigma, cs, r
 
 
 _P = C.POINTER(C.c_double)

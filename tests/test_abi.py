@@ -4,6 +4,8 @@ import glob
 import os
 import re
 
+import pytest
+
 from shark_amd import gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -54,3 +56,18 @@ def test_create_rejects_out_of_scope_configs_without_gpu():
         c = C.make_cfg(**kw)
         assert lib.shark_gpu_create(ctypes.byref(c), ctypes.byref(h)) != 0
         assert frag in lib.shark_gpu_last_error(), lib.shark_gpu_last_error()
+
+
+def test_growth_create_rejects_out_of_scope_configs_without_gpu():
+    """Growth parameters are validated before any device call."""
+    from shark_amd import growth as G
+    p = G.disk_params(nbins=16)
+    p.env = G.ENV_COLLAPSE
+    with pytest.raises(gpu.SharkGPUError, match="disk environment"):
+        G.Growth(p, 10)
+    p = G.disk_params(nbins=16, decomposed=0)
+    with pytest.raises(gpu.SharkGPUError, match="decomposed"):
+        G.Growth(p, 10)
+    p = G.disk_params(nbins=65)
+    with pytest.raises(gpu.SharkGPUError, match="nbins"):
+        G.Growth(p, 10)

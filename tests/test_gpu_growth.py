@@ -1,0 +1,82 @@
+"""GPU dust growth vs the FP64 oracle (oracle/growth_oracle.c, pinned to the reference's
+dust_growth in tests/test_growth_oracle.py).
+
+Mode FP64 must agree bit for bit (densities and sub-step counts).  The MFMA modes are
+reported per field against the FP64 oracle: mass-weighted size-distribution L1 error
+sum_k |rho_k - rho_k,ref| / sum_k rho_k,ref per cell, with tolerances written here; their
+per-cell mass rescale must conserve total dust mass to FP64 round-off (1e-13)."""
+import numpy as np
+import pytest
+
+import pyoracle
+from shark_amd import growth as G
+
+pytestmark = pytest.mark.gpu
+
+
+def cells(nb=64, nr=64, nphi=8, **kw):
+    p = G.disk_params(nbins=nb, max_substeps=100000, **kw)
+    t = pyoracle.growth_tables(p)
+    rhod, sigma, cs, r = G.disk_state(p, nr=nr, nphi=nphi, bins=np.array(t.sdust[:nb]))
+    return p, t, rhod, sigma, cs, r
+
+
+def run_gpu(p, rhod, sigma, cs, r, mode, dt):
+    g = G.Growth(p, sigma.size)
+    try:
+        g.upload(rhod, sigma, cs, r)
+        g.run(mode, dt)
+        return g.download()
+    finally:
+        g.close()
+
+
+def l1_mass(a, ref):
+    return np.abs(a - ref).sum(axis=0) / ref.sum(axis=0)
+
+
+@pytest.mark.parametrize("nb,kw,dt", [
+    (64, {}, 2e-8),                       # velocity-threshold fragmentation (stiff)
+    (64, dict(frag_mode=0), 1e4),         # energy-threshold fragmentation
+    (33, dict(fragmentation=0), 1e4),
+    (16, dict(kernel_type=1, fragmentation=0), 1e-38),
+    (16, dict(kernel_type=2, fragmentation=0), 1e-8),
+    (24, dict(fragmentation=0, decomposed=0, eps_threshold=1e-3), 1e4),
+])
+def test_fp64_bit_exact(nb, kw, dt):
+    p, t, rhod, sigma, cs, r = cells(nb=nb, **kw)
+    ref, nref = pyoracle.growth_disk(p, t, sigma, cs, r, dt, rhod)
+    out, n = run_gpu(p, rhod, sigma, cs, r, G.FP64, dt)
+    print(f"nb={nb} {kw} dt={dt} substeps {nref.min()}..{nref.max()}")
+    assert nref.max() >= 2
+    assert np.array_equal(n, nref)
+    assert np.array_equal(out, ref), np.abs(out / ref - 1).max()
+
+
+def test_fp64_chunked_relaunch_matches():
+    """More sub-steps than one launch runs (64): the relaunch loop resumes every cell."""
+    p, t, rhod, sigma, cs, r = cells(nb=32, nr=16, nphi=4)
+    dt = 4e-7
+    ref, nref = pyoracle.growth_disk(p, t, sigma, cs, r, dt, rhod)
+    assert nref.max() > 64
+    out, n = run_gpu(p, rhod, sigma, cs, r, G.FP64, dt)
+    assert np.array_equal(n, nref)
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("mode,tol", [(G.MFMA_FP32, 1e-7), (G.MFMA_BF16, 2e-6), (G.MFMA_BF16_PLAIN, 2e-3)])
+def test_mfma_error_vs_fp64_oracle(mode, tol):
+    p, t, rhod, sigma, cs, r = cells(nb=64, nr=64, nphi=8)
+    dt = 2e-8
+    ref, _ = pyoracle.growth_disk(p, t, sigma, cs, r, dt, rhod)
+    out, n = run_gpu(p, rhod, sigma, cs, r, mode, dt)
+    err = l1_mass(out, ref)
+    change = l1_mass(ref, rhod)
+    mass = np.abs(out.sum(axis=0) / rhod.sum(axis=0) - 1)
+    print(f"{G.MODE_NAMES[mode]}: L1 err mean {err.mean():.3e} max {err.max():.3e} "
+          f"(change {change.mean():.3e}); mass residual max {mass.max():.3e}; "
+          f"substeps {n.min()}..{n.max()}")
+    assert change.mean() > 1e-3          # the step does something measurable
+    assert err.max() <= tol
+    assert mass.max() <= 1e-13
+    assert (out > 0).all()
