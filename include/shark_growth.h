@@ -64,6 +64,13 @@ int shark_growth_upload(shark_growth* g, const double* rhod, const double* rho_g
 int shark_growth_run(shark_growth* g, int mode, double dt);
 /* rhod after the last run, and per-cell sub-step counts (either may be NULL). */
 int shark_growth_download(shark_growth* g, double* rhod, int64_t* nsub);
+/* Grain charging of the current state (analytical charge, src/chemistry/ionisation.f90:809-922,
+ * with the Newton-Raphson of src/commons/newton_raphson.f90): per cell the number-weighted mean
+ * grain charge and the ion fraction zeta / sum_k k_i,k n_k, in the disk environment's cgs
+ * midplane densities.  zeta: cosmic-ray ionisation rate (s^-1); mu_ions: ion mass in m_H;
+ * electrons: include electron capture.  Outputs (ncells) may be NULL. */
+int shark_growth_charge(shark_growth* g, double zeta, double mu_ions, int electrons, double* zmean,
+                        double* xion);
 /* Device time (ms) of the last run. */
 int shark_growth_kernel_time(shark_growth* g, double* ms);
 int shark_growth_destroy(shark_growth* g);

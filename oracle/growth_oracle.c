@@ -289,3 +289,111 @@ int64_t growth_oracle_disk(const shark_growth_params* p, const growth_tables* t,
   }
   return total;
 }
+
+/* ------------------------------------------------------------------------------- charging
+ * Grain charge of the analytical prescription (src/chemistry/ionisation.f90:809-922, Fujii et
+ * al. 2011): Newton-Raphson (src/commons/newton_raphson.f90:7-97, tolerance 1e-14, start -1) on
+ * f(Z) of functions_NR.f90:15-43 with derivative :61-91.  Constants of phys_const.f90. */
+static const double E_EL = 4.8e-10, M_EL = 9.109e-28;
+
+static double charge_f(double y, double zeta, double rho, double m_d, double rho_d, double s_d,
+                       double T, double mu_ions, double mu_gas, int electrons) {
+  const double m_i = mu_ions * MH, m_e = M_EL;
+  const double e2 = E_EL * E_EL;
+  const double k_e_d = PI * (s_d * s_d) * sqrt(8.0 * KB * T / (PI * m_e)) * exp(e2 * y / (s_d * KB * T));
+  const double k_i_d = PI * (s_d * s_d) * sqrt(8.0 * KB * T / (PI * m_i)) * (1.0 - e2 * y / (s_d * KB * T));
+  const double X = m_d * rho / (rho_d * mu_gas * MH);
+  if (!electrons) return mu_gas * MH * zeta / rho * (X * X) * ((1.0 / k_i_d)) - y;
+  return mu_gas * MH * zeta / rho * (X * X) * ((1.0 / k_i_d) - (1.0 / k_e_d)) - y;
+}
+
+static double charge_fp(double y, double zeta, double rho, double m_d, double rho_d, double s_d,
+                        double T, double mu_ions, double mu_gas, int electrons) {
+  const double m_i = mu_ions * MH, m_e = M_EL;
+  const double e2 = E_EL * E_EL;
+  const double k_e_d = PI * (s_d * s_d) * sqrt(8.0 * KB * T / (PI * m_e)) * exp(e2 * y / (s_d * KB * T));
+  const double k_i_d = PI * (s_d * s_d) * sqrt(8.0 * KB * T / (PI * m_i)) * (1.0 - e2 * y / (s_d * KB * T));
+  const double kp_e = PI * s_d * sqrt(8.0 * KB * T / PI / m_e) * e2 / KB / T * exp(e2 * (y) / s_d / KB / T);
+  const double kp_i = -(PI * s_d * sqrt(8.0 * KB * T / PI / m_i) * e2 / KB / T);
+  const double X = m_d * rho / rho_d / mu_gas / MH;
+  if (!electrons) return mu_gas * MH * zeta / rho * (X * X) * (-kp_i / (k_i_d * k_i_d)) - 1.0;
+  return mu_gas * MH * zeta / rho * (X * X) * (kp_e / (k_e_d * k_e_d) - kp_i / (k_i_d * k_i_d)) - 1.0;
+}
+
+/* solve_newton_raphson with y0 = -1; returns Z, *iters the reference's iteration count.
+ * steptol > 0 adds a stop once a step is below steptol*|y| (the disk environment's criterion:
+ * the reference's absolute 1e-14 on f cannot be met by charges of hundreds of electrons, where
+ * it would run all maxiter iterations); steptol = 0 is the reference's loop. */
+double growth_oracle_charge_root(double zeta, double rho, double m_d, double rho_d, double s_d,
+                                 double T, double mu_ions, double mu_gas, int electrons,
+                                 int maxiter, double steptol, int* iters) {
+  const double tol = 1.e-14;
+  double y = -1.0;
+  int k;
+  for (k = 1; k <= maxiter; ++k) {
+    const double fy = charge_f(y, zeta, rho, m_d, rho_d, s_d, T, mu_ions, mu_gas, electrons);
+    const double fyp = charge_fp(y, zeta, rho, m_d, rho_d, s_d, T, mu_ions, mu_gas, electrons);
+    if (fabs(fy) < tol) break;
+    const double deltay = fy / fyp;
+    y = y - deltay;
+    if (steptol > 0.0 && fabs(deltay) <= steptol * fabs(y)) { ++k; break; }
+  }
+  if (iters) *iters = k - 1;
+  return y;
+}
+
+/* analytical_charge over the collapse cells: T = barotrop(q_rho/(mu_gas mH)) with unit_nH = 1
+ * (setups/Collapse_1D/eos.f90:8), raw code-unit densities, masses and sizes as the reference
+ * passes them; ni/ne of the last bin survive (the reference overwrites them per bin). */
+void growth_oracle_charge_collapse(int64_t nc, int nb, const double* q_rho, const double* rhod,
+                                   const double* mdust, const double* sdust, double T0,
+                                   double zeta, double mu_ions, double mu_gas, int electrons,
+                                   int maxiter, double* zd, double* ni, double* ne) {
+  const double m_i = mu_ions * MH, m_e = M_EL, e2 = E_EL * E_EL;
+  for (int k = 0; k < nb; ++k)
+    for (int64_t c = 0; c < nc; ++c) {
+      const double nH = q_rho[c] / (mu_gas * MH);
+      const double T = T0 * sqrt(1.0 + pow(nH * 1.0 / 1e11, (double)0.8f)) * pow(1.0 + (nH * 1.0 / 1e16), -0.3) *
+                       pow(1.0 + (nH * 1.0 / 1e21), 0.56667);
+      const double rho = q_rho[c], m_d = mdust[k * nc + c], rd = rhod[k * nc + c], s_d = sdust[k * nc + c];
+      const double y = growth_oracle_charge_root(zeta, rho, m_d, rd, s_d, T, mu_ions, mu_gas, electrons, maxiter, 0.0, NULL);
+      zd[k * nc + c] = y;
+      const double k_e_d = PI * (s_d * s_d) * sqrt(8.0 * KB * T / (PI * m_e)) * exp(e2 * y / (s_d * KB * T));
+      const double k_i_d = PI * (s_d * s_d) * sqrt(8.0 * KB * T / (PI * m_i)) * (1.0 - e2 * y / (s_d * KB * T));
+      ni[c] = zeta * rho * m_d / (k_i_d * mu_gas * MH * rd);
+      ne[c] = zeta * rho * m_d / (k_e_d * mu_gas * MH * rd);
+      if (!electrons) ne[c] = 0.0;
+    }
+}
+
+/* Disk environment in cgs: midplane gas density sigma/(sqrt(2 pi) H), dust midplane densities
+ * rhod/(sqrt(2 pi) H), T from cs.  Per cell: number-weighted mean charge sum n_k Z_k / sum n_k
+ * and ion fraction n_i/n_H with n_i = zeta n_H / sum_k k_i,k n_k (ion balance on all grains). */
+void growth_oracle_charge_disk(const shark_growth_params* p, const growth_tables* t, int64_t nc,
+                               const double* sigma, const double* cs, const double* r,
+                               const double* rhod, double zeta, double mu_ions, int electrons,
+                               int maxiter, double* zd, double* zmean, double* xion) {
+  const int nb = t->nb;
+  const double m_i = mu_ions * MH, e2 = E_EL * E_EL;
+  #pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t c = 0; c < nc; ++c) {
+    double T, tL, teta, Re, ts[SHARK_GROWTH_MAX_BINS];
+    growth_oracle_disk_env(p, t, sigma[c], cs[c], r[c], &T, &tL, &teta, &Re, ts);
+    const double H = p->hoverr * r[c];
+    const double rho = sigma[c] / (sqrt(2.0 * PI) * H) * p->unit_d;
+    double nz = 0.0, nn = 0.0, kin = 0.0;
+    for (int k = 0; k < nb; ++k) {
+      const double rd = rhod[k * nc + c] / (sqrt(2.0 * PI) * H) * p->unit_d;
+      const double m_d = t->mdust[k] * p->unit_m, s_d = t->sdust[k] * p->unit_l;
+      const double y = growth_oracle_charge_root(zeta, rho, m_d, rd, s_d, T, mu_ions, p->mu_gas, electrons, maxiter, 1e-15, NULL);
+      if (zd) zd[k * nc + c] = y;
+      const double n = rd / m_d;
+      nz += n * y;
+      nn += n;
+      const double k_i_d = PI * (s_d * s_d) * sqrt(8.0 * KB * T / (PI * m_i)) * (1.0 - e2 * y / (s_d * KB * T));
+      kin += k_i_d * n;
+    }
+    zmean[c] = nz / nn;
+    xion[c] = zeta / kin;
+  }
+}

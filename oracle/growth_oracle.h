@@ -32,4 +32,17 @@ void growth_oracle_disk_env(const shark_growth_params* p, const growth_tables* t
 int64_t growth_oracle_disk(const shark_growth_params* p, const growth_tables* t, int64_t ncells,
                            const double* sigma, const double* cs, const double* r, double dt,
                            double* rhod, int64_t* nsub, double* drhodt_first);
+/* charging (src/chemistry/ionisation.f90:809-922, functions_NR.f90, newton_raphson.f90) */
+double growth_oracle_charge_root(double zeta, double rho, double m_d, double rho_d, double s_d,
+                                 double T, double mu_ions, double mu_gas, int electrons,
+                                 int maxiter, double steptol, int* iters);
+void growth_oracle_charge_collapse(int64_t nc, int nb, const double* q_rho, const double* rhod,
+                                   const double* mdust, const double* sdust, double T0,
+                                   double zeta, double mu_ions, double mu_gas, int electrons,
+                                   int maxiter, double* zd, double* ni, double* ne);
+/* disk environment: stops at maxiter or once a Newton step is below 1e-15 |Z| */
+void growth_oracle_charge_disk(const shark_growth_params* p, const growth_tables* t, int64_t nc,
+                               const double* sigma, const double* cs, const double* r,
+                               const double* rhod, double zeta, double mu_ions, int electrons,
+                               int maxiter, double* zd, double* zmean, double* xion);
 #endif

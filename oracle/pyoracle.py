@@ -186,3 +186,47 @@ def growth_disk_env(params, tables, sigma, cs, r):
     L.growth_oracle_disk_env(C.byref(params), C.byref(tables), sigma, cs, r,
                              *[C.byref(o) for o in out], _ptr(ts))
     return dict(T=out[0].value, t_L=out[1].value, t_eta=out[2].value, Re=out[3].value, tstop=ts)
+
+
+def charge_collapse(q_rho, rhod, mdust, sdust, T0, zeta, mu_ions, mu_gas, electrons=True,
+                    maxiter=100000):
+    """analytical_charge over collapse cells; arrays (nbins, ncells). Returns zd, ni, ne."""
+    L = _glib()
+    nb, nc = rhod.shape
+    zd = np.zeros((nb, nc))
+    ni = np.zeros(nc)
+    ne = np.zeros(nc)
+    L.growth_oracle_charge_collapse.argtypes = [C.c_int64, C.c_int] + [_P] * 4 + [
+        C.c_double] * 4 + [C.c_int, C.c_int] + [_P] * 3
+    L.growth_oracle_charge_collapse(nc, nb, _ptr(_c(q_rho)), _ptr(_c(rhod)), _ptr(_c(mdust)),
+                                    _ptr(_c(sdust)), T0, zeta, mu_ions, mu_gas, int(electrons),
+                                    maxiter, _ptr(zd), _ptr(ni), _ptr(ne))
+    return zd, ni, ne
+
+
+def charge_root(zeta, rho, m_d, rho_d, s_d, T, mu_ions=25.0, mu_gas=2.31, electrons=True,
+                maxiter=100000, steptol=0.0):
+    L = _glib()
+    L.growth_oracle_charge_root.restype = C.c_double
+    L.growth_oracle_charge_root.argtypes = [C.c_double] * 8 + [C.c_int, C.c_int, C.c_double,
+                                                               C.POINTER(C.c_int)]
+    it = C.c_int()
+    z = L.growth_oracle_charge_root(zeta, rho, m_d, rho_d, s_d, T, mu_ions, mu_gas, int(electrons),
+                                    maxiter, steptol, C.byref(it))
+    return z, it.value
+
+
+def charge_disk(params, tables, sigma, cs, r, rhod, zeta=5e-17, mu_ions=25.0, electrons=True,
+                maxiter=100, per_bin=False):
+    """Disk-environment grain charges: (zmean, xion[, zd])."""
+    L = _glib()
+    nb, nc = rhod.shape
+    zd = np.zeros((nb, nc))
+    zm = np.zeros(nc)
+    xi = np.zeros(nc)
+    L.growth_oracle_charge_disk.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + [_P] * 4 + [
+        C.c_double, C.c_double, C.c_int, C.c_int] + [_P] * 3
+    L.growth_oracle_charge_disk(C.byref(params), C.byref(tables), nc, _ptr(_c(sigma)), _ptr(_c(cs)),
+                                _ptr(_c(r)), _ptr(_c(rhod)), zeta, mu_ions, int(electrons), maxiter,
+                                _ptr(zd), _ptr(zm), _ptr(xi))
+    return (zm, xi, zd) if per_bin else (zm, xi)

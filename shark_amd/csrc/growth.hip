@@ -590,6 +590,84 @@ __global__ __launch_bounds__(256) void k_gmfma(GArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ grain charging
+// Analytical charge (src/chemistry/ionisation.f90:809-922, Fujii et al. 2011): per grain,
+// Newton-Raphson from Z = -1 (src/commons/newton_raphson.f90:7-97) on f(Z) of
+// functions_NR.f90:15-43 and its derivative :61-91, in cgs with the disk environment's midplane
+// densities; stops at |f| < 1e-14 (the reference's test), a step below 1e-15 |Z|, or 100
+// iterations.  Per cell: number-weighted mean charge and ion fraction zeta/sum_k k_i,k n_k.
+struct QArgs {
+  int nb, nch, c0;
+  int64_t ld;
+  int electrons, maxiter;
+  double zeta, mu_ions, mu_gas, hoverr, unit_d, unit_m, unit_l, s2pi;
+  const double* rhod;
+  const double* rho_gas;
+  const double* radius;
+  const double* env;     // T at env[cc]
+  const double* mdust;
+  const double* sdust;
+  double* zmean;         // [ncells]
+  double* xion;          // [ncells]
+};
+
+constexpr double kEel = 4.8e-10, kMel = 9.109e-28;
+
+__device__ double charge_f(double y, double zeta, double rho, double m_d, double rho_d, double s_d,
+                           double T, double mu_ions, double mu_gas, int electrons) {
+  const double m_i = mu_ions * kMH, m_e = kMel;
+  const double e2 = kEel * kEel;
+  const double k_e_d = kPi * (s_d * s_d) * sqrt(8.0 * kKB * T / (kPi * m_e)) * exp(e2 * y / (s_d * kKB * T));
+  const double k_i_d = kPi * (s_d * s_d) * sqrt(8.0 * kKB * T / (kPi * m_i)) * (1.0 - e2 * y / (s_d * kKB * T));
+  const double X = m_d * rho / (rho_d * mu_gas * kMH);
+  if (!electrons) return mu_gas * kMH * zeta / rho * (X * X) * ((1.0 / k_i_d)) - y;
+  return mu_gas * kMH * zeta / rho * (X * X) * ((1.0 / k_i_d) - (1.0 / k_e_d)) - y;
+}
+
+__device__ double charge_fp(double y, double zeta, double rho, double m_d, double rho_d, double s_d,
+                            double T, double mu_ions, double mu_gas, int electrons) {
+  const double m_i = mu_ions * kMH, m_e = kMel;
+  const double e2 = kEel * kEel;
+  const double k_e_d = kPi * (s_d * s_d) * sqrt(8.0 * kKB * T / (kPi * m_e)) * exp(e2 * y / (s_d * kKB * T));
+  const double k_i_d = kPi * (s_d * s_d) * sqrt(8.0 * kKB * T / (kPi * m_i)) * (1.0 - e2 * y / (s_d * kKB * T));
+  const double kp_e = kPi * s_d * sqrt(8.0 * kKB * T / kPi / m_e) * e2 / kKB / T * exp(e2 * (y) / s_d / kKB / T);
+  const double kp_i = -(kPi * s_d * sqrt(8.0 * kKB * T / kPi / m_i) * e2 / kKB / T);
+  const double X = m_d * rho / rho_d / mu_gas / kMH;
+  if (!electrons) return mu_gas * kMH * zeta / rho * (X * X) * (-kp_i / (k_i_d * k_i_d)) - 1.0;
+  return mu_gas * kMH * zeta / rho * (X * X) * (kp_e / (k_e_d * k_e_d) - kp_i / (k_i_d * k_i_d)) - 1.0;
+}
+
+__global__ void k_gcharge(QArgs a) {
+  const int cc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cc >= a.nch) return;
+  const int64_t c = a.c0 + cc;
+  const double T = a.env[cc];
+  const double H = a.hoverr * a.radius[c];
+  const double rho = a.rho_gas[c] / (a.s2pi * H) * a.unit_d;
+  const double m_i = a.mu_ions * kMH, e2 = kEel * kEel;
+  double nz = 0.0, nn = 0.0, kin = 0.0;
+  for (int k = 0; k < a.nb; ++k) {
+    const double rd = a.rhod[(int64_t)k * a.ld + c] / (a.s2pi * H) * a.unit_d;
+    const double m_d = a.mdust[k] * a.unit_m, s_d = a.sdust[k] * a.unit_l;
+    double y = -1.0;
+    for (int it = 1; it <= a.maxiter; ++it) {
+      const double fy = charge_f(y, a.zeta, rho, m_d, rd, s_d, T, a.mu_ions, a.mu_gas, a.electrons);
+      const double fyp = charge_fp(y, a.zeta, rho, m_d, rd, s_d, T, a.mu_ions, a.mu_gas, a.electrons);
+      if (fabs(fy) < 1.e-14) break;
+      const double deltay = fy / fyp;
+      y = y - deltay;
+      if (fabs(deltay) <= 1e-15 * fabs(y)) break;
+    }
+    const double n = rd / m_d;
+    nz += n * y;
+    nn += n;
+    const double k_i_d = kPi * (s_d * s_d) * sqrt(8.0 * kKB * T / (kPi * m_i)) * (1.0 - e2 * y / (s_d * kKB * T));
+    kin += k_i_d * n;
+  }
+  a.zmean[c] = nz / nn;
+  a.xion[c] = a.zeta / kin;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ host state and C-ABI
@@ -615,13 +693,14 @@ struct shark_growth {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double last_ms = 0.0;
   bool uploaded = false;
+  double *dzmean = nullptr, *dxion = nullptr;  // grain charging outputs (allocated on first use)
 };
 
 void shark_growth_free(shark_growth* g) {
   if (!g) return;
   void* ptrs[] = {g->dpairs, g->dredis, g->dmdust, g->dsdust, g->drhod, g->drho, g->dcs, g->drad,
                   g->denv, g->dts, g->dA, g->dF, g->dAh, g->dFh, g->dtg, g->dnit, g->dflags,
-                  g->dGf32, g->dGbhi, g->dGblo};
+                  g->dGf32, g->dGbhi, g->dGblo, g->dzmean, g->dxion};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (g->hflags) (void)hipHostFree(g->hflags);
@@ -924,6 +1003,74 @@ int shark_growth_download(shark_growth* g, double* rhod, int64_t* nsub) {
     GCHK(hipMemcpy(n.data(), g->dnit, sizeof(int) * g->ncells, hipMemcpyDeviceToHost));
     for (int64_t c = 0; c < g->ncells; ++c) nsub[c] = n[c];
   }
+  return 0;
+}
+
+int shark_growth_charge(shark_growth* g, double zeta, double mu_ions, int electrons, double* zmean,
+                        double* xion) {
+  if (!g) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_charge: null handle");
+  if (!g->uploaded) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_charge: upload the state first");
+  if (!(zeta > 0.0) || !(mu_ions > 0.0))
+    return gfail(SHARK_GROWTH_EINVAL, "shark_growth_charge: zeta and mu_ions must be positive");
+  if (!g->dzmean) {
+    GCHK(hipMalloc(&g->dzmean, sizeof(double) * g->ncells));
+    GCHK(hipMalloc(&g->dxion, sizeof(double) * g->ncells));
+  }
+  const shark_growth_params& p = g->p;
+  GArgs a;
+  memset(&a, 0, sizeof(a));
+  a.nb = p.nbins;
+  a.ld = g->ncells;
+  a.unit_v = p.unit_v;
+  a.unit_d = p.unit_d;
+  a.mu_gas = p.mu_gas;
+  a.gamma = p.gamma;
+  a.hoverr = p.hoverr;
+  a.mstar = p.mstar;
+  a.kts = sqrt(kPi * p.gamma / 8.0) * (p.rhograin / p.unit_d);
+  a.s2pi = sqrt(2.0 * kPi);
+  a.sdust = g->dsdust;
+  a.rho_gas = g->drho;
+  a.cs = g->dcs;
+  a.radius = g->drad;
+  a.env = g->denv;
+  a.ts = g->dts;
+  QArgs q;
+  memset(&q, 0, sizeof(q));
+  q.nb = p.nbins;
+  q.ld = g->ncells;
+  q.electrons = electrons ? 1 : 0;
+  q.maxiter = 100;
+  q.zeta = zeta;
+  q.mu_ions = mu_ions;
+  q.mu_gas = p.mu_gas;
+  q.hoverr = p.hoverr;
+  q.unit_d = p.unit_d;
+  q.unit_m = p.unit_m;
+  q.unit_l = p.unit_l;
+  q.s2pi = a.s2pi;
+  q.rhod = g->drhod;
+  q.rho_gas = g->drho;
+  q.radius = g->drad;
+  q.env = g->denv;
+  q.mdust = g->dmdust;
+  q.sdust = g->dsdust;
+  q.zmean = g->dzmean;
+  q.xion = g->dxion;
+  hipStream_t s = g->stream;
+  for (int64_t c0 = 0; c0 < g->ncells; c0 += g->nch_max) {
+    const int64_t rem = g->ncells - c0;
+    a.c0 = q.c0 = (int)c0;
+    a.nch = q.nch = (int)(rem < g->nch_max ? rem : g->nch_max);
+    const int nblk = (a.nch + 255) / 256;
+    hipLaunchKernelGGL(k_genv, dim3(nblk), dim3(256), 0, s, a);
+    GCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_gcharge, dim3(nblk), dim3(256), 0, s, q);
+    GCHK(hipGetLastError());
+  }
+  GCHK(hipStreamSynchronize(s));
+  if (zmean) GCHK(hipMemcpy(zmean, g->dzmean, sizeof(double) * g->ncells, hipMemcpyDeviceToHost));
+  if (xion) GCHK(hipMemcpy(xion, g->dxion, sizeof(double) * g->ncells, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -24,7 +24,7 @@ EXPORTS = [
     "shark_gpu_kernel_time", "shark_gpu_set_timing", "shark_gpu_algorithmic_bytes",
     "shark_gpu_destroy", "shark_gpu_last_error", "shark_gpu_version",
     "shark_growth_create", "shark_growth_run", "shark_growth_download", "shark_growth_upload",
-    "shark_growth_destroy", "shark_growth_kernel_time",
+    "shark_growth_destroy", "shark_growth_kernel_time", "shark_growth_charge",
 ]
 
 

@@ -124,6 +124,7 @@ def _lib():
         lib.shark_growth_run.argtypes = [C.c_void_p, C.c_int, C.c_double]
         lib.shark_growth_download.argtypes = [C.c_void_p, _P, C.POINTER(C.c_int64)]
         lib.shark_growth_kernel_time.argtypes = [C.c_void_p, _P]
+        lib.shark_growth_charge.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _P, _P]
         lib.shark_growth_destroy.argtypes = [C.c_void_p]
         lib._growth_bound = True
     return lib
@@ -153,6 +154,14 @@ class Growth:
         gpu.check(self.L.shark_growth_download(self.h, r.ctypes.data_as(_P),
                                                n.ctypes.data_as(C.POINTER(C.c_int64))))
         return r.reshape(self.p.nbins, self.ncells), n
+
+    def charge(self, zeta=5e-17, mu_ions=25.0, electrons=True):
+        """Mean grain charge and ion fraction per cell of the current state."""
+        zm = np.empty(self.ncells)
+        xi = np.empty(self.ncells)
+        gpu.check(self.L.shark_growth_charge(self.h, float(zeta), float(mu_ions), int(electrons),
+                                             zm.ctypes.data_as(_P), xi.ctypes.data_as(_P)))
+        return zm, xi
 
     def kernel_ms(self):
         ms = C.c_double()

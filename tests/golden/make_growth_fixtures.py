@@ -4,7 +4,9 @@
 Builds the reference Collapse_1D setup (NX=8, NDUST=16, GEOM=1) with the growth harness
 oracle/ref/ref_growth_harness.f90, which calls read_params, setup, ctoprim, compute_tstop
 and dust_growth (src/dust/dust_growth.f90:3) once over dt.  Stores the inputs the oracle
-needs and the dust densities after the call.  Development container only.
+needs and the dust densities after the call.  The "charge" case does the same for
+analytical_charge with oracle/ref/ref_charge_harness.f90 (ref_charge.npz).
+Development container only.
 """
 import json
 import os
@@ -127,8 +129,49 @@ def run_case(name, flags, dt):
     return m
 
 
+CHARGE_NAME = "charge_Collapse_1D_8_n16"
+CHARGE_EXE = os.path.join(REPO, "oracle", "_ref", CHARGE_NAME, "shark_ref")
+
+
+def run_charge():
+    """ref_charge.npz: the reference's analytical_charge (src/chemistry/ionisation.f90:809-922)
+    on the Collapse_1D cells rescaled by oracle/ref/ref_charge_harness.f90 (growth and
+    fragmentation off, so the setup calls no charge routine itself)."""
+    if not os.path.exists(CHARGE_EXE):
+        env = dict(os.environ, REF_NAME=CHARGE_NAME, REF_HARNESS="ref_charge_harness.f90")
+        subprocess.run([BUILD, "Collapse_1D", "8", "1", "16", "1", "0", "1", "0"], check=True,
+                       env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    nml = BASE.replace("growth              = .true.", "growth              = .false.")
+    nml = nml.replace("fragmentation       = .true.", "fragmentation       = .false.")
+    with tempfile.TemporaryDirectory() as td:
+        open(os.path.join(td, "c.nml"), "w").write(nml)
+        subprocess.run([CHARGE_EXE, "c.nml", "."], cwd=td, check=True,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"), stdout=subprocess.DEVNULL)
+        m = _meta(os.path.join(td, "meta.txt"))
+        aux = np.fromfile(os.path.join(td, "aux.bin"))
+    nc, nd = m["ncells"], m["ndust"]
+    o = [0]
+
+    def take(n, shape=None):
+        a = aux[o[0]:o[0] + n]
+        o[0] += n
+        return a.reshape(shape) if shape else a
+
+    arrs = dict(q_rho=take(nc), rhod=take(nc * nd, (nd, nc)), mdust=take(nc * nd, (nd, nc)),
+                sdust=take(nc * nd, (nd, nc)), zd=take(nc * nd, (nd, nc)), ni=take(nc), ne=take(nc))
+    assert o[0] == aux.size
+    m.update(namelist=nml)
+    np.savez_compressed(os.path.join(HERE, "ref_charge.npz"), meta=json.dumps(m), **arrs)
+    return m
+
+
 if __name__ == "__main__":
-    for n in (sys.argv[1:] or list(CASES)):
+    names = sys.argv[1:] or list(CASES) + ["charge"]
+    for n in names:
+        if n == "charge":
+            m = run_charge()
+            print("charge cells", m["ncells"], "bins", m["ndust"])
+            continue
         f, dt = CASES[n]
         m = run_case(n, f, dt)
         print(n, "dt", dt, "cells", m["ncells"], "bins", m["ndust"])

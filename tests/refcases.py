@@ -9,7 +9,8 @@ from shark_amd import cfg as C
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 REF_CASES = sorted(f[4:-4] for f in os.listdir(GOLDEN)
-                   if f.startswith("ref_") and f.endswith(".npz") and not f.startswith("ref_growth_"))
+                   if f.startswith("ref_") and f.endswith(".npz")
+                   and not f.startswith(("ref_growth_", "ref_charge")))
 
 
 def _K(nml):

@@ -80,3 +80,19 @@ def test_mfma_error_vs_fp64_oracle(mode, tol):
     assert err.max() <= tol
     assert mass.max() <= 1e-13
     assert (out > 0).all()
+
+
+def test_charge_matches_oracle():
+    """Grain charging on the GPU vs the oracle (same Newton iteration; exp may differ in the last
+    bit between the device and host libraries, so 1e-12 relative)."""
+    p, t, rhod, sigma, cs, r = cells(nb=64, nr=32, nphi=4)
+    zm_ref, xi_ref = pyoracle.charge_disk(p, t, sigma, cs, r, rhod)
+    g = G.Growth(p, sigma.size)
+    try:
+        g.upload(rhod, sigma, cs, r)
+        zm, xi = g.charge()
+    finally:
+        g.close()
+    assert (zm_ref < 0).all()
+    np.testing.assert_allclose(zm, zm_ref, rtol=1e-12)
+    np.testing.assert_allclose(xi, xi_ref, rtol=1e-12)

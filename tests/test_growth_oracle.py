@@ -183,3 +183,48 @@ def test_disk_epstein_stopping_time_is_midplane_stokes_number():
         s = np.array(t.sdust[:8]) * p.unit_l
         expect = np.pi / 2 * p.rhograin * s / (sigma[c] * p.unit_d * p.unit_l)
         np.testing.assert_allclose(st, expect, rtol=1e-14)
+
+
+# ----------------------------------------------------------------------------- charging
+def test_charge_collapse_bit_exact_with_reference():
+    """analytical_charge on the reference's own inputs: every grain charge (including the cells
+    where the reference's Newton iteration diverges to NaN) and the ion/electron densities of the
+    last bin agree bit for bit."""
+    z = np.load(os.path.join(GOLD, "ref_charge.npz"))
+    m = json.loads(str(z["meta"]))
+    zd, ni, ne = pyoracle.charge_collapse(z["q_rho"], z["rhod"], z["mdust"], z["sdust"],
+                                          m["T0_cloud"], m["zeta"], m["mu_ions"], m["mu_gas"],
+                                          m["electrons"])
+    fin = np.isfinite(z["zd"])
+    assert fin.sum() >= 60 and np.abs(z["zd"][fin]).max() > 100  # non-trivial roots present
+    assert np.array_equal(zd, z["zd"], equal_nan=True)
+    assert np.array_equal(ni, z["ni"], equal_nan=True)
+    assert np.array_equal(ne, z["ne"], equal_nan=True)
+
+
+def test_disk_charge_satisfies_current_balance():
+    """In the disk environment (cgs inputs) the root is the charge at which ion and electron
+    currents onto a grain balance the ionisation: n_i - n_e = Z n_d with n_i = zeta n_H/(k_i n_d),
+    n_e = zeta n_H/(k_e n_d) (the reference's sign convention, functions_NR.f90:36).  The balance is
+    evaluated here from the collision rates written out independently; near the root the ion and
+    electron terms nearly cancel, so the residual is bounded relative to their size."""
+    p, t, sigma, cs, r, rhod, _ = disk_cells(nc=6, nb=8)
+    zm, xi, zd = pyoracle.charge_disk(p, t, sigma, cs, r, rhod, per_bin=True)
+    kB, mH, me, e = 1.38e-16, 1.67e-24, 9.109e-28, 4.8e-10
+    for c in range(6):
+        env = pyoracle.growth_disk_env(p, t, sigma[c], cs[c], r[c])
+        T = env["T"]
+        H = p.hoverr * r[c]
+        rho = sigma[c] / np.sqrt(2 * np.pi) / H * p.unit_d
+        nH = rho / (p.mu_gas * mH)
+        for k in range(8):
+            s = t.sdust[k] * p.unit_l
+            md = t.mdust[k] * p.unit_m
+            nd = rhod[k, c] / np.sqrt(2 * np.pi) / H * p.unit_d / md
+            Z = zd[k, c]
+            tau = s * kB * T / e ** 2                      # reduced temperature
+            ke = np.pi * s ** 2 * np.sqrt(8 * kB * T / (np.pi * me)) * np.exp(Z / tau)
+            ki = np.pi * s ** 2 * np.sqrt(8 * kB * T / (np.pi * 25 * mH)) * (1 - Z / tau)
+            ion, ele = 5e-17 * nH / nd / nd / ki, 5e-17 * nH / nd / nd / ke
+            assert abs(ion - ele - Z) <= 1e-12 * (ion + ele)
+        assert zm[c] < 0  # electrons are faster: grains charge negatively
