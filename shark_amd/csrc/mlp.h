@@ -1,0 +1,12 @@
+// mlp.h — device-side entry points of the MLP shared with the growth surrogate (mlp.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+struct shark_mlp;
+namespace shark {
+// y (n x n_out) = net(x (n x n_in)) for device pointers, n <= max_batch, on stream s
+int mlp_forward_device(shark_mlp* m, const float* x, int64_t n, float* y, hipStream_t s);
+int64_t mlp_max_batch(const shark_mlp* m);
+int mlp_n_in(const shark_mlp* m);
+int mlp_n_out(const shark_mlp* m);
+}  // namespace shark

@@ -1,0 +1,555 @@
+// mlp.hip — multilayer perceptron on gfx950: forward, backward and Adam in hand-written kernels.
+//
+// Every product is one kernel, k_gemm: C[M x N] = A[M x K] . B[K x N] for FP32 operands in
+// global memory with arbitrary strides (so W, W^T, dY^T and H all load without copies), staged
+// into LDS as BF16 in the lane layout of v_mfma_f32_32x32x16_bf16 and accumulated in FP32.  A
+// 256-lane workgroup owns a 64 x 64 tile (four waves, one 32 x 32 MFMA block each) and walks K in
+// steps of 32.  The epilogue fuses bias, SiLU (storing the pre-activation for the backward pass)
+// or the SiLU derivative.  Weight gradients reduce over the batch with split-K partials summed in
+// a fixed order, so training is deterministic for a given seed.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <random>
+#include <vector>
+
+#include "../../include/shark_mlp.h"
+#include "growth.h"
+#include "mlp.h"
+
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, BK = 32;
+enum { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_SILU = 2, EPI_DSILU = 3 };
+
+struct GemmArgs {
+  int M, N, K;
+  const float* A;
+  int64_t sAm, sAk;
+  const float* B;
+  int64_t sBk, sBn;
+  float* C;
+  int64_t ldc;
+  const float* bias;  // [N]
+  float* Z;           // pre-activation: written by EPI_BIAS_SILU, read by EPI_DSILU (ldc)
+  int kchunk;         // K range per blockIdx.z
+  int64_t zstride;    // C offset per blockIdx.z (split-K partials)
+};
+
+__device__ __forceinline__ unsigned short to_bf16(float x) {
+  unsigned int u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = 1.0f / (1.0f + expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+  __shared__ unsigned short As[BK / 8][BM][8];
+  __shared__ unsigned short Bs[BK / 8][BN][8];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
+  const int kbeg = blockIdx.z * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+  const bool a_kfast = g.sAk == 1;
+  const bool b_nfast = g.sBn == 1;
+  f32x16 acc;
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    for (int i = 0; i < (BM * BK) / 256; ++i) {
+      const int e = tid + 256 * i;
+      int m, k;
+      if (a_kfast) { m = e >> 5; k = e & 31; } else { m = e & 63; k = e >> 6; }
+      const int gm = m0 + m, gk = k0 + k;
+      float v = 0.f;
+      if (gm < g.M && gk < kend) v = g.A[(int64_t)gm * g.sAm + (int64_t)gk * g.sAk];
+      As[k >> 3][m][k & 7] = to_bf16(v);
+    }
+    for (int i = 0; i < (BN * BK) / 256; ++i) {
+      const int e = tid + 256 * i;
+      int n, k;
+      if (b_nfast) { n = e & 63; k = e >> 6; } else { n = e >> 5; k = e & 31; }
+      const int gn = n0 + n, gk = k0 + k;
+      float v = 0.f;
+      if (gn < g.N && gk < kend) v = g.B[(int64_t)gk * g.sBk + (int64_t)gn * g.sBn];
+      Bs[k >> 3][n][k & 7] = to_bf16(v);
+    }
+    __syncthreads();
+    for (int s = 0; s < BK / 16; ++s) {
+      const int kg = 2 * s + (lane >> 5);
+      const u32x4 a = *reinterpret_cast<const u32x4*>(&As[kg][wr + (lane & 31)][0]);
+      const u32x4 b = *reinterpret_cast<const u32x4*>(&Bs[kg][wc + (lane & 31)][0]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                    __builtin_bit_cast(bf16x8, b), acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  float* C = g.C + blockIdx.z * g.zstride;
+  for (int r = 0; r < 16; ++r) {
+    const int row = m0 + wr + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    const int col = n0 + wc + (lane & 31);
+    if (row >= g.M || col >= g.N) continue;
+    const int64_t o = (int64_t)row * g.ldc + col;
+    float v = acc[r];
+    if constexpr (EPI == EPI_BIAS) {
+      v += g.bias[col];
+    } else if constexpr (EPI == EPI_BIAS_SILU) {
+      v += g.bias[col];
+      g.Z[o] = v;
+      v = silu(v);
+    } else if constexpr (EPI == EPI_DSILU) {
+      v *= dsilu(g.Z[o]);
+    }
+    C[o] = v;
+  }
+}
+
+// out[i] = sum_z P[z*n + i], z in order
+__global__ void k_sum_parts(const float* P, int parts, int64_t n, float* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int z = 0; z < parts; ++z) s += P[(int64_t)z * n + i];
+  out[i] = s;
+}
+
+// out[n] = sum_m X[m*ld + n]; 64 columns per block, 4 row phases reduced in order
+__global__ __launch_bounds__(256) void k_colsum(const float* X, int M, int N, int64_t ld, float* out) {
+  __shared__ float part[4][64];
+  const int c = threadIdx.x & 63, ph = threadIdx.x >> 6;
+  const int n = blockIdx.x * 64 + c;
+  float s = 0.f;
+  if (n < N)
+    for (int m = ph; m < M; m += 4) s += X[(int64_t)m * ld + n];
+  part[ph][c] = s;
+  __syncthreads();
+  if (ph == 0 && n < N) out[n] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
+}
+
+// dY = 2 (Y - T) / (n n_out); per-block partial sums of (Y - T)^2 / (n n_out)
+__global__ __launch_bounds__(256) void k_lossgrad(const float* Y, const float* T, int64_t total,
+                                                  float scale, float* dY, float* partial) {
+  __shared__ float red[256];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  float e2 = 0.f;
+  if (i < total) {
+    const float d = Y[i] - T[i];
+    dY[i] = 2.f * d * scale;
+    e2 = d * d * scale;
+  }
+  red[threadIdx.x] = e2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+                       float b2, float eps, float c1, float c2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ void k_gather(const float* src, const int* idx, int64_t rows, int cols, float* dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * cols) return;
+  const int64_t r = i / cols;
+  const int c = (int)(i - r * cols);
+  dst[i] = src[(int64_t)idx[r] * cols + c];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host object
+struct shark_mlp {
+  shark_mlp_cfg cfg;
+  int L = 0;
+  std::vector<int> din, dout;          // per layer
+  std::vector<int64_t> woff, boff;     // parameter offsets
+  int64_t np = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float *dp = nullptr, *dg = nullptr, *dm = nullptr, *dv = nullptr;  // params, grads, Adam moments
+  float* dx = nullptr;                 // batch input [max_batch x n_in]
+  float* dt = nullptr;                 // batch target [max_batch x n_out]
+  std::vector<float*> Z, H;            // per hidden layer [max_batch x n_hidden]
+  float *dy = nullptr, *ddy = nullptr; // output and its gradient [max_batch x n_out]
+  float *dda = nullptr, *ddb = nullptr;  // backward ping-pong [max_batch x n_hidden]
+  float* dpart = nullptr;              // split-K partials / loss partials
+  int64_t part_cap = 0;
+  float* hloss = nullptr;              // pinned
+  float *dX = nullptr, *dT = nullptr;  // training set
+  int* didx = nullptr;
+  int64_t ndata = 0;
+  int64_t adam_t = 0;
+  double last_ms = 0.0;
+};
+
+static int mfail(const char* msg) { return shark::fail_msg(1, msg); }
+
+#define MCHK(x)                                                                              \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess) {                                                                  \
+      char m_[256];                                                                          \
+      snprintf(m_, sizeof(m_), "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
+               __LINE__);                                                                    \
+      return shark::fail_msg(3, m_);                                                         \
+    }                                                                                        \
+  } while (0)
+
+static void free_mlp(shark_mlp* m) {
+  if (!m) return;
+  float* bufs[] = {m->dp, m->dg, m->dm, m->dv, m->dx, m->dt, m->dy, m->ddy, m->dda, m->ddb, m->dpart,
+                   m->dX, m->dT};
+  for (float* b : bufs)
+    if (b) (void)hipFree(b);
+  for (float* b : m->Z)
+    if (b) (void)hipFree(b);
+  for (float* b : m->H)
+    if (b) (void)hipFree(b);
+  if (m->didx) (void)hipFree(m->didx);
+  if (m->hloss) (void)hipHostFree(m->hloss);
+  if (m->e0) (void)hipEventDestroy(m->e0);
+  if (m->e1) (void)hipEventDestroy(m->e1);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+static int gemm(hipStream_t s, int epi, int M, int N, int K, const float* A, int64_t sAm, int64_t sAk,
+                const float* B, int64_t sBk, int64_t sBn, float* C, int64_t ldc, const float* bias,
+                float* Z, int splits = 1) {
+  GemmArgs g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.A = A;
+  g.sAm = sAm;
+  g.sAk = sAk;
+  g.B = B;
+  g.sBk = sBk;
+  g.sBn = sBn;
+  g.C = C;
+  g.ldc = ldc;
+  g.bias = bias;
+  g.Z = Z;
+  int kchunk = (K + splits - 1) / splits;
+  kchunk = ((kchunk + BK - 1) / BK) * BK;
+  g.kchunk = kchunk;
+  g.zstride = (int64_t)M * ldc;
+  const int nz = (K + kchunk - 1) / kchunk;
+  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, nz > 0 ? nz : 1);
+  switch (epi) {
+    case EPI_STORE: hipLaunchKernelGGL(k_gemm<EPI_STORE>, grid, dim3(256), 0, s, g); break;
+    case EPI_BIAS: hipLaunchKernelGGL(k_gemm<EPI_BIAS>, grid, dim3(256), 0, s, g); break;
+    case EPI_BIAS_SILU: hipLaunchKernelGGL(k_gemm<EPI_BIAS_SILU>, grid, dim3(256), 0, s, g); break;
+    default: hipLaunchKernelGGL(k_gemm<EPI_DSILU>, grid, dim3(256), 0, s, g); break;
+  }
+  MCHK(hipGetLastError());
+  return 0;
+}
+
+namespace shark {
+
+// Forward pass of n rows (device pointers), activations kept for the backward pass.
+int mlp_forward_device(shark_mlp* m, const float* x, int64_t n, float* y, hipStream_t s) {
+  const int L = m->L;
+  const float* in = x;
+  int64_t ld_in = m->din[0];
+  for (int l = 0; l < L; ++l) {
+    const float* W = m->dp + m->woff[l];
+    const float* b = m->dp + m->boff[l];
+    const int di = m->din[l], dout = m->dout[l];
+    if (l < L - 1) {
+      int rc = gemm(s, EPI_BIAS_SILU, (int)n, dout, di, in, ld_in, 1, W, 1, di, m->H[l], dout, b, m->Z[l]);
+      if (rc) return rc;
+      in = m->H[l];
+      ld_in = dout;
+    } else {
+      int rc = gemm(s, EPI_BIAS, (int)n, dout, di, in, ld_in, 1, W, 1, di, y, dout, b, nullptr);
+      if (rc) return rc;
+    }
+  }
+  return 0;
+}
+
+int64_t mlp_max_batch(const shark_mlp* m) { return m->cfg.max_batch; }
+int mlp_n_in(const shark_mlp* m) { return m->cfg.n_in; }
+int mlp_n_out(const shark_mlp* m) { return m->cfg.n_out; }
+
+}  // namespace shark
+
+// gradients of the loss on rows already in m->dx / m->dt (n rows); returns loss via hloss[0]
+static int grads_device(shark_mlp* m, int64_t n) {
+  hipStream_t s = m->stream;
+  const int L = m->L;
+  int rc = shark::mlp_forward_device(m, m->dx, n, m->dy, s);
+  if (rc) return rc;
+  const int no = m->cfg.n_out;
+  const int64_t total = n * no;
+  const int nblk = (int)((total + 255) / 256);
+  hipLaunchKernelGGL(k_lossgrad, dim3(nblk), dim3(256), 0, s, m->dy, m->dt, total,
+                     1.0f / (float)total, m->ddy, m->dpart);
+  MCHK(hipGetLastError());
+  // reduce the loss partials in a fixed order on the device into hloss (pinned, mapped by copy)
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1), 0, s, m->dpart, nblk, (int64_t)1, m->dpart + nblk);
+  MCHK(hipGetLastError());
+  MCHK(hipMemcpyAsync(m->hloss, m->dpart + nblk, sizeof(float), hipMemcpyDeviceToHost, s));
+  // backward: layer l has input a_{l-1} (x for l = 0) and output gradient dZ_l
+  const float* dZ = m->ddy;  // gradient at the output of the last layer (no activation)
+  int64_t ld_dz = no;
+  float* bufs[2] = {m->dda, m->ddb};
+  const int splits = (int)std::min<int64_t>(64, std::max<int64_t>(1, n / 256));
+  for (int l = L - 1; l >= 0; --l) {
+    const int di = m->din[l], dout = m->dout[l];
+    const float* a_in = (l == 0) ? m->dx : m->H[l - 1];
+    const int64_t ld_in = (l == 0) ? m->cfg.n_in : m->cfg.n_hidden;
+    // dW_l (dout x di) = dZ^T (dout x n) . a_in (n x di), split over the batch
+    const int64_t mn = (int64_t)dout * di;
+    if ((int64_t)splits * mn > m->part_cap) return mfail("mlp: split-K partial buffer too small");
+    rc = gemm(s, EPI_STORE, dout, di, (int)n, dZ, 1, ld_dz, a_in, ld_in, 1, m->dpart, di, nullptr,
+              nullptr, splits);
+    if (rc) return rc;
+    const int kchunk = (((int)((n + splits - 1) / splits) + BK - 1) / BK) * BK;
+    const int nz = (int)((n + kchunk - 1) / kchunk);
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, m->dpart, nz,
+                       mn, m->dg + m->woff[l]);
+    MCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_colsum, dim3((dout + 63) / 64), dim3(256), 0, s, dZ, (int)n, dout, ld_dz,
+                       m->dg + m->boff[l]);
+    MCHK(hipGetLastError());
+    if (l > 0) {
+      // dZ_{l-1} = (dZ_l . W_l) * silu'(Z_{l-1})
+      float* out = bufs[l & 1];
+      const float* W = m->dp + m->woff[l];
+      rc = gemm(s, EPI_DSILU, (int)n, di, dout, dZ, ld_dz, 1, W, di, 1, out, di, nullptr, m->Z[l - 1]);
+      if (rc) return rc;
+      dZ = out;
+      ld_dz = di;
+    }
+  }
+  return 0;
+}
+
+extern "C" {
+
+int shark_mlp_create(const shark_mlp_cfg* cfg, shark_mlp** out) {
+  if (!out) return mfail("shark_mlp_create: null output handle");
+  *out = nullptr;
+  if (!cfg) return mfail("shark_mlp_create: null configuration");
+  if (cfg->version != SHARK_MLP_VERSION) return mfail("shark_mlp_cfg.version does not match SHARK_MLP_VERSION");
+  if (cfg->n_in < 1 || cfg->n_out < 1 || cfg->n_hidden < 1 || cfg->n_layers < 2)
+    return mfail("shark_mlp_create: n_in, n_out, n_hidden >= 1 and n_layers >= 2 required");
+  if (cfg->n_in > 4096 || cfg->n_out > 4096 || cfg->n_hidden > 4096)
+    return mfail("shark_mlp_create: layer widths are limited to 4096");
+  if (cfg->max_batch < 1 || cfg->max_batch > (1 << 22))
+    return mfail("shark_mlp_create: max_batch must lie in [1, 4194304]");
+  shark_mlp* m = new shark_mlp();
+  m->cfg = *cfg;
+  m->L = cfg->n_layers;
+  for (int l = 0; l < m->L; ++l) {
+    m->din.push_back(l == 0 ? cfg->n_in : cfg->n_hidden);
+    m->dout.push_back(l == m->L - 1 ? cfg->n_out : cfg->n_hidden);
+    m->woff.push_back(m->np);
+    m->np += (int64_t)m->din[l] * m->dout[l];
+    m->boff.push_back(m->np);
+    m->np += m->dout[l];
+  }
+  const int64_t B = cfg->max_batch;
+  const int64_t wmax = std::max<int64_t>(cfg->n_hidden, std::max(cfg->n_in, cfg->n_out));
+  m->part_cap = 64 * wmax * wmax + ((B * cfg->n_out + 255) / 256) + 16;
+#define ALLOC(p, n)                                       \
+  do {                                                    \
+    hipError_t e_ = hipMalloc(&(p), sizeof(float) * (n)); \
+    if (e_ != hipSuccess) {                               \
+      free_mlp(m);                                        \
+      return shark::fail_msg(3, "shark_mlp_create: device allocation failed"); \
+    }                                                     \
+  } while (0)
+  if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&m->e0) != hipSuccess || hipEventCreate(&m->e1) != hipSuccess) {
+    free_mlp(m);
+    return shark::fail_msg(3, "shark_mlp_create: stream/event creation failed");
+  }
+  ALLOC(m->dp, m->np);
+  ALLOC(m->dg, m->np);
+  ALLOC(m->dm, m->np);
+  ALLOC(m->dv, m->np);
+  ALLOC(m->dx, B * cfg->n_in);
+  ALLOC(m->dt, B * cfg->n_out);
+  ALLOC(m->dy, B * cfg->n_out);
+  ALLOC(m->ddy, B * cfg->n_out);
+  ALLOC(m->dda, B * cfg->n_hidden);
+  ALLOC(m->ddb, B * cfg->n_hidden);
+  ALLOC(m->dpart, m->part_cap);
+  for (int l = 0; l < m->L - 1; ++l) {
+    float *z = nullptr, *h = nullptr;
+    ALLOC(z, B * cfg->n_hidden);
+    m->Z.push_back(z);
+    ALLOC(h, B * cfg->n_hidden);
+    m->H.push_back(h);
+  }
+#undef ALLOC
+  if (hipHostMalloc(&m->hloss, sizeof(float) * 4, hipHostMallocDefault) != hipSuccess) {
+    free_mlp(m);
+    return shark::fail_msg(3, "shark_mlp_create: pinned allocation failed");
+  }
+  if (hipMemset(m->dp, 0, sizeof(float) * m->np) != hipSuccess ||
+      hipMemset(m->dm, 0, sizeof(float) * m->np) != hipSuccess ||
+      hipMemset(m->dv, 0, sizeof(float) * m->np) != hipSuccess) {
+    free_mlp(m);
+    return shark::fail_msg(3, "shark_mlp_create: memset failed");
+  }
+  *out = m;
+  return 0;
+}
+
+int64_t shark_mlp_num_params(const shark_mlp* m) { return m ? m->np : -1; }
+
+int shark_mlp_set_params(shark_mlp* m, const float* params) {
+  if (!m || !params) return mfail("shark_mlp_set_params: null argument");
+  MCHK(hipMemcpy(m->dp, params, sizeof(float) * m->np, hipMemcpyHostToDevice));
+  MCHK(hipMemset(m->dm, 0, sizeof(float) * m->np));
+  MCHK(hipMemset(m->dv, 0, sizeof(float) * m->np));
+  m->adam_t = 0;
+  return 0;
+}
+
+int shark_mlp_get_params(shark_mlp* m, float* params) {
+  if (!m || !params) return mfail("shark_mlp_get_params: null argument");
+  MCHK(hipStreamSynchronize(m->stream));
+  MCHK(hipMemcpy(params, m->dp, sizeof(float) * m->np, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int shark_mlp_forward(shark_mlp* m, const float* x, int64_t n, float* y) {
+  if (!m || !x || !y) return mfail("shark_mlp_forward: null argument");
+  if (n < 1 || n > m->cfg.max_batch) return mfail("shark_mlp_forward: n must lie in [1, max_batch]");
+  hipStream_t s = m->stream;
+  MCHK(hipMemcpyAsync(m->dx, x, sizeof(float) * n * m->cfg.n_in, hipMemcpyHostToDevice, s));
+  MCHK(hipEventRecord(m->e0, s));
+  int rc = shark::mlp_forward_device(m, m->dx, n, m->dy, s);
+  if (rc) return rc;
+  MCHK(hipEventRecord(m->e1, s));
+  MCHK(hipMemcpyAsync(y, m->dy, sizeof(float) * n * m->cfg.n_out, hipMemcpyDeviceToHost, s));
+  MCHK(hipStreamSynchronize(s));
+  float ms = 0.f;
+  MCHK(hipEventElapsedTime(&ms, m->e0, m->e1));
+  m->last_ms = ms;
+  return 0;
+}
+
+int shark_mlp_grads(shark_mlp* m, const float* x, const float* t, int64_t n, float* loss, float* grads) {
+  if (!m || !x || !t) return mfail("shark_mlp_grads: null argument");
+  if (n < 1 || n > m->cfg.max_batch) return mfail("shark_mlp_grads: n must lie in [1, max_batch]");
+  hipStream_t s = m->stream;
+  MCHK(hipMemcpyAsync(m->dx, x, sizeof(float) * n * m->cfg.n_in, hipMemcpyHostToDevice, s));
+  MCHK(hipMemcpyAsync(m->dt, t, sizeof(float) * n * m->cfg.n_out, hipMemcpyHostToDevice, s));
+  int rc = grads_device(m, n);
+  if (rc) return rc;
+  MCHK(hipStreamSynchronize(s));
+  if (loss) *loss = m->hloss[0];
+  if (grads) MCHK(hipMemcpy(grads, m->dg, sizeof(float) * m->np, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int shark_mlp_set_data(shark_mlp* m, const float* x, const float* t, int64_t n) {
+  if (!m || !x || !t || n < 1) return mfail("shark_mlp_set_data: null argument or empty set");
+  if (n > (int64_t)1 << 30) return mfail("shark_mlp_set_data: at most 2^30 rows");
+  if (m->dX) (void)hipFree(m->dX);
+  if (m->dT) (void)hipFree(m->dT);
+  if (m->didx) (void)hipFree(m->didx);
+  m->dX = m->dT = nullptr;
+  m->didx = nullptr;
+  m->ndata = 0;
+  MCHK(hipMalloc(&m->dX, sizeof(float) * n * m->cfg.n_in));
+  MCHK(hipMalloc(&m->dT, sizeof(float) * n * m->cfg.n_out));
+  MCHK(hipMalloc(&m->didx, sizeof(int) * n));
+  MCHK(hipMemcpy(m->dX, x, sizeof(float) * n * m->cfg.n_in, hipMemcpyHostToDevice));
+  MCHK(hipMemcpy(m->dT, t, sizeof(float) * n * m->cfg.n_out, hipMemcpyHostToDevice));
+  m->ndata = n;
+  return 0;
+}
+
+int shark_mlp_train(shark_mlp* m, int epochs, int batch, float lr, uint64_t seed, float* loss_out) {
+  if (!m) return mfail("shark_mlp_train: null handle");
+  if (!m->ndata) return mfail("shark_mlp_train: no training set (shark_mlp_set_data)");
+  if (batch < 1 || batch > m->cfg.max_batch) return mfail("shark_mlp_train: batch must lie in [1, max_batch]");
+  if (epochs < 0) return mfail("shark_mlp_train: epochs must be >= 0");
+  hipStream_t s = m->stream;
+  std::mt19937_64 rng(seed);
+  std::vector<int> perm(m->ndata);
+  const int ni = m->cfg.n_in, no = m->cfg.n_out;
+  const float b1 = (float)m->cfg.beta1, b2 = (float)m->cfg.beta2, eps = (float)m->cfg.adam_eps;
+  MCHK(hipEventRecord(m->e0, s));
+  for (int e = 0; e < epochs; ++e) {
+    for (int64_t i = 0; i < m->ndata; ++i) perm[i] = (int)i;
+    for (int64_t i = m->ndata - 1; i > 0; --i) {
+      const int64_t j = (int64_t)(rng() % (uint64_t)(i + 1));
+      std::swap(perm[i], perm[j]);
+    }
+    MCHK(hipMemcpyAsync(m->didx, perm.data(), sizeof(int) * m->ndata, hipMemcpyHostToDevice, s));
+    MCHK(hipStreamSynchronize(s));  // perm is reused by the next epoch
+    double lsum = 0.0;
+    int nb = 0;
+    for (int64_t b0 = 0; b0 + batch <= m->ndata; b0 += batch) {
+      const int64_t rows = batch;
+      hipLaunchKernelGGL(k_gather, dim3((unsigned)((rows * ni + 255) / 256)), dim3(256), 0, s, m->dX,
+                         m->didx + b0, rows, ni, m->dx);
+      hipLaunchKernelGGL(k_gather, dim3((unsigned)((rows * no + 255) / 256)), dim3(256), 0, s, m->dT,
+                         m->didx + b0, rows, no, m->dt);
+      MCHK(hipGetLastError());
+      int rc = grads_device(m, rows);
+      if (rc) return rc;
+      ++m->adam_t;
+      const float c1 = 1.f - powf(b1, (float)m->adam_t), c2 = 1.f - powf(b2, (float)m->adam_t);
+      hipLaunchKernelGGL(k_adam, dim3((unsigned)((m->np + 255) / 256)), dim3(256), 0, s, m->dp, m->dg,
+                         m->dm, m->dv, m->np, lr, b1, b2, eps, c1, c2);
+      MCHK(hipGetLastError());
+      MCHK(hipStreamSynchronize(s));
+      lsum += m->hloss[0];
+      ++nb;
+    }
+    if (loss_out) loss_out[e] = nb ? (float)(lsum / nb) : 0.f;
+  }
+  MCHK(hipEventRecord(m->e1, s));
+  MCHK(hipEventSynchronize(m->e1));
+  float ms = 0.f;
+  MCHK(hipEventElapsedTime(&ms, m->e0, m->e1));
+  m->last_ms = ms;
+  return 0;
+}
+
+int shark_mlp_kernel_time(shark_mlp* m, double* ms) {
+  if (!m || !ms) return mfail("shark_mlp_kernel_time: null argument");
+  *ms = m->last_ms;
+  return 0;
+}
+
+int shark_mlp_destroy(shark_mlp* m) {
+  free_mlp(m);
+  return 0;
+}
+
+}  // extern "C"

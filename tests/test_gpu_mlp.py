@@ -1,0 +1,79 @@
+"""The MLP's HIP kernels (forward, gradients, Adam) against the numpy oracle
+(oracle/mlp_oracle.py, whose backward pass is pinned by finite differences in
+tests/test_mlp_oracle.py).  The oracle rounds product operands to BF16 where the kernels do, so the
+remaining difference is FP32 accumulation order: 2e-3 relative on outputs and per-layer gradient
+norms (a transposed or mis-indexed operand gives O(1))."""
+import numpy as np
+import pytest
+
+import mlp_oracle as MO
+from shark_amd import mlp as M
+
+pytestmark = pytest.mark.gpu
+
+
+def net(cfg, seed=0):
+    p = M.init_params(cfg, seed)
+    # larger last layer than the init so outputs are O(1) for the comparison
+    layers = M.unpack(cfg, p)
+    layers[-1][0][:] *= 10.0
+    return p
+
+
+@pytest.mark.parametrize("dims,n", [((69, 256, 4, 65), 512), ((5, 33, 3, 7), 77), ((69, 256, 4, 65), 1)])
+def test_forward_matches_oracle(dims, n):
+    cfg = M.make_cfg(n_in=dims[0], n_hidden=dims[1], n_layers=dims[2], n_out=dims[3], max_batch=1024)
+    p = net(cfg)
+    x = np.random.default_rng(2).standard_normal((n, dims[0])).astype(np.float32)
+    m = M.MLP(cfg, p)
+    try:
+        y = m.forward(x)
+    finally:
+        m.close()
+    ref = MO.forward([(W.astype(np.float64), b.astype(np.float64)) for W, b in M.unpack(cfg, p)], x)
+    scale = np.abs(ref).max()
+    assert np.abs(y - ref).max() <= 2e-3 * scale
+
+
+@pytest.mark.parametrize("dims,n", [((69, 256, 4, 65), 1024), ((5, 33, 3, 7), 300)])
+def test_gradients_match_oracle(dims, n):
+    cfg = M.make_cfg(n_in=dims[0], n_hidden=dims[1], n_layers=dims[2], n_out=dims[3], max_batch=1024)
+    p = net(cfg, seed=3)
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((n, dims[0])).astype(np.float32)
+    t = rng.standard_normal((n, dims[3])).astype(np.float32)
+    m = M.MLP(cfg, p)
+    try:
+        loss, g = m.grads(x, t)
+    finally:
+        m.close()
+    layers = [(W.astype(np.float64), b.astype(np.float64)) for W, b in M.unpack(cfg, p)]
+    lref, gref = MO.loss_grads(layers, x, t)
+    assert loss == pytest.approx(lref, rel=2e-3)
+    gl = M.unpack(cfg, g)
+    for l, ((dW, db), (rW, rb)) in enumerate(zip(gl, gref)):
+        for a, r in ((dW, rW), (db, rb)):
+            err = np.linalg.norm(a - r) / np.linalg.norm(r)
+            print(f"layer {l} grad rel err {err:.2e}")
+            assert err <= 1e-3, (l, err)
+
+
+def test_training_reduces_loss_and_is_deterministic():
+    cfg = M.make_cfg(n_in=8, n_hidden=64, n_layers=3, n_out=4, max_batch=256)
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((8, 4))
+    x = rng.standard_normal((4096, 8)).astype(np.float32)
+    t = np.tanh(x @ A).astype(np.float32)
+    runs = []
+    for _ in range(2):
+        m = M.MLP(cfg, M.init_params(cfg, 1))
+        try:
+            m.set_data(x, t)
+            losses = m.train(epochs=20, batch=256, lr=3e-3, seed=11)
+            runs.append((losses, m.get_params()))
+        finally:
+            m.close()
+    losses = runs[0][0]
+    print("losses", losses[0], losses[-1])
+    assert losses[-1] < 0.1 * losses[0]
+    assert np.array_equal(runs[0][1], runs[1][1])
