@@ -25,7 +25,7 @@ extern "C" {
 
 enum { SHARK_GROWTH_ENV_COLLAPSE = 0, SHARK_GROWTH_ENV_DISK = 1 };
 enum { SHARK_GROWTH_FP64 = 0, SHARK_GROWTH_MFMA_FP32 = 1, SHARK_GROWTH_MFMA_BF16 = 2,
-       SHARK_GROWTH_MFMA_BF16_PLAIN = 3 };
+       SHARK_GROWTH_MFMA_BF16_PLAIN = 3, SHARK_GROWTH_SURROGATE = 4 };
 /* return codes (0 = success; shark_gpu_last_error() holds the message) */
 enum { SHARK_GROWTH_EINVAL = 1, SHARK_GROWTH_EHIP = 3, SHARK_GROWTH_ENUM = 6 };
 
@@ -52,6 +52,7 @@ typedef struct shark_growth_params {
 } shark_growth_params;
 
 typedef struct shark_growth shark_growth;
+struct shark_mlp;
 
 /* Growth state for ncells cells.  Rejects (SHARK_GROWTH_EINVAL) the collapse environment, and
  * fragmentation with decomposed=0, whose target bins depend on the cell. */
@@ -71,6 +72,25 @@ int shark_growth_download(shark_growth* g, double* rhod, int64_t* nsub);
  * electrons: include electron capture.  Outputs (ncells) may be NULL. */
 int shark_growth_charge(shark_growth* g, double zeta, double mu_ions, int electrons, double* zmean,
                         double* xion);
+/* Per-cell growth intervals (ncells, >= 0) used instead of run's dt; NULL returns to dt. */
+int shark_growth_set_cell_dt(shark_growth* g, const double* dt_cells);
+/* Raw (unstandardised) network inputs of the current state, (ncells x (nbins+5)) row-major:
+ * log10 midplane dust density per bin (g cm^-3), log10 T, log10 midplane gas density, log10 alpha,
+ * log10 ion fraction (from the charge solve), log10 dt (or the per-cell interval). */
+int shark_growth_features(shark_growth* g, double dt, float* feat);
+/* Attach a trained network for SHARK_GROWTH_SURROGATE (NULL detaches).  norm holds
+ * in_mean[nbins+5], in_std[nbins+5], out_mean[nbins+1], out_std[nbins+1]; outputs are the change of
+ * the natural log of each bin's share of the dust mass and log10(-<Z>).  New shares are the
+ * softmax of old log-shares + change (positive, summing to one).  A cell falls back to mode 0 and
+ * the charge solve when an output is not finite, when its mass residual after that normalisation
+ * exceeds resid_max, or when an input lies beyond 8 standard deviations of the training set.
+ * alpha (<= 0: turb_coef^2), zeta and mu_ions feed the features and the fallback charge solve.
+ * The network must outlive its use here. */
+int shark_growth_set_surrogate(shark_growth* g, struct shark_mlp* m, const float* norm,
+                               double alpha, double zeta, double mu_ions, double resid_max);
+/* Of the last SHARK_GROWTH_SURROGATE run: cells that fell back, and the mean mass residual of the
+ * accepted raw predictions before normalisation, |sum_k share_k exp(change_k) - 1|. */
+int shark_growth_surrogate_stats(shark_growth* g, int64_t* n_fallback, double* raw_resid_mean);
 /* Device time (ms) of the last run. */
 int shark_growth_kernel_time(shark_growth* g, double* ms);
 int shark_growth_destroy(shark_growth* g);

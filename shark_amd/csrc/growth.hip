@@ -31,6 +31,7 @@
 
 #include "../../include/shark_growth.h"
 #include "growth.h"
+#include "mlp.h"
 
 namespace {
 
@@ -176,6 +177,7 @@ struct GArgs {
   double* F;    // [npairs][nch]
   float* Ah;    // [npairs][nch]
   float* Fh;    // [npairs][nch]
+  const double* dtc;  // [ld] per-cell growth interval, or null for dt everywhere
   double* tg;   // [ld] time reached inside the growth call
   int* nit;     // [ld] sub-steps taken
   int* unfinished;
@@ -301,7 +303,8 @@ __global__ __launch_bounds__(64) void k_gfp64(GArgs a) {
   const double rg = valid ? a.rho_gas[c] : 1.0;
   double tgrow = valid ? a.tg[c] : 0.0;
   int niter = valid ? a.nit[c] : 0;
-  bool act = valid && tgrow < a.dt && !(a.max_sub > 0 && niter >= a.max_sub);
+  const double dtl = a.dtc ? a.dtc[c] : a.dt;
+  bool act = valid && tgrow < dtl && !(a.max_sub > 0 && niter >= a.max_sub);
   const int kt = a.kernel_type;
   const double st = a.sticking;
   for (int s = 0; s < a.sub_per_launch; ++s) {
@@ -354,7 +357,7 @@ __global__ __launch_bounds__(64) void k_gfp64(GArgs a) {
         const double d = dd[k * 64 + lane];
         if (fabs(d) != 0.0) dtg = fmn(dtg, rh[(int64_t)k * a.ld] / fabs(d) * a.cfl);
       }
-      dtg = fmn(dtg, a.dt - tgrow);
+      dtg = fmn(dtg, dtl - tgrow);
       for (int k = 0; k < nb; ++k) {
         const double v = fmx(rh[(int64_t)k * a.ld] + dd[k * 64 + lane] * dtg, rg * a.ratio_min);
         rh[(int64_t)k * a.ld] = v;
@@ -364,7 +367,7 @@ __global__ __launch_bounds__(64) void k_gfp64(GArgs a) {
         atomicOr(a.bad, 1);
         act = false;
       } else {
-        act = tgrow < a.dt && !(a.max_sub > 0 && niter >= a.max_sub);
+        act = tgrow < dtl && !(a.max_sub > 0 && niter >= a.max_sub);
       }
     }
   }
@@ -439,7 +442,7 @@ __global__ __launch_bounds__(256) void k_gmfma(GArgs a) {
   const int kt = a.kernel_type;
   // per-cell state lives with thread `tid` for tid < 64
   int64_t c = 0;
-  double tgrow = 0.0, rg = 1.0;
+  double tgrow = 0.0, rg = 1.0, dtl = 0.0;
   int niter = 0;
   bool valid = false;
   if (tid < NCT) {
@@ -449,6 +452,7 @@ __global__ __launch_bounds__(256) void k_gmfma(GArgs a) {
       tgrow = a.tg[c];
       niter = a.nit[c];
       rg = a.rho_gas[c];
+      dtl = a.dtc ? a.dtc[c] : a.dt;
     }
   }
   const int nchunks = a.kpad / KC;
@@ -456,7 +460,7 @@ __global__ __launch_bounds__(256) void k_gmfma(GArgs a) {
     if (tid == 0) sm.any = 0;
     __syncthreads();
     if (tid < NCT) {
-      const bool act = valid && tgrow < a.dt && !(a.max_sub > 0 && niter >= a.max_sub);
+      const bool act = valid && tgrow < dtl && !(a.max_sub > 0 && niter >= a.max_sub);
       sm.act[tid] = act;
       if (act) sm.any = 1;  // benign race: every writer stores 1
       double tot = 0.0;
@@ -566,7 +570,7 @@ __global__ __launch_bounds__(256) void k_gmfma(GArgs a) {
         m_old += r;
         if (fabs(d) != 0.0) dtg = fmn(dtg, r / fabs(d) * a.cfl);
       }
-      dtg = fmn(dtg, a.dt - tgrow);
+      dtg = fmn(dtg, dtl - tgrow);
       for (int k = 0; k < nb; ++k) {
         const double v = fmx(a.rhod[(int64_t)k * a.ld + c] + sm.u.d[k][tid] * dtg, rg * a.ratio_min);
         sm.u.d[k][tid] = v;
@@ -586,7 +590,7 @@ __global__ __launch_bounds__(256) void k_gmfma(GArgs a) {
   if (tid < NCT && valid) {
     a.tg[c] = tgrow;
     a.nit[c] = niter;
-    if (tgrow < a.dt && !(a.max_sub > 0 && niter >= a.max_sub)) atomicAdd(a.unfinished, 1);
+    if (tgrow < dtl && !(a.max_sub > 0 && niter >= a.max_sub)) atomicAdd(a.unfinished, 1);
   }
 }
 
@@ -609,6 +613,7 @@ struct QArgs {
   const double* sdust;
   double* zmean;         // [ncells]
   double* xion;          // [ncells]
+  const int* mask;       // [ncells] or null: only cells with mask != 0 are charged
 };
 
 constexpr double kEel = 4.8e-10, kMel = 9.109e-28;
@@ -641,6 +646,7 @@ __global__ void k_gcharge(QArgs a) {
   const int cc = blockIdx.x * blockDim.x + threadIdx.x;
   if (cc >= a.nch) return;
   const int64_t c = a.c0 + cc;
+  if (a.mask && !a.mask[c]) return;
   const double T = a.env[cc];
   const double H = a.hoverr * a.radius[c];
   const double rho = a.rho_gas[c] / (a.s2pi * H) * a.unit_d;
@@ -666,6 +672,122 @@ __global__ void k_gcharge(QArgs a) {
   }
   a.zmean[c] = nz / nn;
   a.xion[c] = a.zeta / kin;
+}
+
+// ------------------------------------------------------------------ learned growth step
+// Features of a cell (n_in = nbins + 5): log10 of each bin's midplane dust density (g cm^-3),
+// log10 T, log10 midplane gas density, log10 alpha, log10 ion fraction, log10 dt; standardised
+// with the training set's mean and spread.  Outputs (n_out = nbins + 1): the change over dt of the
+// natural log of each bin's share of the cell's dust mass, and log10(-<Z>) of the new state,
+// both standardised.
+struct SArgs {
+  int nb, nch, c0, nin, nout;
+  int64_t ld;
+  double s2pi, hoverr, unit_d, log10_alpha, dt, resid_max, ratio_min;
+  float zmax;  // input envelope in standard deviations
+  const double* dtc;
+  double* rhod;
+  const double* rho_gas;
+  const double* radius;
+  const double* env;
+  const double* xion;
+  float* feat;
+  const float* y;
+  const float* norm;  // in_mean[nin], in_std[nin], out_mean[nout], out_std[nout]; null: raw
+  double* zmean;
+  int* fb;            // [ncells] 1 = plain fallback
+  double* tg;
+  int* nit;
+  int* nfb;           // fallback count
+  double* resid_sum;  // sum over accepted cells of |sum_k share_k,old exp(change_k) - 1|
+};
+
+__global__ void k_surr_feat(SArgs a) {
+  const int cc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cc >= a.nch) return;
+  const int64_t c = a.c0 + cc;
+  const double H = a.hoverr * a.radius[c];
+  const double conv = a.unit_d / (a.s2pi * H);
+  float* f = a.feat + (int64_t)cc * a.nin;
+  const float* mu = a.norm;
+  const float* sd = a.norm ? a.norm + a.nin : nullptr;
+  auto put = [&](int j, double v) {
+    float x = (float)v;
+    if (mu) x = (x - mu[j]) / sd[j];
+    f[j] = x;
+  };
+  for (int k = 0; k < a.nb; ++k) put(k, log10(fmax(a.rhod[(int64_t)k * a.ld + c], 1e-300) * conv));
+  const double dtl = a.dtc ? a.dtc[c] : a.dt;
+  put(a.nb, log10(a.env[cc]));
+  put(a.nb + 1, log10(a.rho_gas[c] * conv));
+  put(a.nb + 2, a.log10_alpha);
+  put(a.nb + 3, log10(a.xion[c]));
+  put(a.nb + 4, log10(dtl));
+}
+
+// New state from the network output.  The network predicts the change of the log of each bin's
+// share of the cell's dust mass; the new shares are the softmax of (old log-shares + change), so
+// they are positive and sum to one by construction and the cell's dust mass is kept in FP64.  A
+// cell falls back to the plain coagulation when an output is not finite, when the mass residual
+// |sum_k share_k - 1| after that normalisation exceeds resid_max (only possible through a
+// non-finite or degenerate prediction), or when an input lies more than zmax standard deviations
+// from the training set's mean (outside the envelope the network was trained on).
+__global__ void k_surr_post(SArgs a) {
+  const int cc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cc >= a.nch) return;
+  const int64_t c = a.c0 + cc;
+  const float* y = a.y + (int64_t)cc * a.nout;
+  const float* f = a.feat + (int64_t)cc * a.nin;
+  const float* om = a.norm + 2 * a.nin;
+  const float* os = om + a.nout;
+  double tot = 0.0;
+  for (int k = 0; k < a.nb; ++k) tot += a.rhod[(int64_t)k * a.ld + c];
+  bool fallback = !(tot > 0.0);
+  for (int j = 0; j < a.nin && !fallback; ++j)
+    if (!(fabsf(f[j]) <= a.zmax)) fallback = true;
+  double lmax = -1e300, raw = 0.0;
+  double ls[MAXB];
+  if (!fallback) {
+    for (int k = 0; k < a.nb; ++k) {
+      const double sh = a.rhod[(int64_t)k * a.ld + c] / tot;
+      const double d = (double)(y[k] * os[k] + om[k]);
+      ls[k] = log(sh) + d;
+      raw += sh * exp(d);
+      if (!(ls[k] == ls[k]) || !(fabs(ls[k]) < 1e30)) fallback = true;
+      lmax = fmax(lmax, ls[k]);
+    }
+  }
+  double S = 0.0;
+  if (!fallback) {
+    for (int k = 0; k < a.nb; ++k) {
+      ls[k] = exp(ls[k] - lmax);
+      S += ls[k];
+    }
+    double chk = 0.0;
+    for (int k = 0; k < a.nb; ++k) chk += ls[k] / S;
+    if (!(fabs(chk - 1.0) <= a.resid_max)) fallback = true;
+  }
+  const double dtl = a.dtc ? a.dtc[c] : a.dt;
+  a.nit[c] = 0;
+  if (fallback) {
+    a.fb[c] = 1;
+    a.tg[c] = 0.0;
+    atomicAdd(a.nfb, 1);
+    return;
+  }
+  a.fb[c] = 0;
+  a.tg[c] = dtl;
+  atomicAdd(a.resid_sum, fabs(raw - 1.0));
+  // new densities share * rho_tot, floored as the plain step floors, rescaled to the cell's mass
+  const double floor_v = a.rho_gas[c] * a.ratio_min;
+  double snew = 0.0;
+  for (int k = 0; k < a.nb; ++k) {
+    ls[k] = fmax(ls[k] / S * tot, floor_v);
+    snew += ls[k];
+  }
+  const double sc = tot / snew;
+  for (int k = 0; k < a.nb; ++k) a.rhod[(int64_t)k * a.ld + c] = ls[k] * sc;
+  a.zmean[c] = -pow(10.0, (double)(y[a.nb] * os[a.nb] + om[a.nb]));
 }
 
 }  // namespace
@@ -694,13 +816,26 @@ struct shark_growth {
   double last_ms = 0.0;
   bool uploaded = false;
   double *dzmean = nullptr, *dxion = nullptr;  // grain charging outputs (allocated on first use)
+  double* ddtc = nullptr;                      // per-cell growth interval
+  bool use_dtc = false;
+  // learned step
+  shark_mlp* mlp = nullptr;
+  float* dnorm = nullptr;
+  float *dfeat = nullptr, *dy = nullptr;
+  int* dfb = nullptr;
+  double* dresid = nullptr;
+  double alpha = 0.0, zeta = 5e-17, mu_ions = 25.0, resid_max = 1e-6;
+  float zmax = 8.0f;
+  int64_t last_nfb = 0;
+  double last_resid = 0.0;
 };
 
 void shark_growth_free(shark_growth* g) {
   if (!g) return;
   void* ptrs[] = {g->dpairs, g->dredis, g->dmdust, g->dsdust, g->drhod, g->drho, g->dcs, g->drad,
                   g->denv, g->dts, g->dA, g->dF, g->dAh, g->dFh, g->dtg, g->dnit, g->dflags,
-                  g->dGf32, g->dGbhi, g->dGblo, g->dzmean, g->dxion};
+                  g->dGf32, g->dGbhi, g->dGblo, g->dzmean, g->dxion, g->ddtc, g->dnorm,
+                  g->dfeat, g->dy, g->dfb, g->dresid};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (g->hflags) (void)hipHostFree(g->hflags);
@@ -889,12 +1024,7 @@ int shark_growth_upload(shark_growth* g, const double* rhod, const double* rho_g
   return 0;
 }
 
-int shark_growth_run(shark_growth* g, int mode, double dt) {
-  if (!g) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: null handle");
-  if (!g->uploaded) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: upload the state first");
-  if (mode < SHARK_GROWTH_FP64 || mode > SHARK_GROWTH_MFMA_BF16_PLAIN)
-    return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: unknown mode");
-  if (!(dt >= 0.0)) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: dt must be >= 0");
+static GArgs make_args(shark_growth* g, int mode, double dt) {
   const shark_growth_params& p = g->p;
   GArgs a;
   memset(&a, 0, sizeof(a));
@@ -911,7 +1041,7 @@ int shark_growth_run(shark_growth* g, int mode, double dt) {
   a.max_sub = p.max_substeps;
   a.sub_per_launch = SUB_PER_LAUNCH;
   a.kpad = g->kpad;
-  a.want64 = mode == SHARK_GROWTH_FP64;
+  a.want64 = mode == SHARK_GROWTH_FP64 || mode == SHARK_GROWTH_SURROGATE;
   a.want32 = !a.want64;
   a.dt = dt;
   a.cfl = p.cfl_growth;
@@ -946,6 +1076,7 @@ int shark_growth_run(shark_growth* g, int mode, double dt) {
   a.F = g->dF;
   a.Ah = g->dAh;
   a.Fh = g->dFh;
+  a.dtc = g->use_dtc ? g->ddtc : nullptr;
   a.tg = g->dtg;
   a.nit = g->dnit;
   a.unfinished = g->dflags;
@@ -953,37 +1084,183 @@ int shark_growth_run(shark_growth* g, int mode, double dt) {
   a.Gf32 = g->dGf32;
   a.Gbhi = g->dGbhi;
   a.Gblo = g->dGblo;
+  return a;
+}
+
+static QArgs make_qargs(shark_growth* g) {
+  const shark_growth_params& p = g->p;
+  QArgs q;
+  memset(&q, 0, sizeof(q));
+  q.nb = p.nbins;
+  q.ld = g->ncells;
+  q.electrons = 1;
+  q.maxiter = 100;
+  q.zeta = g->zeta;
+  q.mu_ions = g->mu_ions;
+  q.mu_gas = p.mu_gas;
+  q.hoverr = p.hoverr;
+  q.unit_d = p.unit_d;
+  q.unit_m = p.unit_m;
+  q.unit_l = p.unit_l;
+  q.s2pi = sqrt(2.0 * kPi);
+  q.rhod = g->drhod;
+  q.rho_gas = g->drho;
+  q.radius = g->drad;
+  q.env = g->denv;
+  q.mdust = g->dmdust;
+  q.sdust = g->dsdust;
+  q.zmean = g->dzmean;
+  q.xion = g->dxion;
+  return q;
+}
+
+static int ensure_charge_buffers(shark_growth* g) {
+  if (!g->dzmean) {
+    GCHK(hipMalloc(&g->dzmean, sizeof(double) * g->ncells));
+    GCHK(hipMalloc(&g->dxion, sizeof(double) * g->ncells));
+  }
+  return 0;
+}
+
+// pair prefactors for the chunk, then the bounded sub-step kernel relaunched until every active
+// cell of the chunk reached its interval
+static int plain_chunk(shark_growth* g, GArgs& a, int mode) {
+  hipStream_t s = g->stream;
+  const int nblk = (a.nch + 255) / 256;
+  hipLaunchKernelGGL(k_gpairs, dim3(nblk, a.npairs), dim3(256), 0, s, a);
+  GCHK(hipGetLastError());
+  for (int launch = 0;; ++launch) {
+    GCHK(hipMemsetAsync(g->dflags, 0, sizeof(int), s));
+    if (mode == SHARK_GROWTH_FP64) {
+      hipLaunchKernelGGL(k_gfp64, dim3((a.nch + 63) / 64), dim3(64), sizeof(double) * 64 * a.nb, s, a);
+    } else if (mode == SHARK_GROWTH_MFMA_FP32) {
+      hipLaunchKernelGGL(k_gmfma<1>, dim3((a.nch + NCT - 1) / NCT), dim3(256), 0, s, a);
+    } else if (mode == SHARK_GROWTH_MFMA_BF16) {
+      hipLaunchKernelGGL(k_gmfma<2>, dim3((a.nch + NCT - 1) / NCT), dim3(256), 0, s, a);
+    } else {
+      hipLaunchKernelGGL(k_gmfma<3>, dim3((a.nch + NCT - 1) / NCT), dim3(256), 0, s, a);
+    }
+    GCHK(hipGetLastError());
+    GCHK(hipMemcpyAsync(g->hflags, g->dflags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    GCHK(hipStreamSynchronize(s));
+    if (g->hflags[1]) return gfail(SHARK_GROWTH_ENUM, "growth produced a non-finite density or a zero sub-step");
+    if (g->hflags[0] == 0) break;
+  }
+  return 0;
+}
+
+static SArgs make_sargs(shark_growth* g, double dt) {
+  const shark_growth_params& p = g->p;
+  SArgs sa;
+  memset(&sa, 0, sizeof(sa));
+  sa.nb = p.nbins;
+  sa.ld = g->ncells;
+  sa.nin = p.nbins + 5;
+  sa.nout = p.nbins + 1;
+  sa.s2pi = sqrt(2.0 * kPi);
+  sa.hoverr = p.hoverr;
+  sa.unit_d = p.unit_d;
+  sa.log10_alpha = log10(g->alpha > 0.0 ? g->alpha : p.turb_coef * p.turb_coef);
+  sa.dt = dt;
+  sa.resid_max = g->resid_max;
+  sa.zmax = g->zmax;
+  sa.ratio_min = p.dust_ratio_min;
+  sa.dtc = g->use_dtc ? g->ddtc : nullptr;
+  sa.rhod = g->drhod;
+  sa.rho_gas = g->drho;
+  sa.radius = g->drad;
+  sa.env = g->denv;
+  sa.xion = g->dxion;
+  sa.zmean = g->dzmean;
+  sa.fb = g->dfb;
+  sa.tg = g->dtg;
+  sa.nit = g->dnit;
+  sa.nfb = g->dflags;
+  sa.resid_sum = g->dresid;
+  return sa;
+}
+
+// learned step over all cells, in chunks of the network's batch: environment, charge (the ion
+// fraction is an input), features, network, new state; then the plain FP64 coagulation and the
+// charge solve on the cells that fell back
+static int surrogate_run(shark_growth* g, double dt) {
+  hipStream_t s = g->stream;
+  GArgs a = make_args(g, SHARK_GROWTH_SURROGATE, dt);
+  QArgs q = make_qargs(g);
+  SArgs sa = make_sargs(g, dt);
+  sa.norm = g->dnorm;
+  sa.feat = g->dfeat;
+  sa.y = g->dy;
+  const int64_t B = std::min<int64_t>(shark::mlp_max_batch(g->mlp), g->nch_max);
+  GCHK(hipMemsetAsync(g->dresid, 0, sizeof(double), s));
+  int64_t nfb_total = 0;
+  for (int64_t c0 = 0; c0 < g->ncells; c0 += B) {
+    const int64_t rem = g->ncells - c0;
+    const int nch = (int)(rem < B ? rem : B);
+    a.c0 = q.c0 = sa.c0 = (int)c0;
+    a.nch = q.nch = sa.nch = nch;
+    const int nblk = (nch + 255) / 256;
+    hipLaunchKernelGGL(k_genv, dim3(nblk), dim3(256), 0, s, a);
+    GCHK(hipGetLastError());
+    q.mask = nullptr;
+    hipLaunchKernelGGL(k_gcharge, dim3(nblk), dim3(256), 0, s, q);
+    GCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_surr_feat, dim3(nblk), dim3(256), 0, s, sa);
+    GCHK(hipGetLastError());
+    int rc = shark::mlp_forward_device(g->mlp, g->dfeat, nch, g->dy, s);
+    if (rc) return rc;
+    GCHK(hipMemsetAsync(g->dflags, 0, sizeof(int) * 2, s));
+    hipLaunchKernelGGL(k_surr_post, dim3(nblk), dim3(256), 0, s, sa);
+    GCHK(hipGetLastError());
+    GCHK(hipMemcpyAsync(g->hflags, g->dflags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    GCHK(hipStreamSynchronize(s));
+    const int nfb = g->hflags[0];
+    nfb_total += nfb;
+    if (nfb > 0) {
+      rc = plain_chunk(g, a, SHARK_GROWTH_FP64);
+      if (rc) return rc;
+      q.mask = g->dfb;
+      hipLaunchKernelGGL(k_gcharge, dim3(nblk), dim3(256), 0, s, q);
+      GCHK(hipGetLastError());
+    }
+  }
+  double rs = 0.0;
+  GCHK(hipMemcpyAsync(&rs, g->dresid, sizeof(double), hipMemcpyDeviceToHost, s));
+  GCHK(hipStreamSynchronize(s));
+  g->last_nfb = nfb_total;
+  const int64_t nacc = g->ncells - nfb_total;
+  g->last_resid = nacc > 0 ? rs / (double)nacc : 0.0;
+  return 0;
+}
+
+int shark_growth_run(shark_growth* g, int mode, double dt) {
+  if (!g) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: null handle");
+  if (!g->uploaded) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: upload the state first");
+  if (mode < SHARK_GROWTH_FP64 || mode > SHARK_GROWTH_SURROGATE)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: unknown mode");
+  if (!(dt >= 0.0)) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: dt must be >= 0");
+  if (mode == SHARK_GROWTH_SURROGATE && !g->mlp)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: no surrogate attached (shark_growth_set_surrogate)");
   hipStream_t s = g->stream;
   GCHK(hipMemsetAsync(g->dtg, 0, sizeof(double) * g->ncells, s));
   GCHK(hipMemsetAsync(g->dnit, 0, sizeof(int) * g->ncells, s));
   GCHK(hipMemsetAsync(g->dflags, 0, sizeof(int) * 2, s));
   GCHK(hipEventRecord(g->e0, s));
-  for (int64_t c0 = 0; c0 < g->ncells; c0 += g->nch_max) {
-    const int64_t rem = g->ncells - c0;
-    a.c0 = (int)c0;
-    a.nch = (int)(rem < g->nch_max ? rem : g->nch_max);
-    const int nblk = (a.nch + 255) / 256;
-    hipLaunchKernelGGL(k_genv, dim3(nblk), dim3(256), 0, s, a);
-    GCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_gpairs, dim3(nblk, a.npairs), dim3(256), 0, s, a);
-    GCHK(hipGetLastError());
-    // relaunch the bounded sub-step kernel until every cell of the chunk reached dt
-    for (int launch = 0;; ++launch) {
-      GCHK(hipMemsetAsync(g->dflags, 0, sizeof(int), s));
-      if (mode == SHARK_GROWTH_FP64) {
-        hipLaunchKernelGGL(k_gfp64, dim3((a.nch + 63) / 64), dim3(64), sizeof(double) * 64 * a.nb, s, a);
-      } else if (mode == SHARK_GROWTH_MFMA_FP32) {
-        hipLaunchKernelGGL(k_gmfma<1>, dim3((a.nch + NCT - 1) / NCT), dim3(256), 0, s, a);
-      } else if (mode == SHARK_GROWTH_MFMA_BF16) {
-        hipLaunchKernelGGL(k_gmfma<2>, dim3((a.nch + NCT - 1) / NCT), dim3(256), 0, s, a);
-      } else {
-        hipLaunchKernelGGL(k_gmfma<3>, dim3((a.nch + NCT - 1) / NCT), dim3(256), 0, s, a);
-      }
+  if (mode == SHARK_GROWTH_SURROGATE) {
+    int rc = ensure_charge_buffers(g);
+    if (rc) return rc;
+    rc = surrogate_run(g, dt);
+    if (rc) return rc;
+  } else {
+    GArgs a = make_args(g, mode, dt);
+    for (int64_t c0 = 0; c0 < g->ncells; c0 += g->nch_max) {
+      const int64_t rem = g->ncells - c0;
+      a.c0 = (int)c0;
+      a.nch = (int)(rem < g->nch_max ? rem : g->nch_max);
+      hipLaunchKernelGGL(k_genv, dim3((a.nch + 255) / 256), dim3(256), 0, s, a);
       GCHK(hipGetLastError());
-      GCHK(hipMemcpyAsync(g->hflags, g->dflags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
-      GCHK(hipStreamSynchronize(s));
-      if (g->hflags[1]) return gfail(SHARK_GROWTH_ENUM, "growth produced a non-finite density or a zero sub-step");
-      if (g->hflags[0] == 0) break;
+      int rc = plain_chunk(g, a, mode);
+      if (rc) return rc;
     }
   }
   GCHK(hipEventRecord(g->e1, s));
@@ -991,6 +1268,100 @@ int shark_growth_run(shark_growth* g, int mode, double dt) {
   float ms = 0.f;
   GCHK(hipEventElapsedTime(&ms, g->e0, g->e1));
   g->last_ms = ms;
+  return 0;
+}
+
+int shark_growth_set_cell_dt(shark_growth* g, const double* dt_cells) {
+  if (!g) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_cell_dt: null handle");
+  if (!dt_cells) {
+    g->use_dtc = false;
+    return 0;
+  }
+  for (int64_t c = 0; c < g->ncells; ++c)
+    if (!(dt_cells[c] >= 0.0)) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_cell_dt: intervals must be >= 0");
+  if (!g->ddtc) GCHK(hipMalloc(&g->ddtc, sizeof(double) * g->ncells));
+  GCHK(hipMemcpy(g->ddtc, dt_cells, sizeof(double) * g->ncells, hipMemcpyHostToDevice));
+  g->use_dtc = true;
+  return 0;
+}
+
+int shark_growth_set_surrogate(shark_growth* g, shark_mlp* m, const float* norm, double alpha,
+                               double zeta, double mu_ions, double resid_max) {
+  if (!g) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate: null handle");
+  if (!m) {
+    g->mlp = nullptr;
+    return 0;
+  }
+  const int nb = g->p.nbins;
+  if (shark::mlp_n_in(m) != nb + 5 || shark::mlp_n_out(m) != nb + 1)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate: the network needs nbins+5 inputs and nbins+1 outputs");
+  if (!norm) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate: null normalisation");
+  if (!(resid_max > 0.0) || !(zeta > 0.0) || !(mu_ions > 0.0))
+    return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate: resid_max, zeta and mu_ions must be positive");
+  const int nn = 2 * (nb + 5) + 2 * (nb + 1);
+  for (int i = 0; i < nn; ++i)
+    if (!(norm[i] == norm[i])) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate: non-finite normalisation");
+  const int64_t B = std::min<int64_t>(shark::mlp_max_batch(m), g->nch_max);
+  if (!g->dnorm) GCHK(hipMalloc(&g->dnorm, sizeof(float) * 2 * (MAXB + 5) + sizeof(float) * 2 * (MAXB + 1)));
+  if (g->dfeat) (void)hipFree(g->dfeat);
+  if (g->dy) (void)hipFree(g->dy);
+  g->dfeat = nullptr;
+  g->dy = nullptr;
+  GCHK(hipMalloc(&g->dfeat, sizeof(float) * B * (nb + 5)));
+  GCHK(hipMalloc(&g->dy, sizeof(float) * B * (nb + 1)));
+  if (!g->dfb) GCHK(hipMalloc(&g->dfb, sizeof(int) * g->ncells));
+  if (!g->dresid) GCHK(hipMalloc(&g->dresid, sizeof(double)));
+  GCHK(hipMemcpy(g->dnorm, norm, sizeof(float) * nn, hipMemcpyHostToDevice));
+  int rc = ensure_charge_buffers(g);
+  if (rc) return rc;
+  g->mlp = m;
+  g->alpha = alpha;
+  g->zeta = zeta;
+  g->mu_ions = mu_ions;
+  g->resid_max = resid_max;
+  return 0;
+}
+
+int shark_growth_features(shark_growth* g, double dt, float* feat) {
+  if (!g || !feat) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_features: null argument");
+  if (!g->uploaded) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_features: upload the state first");
+  int rc = ensure_charge_buffers(g);
+  if (rc) return rc;
+  const int nin = g->p.nbins + 5;
+  hipStream_t s = g->stream;
+  GArgs a = make_args(g, SHARK_GROWTH_FP64, dt);
+  QArgs q = make_qargs(g);
+  SArgs sa = make_sargs(g, dt);
+  const int64_t B = g->nch_max;
+  float* dfe = nullptr;
+  GCHK(hipMalloc(&dfe, sizeof(float) * B * nin));
+  sa.feat = dfe;
+  sa.norm = nullptr;
+  for (int64_t c0 = 0; c0 < g->ncells; c0 += B) {
+    const int64_t rem = g->ncells - c0;
+    const int nch = (int)(rem < B ? rem : B);
+    a.c0 = q.c0 = sa.c0 = (int)c0;
+    a.nch = q.nch = sa.nch = nch;
+    const int nblk = (nch + 255) / 256;
+    hipLaunchKernelGGL(k_genv, dim3(nblk), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_gcharge, dim3(nblk), dim3(256), 0, s, q);
+    hipLaunchKernelGGL(k_surr_feat, dim3(nblk), dim3(256), 0, s, sa);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(feat + c0 * nin, dfe, sizeof(float) * nch * nin, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      (void)hipFree(dfe);
+      GCHK(e);
+    }
+  }
+  GCHK(hipFree(dfe));
+  return 0;
+}
+
+int shark_growth_surrogate_stats(shark_growth* g, int64_t* n_fallback, double* raw_resid_mean) {
+  if (!g) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_surrogate_stats: null handle");
+  if (n_fallback) *n_fallback = g->last_nfb;
+  if (raw_resid_mean) *raw_resid_mean = g->last_resid;
   return 0;
 }
 

@@ -25,6 +25,11 @@ EXPORTS = [
     "shark_gpu_destroy", "shark_gpu_last_error", "shark_gpu_version",
     "shark_growth_create", "shark_growth_run", "shark_growth_download", "shark_growth_upload",
     "shark_growth_destroy", "shark_growth_kernel_time", "shark_growth_charge",
+    "shark_growth_set_cell_dt", "shark_growth_features", "shark_growth_set_surrogate",
+    "shark_growth_surrogate_stats",
+    "shark_mlp_create", "shark_mlp_num_params", "shark_mlp_set_params", "shark_mlp_get_params",
+    "shark_mlp_forward", "shark_mlp_grads", "shark_mlp_set_data", "shark_mlp_train",
+    "shark_mlp_kernel_time", "shark_mlp_destroy",
 ]
 
 

@@ -14,8 +14,9 @@ from . import gpu
 SHARK_GROWTH_VERSION = 1
 MAX_BINS = 64
 ENV_COLLAPSE, ENV_DISK = 0, 1
-FP64, MFMA_FP32, MFMA_BF16, MFMA_BF16_PLAIN = 0, 1, 2, 3
-MODE_NAMES = {FP64: "fp64", MFMA_FP32: "mfma_fp32", MFMA_BF16: "mfma_bf16x3", MFMA_BF16_PLAIN: "mfma_bf16"}
+FP64, MFMA_FP32, MFMA_BF16, MFMA_BF16_PLAIN, SURROGATE = 0, 1, 2, 3, 4
+MODE_NAMES = {FP64: "fp64", MFMA_FP32: "mfma_fp32", MFMA_BF16: "mfma_bf16x3", MFMA_BF16_PLAIN: "mfma_bf16",
+              SURROGATE: "surrogate"}
 
 AU = 1.5e13
 MSUN = 2e33
@@ -125,6 +126,11 @@ def _lib():
         lib.shark_growth_download.argtypes = [C.c_void_p, _P, C.POINTER(C.c_int64)]
         lib.shark_growth_kernel_time.argtypes = [C.c_void_p, _P]
         lib.shark_growth_charge.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _P, _P]
+        lib.shark_growth_set_cell_dt.argtypes = [C.c_void_p, _P]
+        lib.shark_growth_features.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_float)]
+        lib.shark_growth_set_surrogate.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_float),
+                                                   C.c_double, C.c_double, C.c_double, C.c_double]
+        lib.shark_growth_surrogate_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), _P]
         lib.shark_growth_destroy.argtypes = [C.c_void_p]
         lib._growth_bound = True
     return lib
@@ -162,6 +168,35 @@ class Growth:
         gpu.check(self.L.shark_growth_charge(self.h, float(zeta), float(mu_ions), int(electrons),
                                              zm.ctypes.data_as(_P), xi.ctypes.data_as(_P)))
         return zm, xi
+
+    def set_cell_dt(self, dt_cells):
+        """Per-cell growth intervals for the following runs (None: run's dt everywhere)."""
+        if dt_cells is None:
+            gpu.check(self.L.shark_growth_set_cell_dt(self.h, None))
+            return
+        d = np.ascontiguousarray(dt_cells, dtype=np.float64)
+        assert d.size == self.ncells
+        self._dtc = d
+        gpu.check(self.L.shark_growth_set_cell_dt(self.h, d.ctypes.data_as(_P)))
+
+    def features(self, dt):
+        f = np.empty((self.ncells, self.p.nbins + 5), dtype=np.float32)
+        gpu.check(self.L.shark_growth_features(self.h, float(dt), f.ctypes.data_as(C.POINTER(C.c_float))))
+        return f
+
+    def set_surrogate(self, net, norm, alpha=0.0, zeta=5e-17, mu_ions=25.0, resid_max=1e-6):
+        """Attach a shark_amd.mlp.MLP (kept referenced here) with its normalisation vector."""
+        self._net = net
+        self._norm = np.ascontiguousarray(norm, dtype=np.float32)
+        gpu.check(self.L.shark_growth_set_surrogate(
+            self.h, None if net is None else net.h, self._norm.ctypes.data_as(C.POINTER(C.c_float)),
+            float(alpha), float(zeta), float(mu_ions), float(resid_max)))
+
+    def surrogate_stats(self):
+        n = C.c_int64()
+        r = C.c_double()
+        gpu.check(self.L.shark_growth_surrogate_stats(self.h, C.byref(n), C.byref(r)))
+        return n.value, r.value
 
     def kernel_ms(self):
         ms = C.c_double()

@@ -1,0 +1,96 @@
+"""Learned growth-and-charging step: training set from the plain solver, training on the GPU,
+weights on disk, and attachment to a growth state.
+
+Samples come from polar-disk cells (the disk workload's radial grid and surface density): an
+MRN size distribution aged by the plain coagulation for a random time per cell, then advanced by
+the plain coagulation over a random interval dt.  Inputs are the growth module's own features of
+the aged state (shark_growth_features); targets are the change of the natural log of each bin's
+share of the dust mass and log10(-<Z>) of the advanced state (charge solve of
+shark_growth_charge).
+"""
+import json
+import os
+
+import numpy as np
+
+from . import growth as G
+from . import mlp as M
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+def sample_states(params, tables_sdust, nr, nphi, seed, age_max=1e-2, dt_lo=1e-5, dt_hi=1e-3,
+                  gen_mode=G.MFMA_FP32):
+    """(X raw features, Y raw targets, state dict) for nr*nphi cells."""
+    rng = np.random.default_rng(seed)
+    rhod, sigma, cs, r = G.disk_state(params, nr=nr, nphi=nphi, bins=tables_sdust, seed=seed)
+    n = sigma.size
+    age = np.where(rng.random(n) < 0.15, 0.0, 10.0 ** rng.uniform(-7, np.log10(age_max), n))
+    dt = 10.0 ** rng.uniform(np.log10(dt_lo), np.log10(dt_hi), n)
+    g = G.Growth(params, n)
+    try:
+        g.upload(rhod, sigma, cs, r)
+        g.set_cell_dt(age)
+        g.run(gen_mode, 0.0)
+        s1, _ = g.download()
+        g.set_cell_dt(dt)
+        X = g.features(0.0)
+        g.run(gen_mode, 0.0)
+        s2, nsub = g.download()
+        g.upload(s2, sigma, cs, r)
+        zm, _ = g.charge()
+    finally:
+        g.close()
+    sh1 = s1 / s1.sum(axis=0)
+    sh2 = s2 / s2.sum(axis=0)
+    Y = np.concatenate([(np.log(sh2) - np.log(sh1)).T, np.log10(-zm)[:, None]], axis=1).astype(np.float32)
+    state = dict(rhod=s1, sigma=sigma, cs=cs, r=r, dt=dt, rhod_next=s2, nsub=nsub)
+    return X, Y, state
+
+
+def norm_of(X, Y):
+    mi, si = X.mean(axis=0), X.std(axis=0)
+    mo, so = Y.mean(axis=0), Y.std(axis=0)
+    si = np.where(si > 1e-6, si, 1.0)
+    so = np.where(so > 1e-12, so, 1.0)
+    return np.concatenate([mi, si, mo, so]).astype(np.float32)
+
+
+def standardise(X, Y, norm):
+    ni, no = X.shape[1], Y.shape[1]
+    mi, si = norm[:ni], norm[ni:2 * ni]
+    mo, so = norm[2 * ni:2 * ni + no], norm[2 * ni + no:]
+    return ((X - mi) / si).astype(np.float32), ((Y - mo) / so).astype(np.float32)
+
+
+def train(X, Y, norm, hidden=256, layers=4, epochs=(30, 20, 10), lrs=(2e-3, 5e-4, 1e-4),
+          batch=4096, seed=0, max_batch=65536):
+    """Adam on the GPU; returns (MLP, loss history)."""
+    Xs, Ys = standardise(X, Y, norm)
+    cfg = M.make_cfg(n_in=X.shape[1], n_hidden=hidden, n_layers=layers, n_out=Y.shape[1],
+                     max_batch=max_batch)
+    net = M.MLP(cfg, M.init_params(cfg, seed))
+    net.set_data(Xs, Ys)
+    hist = []
+    for i, (e, lr) in enumerate(zip(epochs, lrs)):
+        hist += list(net.train(e, batch, lr, seed=seed + 101 * i))
+    return net, np.array(hist)
+
+
+def save(path, net, norm, meta):
+    np.savez_compressed(path, params=net.get_params(), norm=norm,
+                        dims=np.array([net.cfg.n_in, net.cfg.n_hidden, net.cfg.n_layers, net.cfg.n_out]),
+                        meta=json.dumps(meta))
+
+
+def load(path, max_batch=65536):
+    z = np.load(path, allow_pickle=False)
+    n_in, n_hidden, n_layers, n_out = [int(v) for v in z["dims"]]
+    cfg = M.make_cfg(n_in=n_in, n_hidden=n_hidden, n_layers=n_layers, n_out=n_out, max_batch=max_batch)
+    net = M.MLP(cfg, z["params"])
+    return net, z["norm"], json.loads(str(z["meta"]))
+
+
+def mass_l1(a, ref):
+    """Mass-weighted size-distribution L1 error per cell: sum_k |a_k - ref_k| / sum_k ref_k."""
+    return np.abs(a - ref).sum(axis=0) / ref.sum(axis=0)
