@@ -312,44 +312,57 @@ __global__ __launch_bounds__(64) void k_gfp64(GArgs a) {
     for (int k = 0; k < nb; ++k) dd[k * 64 + lane] = 0.0;
     if (act) {
       niter = niter + 1;
-      int p = 0;
-      for (int i = 0; i < nb; ++i) {
-        const double ri = rh[(int64_t)i * a.ld];
-        for (int j = 0; j <= i; ++j, ++p) {
-          const PairC P = a.pairs[p];
-          const double m1 = P.m1, m2 = P.m2;
-          const double rj = rh[(int64_t)j * a.ld];
-          const int64_t o = (int64_t)p * a.nch + cc;
-          double f = a.frag ? a.F[o] : 0.0, pc = 1.0;
-          double dndt;
-          if (kt == 1) {
-            pc = 1.0; f = 0.0;
-            dndt = ri * rj / m1 / m2;
-          } else if (kt == 2) {
-            pc = 1.0; f = 0.0;
-            dndt = (m1 + m2) * ri * rj / m1 / m2;
-          } else {
-            dndt = a.A[o] * ri * rj / m1 / m2;
-          }
-          if (i == j) dndt = dndt / 2.0;
-          if (kt == 0) {
-            if (a.eps_thr > 0.0 && (ri < a.eps_thr * rg)) pc = 0.0;
-            if (a.eps_thr > 0.0 && (rj < a.eps_thr * rg)) pc = 0.0;
-            if (a.eps_thr_frag > 0.0 && (ri < a.eps_thr_frag * rg)) f = 0.0;
-            if (a.eps_thr_frag > 0.0 && (rj < a.eps_thr_frag * rg)) f = 0.0;
-          }
-          const double eps = P.eps;
-          dd[P.i * 64 + lane] = dd[P.i * 64 + lane] - pc * (1.0 - f) * m1 * dndt * st - pc * f * m1 * dndt;
-          dd[P.j * 64 + lane] = dd[P.j * 64 + lane] - pc * (1.0 - f) * m2 * dndt * st - pc * f * m2 * dndt;
-          if (f < 1.0) {
-            dd[P.t1 * 64 + lane] = dd[P.t1 * 64 + lane] + eps * pc * (1.0 - f) * (m1 + m2) * dndt * st;
-            dd[P.t2 * 64 + lane] = dd[P.t2 * 64 + lane] + (1. - eps) * pc * (1.0 - f) * (m1 + m2) * dndt * st;
-          }
-          if ((a.frag && f > 0.0) && pc > 0.) {
-            const double base = pc * f * (m1 + m2) * dndt;
-            for (int k = 0; k < P.nfrag; ++k)
-              dd[k * 64 + lane] = dd[k * 64 + lane] + base * a.redis[P.rrow + k];
-          }
+      // flat pair loop (same order as i outer, j <= i inner); the loads of pair p+1 are issued
+      // before pair p is processed so their latency overlaps its arithmetic and LDS updates
+      const int np = a.npairs;
+      const double* __restrict__ Ap = a.A + cc;
+      const double* __restrict__ Fp = a.F + cc;
+      double An = (kt == 0) ? Ap[0] : 0.0;
+      double Fn = a.frag ? Fp[0] : 0.0;
+      double rin = rh[0], rjn = rh[0];
+      for (int p = 0; p < np; ++p) {
+        const PairC P = a.pairs[p];
+        const double Ac = An, Fc = Fn, ri = rin, rj = rjn;
+        if (p + 1 < np) {
+          const PairC Q = a.pairs[p + 1];
+          const int64_t on = (int64_t)(p + 1) * a.nch;
+          An = (kt == 0) ? Ap[on] : 0.0;
+          Fn = a.frag ? Fp[on] : 0.0;
+          rin = rh[(int64_t)Q.i * a.ld];
+          rjn = rh[(int64_t)Q.j * a.ld];
+        }
+        const int i = P.i, j = P.j;
+        const double m1 = P.m1, m2 = P.m2;
+        double f = Fc, pc = 1.0;
+        double dndt;
+        if (kt == 1) {
+          pc = 1.0; f = 0.0;
+          dndt = ri * rj / m1 / m2;
+        } else if (kt == 2) {
+          pc = 1.0; f = 0.0;
+          dndt = (m1 + m2) * ri * rj / m1 / m2;
+        } else {
+          dndt = Ac * ri * rj / m1 / m2;
+        }
+        if (i == j) dndt = dndt / 2.0;
+        if (kt == 0) {
+          if (a.eps_thr > 0.0 && (ri < a.eps_thr * rg)) pc = 0.0;
+          if (a.eps_thr > 0.0 && (rj < a.eps_thr * rg)) pc = 0.0;
+          if (a.eps_thr_frag > 0.0 && (ri < a.eps_thr_frag * rg)) f = 0.0;
+          if (a.eps_thr_frag > 0.0 && (rj < a.eps_thr_frag * rg)) f = 0.0;
+        }
+        const double eps = P.eps;
+        dd[i * 64 + lane] = dd[i * 64 + lane] - pc * (1.0 - f) * m1 * dndt * st - pc * f * m1 * dndt;
+        dd[j * 64 + lane] = dd[j * 64 + lane] - pc * (1.0 - f) * m2 * dndt * st - pc * f * m2 * dndt;
+        if (f < 1.0) {
+          dd[P.t1 * 64 + lane] = dd[P.t1 * 64 + lane] + eps * pc * (1.0 - f) * (m1 + m2) * dndt * st;
+          dd[P.t2 * 64 + lane] = dd[P.t2 * 64 + lane] + (1. - eps) * pc * (1.0 - f) * (m1 + m2) * dndt * st;
+        }
+        if ((a.frag && f > 0.0) && pc > 0.) {
+          const double base = pc * f * (m1 + m2) * dndt;
+          const double* __restrict__ rr = a.redis + P.rrow;
+#pragma unroll 4
+          for (int k = 0; k < P.nfrag; ++k) dd[k * 64 + lane] = dd[k * 64 + lane] + base * rr[k];
         }
       }
       double dtg = 1e32;
