@@ -12,6 +12,14 @@ synchronised on both sides.  One JSON line is printed.
 
 cpu_baseline: the CPU restatement of the same step (oracle/liboracle.so, OpenMP over the
 host cores) on the same workload for a bounded number of steps.
+
+growth (second half of the metric, "growth-step speedup at fixed L1 err"): on the 512 x 512
+polar disk with 64 mass bins (MRN initial distribution), one growth call per mode over the
+same cells and interval: the plain FP64 coagulation (reference order) and the FP32 / BF16
+MFMA contractions, each with its sub-step count, time per cell-sub-step, speedup over FP64 and
+mass-weighted L1 error against it.  With trained weights in shark_amd/data, the learned step is
+timed against the plain FP64 step on cells aged by the plain solver, at intervals in the disk's
+hydro-step range, with its L1 error and fallback count.
 """
 import argparse
 import json
@@ -60,6 +68,77 @@ def cpu_baseline(cfg, geom, state, zones_per_step, steps):
                       f"{dt:.1f} s, OpenMP over {cores} threads"}
 
 
+def growth_block(steps_probe_dt=2e-8, surrogate_cells=16384, seed=99):
+    """Growth timing and error per mode (see module docstring)."""
+    import numpy as np
+    from shark_amd import growth as G
+    from shark_amd import surrogate as S
+    from shark_amd.train_surrogate import tables_sdust
+
+    nb = 64
+    p = G.disk_params(nbins=nb, max_substeps=1000000)
+    sd = tables_sdust(p)
+    rhod, sigma, cs, r = G.disk_state(p, nr=512, nphi=512, bins=sd)
+    n = sigma.size
+    g = G.Growth(p, n)
+    modes = {}
+    ref = None
+    try:
+        for mode in (G.FP64, G.MFMA_FP32, G.MFMA_BF16, G.MFMA_BF16_PLAIN):
+            g.upload(rhod, sigma, cs, r)
+            g.run(mode, steps_probe_dt)
+            out, nsub = g.download()
+            ms = g.kernel_ms()
+            if mode == G.FP64:
+                ref = out
+            err = S.mass_l1(out, ref)
+            modes[G.MODE_NAMES[mode]] = {
+                "ms": ms, "substeps_max": int(nsub.max()), "substeps_mean": float(nsub.mean()),
+                "ns_per_cell_substep": ms * 1e6 / float(nsub.sum()),
+                "l1_vs_fp64_mean": float(err.mean()), "l1_vs_fp64_max": float(err.max()),
+                "mass_residual_max": float(np.abs(out.sum(0) / rhod.sum(0) - 1).max())}
+    finally:
+        g.close()
+    base = modes["fp64"]["ms"]
+    for v in modes.values():
+        v["speedup_vs_fp64"] = base / v["ms"]
+    out = {"workload": "disk_512x512_b64_mrn", "cells": n, "dt": steps_probe_dt, "modes": modes}
+
+    wpath = os.path.join(ROOT, "shark_amd", "data", "surrogate_disk64.npz")
+    if os.path.exists(wpath):
+        net, norm, meta = S.load(wpath)
+        nphi = max(1, surrogate_cells // 1024)
+        _, _, st = S.sample_states(p, sd, 1024, nphi, seed=seed, age_max=meta["age_max"],
+                                   dt_lo=meta["dt_range"][0], dt_hi=meta["dt_range"][1])
+        ns = st["sigma"].size
+        g = G.Growth(p, ns)
+        try:
+            g.set_surrogate(net, norm)
+            g.set_cell_dt(st["dt"])
+            g.upload(st["rhod"], st["sigma"], st["cs"], st["r"])
+            g.run(G.SURROGATE, 0.0)
+            so, _ = g.download()
+            ms_s = g.kernel_ms()
+            nfb, raw = g.surrogate_stats()
+            g.upload(st["rhod"], st["sigma"], st["cs"], st["r"])
+            g.run(G.FP64, 0.0)
+            fo, fsub = g.download()
+            ms_f = g.kernel_ms()
+        finally:
+            g.close()
+            net.close()
+        err = S.mass_l1(so, fo)
+        out["surrogate"] = {
+            "cells": ns, "dt_range": meta["dt_range"], "age_max": meta["age_max"],
+            "train_samples": meta["samples"], "ms": ms_s, "fp64_ms": ms_f,
+            "speedup_vs_fp64": ms_f / ms_s, "fp64_substeps_mean": float(fsub.mean()),
+            "l1_vs_fp64_mean": float(err.mean()), "l1_vs_fp64_p99": float(np.quantile(err, 0.99)),
+            "no_change_l1_mean": float(S.mass_l1(st["rhod"], fo).mean()),
+            "fallback_cells": int(nfb), "raw_mass_residual_mean": raw,
+            "mass_residual_max": float(np.abs(so.sum(0) / st["rhod"].sum(0) - 1).max())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +150,7 @@ def main():
     ap.add_argument("--graph", action="store_true", help="time the hipGraph replay path instead "
                     "of per-kernel event-bracketed launches")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--no-growth", action="store_true", help="skip the growth timing block")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -140,6 +220,8 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, geom, state, zones_per_step, args.cpu_steps)
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    if not args.no_growth:
+        out["growth"] = growth_block()
     print(json.dumps(out))
 
 
