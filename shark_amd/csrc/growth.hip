@@ -58,7 +58,18 @@ struct PairC {
   int i, j, t1, t2;  // partner bins and the two coagulation target bins (0-based)
   int nfrag, rrow;   // fragments go to bins [0, nfrag); their weights start at redis[rrow]
   double eps, m1, m2, geom;  // geom = clustered^2 pi (s1+s2)^2
+  double ym1, ym2;   // correctly rounded 1/m1, 1/m2 (exact quotients through exdiv)
 };
+
+// a / b correctly rounded from y = RN(1/b): q0 = a*y is within one ulp, the FMA residual is exact
+// and q0 + r*y rounds to the correctly rounded quotient (Markstein's theorem) in the normal range;
+// a zero dividend keeps the sign of a/b.
+__host__ __device__ __forceinline__ double exdiv(double a, double b, double y) {
+  const double q0 = a * y;
+  const double r = fma(-b, q0, a);
+  const double q = fma(r, y, q0);
+  return a == 0.0 ? q0 : q;
+}
 
 static double powi(double a, int b) {  // x**n for integer n, binary powering
   int recip = b < 0;
@@ -147,6 +158,8 @@ static void build_pairs(const shark_growth_params* p, const Tables* t, std::vect
       c.eps = eps;
       c.m1 = m1;
       c.m2 = m2;
+      c.ym1 = 1.0 / m1;
+      c.ym2 = 1.0 / m2;
       c.geom = p->clustered * p->clustered * kPi * ((s1 + s2) * (s1 + s2));
       out->push_back(c);
     }
@@ -337,12 +350,12 @@ __global__ __launch_bounds__(64) void k_gfp64(GArgs a) {
         double dndt;
         if (kt == 1) {
           pc = 1.0; f = 0.0;
-          dndt = ri * rj / m1 / m2;
+          dndt = exdiv(exdiv(ri * rj, m1, P.ym1), m2, P.ym2);
         } else if (kt == 2) {
           pc = 1.0; f = 0.0;
-          dndt = (m1 + m2) * ri * rj / m1 / m2;
+          dndt = exdiv(exdiv((m1 + m2) * ri * rj, m1, P.ym1), m2, P.ym2);
         } else {
-          dndt = Ac * ri * rj / m1 / m2;
+          dndt = exdiv(exdiv(Ac * ri * rj, m1, P.ym1), m2, P.ym2);
         }
         if (i == j) dndt = dndt / 2.0;
         if (kt == 0) {

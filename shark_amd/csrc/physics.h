@@ -15,6 +15,19 @@ constexpr double kPi = 3.141592653589793238462643383279;  // src/commons/phys_co
 __device__ __forceinline__ double fmaxf_(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double fminf_(double a, double b) { return a < b ? a : b; }
 
+// a / b, correctly rounded, for a divisor whose correctly rounded reciprocal y = 1/b is at hand
+// (one IEEE division shared by several quotients).  q0 = a*y is within one ulp of a/b, the
+// residual a - b*q0 is exact in an FMA, and q0 + (a - b*q0)*y rounds to the correctly rounded
+// quotient (Markstein, IBM J. Res. Dev. 34 (1990) 111-119; Muller et al., Handbook of
+// Floating-Point Arithmetic, division with FMA) for quotients in the normal range.  A zero
+// dividend takes the product, so the sign of zero is that of a/b.
+__device__ __forceinline__ double exdiv(double a, double b, double y) {
+  const double q0 = a * y;
+  const double r = fma(-b, q0, a);
+  const double q = fma(r, y, q0);
+  return a == 0.0 ? q0 : q;
+}
+
 // slope_limit, src/solver/slope_limiters.f90:5-37
 __device__ __forceinline__ double slope_limit(int type, double sl, double sr) {
   double s;

@@ -139,6 +139,8 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   a.gamma = cfg->gamma; a.cfl = cfg->cfl; a.smallr = cfg->smallr; a.dust_ratio_min = cfg->dust_ratio_min;
   a.q_shear = cfg->q_shear; a.omega_shear = cfg->omega_shear; a.box_l = cfg->box_l; a.rad0 = cfg->rad0;
   a.eta_stream = cfg->eta_stream; a.rho_init = cfg->rho_init; a.hoverr = cfg->hoverr;
+  a.eps_k1 = sqrt((3.141592653589793238462643383279 * cfg->gamma) / 8.0) * cfg->rhograin;
+  a.s2pi = sqrt(2.0 * 3.141592653589793238462643383279);
   a.rhograin = cfg->rhograin; a.mstar = cfg->mstar; a.r_relax = cfg->r_relax; a.n_rel = cfg->n_rel;
   a.smooth_r = cfg->smooth_r;
   for (int k = 0; k < SHARK_MAX_DUST; ++k) a.K[k] = cfg->K[k];

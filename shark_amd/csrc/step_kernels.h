@@ -48,6 +48,8 @@ struct StepArgs {
   double mstar, r_relax, n_rel, smooth_r;
   double K[64];
   double sdust_k[64];   // per-species grain size when uniform over cells
+  double eps_k1;        // sqrt(pi gamma / 8) * rhograin: the Epstein prefactor (disk tstop)
+  double s2pi;          // sqrt(2 pi)
 };
 
 void launch_bc(const StepArgs& a, hipStream_t s);
