@@ -68,6 +68,24 @@ def cpu_baseline(cfg, geom, state, zones_per_step, steps):
                       f"{dt:.1f} s, OpenMP over {cores} threads"}
 
 
+def profiled_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of the disk
+    workload (FETCH_SIZE/WRITE_SIZE runs scaled by the copy calibration), or None."""
+    if workload != "disk":
+        return None
+    path = os.path.join(ROOT, "profiles", "disk_1024x2048_n16_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes":
+            vals[parts[1]] = int(parts[2])
+    if kernel == "hydro" and "hydro_gas" in vals and "hydro_dust" in vals:
+        return vals["hydro_gas"] + vals["hydro_dust"]
+    return vals.get(kernel)
+
+
 def growth_block(steps_probe_dt=2e-8, surrogate_cells=16384, seed=99):
     """Growth timing and error per mode (see module docstring)."""
     import numpy as np
@@ -209,7 +227,9 @@ def main():
                    "fargo": bool(cfg.fargo), "drag": bool(cfg.drag), "solver": "hll",
                    "solverdust": "huang_bai", "slope_type": cfg.slope_type},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(args.workload, dom),
+                     "traffic_source": "profiles/disk_1024x2048_n16_hbm_traffic.md",
                      "algorithmic_bytes": dom_bytes, "avg_ms": dom_ms},
         "kernels_ms": kt,
         "step_roofline": {"algorithmic_bytes": step_alg,
