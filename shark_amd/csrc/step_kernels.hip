@@ -619,8 +619,21 @@ __global__ __launch_bounds__(256, (TW >= 64 ? 3 : 5)) void k_post2(StepArgs a) {
   double (*fmS)[TW] = akS + NS;
   double* vvlast = sumS[0];
   const int c = threadIdx.x % TW, s = threadIdx.x / TW;
-  const int ix = 3 + blockIdx.x * TW + c;
-  const int iy = (NDIM == 2) ? 3 + blockIdx.y : 1;
+  // XCD-aware block order: the hardware deals consecutive blocks to the 8 XCDs in turn; the
+  // swizzle gives each XCD a contiguous band of rows, so the FARGO source rows a block reads were
+  // loaded into the same XCD's L2 by its neighbours in the band
+  int bx = blockIdx.x, by = blockIdx.y;
+  if (NDIM == 2) {
+    const unsigned nbx = gridDim.x, nb = nbx * gridDim.y;
+    if ((nb & 7u) == 0) {
+      const unsigned b = blockIdx.x + blockIdx.y * nbx;
+      const unsigned l = (b & 7u) * (nb >> 3) + (b >> 3);
+      bx = (int)(l % nbx);
+      by = (int)(l / nbx);
+    }
+  }
+  const int ix = 3 + bx * TW + c;
+  const int iy = (NDIM == 2) ? 3 + by : 1;
   const bool valid = ix <= a.nx + 2;
   const int ixs = valid ? ix : a.nx + 2;  // clamped column for loads of idle lanes
   const size_t nc = a.nc;
