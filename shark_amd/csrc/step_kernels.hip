@@ -382,7 +382,12 @@ template <int NDIM, int GEOM, bool GAS, int SLOPE, int SOLV>
 __global__ __launch_bounds__(256, (GAS ? 2 : 1)) void k_hydro(StepArgs a) {
   if (!a.sc->active) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const long item = (long)blockIdx.x * 4 + wave;
+  // XCD-aware order: each of the 8 XCDs takes a contiguous range of work items, so the halo rows
+  // shared by neighbouring strips of a column group are read through one L2
+  unsigned b = blockIdx.x;
+  const unsigned nb = gridDim.x;
+  if ((nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+  const long item = (long)b * 4 + wave;
   const long G = a.ngroups, S = a.nstrips, F = GAS ? 1 : a.nd;
   if (item >= G * S * F) return;
   const int g = (int)(item % G);
@@ -604,7 +609,7 @@ __global__ __launch_bounds__(256) void k_post(StepArgs a) {
 // species order in one thread per (cell, component), so results equal k_post / the
 // reference bit for bit; loads, divisions and the Courant work are shared by all slots.
 template <int ND, bool FARGO, int NDIM, int TW>
-__global__ __launch_bounds__(256, (TW >= 64 ? 3 : 5)) void k_post2(StepArgs a) {
+__global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
   if (!a.sc->active) return;
   constexpr int NV = 5 + 4 * ND;
   constexpr int NS = 256 / TW;
@@ -954,8 +959,9 @@ void launch_hydro(const StepArgs& a, hipStream_t s) {
 template <int ND>
 static void post_nd(const StepArgs& a, hipStream_t s) {
   // 32-cell row segments with 8 slots: 27 KB of LDS and <= 96 VGPRs, so five blocks share a CU
-  // (measured on the 1024 x 2048 16-species disk: 1.45 ms against 1.79 ms for 64-cell segments)
-  constexpr int TW = 32;
+  // (measured on the 1024 x 2048 16-species disk: 1.45 ms against 1.79 ms for 64-cell segments);
+  // 64 species (261 variables) take 16-cell segments with 16 slots to stay within LDS
+  constexpr int TW = ND >= 32 ? 16 : 32;
   if (a.ndim == 1) {
     dim3 grid((a.nx + TW - 1) / TW);
     hipLaunchKernelGGL((k_post2<ND, false, 1, TW>), grid, dim3(256), 0, s, a);
@@ -981,6 +987,7 @@ static void courant_nd(const StepArgs& a, hipStream_t s) {
     case 4: FN<4>(a, s); break;                      \
     case 8: FN<8>(a, s); break;                      \
     case 16: FN<16>(a, s); break;                    \
+    case 64: FN<64>(a, s); break;                    \
     default: break;                                  \
   }
 
@@ -1002,6 +1009,8 @@ void launch_ctoprim(const StepArgs& a, double* q, hipStream_t s) {
   hipLaunchKernelGGL(k_ctoprim, dim3((a.nc + 255) / 256), dim3(256), 0, s, a, q);
 }
 
-bool nd_supported(int nd) { return nd == 0 || nd == 1 || nd == 2 || nd == 3 || nd == 4 || nd == 8 || nd == 16; }
+bool nd_supported(int nd) {
+  return nd == 0 || nd == 1 || nd == 2 || nd == 3 || nd == 4 || nd == 8 || nd == 16 || nd == 64;
+}
 
 }  // namespace shark

@@ -64,7 +64,7 @@ def standardise(X, Y, norm):
 
 
 def train(X, Y, norm, hidden=256, layers=4, epochs=(30, 20, 10), lrs=(2e-3, 5e-4, 1e-4),
-          batch=4096, seed=0, max_batch=65536):
+          batch=4096, seed=0, max_batch=65536, progress_every=5, verbose=False):
     """Adam on the GPU; returns (MLP, loss history)."""
     Xs, Ys = standardise(X, Y, norm)
     cfg = M.make_cfg(n_in=X.shape[1], n_hidden=hidden, n_layers=layers, n_out=Y.shape[1],
@@ -73,7 +73,13 @@ def train(X, Y, norm, hidden=256, layers=4, epochs=(30, 20, 10), lrs=(2e-3, 5e-4
     net.set_data(Xs, Ys)
     hist = []
     for i, (e, lr) in enumerate(zip(epochs, lrs)):
-        hist += list(net.train(e, batch, lr, seed=seed + 101 * i))
+        done = 0
+        while done < e:                      # a few epochs per call, reporting as it goes
+            n = min(progress_every, e - done)
+            hist += list(net.train(n, batch, lr, seed=seed + 101 * i + 7 * done))
+            done += n
+            if verbose:
+                print(f"  lr {lr:g}: epoch {len(hist)} loss {hist[-1]:.4e}", flush=True)
     return net, np.array(hist)
 
 

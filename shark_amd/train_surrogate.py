@@ -66,7 +66,7 @@ def main(argv=None):
     t1 = time.time()
     epochs = [int(e) for e in a.epochs.split(",")]
     lrs = [float(v) for v in a.lrs.split(",")]
-    net, hist = S.train(X, Y, norm, epochs=epochs, lrs=lrs, batch=a.batch, seed=a.seed)
+    net, hist = S.train(X, Y, norm, epochs=epochs, lrs=lrs, batch=a.batch, seed=a.seed, verbose=True)
     t_train = time.time() - t1
     print(f"trained on {X.shape[0]} samples in {t_train:.0f} s: loss {hist[0]:.3e} -> {hist[-1]:.3e}",
           flush=True)
