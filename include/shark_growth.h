@@ -91,6 +91,17 @@ int shark_growth_set_surrogate(shark_growth* g, struct shark_mlp* m, const float
 /* Of the last SHARK_GROWTH_SURROGATE run: cells that fell back, and the mean mass residual of the
  * accepted raw predictions before normalisation, |sum_k share_k exp(change_k) - 1|. */
 int shark_growth_surrogate_stats(shark_growth* g, int64_t* n_fallback, double* raw_resid_mean);
+/* Dust growth inside the step (src/solver/solve.f90:112): after the drag of every shark_gpu_step /
+ * shark_gpu_run of ctx, a growth call of `mode` over the step's dt on the step's dust densities
+ * (u_prim(:,irhod), active cells only; the gas density u_prim(:,irho) and cs of the step, the
+ * cell radius of the grid).  g must have been created with the step's ncells (ghost cells
+ * included) and nbins == ndust; it then belongs to the step (shark_growth_upload/run refuse it).
+ * g == NULL detaches.  The caller keeps ownership: destroy the context (or detach) before g.
+ * Each species keeps its velocity through growth (momentum scaled with its density; the reference
+ * changes the densities only, dust_growth.f90:243-245, which collapses the next Courant step).
+ * shark_gpu_kernel_time(ctx, "growth", ...) reports its time per step. */
+struct shark_ctx;
+int shark_gpu_attach_growth(struct shark_ctx* ctx, shark_growth* g, int mode);
 /* Device time (ms) of the last run. */
 int shark_growth_kernel_time(shark_growth* g, double* ms);
 int shark_growth_destroy(shark_growth* g);

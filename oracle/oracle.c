@@ -30,6 +30,7 @@
 #include <string.h>
 #include <stdio.h>
 #include "oracle.h"
+#include "growth_oracle.h"
 
 #define HALF 0.5
 static const double PI = 3.141592653589793238462643383279;  /* phys_const.f90:14 */
@@ -43,6 +44,9 @@ struct oracle {
   double *dx, *surf, *vol, *radii_c, *position, *condinit, *fargo_v, *S_U, *uold, *duprim;
   unsigned char *active, *active_pred;
   double dt, time;
+  int growth_on;                     /* dust_growth after the drag (solve.f90:112), disk env */
+  shark_growth_params gp;
+  growth_tables gt;
 };
 
 static inline int cidx(const oracle* o, int ix, int iy) {
@@ -856,6 +860,55 @@ static void setup_inloop(oracle* o) {
   }
 }
 
+/* ---------------------------------------------------------------- growth */
+/* dust_growth over the active cells (solve.f90:112) in the disk environment of growth_oracle.c:
+ * surface density u_prim(:,irho), the step's cs and cell radius, the step's dt.  The reference
+ * changes only the dust densities (dust_growth.f90:243-245), so each species' velocity m/rho_d
+ * jumps by the density ratio and the next Courant step collapses (1.8e-2 -> 4e-10 on a 32x48
+ * disk); here every species keeps its velocity, its momentum scaled with its density. */
+static void growth_disk_step(oracle* o) {
+  const int nb = o->nd;
+  int64_t na = 0;
+  for (int c = 0; c < o->nc; ++c) na += o->active[c] ? 1 : 0;
+  double* sg = (double*)malloc(sizeof(double) * na);
+  double* cs = (double*)malloc(sizeof(double) * na);
+  double* r = (double*)malloc(sizeof(double) * na);
+  double* rd = (double*)malloc(sizeof(double) * na * nb);
+  int* map = (int*)malloc(sizeof(int) * na);
+  int64_t n = 0;
+  for (int c = 0; c < o->nc; ++c)
+    if (o->active[c]) {
+      map[n] = c;
+      sg[n] = U(0, c);
+      cs[n] = o->cs[c];
+      r[n] = o->radii_c[c];
+      ++n;
+    }
+  for (int k = 0; k < nb; ++k)
+    for (int64_t i = 0; i < na; ++i) rd[k * na + i] = U(IRHOD(k), map[i]);
+  growth_oracle_disk(&o->gp, &o->gt, na, sg, cs, r, o->dt, rd, NULL, NULL);
+  /* each species keeps its velocity: momentum = (momentum / old density) * new density */
+  for (int k = 0; k < nb; ++k)
+    for (int64_t i = 0; i < na; ++i) {
+      const int c = map[i];
+      const double r0 = U(IRHOD(k), c), r1 = rd[k * na + i];
+      U(IVDX(k), c) = (U(IVDX(k), c) / r0) * r1;
+      U(IVDY(k), c) = (U(IVDY(k), c) / r0) * r1;
+      U(IVDZ(k), c) = (U(IVDZ(k), c) / r0) * r1;
+      U(IRHOD(k), c) = r1;
+    }
+  free(sg); free(cs); free(r); free(rd); free(map);
+}
+
+int oracle_set_growth(oracle* o, const shark_growth_params* gp) {
+  if (!gp) { o->growth_on = 0; return 0; }
+  if (gp->nbins != o->nd) return 1;
+  o->gp = *gp;
+  growth_oracle_tables(gp, &o->gt);
+  o->growth_on = 1;
+  return 0;
+}
+
 /* ---------------------------------------------------------------- solve */
 /* solve, src/solver/solve.f90:10-181 (no charging, MHD, TURB) */
 double oracle_step(oracle* o) {
@@ -874,6 +927,7 @@ double oracle_step(oracle* o) {
   source_terms(o);
   if (p->fargo) fargo_scheme(o);
   if (o->nd > 0 && p->drag) dust_drag(o, 1.0);
+  if (o->growth_on) growth_disk_step(o);
   setup_inloop(o);
   o->time = o->time + o->dt;
   return o->dt;

@@ -6,6 +6,9 @@
 typedef struct oracle oracle;
 oracle* oracle_create(const shark_cfg* cfg);
 void oracle_destroy(oracle* o);
+/* dust growth after the drag in every step (disk environment); NULL switches it off */
+struct shark_growth_params;
+int oracle_set_growth(oracle* o, const struct shark_growth_params* gp);
 int oracle_ncells(const oracle* o);
 int oracle_nvar(const oracle* o);
 void oracle_set_geometry(oracle* o, const double* dx, const double* surf, const double* vol,

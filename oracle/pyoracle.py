@@ -78,6 +78,13 @@ class Oracle:
         self.L.oracle_upload(self.h, _ptr(s["u"]), _ptr(s["cs"]), _ptr(s.get("sdust")),
                              _ptr(s.get("condinit")), _ptr(s.get("fargo_velocity")))
 
+    def set_growth(self, params):
+        """Dust growth (disk environment, FP64) after the drag of every step."""
+        self.L.oracle_set_growth.argtypes = [C.c_void_p, C.c_void_p]
+        self._gp = params
+        if self.L.oracle_set_growth(self.h, None if params is None else C.byref(params)) != 0:
+            raise ValueError("growth bins must equal the number of dust species")
+
     def step(self):
         return self.L.oracle_step(self.h)
 

@@ -191,6 +191,8 @@ struct GArgs {
   float* Ah;    // [npairs][nch]
   float* Fh;    // [npairs][nch]
   const double* dtc;  // [ld] per-cell growth interval, or null for dt everywhere
+  const double* dt_dev;  // device scalar holding dt (bound to a step), or null for dt
+  const int* cmask;      // [ld] 0 for cells that do not grow (ghost cells of a step), or null
   double* tg;   // [ld] time reached inside the growth call
   int* nit;     // [ld] sub-steps taken
   int* unfinished;
@@ -201,6 +203,14 @@ struct GArgs {
 };
 
 __device__ __forceinline__ double fmn(double a, double b) { return a < b ? a : b; }
+// growth interval of a cell: the per-cell interval when set, else dt (or the step's device dt);
+// zero for masked cells
+template <class Args>
+__device__ __forceinline__ double cell_dt(const Args& a, int64_t c) {
+  if (a.cmask && !a.cmask[c]) return 0.0;
+  if (a.dtc) return a.dtc[c];
+  return a.dt_dev ? *a.dt_dev : a.dt;
+}
 __device__ __forceinline__ double fmx(double a, double b) { return a > b ? a : b; }
 
 // Disk environment of one cell: T from the local isothermal sound speed, t_L = 1/Omega, the
@@ -316,7 +326,7 @@ __global__ __launch_bounds__(64) void k_gfp64(GArgs a) {
   const double rg = valid ? a.rho_gas[c] : 1.0;
   double tgrow = valid ? a.tg[c] : 0.0;
   int niter = valid ? a.nit[c] : 0;
-  const double dtl = a.dtc ? a.dtc[c] : a.dt;
+  const double dtl = cell_dt(a, c);
   bool act = valid && tgrow < dtl && !(a.max_sub > 0 && niter >= a.max_sub);
   const int kt = a.kernel_type;
   const double st = a.sticking;
@@ -478,7 +488,7 @@ __global__ __launch_bounds__(256) void k_gmfma(GArgs a) {
       tgrow = a.tg[c];
       niter = a.nit[c];
       rg = a.rho_gas[c];
-      dtl = a.dtc ? a.dtc[c] : a.dt;
+      dtl = cell_dt(a, c);
     }
   }
   const int nchunks = a.kpad / KC;
@@ -712,6 +722,8 @@ struct SArgs {
   double s2pi, hoverr, unit_d, log10_alpha, dt, resid_max, ratio_min;
   float zmax;  // input envelope in standard deviations
   const double* dtc;
+  const double* dt_dev;
+  const int* cmask;
   double* rhod;
   const double* rho_gas;
   const double* radius;
@@ -743,7 +755,7 @@ __global__ void k_surr_feat(SArgs a) {
     f[j] = x;
   };
   for (int k = 0; k < a.nb; ++k) put(k, log10(fmax(a.rhod[(int64_t)k * a.ld + c], 1e-300) * conv));
-  const double dtl = a.dtc ? a.dtc[c] : a.dt;
+  const double dtl = cell_dt(a, c);
   put(a.nb, log10(a.env[cc]));
   put(a.nb + 1, log10(a.rho_gas[c] * conv));
   put(a.nb + 2, a.log10_alpha);
@@ -766,6 +778,12 @@ __global__ void k_surr_post(SArgs a) {
   const float* f = a.feat + (int64_t)cc * a.nin;
   const float* om = a.norm + 2 * a.nin;
   const float* os = om + a.nout;
+  if (!(cell_dt(a, c) > 0.0)) {   // no growth interval (a masked ghost cell): unchanged
+    a.fb[c] = 0;
+    a.tg[c] = 0.0;
+    a.nit[c] = 0;
+    return;
+  }
   double tot = 0.0;
   for (int k = 0; k < a.nb; ++k) tot += a.rhod[(int64_t)k * a.ld + c];
   bool fallback = !(tot > 0.0);
@@ -793,7 +811,7 @@ __global__ void k_surr_post(SArgs a) {
     for (int k = 0; k < a.nb; ++k) chk += ls[k] / S;
     if (!(fabs(chk - 1.0) <= a.resid_max)) fallback = true;
   }
-  const double dtl = a.dtc ? a.dtc[c] : a.dt;
+  const double dtl = cell_dt(a, c);
   a.nit[c] = 0;
   if (fallback) {
     a.fb[c] = 1;
@@ -850,6 +868,12 @@ struct shark_growth {
   float *dfeat = nullptr, *dy = nullptr;
   int* dfb = nullptr;
   double* dresid = nullptr;
+  // bound to a step context (shark_gpu_attach_growth): rhod, gas density and cs are the step's
+  // device arrays, dt is the step's device scalar, ghost cells are masked out
+  bool ext = false;
+  hipStream_t run_stream = nullptr;
+  const double* dt_dev = nullptr;
+  int* dmask = nullptr;
   double alpha = 0.0, zeta = 5e-17, mu_ions = 25.0, resid_max = 1e-6;
   float zmax = 8.0f;
   int64_t last_nfb = 0;
@@ -858,10 +882,11 @@ struct shark_growth {
 
 void shark_growth_free(shark_growth* g) {
   if (!g) return;
+  if (g->ext) g->drhod = g->drho = g->dcs = nullptr;  // the step context owns these
   void* ptrs[] = {g->dpairs, g->dredis, g->dmdust, g->dsdust, g->drhod, g->drho, g->dcs, g->drad,
                   g->denv, g->dts, g->dA, g->dF, g->dAh, g->dFh, g->dtg, g->dnit, g->dflags,
                   g->dGf32, g->dGbhi, g->dGblo, g->dzmean, g->dxion, g->ddtc, g->dnorm,
-                  g->dfeat, g->dy, g->dfb, g->dresid};
+                  g->dfeat, g->dy, g->dfb, g->dresid, g->dmask};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (g->hflags) (void)hipHostFree(g->hflags);
@@ -977,6 +1002,7 @@ int shark_growth_create(const shark_growth_params* p, int64_t ncells, shark_grow
     }                                             \
   } while (0)
   GC(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+  g->run_stream = g->stream;
   GC(hipEventCreate(&g->e0));
   GC(hipEventCreate(&g->e1));
   GC(hipMalloc(&g->dpairs, sizeof(PairC) * np));
@@ -1103,6 +1129,8 @@ static GArgs make_args(shark_growth* g, int mode, double dt) {
   a.Ah = g->dAh;
   a.Fh = g->dFh;
   a.dtc = g->use_dtc ? g->ddtc : nullptr;
+  a.dt_dev = g->dt_dev;
+  a.cmask = g->dmask;
   a.tg = g->dtg;
   a.nit = g->dnit;
   a.unfinished = g->dflags;
@@ -1151,7 +1179,7 @@ static int ensure_charge_buffers(shark_growth* g) {
 // pair prefactors for the chunk, then the bounded sub-step kernel relaunched until every active
 // cell of the chunk reached its interval
 static int plain_chunk(shark_growth* g, GArgs& a, int mode) {
-  hipStream_t s = g->stream;
+  hipStream_t s = g->run_stream;
   const int nblk = (a.nch + 255) / 256;
   hipLaunchKernelGGL(k_gpairs, dim3(nblk, a.npairs), dim3(256), 0, s, a);
   GCHK(hipGetLastError());
@@ -1192,6 +1220,8 @@ static SArgs make_sargs(shark_growth* g, double dt) {
   sa.zmax = g->zmax;
   sa.ratio_min = p.dust_ratio_min;
   sa.dtc = g->use_dtc ? g->ddtc : nullptr;
+  sa.dt_dev = g->dt_dev;
+  sa.cmask = g->dmask;
   sa.rhod = g->drhod;
   sa.rho_gas = g->drho;
   sa.radius = g->drad;
@@ -1210,7 +1240,7 @@ static SArgs make_sargs(shark_growth* g, double dt) {
 // fraction is an input), features, network, new state; then the plain FP64 coagulation and the
 // charge solve on the cells that fell back
 static int surrogate_run(shark_growth* g, double dt) {
-  hipStream_t s = g->stream;
+  hipStream_t s = g->run_stream;
   GArgs a = make_args(g, SHARK_GROWTH_SURROGATE, dt);
   QArgs q = make_qargs(g);
   SArgs sa = make_sargs(g, dt);
@@ -1259,7 +1289,7 @@ static int surrogate_run(shark_growth* g, double dt) {
   return 0;
 }
 
-int shark_growth_run(shark_growth* g, int mode, double dt) {
+static int run_impl(shark_growth* g, int mode, double dt) {
   if (!g) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: null handle");
   if (!g->uploaded) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: upload the state first");
   if (mode < SHARK_GROWTH_FP64 || mode > SHARK_GROWTH_SURROGATE)
@@ -1267,7 +1297,7 @@ int shark_growth_run(shark_growth* g, int mode, double dt) {
   if (!(dt >= 0.0)) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: dt must be >= 0");
   if (mode == SHARK_GROWTH_SURROGATE && !g->mlp)
     return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: no surrogate attached (shark_growth_set_surrogate)");
-  hipStream_t s = g->stream;
+  hipStream_t s = g->run_stream;
   GCHK(hipMemsetAsync(g->dtg, 0, sizeof(double) * g->ncells, s));
   GCHK(hipMemsetAsync(g->dnit, 0, sizeof(int) * g->ncells, s));
   GCHK(hipMemsetAsync(g->dflags, 0, sizeof(int) * 2, s));
@@ -1296,6 +1326,57 @@ int shark_growth_run(shark_growth* g, int mode, double dt) {
   g->last_ms = ms;
   return 0;
 }
+
+int shark_growth_run(shark_growth* g, int mode, double dt) {
+  if (g && g->ext)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_growth_run: this state is attached to a step context and grows inside its steps");
+  return run_impl(g, mode, dt);
+}
+
+}  // extern "C"
+
+namespace shark {
+// Binds a growth state to a step's device arrays: the dust-density planes of U (rhod, leading
+// dimension ncells), the gas density plane, cs, the per-column radius, the step's device dt, and a
+// mask of the active cells (ghost cells do not grow).
+int growth_bind(shark_growth* g, double* dU, double* dcs, const double* rc_cols, int nxm, int nym, int nx,
+                int ny, int ndim, const double* dt_dev, hipStream_t s) {
+  const int64_t nc = (int64_t)nxm * nym;
+  if (g->ncells != nc)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_gpu_attach_growth: the growth state must be created for the step's ncells (ghosts included)");
+  std::vector<double> rad(nc);
+  std::vector<int> mask(nc);
+  for (int64_t c = 0; c < nc; ++c) {
+    const int ix = (int)(c % nxm) + 1, iy = (ndim == 1) ? 1 : (int)(c / nxm) + 1;
+    rad[c] = rc_cols[ix - 1];
+    mask[c] = (ix >= 3 && ix <= nx + 2 && (ndim == 1 || (iy >= 3 && iy <= ny + 2))) ? 1 : 0;
+  }
+  if (!g->ext) {
+    (void)hipFree(g->drhod);
+    (void)hipFree(g->drho);
+    (void)hipFree(g->dcs);
+  }
+  g->drhod = dU + 5 * nc;
+  g->drho = dU;
+  g->dcs = dcs;
+  g->ext = true;
+  if (!g->dmask) GCHK(hipMalloc(&g->dmask, sizeof(int) * nc));
+  GCHK(hipMemcpy(g->drad, rad.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+  GCHK(hipMemcpy(g->dmask, mask.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+  g->dt_dev = dt_dev;
+  g->run_stream = s;
+  g->use_dtc = false;
+  g->uploaded = true;
+  return 0;
+}
+
+// One growth call of the bound state over the step's dt, on the step's stream.
+int growth_step(shark_growth* g, int mode) { return run_impl(g, mode, 0.0); }
+
+int growth_nbins(const shark_growth* g) { return g->p.nbins; }
+}  // namespace shark
+
+extern "C" {
 
 int shark_growth_set_cell_dt(shark_growth* g, const double* dt_cells) {
   if (!g) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_cell_dt: null handle");

@@ -12,6 +12,7 @@
 
 #include "../../include/shark_gpu.h"
 #include "growth.h"
+#include "../../include/shark_growth.h"
 #include "physics.h"
 #include "step_kernels.h"
 
@@ -48,8 +49,8 @@ int fail_msg(int code, const char* msg) {
                   __FILE__, __LINE__);                                              \
   } while (0)
 
-enum { KT_BC = 0, KT_HYDRO, KT_POST, KT_DT, KT_N };
-static const char* kKernelNames[KT_N] = {"bc", "hydro", "post", "dt"};
+enum { KT_BC = 0, KT_HYDRO, KT_POST, KT_DT, KT_GROWTH, KT_N };
+static const char* kKernelNames[KT_N] = {"bc", "hydro", "post", "dt", "growth"};
 
 struct shark_ctx {
   shark_cfg cfg;
@@ -67,9 +68,11 @@ struct shark_ctx {
   bool geom_set = false, uploaded = false;
   hipGraphExec_t gexec = nullptr;
   bool timing = false;
-  double kt_ms[KT_N] = {0, 0, 0, 0};
-  int64_t kt_n[KT_N] = {0, 0, 0, 0};
-  shark_growth* growth = nullptr;
+  double kt_ms[KT_N] = {0, 0, 0, 0, 0};
+  int64_t kt_n[KT_N] = {0, 0, 0, 0, 0};
+  shark_growth* growth = nullptr;   // attached growth state (dust_growth inside each step)
+  double* dRold = nullptr;          // dust densities before growth, for the velocity-keeping momenta
+  int growth_mode = 0;
 };
 
 static int nvar_of(const shark_cfg* c) { return 5 + 4 * c->ndust; }
@@ -305,9 +308,53 @@ static int build_graph(shark_ctx* c) {
   return SHARK_OK;
 }
 
+// Steps with dust growth between the drag and the in-loop hook (solve.f90:110-152): the post pass
+// runs in two parts around the growth call, whose sub-step loop is driven from the host, so these
+// steps are launched directly (no graph) and the run checks the device's active flag every step.
+static int run_steps_growth(shark_ctx* c, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    HIPCHK(hipMemcpyAsync(c->hsc, c->dsc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!c->hsc->active) break;
+    hipEvent_t e[KT_N + 2];
+    if (c->timing)
+      for (auto& x : e) HIPCHK(hipEventCreate(&x));
+    if (c->timing) HIPCHK(hipEventRecord(e[0], c->stream));
+    shark::launch_bc(c->a, c->stream);
+    if (c->timing) HIPCHK(hipEventRecord(e[1], c->stream));
+    shark::launch_hydro(c->a, c->stream);
+    if (c->timing) HIPCHK(hipEventRecord(e[2], c->stream));
+    shark::launch_post_part(c->a, 1, c->stream);
+    if (c->timing) HIPCHK(hipEventRecord(e[3], c->stream));
+    HIPCHK(hipGetLastError());
+    int rc = shark::growth_step(c->growth, c->growth_mode);
+    if (rc) return rc;
+    if (c->timing) HIPCHK(hipEventRecord(e[4], c->stream));
+    shark::launch_post_part(c->a, 2, c->stream);
+    if (c->timing) HIPCHK(hipEventRecord(e[5], c->stream));
+    shark::launch_finalize(c->a, c->stream);
+    if (c->timing) HIPCHK(hipEventRecord(e[6], c->stream));
+    HIPCHK(hipGetLastError());
+    if (c->timing) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      float t[6];
+      for (int k = 0; k < 6; ++k) HIPCHK(hipEventElapsedTime(&t[k], e[k], e[k + 1]));
+      c->kt_ms[KT_BC] += t[0];
+      c->kt_ms[KT_HYDRO] += t[1];
+      c->kt_ms[KT_POST] += t[2] + t[4];
+      c->kt_ms[KT_GROWTH] += t[3];
+      c->kt_ms[KT_DT] += t[5];
+      for (int k = 0; k < KT_N; ++k) c->kt_n[k] += 1;
+      for (auto& x : e) hipEventDestroy(x);
+    }
+  }
+  return SHARK_OK;
+}
+
 static int run_steps(shark_ctx* c, int64_t n, double tend) {
   shark::launch_arm(c->a, tend, c->stream);
   HIPCHK(hipGetLastError());
+  if (c->growth) return run_steps_growth(c, n);
   if (c->timing) {
     std::vector<hipEvent_t> ev((size_t)n * (KT_N + 1));
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
@@ -424,12 +471,39 @@ extern "C" int shark_gpu_kernel_time(shark_ctx* c, const char* name, double* avg
   return fail(SHARK_EINVAL, "unknown kernel name '%s'", name);
 }
 
+extern "C" int shark_gpu_attach_growth(shark_ctx* c, shark_growth* g, int mode) {
+  if (!c) return fail(SHARK_EINVAL, "null ctx");
+  if (!g) {
+    c->growth = nullptr;
+    return SHARK_OK;
+  }
+  if (!c->geom_set) return fail(SHARK_ESTATE, "set the geometry before attaching growth");
+  if (mode < 0 || mode > 4) return fail(SHARK_EINVAL, "growth mode %d out of range", mode);
+  if (shark::growth_nbins(g) != c->cfg.ndust)
+    return fail(SHARK_EINVAL, "growth bins (%d) must equal ndust (%d)", shark::growth_nbins(g), c->cfg.ndust);
+  std::vector<double> rc((size_t)c->a.nxm);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(rc.data(), c->dgeom + 5 * (size_t)c->a.nxm, rc.size() * 8, hipMemcpyDeviceToHost));
+  int r = shark::growth_bind(g, c->dU, c->dcs, rc.data(), c->a.nxm, c->a.nym, c->a.nx, c->a.ny, c->a.ndim,
+                             &c->dsc->dt, c->stream);
+  if (r) return r;
+  if (!c->dRold) {
+    hipError_t e = hipMalloc(&c->dRold, sizeof(double) * (size_t)c->cfg.ndust * c->a.nc);
+    if (e != hipSuccess) return fail(SHARK_ENOMEM, "growth scratch: %s", hipGetErrorString(e));
+  }
+  c->a.Rold = c->dRold;
+  c->growth = g;
+  c->growth_mode = mode;
+  return SHARK_OK;
+}
+
 extern "C" int shark_gpu_destroy(shark_ctx* c) {
   if (!c) return SHARK_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
-  if (c->growth) shark_growth_free(c->growth);
+  // an attached growth state belongs to the caller (shark_growth_destroy after this call)
   hipFree(c->dU); hipFree(c->dB); hipFree(c->dcs); hipFree(c->dgeom); hipFree(c->dsc);
+  if (c->dRold) hipFree(c->dRold);
   if (c->dsdust) hipFree(c->dsdust);
   if (c->dcond) hipFree(c->dcond);
   if (c->dfargo) hipFree(c->dfargo);

@@ -37,6 +37,7 @@ struct StepArgs {
   const double* sdust;  // (ndust, ncells) or nullptr when sdust_uniform
   const double* condinit;   // (nvar, ncells), disk in-loop hook
   const double* fargo_vk;   // fargo_velocity per column
+  double* Rold;             // (ndust, ncells) dust densities before growth (growth in the step)
   DevScalars* sc;
   int nx, ny, nd, nxm, nym, nc, nvar, ndim;
   int geom, solver, solverdust, slope, force_kick, force_kind, tstop_kind, inloop_kind;
@@ -55,6 +56,8 @@ struct StepArgs {
 void launch_bc(const StepArgs& a, hipStream_t s);
 void launch_hydro(const StepArgs& a, hipStream_t s);
 void launch_post(const StepArgs& a, hipStream_t s);
+// the post pass split around a growth step: part 1 remap + drag, part 2 in-loop hook + Courant
+void launch_post_part(const StepArgs& a, int part, hipStream_t s);
 void launch_courant_init(const StepArgs& a, hipStream_t s);
 void launch_finalize(const StepArgs& a, hipStream_t s);
 void launch_set_dt(const StepArgs& a, hipStream_t s);

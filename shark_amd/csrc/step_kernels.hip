@@ -608,7 +608,10 @@ __global__ __launch_bounds__(256) void k_post(StepArgs a) {
 // species spread over the NS = 256/TW thread slots of the block.  The drag sums run in
 // species order in one thread per (cell, component), so results equal k_post / the
 // reference bit for bit; loads, divisions and the Courant work are shared by all slots.
-template <int ND, bool FARGO, int NDIM, int TW>
+// PART 0: the whole post pass.  With a growth step between drag and the in-loop hook
+// (solve.f90:110-152) the pass is split: PART 1 remaps and drags (writes U, no Courant candidate),
+// PART 2 reloads U after growth, applies the in-loop hook and computes the Courant candidate.
+template <int ND, bool FARGO, int NDIM, int TW, int PART>
 __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
   if (!a.sc->active) return;
   constexpr int NV = 5 + 4 * ND;
@@ -646,7 +649,20 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
   const double dt = a.sc->dt;
 
   // ---- phase 1: remap (FARGO) or copy every variable of the segment into LDS
-  if (FARGO) {
+  if (PART == 2) {
+#pragma unroll
+    for (int v = s; v < NV; v += NS) un[v][c] = a.U[v * nc + cell];
+    __syncthreads();   // the planes below were loaded by other slots
+    // growth changed the dust densities only; each species keeps its velocity:
+    // momentum = (momentum / density before growth) * density after growth
+    for (int k = s; k < ND; k += NS) {
+      const double r0 = a.Rold[(size_t)k * nc + cell], r1 = un[5 + k][c];
+      un[5 + ND + k][c] = (un[5 + ND + k][c] / r0) * r1;
+      un[5 + 2 * ND + k][c] = (un[5 + 2 * ND + k][c] / r0) * r1;
+      un[5 + 3 * ND + k][c] = (un[5 + 3 * ND + k][c] / r0) * r1;
+    }
+    __syncthreads();
+  } else if (FARGO) {
     const double rp = (a.geom == 2) ? a.g.rc[ixs - 1] : 1.0;
     const double vk = a.fargo_vk[ixs - 1];
     const double dy = a.g.dx2[ixs - 1] * rp;
@@ -689,7 +705,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
 #pragma unroll
     for (int v = s; v < NV; v += NS) un[v][c] = a.B[v * nc + cell];
   }
-  const bool do_drag = ND > 0 && a.drag;
+  const bool do_drag = ND > 0 && a.drag && PART != 2;
   if (do_drag) {
     // stopping times of the step (compute_tstop on the start-of-step primitives); alpha_k and
     // alpha_k / (1 + alpha_k) are shared by the drag sums and the momentum update
@@ -749,7 +765,9 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
     __syncthreads();
   }
   // ---- phase 4: in-loop hook, per fluid
-  if (a.inloop_kind == 2) {
+  if (PART == 1) {
+    // growth runs next; the hook and the Courant candidate follow it in PART 2
+  } else if (a.inloop_kind == 2) {
     const double r = a.g.rc[ixs - 1];
     if (r < a.r_relax) {
       const double t_rel = a.n_rel * 2.0 * kPi / (sqrt(a.mstar / (r * r * r))) -
@@ -791,6 +809,11 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
   // ---- write the final state
   if (valid)
     for (int v = s; v < NV; v += NS) a.U[v * nc + cell] = un[v][c];
+  if (PART == 1) {
+    if (valid)
+      for (int k = s; k < ND; k += NS) a.Rold[(size_t)k * nc + cell] = un[5 + k][c];
+    return;
+  }
   // ---- phase 5: next step's Courant candidate (courant.f90:68-158), species split over slots
   {
     double f1, f2, f3;
@@ -956,19 +979,27 @@ void launch_hydro(const StepArgs& a, hipStream_t s) {
   if (a.nd > 0) hydro_launch_geom<false>(a, s);
 }
 
+template <int ND, int PART>
+static void post_nd_part(const StepArgs& a, hipStream_t s);
 template <int ND>
-static void post_nd(const StepArgs& a, hipStream_t s) {
+static void post_nd(const StepArgs& a, hipStream_t s) { post_nd_part<ND, 0>(a, s); }
+template <int ND>
+static void post1_nd(const StepArgs& a, hipStream_t s) { post_nd_part<ND, 1>(a, s); }
+template <int ND>
+static void post2_nd(const StepArgs& a, hipStream_t s) { post_nd_part<ND, 2>(a, s); }
+template <int ND, int PART>
+static void post_nd_part(const StepArgs& a, hipStream_t s) {
   // 32-cell row segments with 8 slots: 27 KB of LDS and <= 96 VGPRs, so five blocks share a CU
   // (measured on the 1024 x 2048 16-species disk: 1.45 ms against 1.79 ms for 64-cell segments);
   // 64 species (261 variables) take 16-cell segments with 16 slots to stay within LDS
   constexpr int TW = ND >= 32 ? 16 : 32;
   if (a.ndim == 1) {
     dim3 grid((a.nx + TW - 1) / TW);
-    hipLaunchKernelGGL((k_post2<ND, false, 1, TW>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_post2<ND, false, 1, TW, PART>), grid, dim3(256), 0, s, a);
   } else {
     dim3 grid((a.nx + TW - 1) / TW, a.ny);
-    if (a.fargo) hipLaunchKernelGGL((k_post2<ND, true, 2, TW>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_post2<ND, false, 2, TW>), grid, dim3(256), 0, s, a);
+    if (a.fargo && PART != 2) hipLaunchKernelGGL((k_post2<ND, true, 2, TW, PART>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_post2<ND, false, 2, TW, PART>), grid, dim3(256), 0, s, a);
   }
 }
 
@@ -992,6 +1023,10 @@ static void courant_nd(const StepArgs& a, hipStream_t s) {
   }
 
 void launch_post(const StepArgs& a, hipStream_t s) { SHARK_ND_SWITCH(post_nd) }
+void launch_post_part(const StepArgs& a, int part, hipStream_t s) {
+  if (part == 1) { SHARK_ND_SWITCH(post1_nd) }
+  else { SHARK_ND_SWITCH(post2_nd) }
+}
 void launch_courant_init(const StepArgs& a, hipStream_t s) {
   SHARK_ND_SWITCH(courant_nd)
   hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, s, a.sc);

@@ -26,7 +26,7 @@ EXPORTS = [
     "shark_growth_create", "shark_growth_run", "shark_growth_download", "shark_growth_upload",
     "shark_growth_destroy", "shark_growth_kernel_time", "shark_growth_charge",
     "shark_growth_set_cell_dt", "shark_growth_features", "shark_growth_set_surrogate",
-    "shark_growth_surrogate_stats",
+    "shark_growth_surrogate_stats", "shark_gpu_attach_growth",
     "shark_mlp_create", "shark_mlp_num_params", "shark_mlp_set_params", "shark_mlp_get_params",
     "shark_mlp_forward", "shark_mlp_grads", "shark_mlp_set_data", "shark_mlp_train",
     "shark_mlp_kernel_time", "shark_mlp_destroy",
@@ -105,6 +105,12 @@ class Step:
              for k, v in state.items()}
         check(self.L.shark_gpu_upload(self.h, _ptr(s["u"]), _ptr(s["cs"]), _ptr(s.get("sdust")),
                                       _ptr(s.get("condinit")), _ptr(s.get("fargo_velocity"))))
+
+    def attach_growth(self, growth, mode):
+        """Dust growth inside every step (shark_gpu_attach_growth); growth=None detaches."""
+        self.L.shark_gpu_attach_growth.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        self._growth = growth
+        check(self.L.shark_gpu_attach_growth(self.h, None if growth is None else growth.h, int(mode)))
 
     def step(self):
         dt = C.c_double()

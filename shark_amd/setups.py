@@ -173,7 +173,8 @@ def si_box(n=1024, nd=8, st_min=1e-3, st_max=1.0, eps_tot=0.01, hoverr=0.1, nois
 
 def disk(nr=1024, nphi=2048, nd=16, st_min=1e-4, st_max=1.0, eps_tot=0.01, hoverr=0.05,
          r_in=0.4, r_out=2.5, sigma_r0=1700.0, noise=1e-3, fargo=True, frame="reference",
-         seed=20261202, cfl=0.5, slope_type=1, gamma=1.666667, r_relax=0.3, n_rel=8.0):
+         seed=20261202, cfl=0.5, slope_type=1, gamma=1.666667, r_relax=0.3, n_rel=8.0,
+         sizes=None, weights=None):
     """Polar disk (disk_polar setup.f90:12-84, linear radial grid): Sigma ~ r^-1 (R_out_disk
     -> infinity), H/R = hoverr, Keplerian rotation about Mstar = 1, nd species whose Epstein
     Stokes number at r = 1 is log-spaced in [st_min, st_max], dust-to-gas eps_tot split equally,
@@ -185,7 +186,10 @@ def disk(nr=1024, nphi=2048, nd=16, st_min=1e-4, st_max=1.0, eps_tot=0.01, hover
     frame="residual": every fluid stores its velocity relative to Omega*r and the FARGO shift is
     +Omega*r.
     r_relax <= r_in keeps the in-loop relaxation of setup.f90:228 off (its default 3.0 would
-    relax every cell toward the initial state)."""
+    relax every cell toward the initial state).
+    sizes / weights: explicit grain sizes (code units) and dust-mass fractions per species (the
+    growth bins' sizes with an MRN split, for runs with dust growth) instead of the Stokes grid
+    and the equal split."""
     unit_l = AU
     unit_dcol = MSUN / AU ** 2
     unit_d = MSUN / AU ** 3
@@ -206,7 +210,8 @@ def disk(nr=1024, nphi=2048, nd=16, st_min=1e-4, st_max=1.0, eps_tot=0.01, hover
     # grain size giving Stokes number st at r=1 (Epstein law, setup.f90:60 / :436)
     st = np.logspace(math.log10(st_min), math.log10(st_max), nd) if nd > 1 else np.array([st_min])
     rho1 = sig0 / (math.sqrt(2.0 * PI) * hoverr)
-    sizes = st / (math.sqrt(PI * gamma / 8.0) * rhograin / (rho1 * hoverr))
+    if sizes is None:
+        sizes = st / (math.sqrt(PI * gamma / 8.0) * rhograin / (rho1 * hoverr))
     eta_drift = np.sqrt(1.0 - hoverr ** 2 * 2.0) - 1.0
     nv = 5 + 4 * nd
     q = np.zeros((nv, nc))
@@ -224,7 +229,7 @@ def disk(nr=1024, nphi=2048, nd=16, st_min=1e-4, st_max=1.0, eps_tot=0.01, hover
     for k in range(nd):
         ts = math.sqrt(PI * gamma / 8.0) * rhograin * sizes[k] / (rho / (math.sqrt(2.0 * PI) * H) * cs0)
         stk = ts * om
-        q[5 + k] = eps_tot / nd * rho
+        q[5 + k] = eps_tot / nd * rho if weights is None else (eps_tot * weights[k]) * rho
         q[5 + nd + k] = eta_drift * om * rr / (stk + 1.0 / stk)
         q[5 + 2 * nd + k] = (vkep + vfarg) if frame == "reference" else 0.0
         sd[k] = sizes[k]
@@ -236,6 +241,19 @@ def disk(nr=1024, nphi=2048, nd=16, st_min=1e-4, st_max=1.0, eps_tot=0.01, hover
                      smooth_r=r_in, hoverr=hoverr, rhograin=rhograin, disk_bc_mode=0)
     fv = vfarg if fargo else None
     return cfg, g, dict(u=u, cs=cs0.copy(), sdust=sd, condinit=u.copy(), fargo_velocity=fv)
+
+
+def disk_growth(nr=1024, nphi=2048, nbins=64, **kw):
+    """The polar disk with nbins dust species that are the growth bins (1e-4..10 cm, MRN mass
+    split), for steps with dust growth; returns (cfg, geom, state, growth params)."""
+    from . import growth as G
+    p = G.disk_params(nbins=nbins, max_substeps=1000000)
+    zeta = (p.smax / p.smin) ** (1.0 / nbins)
+    k = np.arange(nbins)
+    sz = np.sqrt((p.smin * zeta ** k / p.unit_l) * (p.smin * zeta ** (k + 1) / p.unit_l))
+    w = sz ** 0.5
+    cfg, geom, state = disk(nr=nr, nphi=nphi, nd=nbins, sizes=sz, weights=w / w.sum(), **kw)
+    return cfg, geom, state, p
 
 
 WORKLOADS = {
