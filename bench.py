@@ -4,7 +4,13 @@
 Default workload: the 2D polar disk with FARGO, Nr=1024 x Nphi=2048, 16 dust species
 (17 fluids), per-cell 17-fluid implicit drag — the configuration the 60%-of-HBM and 20x-CPU
 targets are stated on.  `--workload si` runs the 1024^2 8-species shearing box, `dustywave`
-the 512-cell 1D wave.
+the 512-cell 1D wave, `c5` the 1024 x 2048 disk with 64 dust bins and the learned
+growth-and-charging step inside every step (velocity-keeping density update between drag and the
+in-loop hook).
+
+Launch shapes: before the warm-up the library's autotuner (shark_gpu_autotune) times the
+candidate hydro strip heights, post segment widths and block orders on the workload's own state
+and keeps the fastest; the state is restored, and the chosen shape is reported under "tune".
 
 Timed region: W warm-up steps, then exactly K steps with the step kernels bracketed by HIP
 events on the library's stream (per-kernel average durations feed the roofline), host
@@ -36,7 +42,9 @@ WORKLOADS = {
     "disk": ("disk_1024x2048_n16_fargo", lambda s: s.disk()),
     "si": ("si_1024x1024_n8", lambda s: s.si_box()),
     "dustywave": ("dustywave_512_n1", lambda s: s.dustywave()),
+    "c5": ("disk_1024x2048_b64_learned_growth", lambda s: s.disk_growth(nr=1024, nphi=2048, nbins=64)),
 }
+MLP_DIMS = (69, 256, 256, 256, 65)   # the learned step's layer widths
 
 
 def copy_probe(nbytes=1 << 30, reps=10):
@@ -169,6 +177,7 @@ def main():
                     "of per-kernel event-bracketed launches")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-growth", action="store_true", help="skip the growth timing block")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -183,11 +192,22 @@ def main():
     from shark_amd.cfg import algorithmic_bytes
 
     name, make = WORKLOADS[args.workload]
-    cfg, geom, state = make(setups)
+    learned = args.workload == "c5"
+    made = make(setups)
+    cfg, geom, state = made[:3]
     cells = cfg.nx * (cfg.ny if cfg.ny > 1 else 1)
     fluids = 1 + cfg.ndust
     zones_per_step = cells * fluids
     g = gpu.Step(cfg, geom, state)
+    tune = None if args.no_tune else g.autotune(reps=5)
+    gr = net = None
+    if learned:
+        from shark_amd import growth as G
+        from shark_amd import surrogate as S
+        net, norm, _ = S.load(os.path.join(ROOT, "shark_amd", "data", "surrogate_disk64.npz"))
+        gr = G.Growth(made[3], (cfg.nx + 4) * (cfg.ny + 4))
+        gr.set_surrogate(net, norm)
+        g.attach_growth(gr, G.SURROGATE)
     g.run(args.warmup)
 
     g.set_timing(not args.graph)
@@ -195,7 +215,12 @@ def main():
     n, _ = g.run(args.steps)
     wall = time.perf_counter() - t0
     assert n == args.steps, (n, args.steps)
-    kt = {k: g.kernel_time(k)[0] for k in ("bc", "hydro", "post", "dt")}
+    kt = {k: g.kernel_time(k)[0] for k in ("bc", "hydro", "post", "dt") + (("growth",) if learned else ())}
+    learned_info = None
+    if learned:
+        nfb, raw = gr.surrogate_stats()
+        learned_info = {"growth_ms_per_step": kt["growth"], "fallback_cells_last_step": int(nfb),
+                        "raw_mass_residual_mean": raw}
 
     nv = 5 + 4 * cfg.ndust
     bytes_hydro = cells * 8 * (2 * nv + 1)            # read u_prim + cs, write u_prim
@@ -209,6 +234,20 @@ def main():
         dom_ms = kt[dom]
         dom_bytes = bytes_hydro if dom == "hydro" else bytes_post
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": profiled_traffic(args.workload, dom),
+            "traffic_source": "profiles/disk_1024x2048_n16_hbm_traffic.md",
+            "algorithmic_bytes": dom_bytes, "avg_ms": dom_ms}
+    if learned and kt["growth"] > dom_ms:
+        # the learned step dominates: rate its network products against the dense BF16 peak
+        # (2 flops per multiply-add over every layer, every cell incl. ghosts)
+        ncell_all = (cfg.nx + 4) * (cfg.ny + 4)
+        flops = 2.0 * ncell_all * sum(a * b for a, b in zip(MLP_DIMS[:-1], MLP_DIMS[1:]))
+        tf = flops / (kt["growth"] * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": "growth", "achieved": tf, "peak": 2500.0,
+                "unit": "TFLOP/s", "frac": tf / 2500.0, "traffic": None,
+                "algorithmic_flops": flops, "avg_ms": kt["growth"]}
     out = {
         "metric": METRIC,
         "value": zones_per_step * args.steps / wall,
@@ -226,21 +265,29 @@ def main():
                    "fluids": fluids, "geom": "polar" if cfg.geom == 2 else "cartesian",
                    "fargo": bool(cfg.fargo), "drag": bool(cfg.drag), "solver": "hll",
                    "solverdust": "huang_bai", "slope_type": cfg.slope_type},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(args.workload, dom),
-                     "traffic_source": "profiles/disk_1024x2048_n16_hbm_traffic.md",
-                     "algorithmic_bytes": dom_bytes, "avg_ms": dom_ms},
+        "roofline": roof,
         "kernels_ms": kt,
+        "tune": tune,
         "step_roofline": {"algorithmic_bytes": step_alg,
                           "achieved_GBs": step_alg / (wall / args.steps) / 1e9,
                           "frac": step_alg / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},
     }
+    if learned:
+        out["learned_growth"] = learned_info
+        out["dtype"] = "f64 step; learned growth bf16 x bf16 -> f32"
     g.close()
-    if not args.no_cpu_baseline:
+    if gr is not None:
+        gr.close()
+        net.close()
+    if learned:
+        # the oracle's FP64 coagulation needs up to ~1700 sub-steps per cell at this dt: a CPU
+        # sample of the coupled step would not finish in a bounded time
+        out["cpu_baseline"] = None
+        out["cpu_baseline_note"] = "not run for the coupled learned-growth workload"
+    elif not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, geom, state, zones_per_step, args.cpu_steps)
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
-    if not args.no_growth:
+    if not args.no_growth and not learned:
         out["growth"] = growth_block()
     print(json.dumps(out))
 

@@ -43,8 +43,12 @@ int64_t shark_mlp_num_params(const shark_mlp* m);
 /* FP32 master parameters in and out (host arrays of shark_mlp_num_params floats). */
 int shark_mlp_set_params(shark_mlp* m, const float* params);
 int shark_mlp_get_params(shark_mlp* m, float* params);
-/* y (n x n_out) = net(x (n x n_in)); n <= max_batch; host arrays. */
+/* y (n x n_out) = net(x (n x n_in)); n <= max_batch; host arrays.  Runs the fused inference
+ * kernel (all layers in one launch, activations in LDS) when every width is <= 256. */
 int shark_mlp_forward(shark_mlp* m, const float* x, int64_t n, float* y);
+/* The same forward as the training path computes it (one product kernel per layer).  Its outputs
+ * equal shark_mlp_forward's bit for bit. */
+int shark_mlp_forward_layered(shark_mlp* m, const float* x, int64_t n, float* y);
 /* Loss and its gradient w.r.t. every parameter on one batch (n <= max_batch). */
 int shark_mlp_grads(shark_mlp* m, const float* x, const float* t, int64_t n, float* loss,
                     float* grads);

@@ -710,6 +710,64 @@ __global__ void k_gcharge(QArgs a) {
   a.xion[c] = a.zeta / kin;
 }
 
+// Gas temperature of each chunk cell (the T of k_genv), for the learned step's chunks.
+__global__ void k_senv(GArgs a, double* Tout) {
+  const int cc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cc >= a.nch) return;
+  const double csp = a.cs[a.c0 + cc] * a.unit_v;
+  Tout[cc] = csp * csp * (a.mu_gas * kMH) / (a.gamma * kKB);
+}
+
+// Ion fraction of a cell for the learned step's features.  The same charge equation as k_gcharge
+// with its terms regrouped (k_e = C_e exp(A y), k_i = C_i (1 - A y), A = e^2 / (s k T):
+// f = Q (1/k_i - 1/k_e) - y, f' = Q (A/k_e + A C_i/k_i^2) - 1, one exp per iteration), each bin's
+// Newton started from the previous bin's root scaled by the size ratio (the surface potential
+// A y changes slowly from bin to bin), to |dy| <= 1e-12 |y|; a bin whose warm start does not
+// converge in 30 iterations is solved again from y = -1.  The features are rounded to FP32, far
+// above the difference to the reference-order solve of k_gcharge.
+__global__ void k_fcharge(QArgs a) {
+  const int cc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cc >= a.nch) return;
+  const int64_t c = a.c0 + cc;
+  const double T = a.env[cc];
+  const double H = a.hoverr * a.radius[c];
+  const double rho = a.rho_gas[c] / (a.s2pi * H) * a.unit_d;
+  const double kT = kKB * T, e2 = kEel * kEel;
+  const double ve = sqrt(8.0 * kT / (kPi * kMel)), vi = sqrt(8.0 * kT / (kPi * (a.mu_ions * kMH)));
+  const double base = a.mu_gas * kMH * a.zeta / rho;
+  const double ee = a.electrons ? 1.0 : 0.0;
+  double kin = 0.0, y = -1.0, s_prev = 0.0;
+  for (int k = 0; k < a.nb; ++k) {
+    const double rd = a.rhod[(int64_t)k * a.ld + c] / (a.s2pi * H) * a.unit_d;
+    const double m_d = a.mdust[k] * a.unit_m, s_d = a.sdust[k] * a.unit_l;
+    const double A = e2 / (s_d * kT);
+    const double Ce = kPi * s_d * s_d * ve, Ci = kPi * s_d * s_d * vi;
+    const double X = m_d * rho / (rd * a.mu_gas * kMH);
+    const double Q = base * X * X;
+    const double y_warm = k > 0 ? y * (s_d / s_prev) : -1.0;
+    bool ok = false;
+    for (int pass = 0; pass < 2 && !ok; ++pass) {
+      y = pass == 0 ? y_warm : -1.0;
+      const int lim = pass == 0 ? 30 : a.maxiter;
+      for (int it = 0; it < lim; ++it) {
+        const double yke = exp(-A * y) / Ce, yki = 1.0 / (Ci * (1.0 - A * y));
+        const double f = Q * (yki - ee * yke) - y;
+        const double fp = Q * (ee * A * yke + Ci * A * (yki * yki)) - 1.0;
+        const double d = f / fp;
+        y -= d;
+        if (!(fabs(y) < 1e300)) break;
+        if (fabs(d) <= 1e-12 * fabs(y)) {
+          ok = true;
+          break;
+        }
+      }
+    }
+    s_prev = s_d;
+    kin += Ci * (1.0 - A * y) * (rd / m_d);
+  }
+  a.xion[c] = a.zeta / kin;
+}
+
 // ------------------------------------------------------------------ learned growth step
 // Features of a cell (n_in = nbins + 5): log10 of each bin's midplane dust density (g cm^-3),
 // log10 T, log10 midplane gas density, log10 alpha, log10 ion fraction, log10 dt; standardised
@@ -729,8 +787,9 @@ struct SArgs {
   const double* radius;
   const double* env;
   const double* xion;
-  float* feat;
-  const float* y;
+  float* feat;           // features, [j * fld + cc] (feature-major)
+  const float* y;        // network outputs, [k * fld + cc]
+  int64_t fld;
   const float* norm;  // in_mean[nin], in_std[nin], out_mean[nout], out_std[nout]; null: raw
   double* zmean;
   int* fb;            // [ncells] 1 = plain fallback
@@ -746,13 +805,13 @@ __global__ void k_surr_feat(SArgs a) {
   const int64_t c = a.c0 + cc;
   const double H = a.hoverr * a.radius[c];
   const double conv = a.unit_d / (a.s2pi * H);
-  float* f = a.feat + (int64_t)cc * a.nin;
+  float* f = a.feat + cc;
   const float* mu = a.norm;
   const float* sd = a.norm ? a.norm + a.nin : nullptr;
   auto put = [&](int j, double v) {
     float x = (float)v;
     if (mu) x = (x - mu[j]) / sd[j];
-    f[j] = x;
+    f[(int64_t)j * a.fld] = x;
   };
   for (int k = 0; k < a.nb; ++k) put(k, log10(fmax(a.rhod[(int64_t)k * a.ld + c], 1e-300) * conv));
   const double dtl = cell_dt(a, c);
@@ -764,51 +823,53 @@ __global__ void k_surr_feat(SArgs a) {
 }
 
 // New state from the network output.  The network predicts the change of the log of each bin's
-// share of the cell's dust mass; the new shares are the softmax of (old log-shares + change), so
-// they are positive and sum to one by construction and the cell's dust mass is kept in FP64.  A
-// cell falls back to the plain coagulation when an output is not finite, when the mass residual
-// |sum_k share_k - 1| after that normalisation exceeds resid_max (only possible through a
+// share of the cell's dust mass; the new shares are the softmax of (old log-shares + change),
+// i.e. w_k = rho_k exp(change_k) normalised by sum_j w_j, so they are positive and sum to one by
+// construction and the cell's dust mass is kept in FP64.  A cell falls back to the plain
+// coagulation when an output is not finite, when an old density is not positive, when the mass
+// residual |sum_k share_k - 1| after that normalisation exceeds resid_max (only possible through a
 // non-finite or degenerate prediction), or when an input lies more than zmax standard deviations
-// from the training set's mean (outside the envelope the network was trained on).
-__global__ void k_surr_post(SArgs a) {
+// from the training set's mean (outside the envelope the network was trained on).  The per-bin
+// values live in registers: every loop over bins is unrolled over MAXB with a bound check.
+__global__ __launch_bounds__(256) void k_surr_post(SArgs a) {
   const int cc = blockIdx.x * blockDim.x + threadIdx.x;
   if (cc >= a.nch) return;
   const int64_t c = a.c0 + cc;
-  const float* y = a.y + (int64_t)cc * a.nout;
-  const float* f = a.feat + (int64_t)cc * a.nin;
+  const float* y = a.y + cc;
+  const float* f = a.feat + cc;
+  const int64_t ld = a.fld;
   const float* om = a.norm + 2 * a.nin;
   const float* os = om + a.nout;
+  const int nb = a.nb;
   if (!(cell_dt(a, c) > 0.0)) {   // no growth interval (a masked ghost cell): unchanged
     a.fb[c] = 0;
     a.tg[c] = 0.0;
     a.nit[c] = 0;
     return;
   }
-  double tot = 0.0;
-  for (int k = 0; k < a.nb; ++k) tot += a.rhod[(int64_t)k * a.ld + c];
-  bool fallback = !(tot > 0.0);
-  for (int j = 0; j < a.nin && !fallback; ++j)
-    if (!(fabsf(f[j]) <= a.zmax)) fallback = true;
-  double lmax = -1e300, raw = 0.0;
-  double ls[MAXB];
-  if (!fallback) {
-    for (int k = 0; k < a.nb; ++k) {
-      const double sh = a.rhod[(int64_t)k * a.ld + c] / tot;
-      const double d = (double)(y[k] * os[k] + om[k]);
-      ls[k] = log(sh) + d;
-      raw += sh * exp(d);
-      if (!(ls[k] == ls[k]) || !(fabs(ls[k]) < 1e30)) fallback = true;
-      lmax = fmax(lmax, ls[k]);
+  // w_k = rho_k exp(change_k), kept in FP32 registers: the network's prediction carries ~1e-3,
+  // and the cell's mass is restored in FP64 from the final densities
+  float w[MAXB];
+  double tot = 0.0, W = 0.0;
+  bool fallback = false;
+#pragma unroll
+  for (int k = 0; k < MAXB; ++k) {
+    if (k < nb) {
+      const double r = a.rhod[(int64_t)k * a.ld + c];
+      tot += r;
+      if (!(r > 0.0)) fallback = true;
+      w[k] = (float)(r * exp((double)(y[k * ld] * os[k] + om[k])));
+      W += (double)w[k];
     }
   }
-  double S = 0.0;
+  for (int j = 0; j < a.nin && !fallback; ++j)
+    if (!(fabsf(f[j * ld]) <= a.zmax)) fallback = true;
+  if (!(W > 0.0) || !(W < INFINITY)) fallback = true;
   if (!fallback) {
-    for (int k = 0; k < a.nb; ++k) {
-      ls[k] = exp(ls[k] - lmax);
-      S += ls[k];
-    }
     double chk = 0.0;
-    for (int k = 0; k < a.nb; ++k) chk += ls[k] / S;
+#pragma unroll
+    for (int k = 0; k < MAXB; ++k)
+      if (k < nb) chk += (double)w[k] / W;
     if (!(fabs(chk - 1.0) <= a.resid_max)) fallback = true;
   }
   const double dtl = cell_dt(a, c);
@@ -821,17 +882,19 @@ __global__ void k_surr_post(SArgs a) {
   }
   a.fb[c] = 0;
   a.tg[c] = dtl;
-  atomicAdd(a.resid_sum, fabs(raw - 1.0));
+  atomicAdd(a.resid_sum, fabs(W / tot - 1.0));   // sum_k share_k,old exp(change_k) - 1
   // new densities share * rho_tot, floored as the plain step floors, rescaled to the cell's mass
   const double floor_v = a.rho_gas[c] * a.ratio_min;
+  const double sc0 = tot / W;
   double snew = 0.0;
-  for (int k = 0; k < a.nb; ++k) {
-    ls[k] = fmax(ls[k] / S * tot, floor_v);
-    snew += ls[k];
-  }
+#pragma unroll
+  for (int k = 0; k < MAXB; ++k)
+    if (k < nb) snew += fmax((double)w[k] * sc0, floor_v);
   const double sc = tot / snew;
-  for (int k = 0; k < a.nb; ++k) a.rhod[(int64_t)k * a.ld + c] = ls[k] * sc;
-  a.zmean[c] = -pow(10.0, (double)(y[a.nb] * os[a.nb] + om[a.nb]));
+#pragma unroll
+  for (int k = 0; k < MAXB; ++k)
+    if (k < nb) a.rhod[(int64_t)k * a.ld + c] = fmax((double)w[k] * sc0, floor_v) * sc;
+  a.zmean[c] = -pow(10.0, (double)(y[nb * ld] * os[nb] + om[nb]));
 }
 
 }  // namespace
@@ -854,6 +917,10 @@ struct shark_growth {
   int* dnit = nullptr;
   int* dflags = nullptr;  // [0] unfinished, [1] bad
   int* hflags = nullptr;  // pinned
+  int* dcfb = nullptr;    // learned step: fallback count per chunk
+  int* hcfb = nullptr;    // pinned mirror
+  int64_t sch = 0;        // learned step: cells per chunk (features, outputs, temperatures)
+  double* dsT = nullptr;  // learned step: gas temperature per chunk cell
   float* dGf32 = nullptr;
   u32x4 *dGbhi = nullptr, *dGblo = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -886,10 +953,12 @@ void shark_growth_free(shark_growth* g) {
   void* ptrs[] = {g->dpairs, g->dredis, g->dmdust, g->dsdust, g->drhod, g->drho, g->dcs, g->drad,
                   g->denv, g->dts, g->dA, g->dF, g->dAh, g->dFh, g->dtg, g->dnit, g->dflags,
                   g->dGf32, g->dGbhi, g->dGblo, g->dzmean, g->dxion, g->ddtc, g->dnorm,
-                  g->dfeat, g->dy, g->dfb, g->dresid, g->dmask};
+                  g->dfeat, g->dy, g->dfb, g->dresid, g->dmask, g->dsT};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (g->hflags) (void)hipHostFree(g->hflags);
+  if (g->dcfb) (void)hipFree(g->dcfb);
+  if (g->hcfb) (void)hipHostFree(g->hcfb);
   if (g->e0) (void)hipEventDestroy(g->e0);
   if (g->e1) (void)hipEventDestroy(g->e1);
   if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -1236,9 +1305,10 @@ static SArgs make_sargs(shark_growth* g, double dt) {
   return sa;
 }
 
-// learned step over all cells, in chunks of the network's batch: environment, charge (the ion
-// fraction is an input), features, network, new state; then the plain FP64 coagulation and the
-// charge solve on the cells that fell back
+// learned step over all cells in chunks of up to 4M cells: temperature, ion fraction (an input),
+// features, network (one fused kernel), new state.  One host synchronisation reads every chunk's
+// fallback count; the cells that fell back then run the plain FP64 coagulation (in the plain
+// step's own chunks) and the charge solve.
 static int surrogate_run(shark_growth* g, double dt) {
   hipStream_t s = g->run_stream;
   GArgs a = make_args(g, SHARK_GROWTH_SURROGATE, dt);
@@ -1247,42 +1317,56 @@ static int surrogate_run(shark_growth* g, double dt) {
   sa.norm = g->dnorm;
   sa.feat = g->dfeat;
   sa.y = g->dy;
-  const int64_t B = std::min<int64_t>(shark::mlp_max_batch(g->mlp), g->nch_max);
+  sa.env = g->dsT;
+  sa.fld = g->sch;
+  q.env = g->dsT;
+  const int64_t B = g->sch;
+  const int nchunk = (int)((g->ncells + B - 1) / B);
   GCHK(hipMemsetAsync(g->dresid, 0, sizeof(double), s));
-  int64_t nfb_total = 0;
-  for (int64_t c0 = 0; c0 < g->ncells; c0 += B) {
-    const int64_t rem = g->ncells - c0;
+  GCHK(hipMemsetAsync(g->dcfb, 0, sizeof(int) * nchunk, s));
+  for (int i = 0; i < nchunk; ++i) {
+    const int64_t c0 = (int64_t)i * B, rem = g->ncells - c0;
     const int nch = (int)(rem < B ? rem : B);
     a.c0 = q.c0 = sa.c0 = (int)c0;
     a.nch = q.nch = sa.nch = nch;
+    sa.nfb = g->dcfb + i;
     const int nblk = (nch + 255) / 256;
-    hipLaunchKernelGGL(k_genv, dim3(nblk), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_senv, dim3(nblk), dim3(256), 0, s, a, g->dsT);
     GCHK(hipGetLastError());
-    q.mask = nullptr;
-    hipLaunchKernelGGL(k_gcharge, dim3(nblk), dim3(256), 0, s, q);
+    hipLaunchKernelGGL(k_fcharge, dim3(nblk), dim3(256), 0, s, q);
     GCHK(hipGetLastError());
     hipLaunchKernelGGL(k_surr_feat, dim3(nblk), dim3(256), 0, s, sa);
     GCHK(hipGetLastError());
-    int rc = shark::mlp_forward_device(g->mlp, g->dfeat, nch, g->dy, s);
+    int rc = shark::mlp_infer_device(g->mlp, g->dfeat, nch, g->dy, s, B, B);
     if (rc) return rc;
-    GCHK(hipMemsetAsync(g->dflags, 0, sizeof(int) * 2, s));
     hipLaunchKernelGGL(k_surr_post, dim3(nblk), dim3(256), 0, s, sa);
     GCHK(hipGetLastError());
-    GCHK(hipMemcpyAsync(g->hflags, g->dflags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
-    GCHK(hipStreamSynchronize(s));
-    const int nfb = g->hflags[0];
-    nfb_total += nfb;
-    if (nfb > 0) {
-      rc = plain_chunk(g, a, SHARK_GROWTH_FP64);
+  }
+  double rs = 0.0;
+  GCHK(hipMemcpyAsync(g->hcfb, g->dcfb, sizeof(int) * nchunk, hipMemcpyDeviceToHost, s));
+  GCHK(hipMemcpyAsync(&rs, g->dresid, sizeof(double), hipMemcpyDeviceToHost, s));
+  GCHK(hipStreamSynchronize(s));
+  int64_t nfb_total = 0;
+  QArgs qf = make_qargs(g);   // exact charge solve on the fallback cells (plain environment)
+  qf.mask = g->dfb;
+  for (int i = 0; i < nchunk; ++i) {
+    if (g->hcfb[i] == 0) continue;
+    nfb_total += g->hcfb[i];
+    const int64_t end = std::min<int64_t>(g->ncells, (int64_t)(i + 1) * B);
+    for (int64_t c0 = (int64_t)i * B; c0 < end; c0 += g->nch_max) {
+      const int nch = (int)std::min<int64_t>(g->nch_max, end - c0);
+      a.c0 = qf.c0 = (int)c0;
+      a.nch = qf.nch = nch;
+      a.env = g->denv;
+      const int nblk = (nch + 255) / 256;
+      hipLaunchKernelGGL(k_genv, dim3(nblk), dim3(256), 0, s, a);
+      GCHK(hipGetLastError());
+      int rc = plain_chunk(g, a, SHARK_GROWTH_FP64);
       if (rc) return rc;
-      q.mask = g->dfb;
-      hipLaunchKernelGGL(k_gcharge, dim3(nblk), dim3(256), 0, s, q);
+      hipLaunchKernelGGL(k_gcharge, dim3(nblk), dim3(256), 0, s, qf);
       GCHK(hipGetLastError());
     }
   }
-  double rs = 0.0;
-  GCHK(hipMemcpyAsync(&rs, g->dresid, sizeof(double), hipMemcpyDeviceToHost, s));
-  GCHK(hipStreamSynchronize(s));
   g->last_nfb = nfb_total;
   const int64_t nacc = g->ncells - nfb_total;
   g->last_resid = nacc > 0 ? rs / (double)nacc : 0.0;
@@ -1402,14 +1486,23 @@ int shark_growth_set_surrogate(shark_growth* g, shark_mlp* m, const float* norm,
   const int nb = g->p.nbins;
   if (shark::mlp_n_in(m) != nb + 5 || shark::mlp_n_out(m) != nb + 1)
     return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate: the network needs nbins+5 inputs and nbins+1 outputs");
+  if (!shark::mlp_fused_ok(m))
+    return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate: the network needs <= 8 layers, widths <= 256 and hidden widths that are multiples of 32");
   if (!norm) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate: null normalisation");
   if (!(resid_max > 0.0) || !(zeta > 0.0) || !(mu_ions > 0.0))
     return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate: resid_max, zeta and mu_ions must be positive");
   const int nn = 2 * (nb + 5) + 2 * (nb + 1);
   for (int i = 0; i < nn; ++i)
     if (!(norm[i] == norm[i])) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate: non-finite normalisation");
-  const int64_t B = std::min<int64_t>(shark::mlp_max_batch(m), g->nch_max);
+  const int64_t B = std::min<int64_t>(g->ncells, (int64_t)1 << 22);   // learned step's chunk
+  const int64_t nchunk = (g->ncells + B - 1) / B;
   if (!g->dnorm) GCHK(hipMalloc(&g->dnorm, sizeof(float) * 2 * (MAXB + 5) + sizeof(float) * 2 * (MAXB + 1)));
+  if (!g->dcfb) {
+    GCHK(hipMalloc(&g->dcfb, sizeof(int) * nchunk));
+    GCHK(hipHostMalloc(&g->hcfb, sizeof(int) * nchunk, hipHostMallocDefault));
+  }
+  if (!g->dsT) GCHK(hipMalloc(&g->dsT, sizeof(double) * B));
+  g->sch = B;
   if (g->dfeat) (void)hipFree(g->dfeat);
   if (g->dy) (void)hipFree(g->dy);
   g->dfeat = nullptr;
@@ -1443,7 +1536,9 @@ int shark_growth_features(shark_growth* g, double dt, float* feat) {
   float* dfe = nullptr;
   GCHK(hipMalloc(&dfe, sizeof(float) * B * nin));
   sa.feat = dfe;
+  sa.fld = B;
   sa.norm = nullptr;
+  std::vector<float> tmp((size_t)B * nin);
   for (int64_t c0 = 0; c0 < g->ncells; c0 += B) {
     const int64_t rem = g->ncells - c0;
     const int nch = (int)(rem < B ? rem : B);
@@ -1451,15 +1546,19 @@ int shark_growth_features(shark_growth* g, double dt, float* feat) {
     a.nch = q.nch = sa.nch = nch;
     const int nblk = (nch + 255) / 256;
     hipLaunchKernelGGL(k_genv, dim3(nblk), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_gcharge, dim3(nblk), dim3(256), 0, s, q);
+    hipLaunchKernelGGL(k_fcharge, dim3(nblk), dim3(256), 0, s, q);
     hipLaunchKernelGGL(k_surr_feat, dim3(nblk), dim3(256), 0, s, sa);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(feat + c0 * nin, dfe, sizeof(float) * nch * nin, hipMemcpyDeviceToHost, s);
+    // feature-major on the device, row-major for the caller
+    if (e == hipSuccess) e = hipMemcpy2DAsync(tmp.data(), sizeof(float) * nch, dfe, sizeof(float) * B,
+                                              sizeof(float) * nch, nin, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
       (void)hipFree(dfe);
       GCHK(e);
     }
+    for (int j = 0; j < nin; ++j)
+      for (int i = 0; i < nch; ++i) feat[(c0 + i) * nin + j] = tmp[(size_t)j * nch + i];
   }
   GCHK(hipFree(dfe));
   return 0;

@@ -13,6 +13,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <random>
 #include <vector>
 
@@ -49,7 +50,12 @@ __device__ __forceinline__ unsigned short to_bf16(float x) {
   return (unsigned short)(u >> 16);
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+// SiLU from the hardware exp2 and reciprocal (v_exp_f32, v_rcp_f32: about 1 ulp each), shared
+// by the training forward and the fused inference so the two stay bit-identical; the activations
+// are rounded to BF16 right after, far coarser than these approximations.
+__device__ __forceinline__ float silu(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-z * 1.44269504088896341f));
+}
 __device__ __forceinline__ float dsilu(float z) {
   const float s = 1.0f / (1.0f + expf(-z));
   return s * (1.0f + z * (1.0f - s));
@@ -179,6 +185,169 @@ __global__ void k_gather(const float* src, const int* idx, int64_t rows, int col
   dst[i] = src[(int64_t)idx[r] * cols + c];
 }
 
+// ------------------------------------------------------------------ fused inference
+// The whole network in one kernel for widths <= 256: a workgroup of 256 lanes takes FM = 128
+// rows (samples), keeps their activations in LDS as BF16 between layers, and reads the weights
+// from a copy packed in the operand order of v_mfma_f32_32x32x16_bf16 (one 1 KB fragment per wave
+// and k-step, fetched FPF k-steps ahead through L2).  Each product is computed transposed,
+// out^T = W . act^T: the weights are the A operand and the activations the B operand, so a lane's
+// accumulators hold four consecutive features of one sample, which go to LDS as one 8-byte store
+// and to a feature-major output as coalesced rows.  Wave w computes the 32-feature blocks w, w+4
+// of a layer for all 128 samples.  Operand values, k order and epilogue equal k_gemm's, so the
+// outputs equal the layer-by-layer forward bit for bit.
+constexpr int FM = 128;      // samples per workgroup
+constexpr int FW = 256;      // widest layer of the fused path
+constexpr int FLD = FW + 8;  // LDS row pitch in BF16 elements (16-byte rows, 4-bank shift per row)
+constexpr int FMAXL = 8;     // layers
+constexpr int FPF = 4;       // k-steps of weight fragments in flight per wave
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+struct InferArgs {
+  const float* x;
+  float* y;
+  int64_t n;
+  int64_t xr, xc;            // element strides of x: per sample, per feature
+  int64_t yr, yc;            // element strides of y
+  int n_in, n_out, L;
+  const unsigned short* wf;  // packed BF16 weights
+  const float* p;            // FP32 parameters (biases)
+  int64_t foff[FMAXL];       // fragment offset of each layer (BF16 elements)
+  int64_t boff[FMAXL];       // bias offset of each layer (floats)
+  int kpad[FMAXL];           // input width rounded up to 16
+  int nbn[FMAXL];            // output width in 32-feature blocks
+  int dout[FMAXL];
+};
+
+// fragment (nb, ks) of layer l, lane j, element e  <-  W[n = 32 nb + j % 32][k = 16 ks + 8 (j / 32) + e]
+__global__ void k_pack_frag(const float* W, int din, int dout, int nks, int nbn, unsigned short* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)nbn * nks * 64 * 8;
+  if (i >= total) return;
+  const int e = (int)(i & 7), j = (int)((i >> 3) & 63);
+  const int64_t f = i >> 9;
+  const int ks = (int)(f % nks), nb = (int)(f / nks);
+  const int n = 32 * nb + (j & 31), k = 16 * ks + 8 * (j >> 5) + e;
+  out[i] = (n < dout && k < din) ? to_bf16(W[(int64_t)n * din + k]) : (unsigned short)0;
+}
+
+__device__ __forceinline__ unsigned int pack_bf16x2(float a, float b) {
+  const f32x2 v = {a, b};
+  return __builtin_bit_cast(unsigned int, __builtin_convertvector(v, bf16x2));
+}
+
+// One layer for all FM samples of the workgroup.  Hidden layers (every width a multiple of 32):
+// bias + SiLU, BF16 back into act once every wave has read the layer's input.  Last layer: bias,
+// FP32 out to y.
+template <bool LAST>
+__device__ __forceinline__ void infer_layer(const InferArgs& g, int l, unsigned short (*act)[FLD], int64_t r0,
+                                            int lane, int w) {
+  const int nks = g.kpad[l] >> 4, nbn = g.nbn[l];
+  const float* bias = g.p + g.boff[l];
+  const int dout = g.dout[l];
+  const u32x4* wf = reinterpret_cast<const u32x4*>(g.wf + g.foff[l]);
+  // two passes, feature blocks w and w + 4; a hidden layer's outputs wait in registers as packed
+  // BF16 until every wave has finished reading the layer's input
+  uint2 pk[2][4][4];
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
+    const int nb = w + 4 * pass;
+    if (nb >= nbn) break;
+    f32x16 acc[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[mb][r] = 0.f;
+    const u32x4* wp = wf + (int64_t)nb * nks * 64 + lane;
+    u32x4 b[FPF];
+#pragma unroll
+    for (int u = 0; u < FPF; ++u)
+      if (u < nks) b[u] = wp[u * 64];
+    for (int ks0 = 0; ks0 < nks; ks0 += FPF) {
+#pragma unroll
+      for (int u = 0; u < FPF; ++u) {
+        const int ks = ks0 + u;
+        if (ks < nks) {
+          const u32x4 c = b[u];
+          if (ks + FPF < nks) b[u] = wp[(ks + FPF) * 64];
+#pragma unroll
+          for (int mb = 0; mb < 4; ++mb) {
+            const u32x4 a = *reinterpret_cast<const u32x4*>(&act[32 * mb + (lane & 31)][16 * ks + 8 * (lane >> 5)]);
+            acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, c),
+                                                              __builtin_bit_cast(bf16x8, a), acc[mb], 0, 0, 0);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      // accumulators 4q..4q+3: features n0..n0+3 of sample 32 mb + lane % 32
+      const int n0 = 32 * nb + 8 * q + 4 * (lane >> 5);
+      if constexpr (!LAST) {
+        const float4 bv = *reinterpret_cast<const float4*>(bias + n0);
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+          float v0 = acc[mb][4 * q], v1 = acc[mb][4 * q + 1], v2 = acc[mb][4 * q + 2], v3 = acc[mb][4 * q + 3];
+          v0 += bv.x;
+          v1 += bv.y;
+          v2 += bv.z;
+          v3 += bv.w;
+          pk[pass][mb][q] = uint2{pack_bf16x2(silu(v0), silu(v1)), pack_bf16x2(silu(v2), silu(v3))};
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (n0 + i >= dout) break;
+          const float bv = bias[n0 + i];
+          float* yc = g.y + (int64_t)(n0 + i) * g.yc;
+#pragma unroll
+          for (int mb = 0; mb < 4; ++mb) {
+            const int64_t row = r0 + 32 * mb + (lane & 31);
+            float v = acc[mb][4 * q + i];
+            v += bv;
+            if (row < g.n) yc[row * g.yr] = v;
+          }
+        }
+      }
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();  // every wave has read this layer's input rows
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+      const int nb = w + 4 * pass;
+      if (nb >= nbn) break;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb)
+          *reinterpret_cast<uint2*>(&act[32 * mb + (lane & 31)][32 * nb + 8 * q + 4 * (lane >> 5)]) = pk[pass][mb][q];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
+  __shared__ __attribute__((aligned(16))) unsigned short act[FM][FLD];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t r0 = (int64_t)blockIdx.x * FM;
+  {
+    const int kp = g.kpad[0];
+    const bool feat_major = g.xc != 1;   // consecutive lanes take consecutive samples
+    for (int e = tid; e < FM * kp; e += 256) {
+      int m, k;
+      if (feat_major) { m = e % FM; k = e / FM; } else { m = e / kp; k = e - m * kp; }
+      const int64_t row = r0 + m;
+      const float v = (row < g.n && k < g.n_in) ? g.x[row * g.xr + k * g.xc] : 0.f;
+      act[m][k] = to_bf16(v);
+    }
+  }
+  __syncthreads();
+  for (int l = 0; l < g.L - 1; ++l) infer_layer<false>(g, l, act, r0, lane, w);
+  infer_layer<true>(g, g.L - 1, act, r0, lane, w);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ host object
@@ -204,6 +373,8 @@ struct shark_mlp {
   int64_t ndata = 0;
   int64_t adam_t = 0;
   double last_ms = 0.0;
+  unsigned short* dwf = nullptr;       // BF16 weights packed for k_mlp_infer
+  bool wf_valid = false;               // false after any change of the parameters
 };
 
 static int mfail(const char* msg) { return shark::fail_msg(1, msg); }
@@ -230,6 +401,7 @@ static void free_mlp(shark_mlp* m) {
   for (float* b : m->H)
     if (b) (void)hipFree(b);
   if (m->didx) (void)hipFree(m->didx);
+  if (m->dwf) (void)hipFree(m->dwf);
   if (m->hloss) (void)hipHostFree(m->hloss);
   if (m->e0) (void)hipEventDestroy(m->e0);
   if (m->e1) (void)hipEventDestroy(m->e1);
@@ -294,6 +466,86 @@ int mlp_forward_device(shark_mlp* m, const float* x, int64_t n, float* y, hipStr
   return 0;
 }
 
+// the fused kernel: at most FMAXL layers, every width <= 256, hidden widths multiples of 32 (so
+// a hidden layer fills whole feature blocks and the next layer's padded input is exact)
+static bool fused_ok(const shark_mlp* m) {
+  if (m->L > FMAXL) return false;
+  for (int l = 0; l < m->L; ++l) {
+    if (m->din[l] > FW || m->dout[l] > FW) return false;
+    if (l < m->L - 1 && m->dout[l] % 32 != 0) return false;
+  }
+  return true;
+}
+
+static int pack_weights(shark_mlp* m, hipStream_t s) {
+  int64_t total = 0;
+  for (int l = 0; l < m->L; ++l) total += (int64_t)((m->dout[l] + 31) / 32) * ((m->din[l] + 15) / 16) * 512;
+  if (!m->dwf) MCHK(hipMalloc(&m->dwf, sizeof(unsigned short) * total));
+  int64_t off = 0;
+  for (int l = 0; l < m->L; ++l) {
+    const int nks = (m->din[l] + 15) / 16, nbn = (m->dout[l] + 31) / 32;
+    const int64_t cnt = (int64_t)nbn * nks * 512;
+    hipLaunchKernelGGL(k_pack_frag, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, m->dp + m->woff[l],
+                       m->din[l], m->dout[l], nks, nbn, m->dwf + off);
+    MCHK(hipGetLastError());
+    off += cnt;
+  }
+  m->wf_valid = true;
+  return 0;
+}
+
+// Inference of n rows (device pointers, any n): the fused kernel when every width is <= 256,
+// else the layer-by-layer forward in chunks of max_batch rows (row-major data only).
+// ldx / ldy > 0: x / y are feature-major, element (row, feature) at [feature * ld + row].
+int mlp_infer_device(shark_mlp* m, const float* x, int64_t n, float* y, hipStream_t s, int64_t ldx,
+                     int64_t ldy) {
+  if (n <= 0) return 0;
+  if (!fused_ok(m)) {
+    if (ldx > 0 || ldy > 0) return mfail("mlp inference: feature-major data needs every width <= 256");
+    for (int64_t r = 0; r < n; r += m->cfg.max_batch) {
+      const int64_t k = std::min<int64_t>(m->cfg.max_batch, n - r);
+      int rc = mlp_forward_device(m, x + r * m->cfg.n_in, k, y + r * m->cfg.n_out, s);
+      if (rc) return rc;
+    }
+    return 0;
+  }
+  if (!m->wf_valid) {
+    // the parameters are written on the MLP's own stream (training, set_params)
+    MCHK(hipStreamSynchronize(m->stream));
+    int rc = pack_weights(m, s);
+    if (rc) return rc;
+  }
+  InferArgs g;
+  memset(&g, 0, sizeof(g));
+  g.x = x;
+  g.y = y;
+  g.n = n;
+  g.xr = ldx > 0 ? 1 : m->cfg.n_in;
+  g.xc = ldx > 0 ? ldx : 1;
+  g.yr = ldy > 0 ? 1 : m->cfg.n_out;
+  g.yc = ldy > 0 ? ldy : 1;
+  g.n_in = m->cfg.n_in;
+  g.n_out = m->cfg.n_out;
+  g.L = m->L;
+  g.wf = m->dwf;
+  g.p = m->dp;
+  int64_t off = 0;
+  for (int l = 0; l < m->L; ++l) {
+    g.kpad[l] = ((m->din[l] + 15) / 16) * 16;
+    g.nbn[l] = (m->dout[l] + 31) / 32;
+    g.dout[l] = m->dout[l];
+    g.foff[l] = off;
+    g.boff[l] = m->boff[l];
+    off += (int64_t)g.nbn[l] * (g.kpad[l] / 16) * 512;
+  }
+  const int64_t nblk = (n + FM - 1) / FM;
+  if (nblk > 0x7fffffff) return mfail("mlp inference: too many rows");
+  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)nblk), dim3(256), 0, s, g);
+  MCHK(hipGetLastError());
+  return 0;
+}
+
+bool mlp_fused_ok(const shark_mlp* m) { return fused_ok(m); }
 int64_t mlp_max_batch(const shark_mlp* m) { return m->cfg.max_batch; }
 int mlp_n_in(const shark_mlp* m) { return m->cfg.n_in; }
 int mlp_n_out(const shark_mlp* m) { return m->cfg.n_out; }
@@ -429,6 +681,7 @@ int64_t shark_mlp_num_params(const shark_mlp* m) { return m ? m->np : -1; }
 
 int shark_mlp_set_params(shark_mlp* m, const float* params) {
   if (!m || !params) return mfail("shark_mlp_set_params: null argument");
+  m->wf_valid = false;
   MCHK(hipMemcpy(m->dp, params, sizeof(float) * m->np, hipMemcpyHostToDevice));
   MCHK(hipMemset(m->dm, 0, sizeof(float) * m->np));
   MCHK(hipMemset(m->dv, 0, sizeof(float) * m->np));
@@ -443,13 +696,13 @@ int shark_mlp_get_params(shark_mlp* m, float* params) {
   return 0;
 }
 
-int shark_mlp_forward(shark_mlp* m, const float* x, int64_t n, float* y) {
+static int forward_host(shark_mlp* m, const float* x, int64_t n, float* y, bool fused) {
   if (!m || !x || !y) return mfail("shark_mlp_forward: null argument");
   if (n < 1 || n > m->cfg.max_batch) return mfail("shark_mlp_forward: n must lie in [1, max_batch]");
   hipStream_t s = m->stream;
   MCHK(hipMemcpyAsync(m->dx, x, sizeof(float) * n * m->cfg.n_in, hipMemcpyHostToDevice, s));
   MCHK(hipEventRecord(m->e0, s));
-  int rc = shark::mlp_forward_device(m, m->dx, n, m->dy, s);
+  int rc = fused ? shark::mlp_infer_device(m, m->dx, n, m->dy, s) : shark::mlp_forward_device(m, m->dx, n, m->dy, s);
   if (rc) return rc;
   MCHK(hipEventRecord(m->e1, s));
   MCHK(hipMemcpyAsync(y, m->dy, sizeof(float) * n * m->cfg.n_out, hipMemcpyDeviceToHost, s));
@@ -458,6 +711,11 @@ int shark_mlp_forward(shark_mlp* m, const float* x, int64_t n, float* y) {
   MCHK(hipEventElapsedTime(&ms, m->e0, m->e1));
   m->last_ms = ms;
   return 0;
+}
+
+int shark_mlp_forward(shark_mlp* m, const float* x, int64_t n, float* y) { return forward_host(m, x, n, y, true); }
+int shark_mlp_forward_layered(shark_mlp* m, const float* x, int64_t n, float* y) {
+  return forward_host(m, x, n, y, false);
 }
 
 int shark_mlp_grads(shark_mlp* m, const float* x, const float* t, int64_t n, float* loss, float* grads) {
@@ -498,6 +756,7 @@ int shark_mlp_train(shark_mlp* m, int epochs, int batch, float lr, uint64_t seed
   if (batch < 1 || batch > m->cfg.max_batch) return mfail("shark_mlp_train: batch must lie in [1, max_batch]");
   if (epochs < 0) return mfail("shark_mlp_train: epochs must be >= 0");
   hipStream_t s = m->stream;
+  m->wf_valid = false;   // Adam rewrites the parameters
   std::mt19937_64 rng(seed);
   std::vector<int> perm(m->ndata);
   const int ni = m->cfg.n_in, no = m->cfg.n_out;

@@ -3,6 +3,7 @@
 // bracketed by HIP events when timing is on).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -147,10 +148,12 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   a.rhograin = cfg->rhograin; a.mstar = cfg->mstar; a.r_relax = cfg->r_relax; a.n_rel = cfg->n_rel;
   a.smooth_r = cfg->smooth_r;
   for (int k = 0; k < SHARK_MAX_DUST; ++k) a.K[k] = cfg->K[k];
-  // rows per hydro work item (reserved[0] overrides; the autotuner sweeps it)
-  a.strip_h = (int)(cfg->reserved[0] > 0 ? cfg->reserved[0] : 32);
+  // launch shapes (shark_gpu_set_tune / shark_gpu_autotune change them)
+  a.strip_h = 32;
   a.ngroups = (a.nx + 59) / 60;
   a.nstrips = a.ndim == 2 ? (a.ny + a.strip_h - 1) / a.strip_h : 1;
+  a.post_tw = shark::post_tw_default(a.nd);
+  a.xcd_order = 1;
 
   const size_t nbytes = (size_t)a.nc * a.nvar * sizeof(double);
   hipError_t e = hipSuccess;
@@ -164,11 +167,11 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
     shark_gpu_destroy(c);
     return rc;
   }
-  hipMemsetAsync(c->dU, 0, nbytes, c->stream);
-  hipMemsetAsync(c->dB, 0, nbytes, c->stream);
+  (void)hipMemsetAsync(c->dU, 0, nbytes, c->stream);
+  (void)hipMemsetAsync(c->dB, 0, nbytes, c->stream);
   memset(c->hsc, 0, sizeof(DevScalars));
   c->hsc->dtbits = 0;
-  hipMemcpyAsync(c->dsc, c->hsc, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream);
+  (void)hipMemcpyAsync(c->dsc, c->hsc, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream);
   a.U = c->dU; a.B = c->dB; a.cs = c->dcs; a.sc = c->dsc;
   const double* gp = c->dgeom;
   a.g.dx1 = gp; a.g.dx2 = gp + a.nxm; a.g.surf1 = gp + 2 * a.nxm; a.g.surf2 = gp + 3 * a.nxm;
@@ -220,7 +223,7 @@ extern "C" int shark_gpu_set_geometry(shark_ctx* c, const double* dx, const doub
 
 static void drop_graph(shark_ctx* c) {
   if (c->gexec) {
-    hipGraphExecDestroy(c->gexec);
+    (void)hipGraphExecDestroy(c->gexec);
     c->gexec = nullptr;
   }
 }
@@ -300,7 +303,7 @@ static int build_graph(shark_ctx* c) {
   enqueue_step(c);
   HIPCHK(hipStreamEndCapture(c->stream, &g));
   hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
     c->gexec = nullptr;
     return fail(SHARK_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
@@ -345,7 +348,7 @@ static int run_steps_growth(shark_ctx* c, int64_t n) {
       c->kt_ms[KT_GROWTH] += t[3];
       c->kt_ms[KT_DT] += t[5];
       for (int k = 0; k < KT_N; ++k) c->kt_n[k] += 1;
-      for (auto& x : e) hipEventDestroy(x);
+      for (auto& x : e) (void)hipEventDestroy(x);
     }
   }
   return SHARK_OK;
@@ -356,10 +359,12 @@ static int run_steps(shark_ctx* c, int64_t n, double tend) {
   HIPCHK(hipGetLastError());
   if (c->growth) return run_steps_growth(c, n);
   if (c->timing) {
-    std::vector<hipEvent_t> ev((size_t)n * (KT_N + 1));
+    // bc, hydro, post, dt: four intervals between five events per step (no growth here)
+    constexpr int NK = KT_DT + 1;
+    std::vector<hipEvent_t> ev((size_t)n * (NK + 1));
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     for (int64_t i = 0; i < n; ++i) {
-      hipEvent_t* e = &ev[(size_t)i * (KT_N + 1)];
+      hipEvent_t* e = &ev[(size_t)i * (NK + 1)];
       HIPCHK(hipEventRecord(e[0], c->stream));
       shark::launch_bc(c->a, c->stream);
       HIPCHK(hipEventRecord(e[1], c->stream));
@@ -373,15 +378,15 @@ static int run_steps(shark_ctx* c, int64_t n, double tend) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int64_t i = 0; i < n; ++i) {
-      hipEvent_t* e = &ev[(size_t)i * (KT_N + 1)];
-      for (int k = 0; k < KT_N; ++k) {
+      hipEvent_t* e = &ev[(size_t)i * (NK + 1)];
+      for (int k = 0; k < NK; ++k) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, e[k], e[k + 1]));
         c->kt_ms[k] += ms;
         c->kt_n[k] += 1;
       }
     }
-    for (auto& e : ev) hipEventDestroy(e);
+    for (auto& e : ev) (void)hipEventDestroy(e);
     return SHARK_OK;
   }
   if (!c->gexec) {
@@ -497,18 +502,174 @@ extern "C" int shark_gpu_attach_growth(shark_ctx* c, shark_growth* g, int mode) 
   return SHARK_OK;
 }
 
+// ------------------------------------------------------------------ launch shapes and the tuner
+static void apply_shape(shark_ctx* c, int strip_h, int tw, int xcd) {
+  StepArgs& a = c->a;
+  a.strip_h = strip_h;
+  a.nstrips = a.ndim == 2 ? (a.ny + strip_h - 1) / strip_h : 1;
+  a.post_tw = tw;
+  a.xcd_order = xcd;
+  drop_graph(c);   // the captured step holds the old shape
+}
+
+extern "C" int shark_gpu_get_tune(shark_ctx* c, shark_tune* t) {
+  if (!c || !t) return fail(SHARK_EINVAL, "null argument");
+  memset(t, 0, sizeof(*t));
+  t->strip_h = c->a.strip_h;
+  t->post_tw = c->a.post_tw;
+  t->xcd_order = c->a.xcd_order;
+  return SHARK_OK;
+}
+
+extern "C" int shark_gpu_set_tune(shark_ctx* c, const shark_tune* t) {
+  if (!c || !t) return fail(SHARK_EINVAL, "null argument");
+  if (t->strip_h < 1 || t->strip_h > 4096) return fail(SHARK_EINVAL, "strip_h %d outside 1..4096", t->strip_h);
+  if (!shark::post_tw_ok(c->a.nd, t->post_tw))
+    return fail(SHARK_EINVAL, "post_tw %d not built for %d species", t->post_tw, c->a.nd);
+  if (t->xcd_order != 0 && t->xcd_order != 1) return fail(SHARK_EINVAL, "xcd_order must be 0 or 1");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  apply_shape(c, t->strip_h, t->post_tw, t->xcd_order);
+  return SHARK_OK;
+}
+
+namespace {
+struct TuneSave {
+  double* U = nullptr;
+  double* B = nullptr;
+  DevScalars* sc = nullptr;
+  ~TuneSave() {
+    if (U) (void)hipFree(U);
+    if (B) (void)hipFree(B);
+    if (sc) (void)hipFree(sc);
+  }
+};
+
+float median(std::vector<float>& v) {
+  std::sort(v.begin(), v.end());
+  return v[v.size() / 2];
+}
+}  // namespace
+
+static int restore_state(shark_ctx* c, const TuneSave& sv, size_t nbytes) {
+  HIPCHK(hipMemcpyAsync(c->dU, sv.U, nbytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dB, sv.B, nbytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dsc, sv.sc, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream));
+  return SHARK_OK;
+}
+
+// Median hydro and post times (ms) of `reps` steps of the current shape, each run from the saved
+// state after one untimed step.
+static int time_shape(shark_ctx* c, const TuneSave& sv, size_t nbytes, int reps, double* hyd, double* post) {
+  int rc = restore_state(c, sv, nbytes);
+  if (rc) return rc;
+  shark::launch_arm(c->a, INFINITY, c->stream);
+  std::vector<hipEvent_t> ev((size_t)3 * (reps + 1));
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  for (int i = 0; i <= reps; ++i) {
+    hipEvent_t* e = &ev[(size_t)3 * i];
+    shark::launch_bc(c->a, c->stream);
+    HIPCHK(hipEventRecord(e[0], c->stream));
+    shark::launch_hydro(c->a, c->stream);
+    HIPCHK(hipEventRecord(e[1], c->stream));
+    shark::launch_post(c->a, c->stream);
+    HIPCHK(hipEventRecord(e[2], c->stream));
+    shark::launch_finalize(c->a, c->stream);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<float> th, tp;
+  for (int i = 1; i <= reps; ++i) {   // step 0 is the untimed warm-up
+    hipEvent_t* e = &ev[(size_t)3 * i];
+    float a = 0.f, b = 0.f;
+    HIPCHK(hipEventElapsedTime(&a, e[0], e[1]));
+    HIPCHK(hipEventElapsedTime(&b, e[1], e[2]));
+    th.push_back(a);
+    tp.push_back(b);
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *hyd = median(th);
+  *post = median(tp);
+  return SHARK_OK;
+}
+
+extern "C" int shark_gpu_autotune(shark_ctx* c, int reps, shark_tune* chosen) {
+  if (!c) return fail(SHARK_EINVAL, "null ctx");
+  if (!c->uploaded) return fail(SHARK_ESTATE, "upload before tuning");
+  if (reps < 1 || reps > 1000) return fail(SHARK_EINVAL, "reps %d outside 1..1000", reps);
+  StepArgs& a = c->a;
+  const size_t nbytes = (size_t)a.nc * a.nvar * sizeof(double);
+  TuneSave sv;
+  hipError_t e;
+  if ((e = hipMalloc(&sv.U, nbytes)) != hipSuccess || (e = hipMalloc(&sv.B, nbytes)) != hipSuccess ||
+      (e = hipMalloc(&sv.sc, sizeof(DevScalars))) != hipSuccess)
+    return fail(SHARK_ENOMEM, "autotune state copy: %s", hipGetErrorString(e));
+  HIPCHK(hipMemcpyAsync(sv.U, c->dU, nbytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(sv.B, c->dB, nbytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(sv.sc, c->dsc, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream));
+
+  int best_h = a.strip_h, best_tw = a.post_tw, best_x = a.xcd_order;
+  double th = 0, tp = 0, bh = 0, bp = 0;
+  int rc = 0;
+  // 1. strip height by hydro time (2D only: a 1D run has one strip)
+  if (a.ndim == 2) {
+    static const int kStrip[] = {4, 8, 16, 24, 32, 48, 64, 96, 128, 256};
+    double bt = INFINITY;
+    for (int h : kStrip) {
+      if (h > a.ny && h != kStrip[0]) break;
+      apply_shape(c, h, best_tw, best_x);
+      if ((rc = time_shape(c, sv, nbytes, reps, &th, &tp))) break;
+      if (th < bt) { bt = th; best_h = h; }
+    }
+  }
+  // 2. post segment width by post time
+  if (!rc) {
+    int tws[4];
+    const int ntw = shark::post_tw_candidates(a.nd, tws);
+    double bt = INFINITY;
+    for (int k = 0; k < ntw && !rc; ++k) {
+      apply_shape(c, best_h, tws[k], best_x);
+      if ((rc = time_shape(c, sv, nbytes, reps, &th, &tp))) break;
+      if (tp < bt) { bt = tp; best_tw = tws[k]; }
+    }
+  }
+  // 3. XCD order by hydro + post time
+  if (!rc) {
+    double bt = INFINITY;
+    for (int x = 1; x >= 0 && !rc; --x) {
+      apply_shape(c, best_h, best_tw, x);
+      if ((rc = time_shape(c, sv, nbytes, reps, &th, &tp))) break;
+      if (th + tp < bt) { bt = th + tp; best_x = x; bh = th; bp = tp; }
+    }
+  }
+  // the run continues from the state it had before tuning, with the chosen shape
+  const int rr = restore_state(c, sv, nbytes);
+  apply_shape(c, best_h, best_tw, best_x);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (rc) return rc;
+  if (rr) return rr;
+  if (chosen) {
+    memset(chosen, 0, sizeof(*chosen));
+    chosen->strip_h = best_h;
+    chosen->post_tw = best_tw;
+    chosen->xcd_order = best_x;
+    chosen->hydro_ms = bh;
+    chosen->post_ms = bp;
+  }
+  return SHARK_OK;
+}
+
 extern "C" int shark_gpu_destroy(shark_ctx* c) {
   if (!c) return SHARK_OK;
-  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
   // an attached growth state belongs to the caller (shark_growth_destroy after this call)
-  hipFree(c->dU); hipFree(c->dB); hipFree(c->dcs); hipFree(c->dgeom); hipFree(c->dsc);
-  if (c->dRold) hipFree(c->dRold);
-  if (c->dsdust) hipFree(c->dsdust);
-  if (c->dcond) hipFree(c->dcond);
-  if (c->dfargo) hipFree(c->dfargo);
-  if (c->hsc) hipHostFree(c->hsc);
-  if (c->stream) hipStreamDestroy(c->stream);
+  (void)hipFree(c->dU); (void)hipFree(c->dB); (void)hipFree(c->dcs); (void)hipFree(c->dgeom); (void)hipFree(c->dsc);
+  if (c->dRold) (void)hipFree(c->dRold);
+  if (c->dsdust) (void)hipFree(c->dsdust);
+  if (c->dcond) (void)hipFree(c->dcond);
+  if (c->dfargo) (void)hipFree(c->dfargo);
+  if (c->hsc) (void)hipHostFree(c->hsc);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return SHARK_OK;
 }

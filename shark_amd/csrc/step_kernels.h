@@ -44,6 +44,8 @@ struct StepArgs {
   int bc_kind, drag, back_reaction, fargo, disk_bc_mode, sdust_uniform;
   int strip_h;          // rows per hydro work item
   int ngroups, nstrips; // hydro decomposition
+  int post_tw;          // cells per post row segment (post_tw_ok)
+  int xcd_order;        // 1: XCD-aware block order in the hydro and post launches
   double gamma, cfl, smallr, dust_ratio_min;
   double q_shear, omega_shear, box_l, rad0, eta_stream, rho_init, hoverr, rhograin;
   double mstar, r_relax, n_rel, smooth_r;
@@ -52,6 +54,11 @@ struct StepArgs {
   double eps_k1;        // sqrt(pi gamma / 8) * rhograin: the Epstein prefactor (disk tstop)
   double s2pi;          // sqrt(2 pi)
 };
+
+// launch shapes: every shape gives the same results bit for bit, only the speed differs
+bool post_tw_ok(int nd, int tw);
+int post_tw_default(int nd);
+int post_tw_candidates(int nd, int* out);   // returns the count (<= 4)
 
 void launch_bc(const StepArgs& a, hipStream_t s);
 void launch_hydro(const StepArgs& a, hipStream_t s);

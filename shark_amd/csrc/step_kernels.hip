@@ -386,7 +386,7 @@ __global__ __launch_bounds__(256, (GAS ? 2 : 1)) void k_hydro(StepArgs a) {
   // shared by neighbouring strips of a column group are read through one L2
   unsigned b = blockIdx.x;
   const unsigned nb = gridDim.x;
-  if ((nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+  if (a.xcd_order && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
   const long item = (long)b * 4 + wave;
   const long G = a.ngroups, S = a.nstrips, F = GAS ? 1 : a.nd;
   if (item >= G * S * F) return;
@@ -631,7 +631,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
   // swizzle gives each XCD a contiguous band of rows, so the FARGO source rows a block reads were
   // loaded into the same XCD's L2 by its neighbours in the band
   int bx = blockIdx.x, by = blockIdx.y;
-  if (NDIM == 2) {
+  if (NDIM == 2 && a.xcd_order) {
     const unsigned nbx = gridDim.x, nb = nbx * gridDim.y;
     if ((nb & 7u) == 0) {
       const unsigned b = blockIdx.x + blockIdx.y * nbx;
@@ -987,12 +987,33 @@ template <int ND>
 static void post1_nd(const StepArgs& a, hipStream_t s) { post_nd_part<ND, 1>(a, s); }
 template <int ND>
 static void post2_nd(const StepArgs& a, hipStream_t s) { post_nd_part<ND, 2>(a, s); }
+// Row-segment widths of the post pass.  Up to 16 species: 16 or 32 cells (32 cells with 8 slots
+// take 27 KB of LDS and <= 96 VGPRs at 16 species, so five blocks share a CU); 64 species
+// (261 variables): 8 or 16 cells, which keep the segment's LDS under 64 KB.
+template <int ND>
+struct PostTW {
+  static constexpr int A = ND >= 32 ? 8 : 16;
+  static constexpr int B = ND >= 32 ? 16 : 32;
+};
+bool post_tw_ok(int nd, int tw) {
+  return nd >= 32 ? (tw == 8 || tw == 16) : (tw == 16 || tw == 32);
+}
+int post_tw_default(int nd) { return nd >= 32 ? 16 : 32; }
+int post_tw_candidates(int nd, int* out) {
+  out[0] = nd >= 32 ? 8 : 16;
+  out[1] = nd >= 32 ? 16 : 32;
+  return 2;
+}
+
+template <int ND, int PART, int TW>
+static void post_nd_tw(const StepArgs& a, hipStream_t s);
 template <int ND, int PART>
 static void post_nd_part(const StepArgs& a, hipStream_t s) {
-  // 32-cell row segments with 8 slots: 27 KB of LDS and <= 96 VGPRs, so five blocks share a CU
-  // (measured on the 1024 x 2048 16-species disk: 1.45 ms against 1.79 ms for 64-cell segments);
-  // 64 species (261 variables) take 16-cell segments with 16 slots to stay within LDS
-  constexpr int TW = ND >= 32 ? 16 : 32;
+  if (a.post_tw == PostTW<ND>::A) post_nd_tw<ND, PART, PostTW<ND>::A>(a, s);
+  else post_nd_tw<ND, PART, PostTW<ND>::B>(a, s);
+}
+template <int ND, int PART, int TW>
+static void post_nd_tw(const StepArgs& a, hipStream_t s) {
   if (a.ndim == 1) {
     dim3 grid((a.nx + TW - 1) / TW);
     hipLaunchKernelGGL((k_post2<ND, false, 1, TW, PART>), grid, dim3(256), 0, s, a);

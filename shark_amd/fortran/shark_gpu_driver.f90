@@ -17,6 +17,7 @@ program shark_gpu_driver
   implicit none
   type(shark_cfg), target :: cfg
   type(c_ptr) :: ctx
+  type(shark_tune) :: tune
   character(len=512) :: mode, arg, outfile
   integer :: nx, ncells, ndim, nvar, i, u_out
   integer(c_int64_t) :: nsteps, steps_done
@@ -52,6 +53,10 @@ program shark_gpu_driver
   call shark_check(shark_gpu_set_geometry(ctx, dx, surf, vol, radii_c, position), 'shark_gpu_set_geometry')
   call shark_check(shark_gpu_upload(ctx, c_loc(u), c_loc(cs), opt(has_sdust, sdust), &
        opt(has_cond, cond), opt1(has_fargo, fargo_v)), 'shark_gpu_upload')
+  ! launch shapes picked on this state (the state is restored, so the run is unchanged)
+  call shark_check(shark_gpu_autotune(ctx, 3_c_int, tune), 'shark_gpu_autotune')
+  print '(A,I0,A,I0,A,I0)', 'shark_gpu_driver: tuned strip_h=', tune%strip_h, ' post_tw=', &
+       tune%post_tw, ' xcd_order=', tune%xcd_order
   call shark_check(shark_gpu_run(ctx, nsteps, tend, steps_done, time_out), 'shark_gpu_run')
   call shark_check(shark_gpu_download(ctx, c_loc(u), c_null_ptr), 'shark_gpu_download')
   call shark_check(shark_gpu_destroy(ctx), 'shark_gpu_destroy')

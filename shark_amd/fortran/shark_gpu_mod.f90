@@ -31,6 +31,14 @@ module shark_gpu_mod
      real(c_double) :: reserved(16)
   end type shark_cfg
 
+  ! same layout as struct shark_tune (launch shapes of the step kernels)
+  type, bind(C) :: shark_tune
+     integer(c_int32_t) :: strip_h, post_tw, xcd_order
+     integer(c_int32_t) :: reserved_i(5)
+     real(c_double) :: hydro_ms, post_ms
+     real(c_double) :: reserved(6)
+  end type shark_tune
+
   interface
      integer(c_int) function shark_gpu_create(cfg, ctx) bind(C, name="shark_gpu_create")
        import :: c_int, c_ptr, shark_cfg
@@ -71,6 +79,12 @@ module shark_gpu_mod
        type(c_ptr), value :: ctx
        real(c_double), intent(out) :: time, dt_last
      end function shark_gpu_get_time
+     integer(c_int) function shark_gpu_autotune(ctx, reps, chosen) bind(C, name="shark_gpu_autotune")
+       import :: c_int, c_ptr, shark_tune
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: reps
+       type(shark_tune), intent(out) :: chosen
+     end function shark_gpu_autotune
      integer(c_int) function shark_gpu_destroy(ctx) bind(C, name="shark_gpu_destroy")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

@@ -23,18 +23,30 @@ EXPORTS = [
     "shark_gpu_run", "shark_gpu_download", "shark_gpu_get_time", "shark_gpu_set_time",
     "shark_gpu_kernel_time", "shark_gpu_set_timing", "shark_gpu_algorithmic_bytes",
     "shark_gpu_destroy", "shark_gpu_last_error", "shark_gpu_version",
+    "shark_gpu_get_tune", "shark_gpu_set_tune", "shark_gpu_autotune",
     "shark_growth_create", "shark_growth_run", "shark_growth_download", "shark_growth_upload",
     "shark_growth_destroy", "shark_growth_kernel_time", "shark_growth_charge",
     "shark_growth_set_cell_dt", "shark_growth_features", "shark_growth_set_surrogate",
     "shark_growth_surrogate_stats", "shark_gpu_attach_growth",
     "shark_mlp_create", "shark_mlp_num_params", "shark_mlp_set_params", "shark_mlp_get_params",
-    "shark_mlp_forward", "shark_mlp_grads", "shark_mlp_set_data", "shark_mlp_train",
+    "shark_mlp_forward", "shark_mlp_forward_layered", "shark_mlp_grads", "shark_mlp_set_data", "shark_mlp_train",
     "shark_mlp_kernel_time", "shark_mlp_destroy",
 ]
 
 
 class SharkGPUError(RuntimeError):
     pass
+
+
+class ShARKTune(C.Structure):
+    """struct shark_tune: launch shapes of the step kernels."""
+    _fields_ = [("strip_h", C.c_int32), ("post_tw", C.c_int32), ("xcd_order", C.c_int32),
+                ("reserved_i", C.c_int32 * 5), ("hydro_ms", C.c_double), ("post_ms", C.c_double),
+                ("reserved", C.c_double * 6)]
+
+    def as_dict(self):
+        return dict(strip_h=self.strip_h, post_tw=self.post_tw, xcd_order=self.xcd_order,
+                    hydro_ms=self.hydro_ms, post_ms=self.post_ms)
 
 
 _LIB = None
@@ -71,6 +83,9 @@ def load():
     lib.shark_gpu_algorithmic_bytes.restype = C.c_int64
     lib.shark_gpu_algorithmic_bytes.argtypes = [C.c_void_p]
     lib.shark_gpu_destroy.argtypes = [C.c_void_p]
+    lib.shark_gpu_get_tune.argtypes = [C.c_void_p, C.POINTER(ShARKTune)]
+    lib.shark_gpu_set_tune.argtypes = [C.c_void_p, C.POINTER(ShARKTune)]
+    lib.shark_gpu_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(ShARKTune)]
     _LIB = lib
     return lib
 
@@ -145,6 +160,29 @@ class Step:
         n = C.c_int64()
         check(self.L.shark_gpu_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def get_tune(self):
+        t = ShARKTune()
+        check(self.L.shark_gpu_get_tune(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tune(self, strip_h=None, post_tw=None, xcd_order=None):
+        t = ShARKTune()
+        check(self.L.shark_gpu_get_tune(self.h, C.byref(t)))
+        if strip_h is not None:
+            t.strip_h = int(strip_h)
+        if post_tw is not None:
+            t.post_tw = int(post_tw)
+        if xcd_order is not None:
+            t.xcd_order = int(xcd_order)
+        check(self.L.shark_gpu_set_tune(self.h, C.byref(t)))
+
+    def autotune(self, reps=5):
+        """Time the candidate launch shapes on this state and keep the fastest (the state is
+        restored afterwards); returns the chosen shape and its kernel times."""
+        t = ShARKTune()
+        check(self.L.shark_gpu_autotune(self.h, int(reps), C.byref(t)))
+        return t.as_dict()
 
     def algorithmic_bytes(self):
         return self.L.shark_gpu_algorithmic_bytes(C.byref(self.cfg))

@@ -75,6 +75,7 @@ def _lib():
         lib.shark_mlp_set_params.argtypes = [C.c_void_p, _P32]
         lib.shark_mlp_get_params.argtypes = [C.c_void_p, _P32]
         lib.shark_mlp_forward.argtypes = [C.c_void_p, _P32, C.c_int64, _P32]
+        lib.shark_mlp_forward_layered.argtypes = [C.c_void_p, _P32, C.c_int64, _P32]
         lib.shark_mlp_grads.argtypes = [C.c_void_p, _P32, _P32, C.c_int64, _P32, _P32]
         lib.shark_mlp_set_data.argtypes = [C.c_void_p, _P32, _P32, C.c_int64]
         lib.shark_mlp_train.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_uint64, _P32]
@@ -110,16 +111,18 @@ class MLP:
         gpu.check(self.L.shark_mlp_get_params(self.h, p.ctypes.data_as(_P32)))
         return p
 
-    def forward(self, x):
+    def forward(self, x, layered=False):
+        """Network outputs: the fused inference kernel (shark_mlp_forward), or with layered=True
+        the training path's one-product-per-layer forward (shark_mlp_forward_layered)."""
         x = _f32(x)
         n = x.shape[0]
         y = np.empty((n, self.cfg.n_out), dtype=np.float32)
         mb = self.cfg.max_batch
+        fn = self.L.shark_mlp_forward_layered if layered else self.L.shark_mlp_forward
         for i in range(0, n, mb):
             xi = np.ascontiguousarray(x[i:i + mb])
             yi = np.empty((xi.shape[0], self.cfg.n_out), dtype=np.float32)
-            gpu.check(self.L.shark_mlp_forward(self.h, xi.ctypes.data_as(_P32), xi.shape[0],
-                                               yi.ctypes.data_as(_P32)))
+            gpu.check(fn(self.h, xi.ctypes.data_as(_P32), xi.shape[0], yi.ctypes.data_as(_P32)))
             y[i:i + mb] = yi
         return y
 

@@ -96,3 +96,26 @@ def test_charge_matches_oracle():
     assert (zm_ref < 0).all()
     np.testing.assert_allclose(zm, zm_ref, rtol=1e-12)
     np.testing.assert_allclose(xi, xi_ref, rtol=1e-12)
+
+
+def test_feature_ion_fraction_matches_charge_solve():
+    """The learned step's ion-fraction feature (regrouped Newton, warm-started per bin) agrees with
+    the reference-order charge solve far below the feature's FP32 rounding, on the initial MRN
+    state and on a state aged by the plain coagulation."""
+    p, t, rhod, sigma, cs, r = cells(nb=64, nr=64, nphi=8)
+    g = G.Growth(p, sigma.size)
+    try:
+        for aged in (False, True):
+            g.upload(rhod, sigma, cs, r)
+            if aged:
+                g.run(G.MFMA_FP32, 1e-3)
+                s2, _ = g.download()
+                g.upload(s2, sigma, cs, r)
+            _, xi = g.charge()
+            f = g.features(1e-3)
+            lx = f[:, p.nbins + 3].astype(np.float64)
+            ref = np.log10(xi)
+            assert np.all(np.isfinite(lx))
+            assert np.abs(lx - ref).max() <= 4 * np.spacing(np.abs(ref).astype(np.float32)).max(), aged
+    finally:
+        g.close()

@@ -77,3 +77,28 @@ def test_training_reduces_loss_and_is_deterministic():
     print("losses", losses[0], losses[-1])
     assert losses[-1] < 0.1 * losses[0]
     assert np.array_equal(runs[0][1], runs[1][1])
+
+
+@pytest.mark.parametrize("dims,n", [((69, 256, 4, 65), 1000), ((69, 256, 4, 65), 1), ((5, 33, 3, 7), 257),
+                                    ((200, 256, 2, 256), 129), ((69, 300, 3, 65), 70)])
+def test_fused_inference_equals_layered_forward(dims, n):
+    """The fused inference kernel (all layers in one launch) reproduces the training path's
+    forward bit for bit: same BF16 operands, k order, MFMA and epilogue.  A width above 256 takes
+    the layered path inside shark_mlp_forward."""
+    cfg = M.make_cfg(n_in=dims[0], n_hidden=dims[1], n_layers=dims[2], n_out=dims[3], max_batch=1024)
+    p = net(cfg, seed=5)
+    x = np.random.default_rng(6).standard_normal((n, dims[0])).astype(np.float32)
+    m = M.MLP(cfg, p)
+    try:
+        yf = m.forward(x)
+        yl = m.forward(x, layered=True)
+        # after a training step the packed weights must follow the new parameters
+        m.set_data(x, yl + 1.0)
+        m.train(1, min(n, 64), 1e-3, seed=1)
+        yf2 = m.forward(x)
+        yl2 = m.forward(x, layered=True)
+    finally:
+        m.close()
+    assert np.array_equal(yf, yl)
+    assert np.array_equal(yf2, yl2)
+    assert not np.array_equal(yf2, yf)

@@ -83,3 +83,47 @@ def test_learned_growth_in_step():
     assert err.mean() < 5e-3
     assert mass.max() < 1e-12
     assert tf > 10 * ts
+
+
+def test_learned_growth_in_step_full_size():
+    """The 1024 x 2048 disk with 64 bins and the learned step in every step, checked through
+    properties that hold at any size against the same step without growth: the gas and the dt are
+    untouched, every cell keeps its total dust mass, every species keeps its velocity."""
+    cfg, geom, state, p = setups.disk_growth(nr=1024, nphi=2048, nbins=64)
+    nc = (cfg.nx + 4) * (cfg.ny + 4)
+    nd = cfg.ndust
+    print("learned growth in the 1024x2048 64-bin disk step", flush=True)
+    g = gpu.Step(cfg, geom, state)
+    try:
+        dt0 = g.step()
+        u0 = g.download()
+    finally:
+        g.close()
+    net, norm, _ = S.load(S.DATA + "/surrogate_disk64.npz")
+    g = gpu.Step(cfg, geom, state)
+    gr = G.Growth(p, nc)
+    try:
+        gr.set_surrogate(net, norm)
+        g.attach_growth(gr, G.SURROGATE)
+        g.set_timing(True)
+        dt1 = g.step()
+        u1 = g.download()
+        ms = g.kernel_time("growth")[0]
+        nfb, _ = gr.surrogate_stats()
+    finally:
+        g.close()
+        gr.close()
+        net.close()
+    act = active(cfg)
+    print(f"learned step {ms:.2f} ms for {act.sum()} cells, fallback {nfb}", flush=True)
+    assert dt1 == dt0
+    assert np.array_equal(u1[:5], u0[:5])
+    r0, r1 = u0[5:5 + nd][:, act], u1[5:5 + nd][:, act]
+    tot0, tot1 = r0.sum(0), r1.sum(0)
+    assert np.abs(tot1 / tot0 - 1).max() < 1e-12
+    assert S.mass_l1(r1, r0).mean() > 1e-3          # the bins did change
+    for c in range(3):
+        m0 = u0[5 + nd + c * nd:5 + nd + (c + 1) * nd][:, act]
+        m1 = u1[5 + nd + c * nd:5 + nd + (c + 1) * nd][:, act]
+        v0, v1 = m0 / r0, m1 / r1
+        assert np.all(np.abs(v1 - v0) <= 1e-12 * np.abs(v0) + 1e-300)
