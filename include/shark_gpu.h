@@ -35,7 +35,8 @@ enum {
   SHARK_EUNSUP = 2,   /* configuration outside the implemented scope */
   SHARK_EHIP = 3,     /* HIP runtime error */
   SHARK_ESTATE = 4,   /* call out of order (e.g. step before upload) */
-  SHARK_ENOMEM = 5
+  SHARK_ENOMEM = 5,
+  SHARK_EIO = 6       /* file could not be written or read */
 };
 
 /* geometry (reference GEOM macro, bin/Makefile:14) */

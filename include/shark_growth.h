@@ -70,6 +70,35 @@ int shark_growth_download(shark_growth* g, double* rhod, int64_t* nsub);
  * grain charge and the ion fraction zeta / sum_k k_i,k n_k, in the disk environment's cgs
  * midplane densities.  zeta: cosmic-ray ionisation rate (s^-1); mu_ions: ion mass in m_H;
  * electrons: include electron capture.  Outputs (ncells) may be NULL. */
+/* Grain charging by the model of Marchand et al. 2021 (`charge`, src/chemistry/ionisation.f90:4-214):
+ * per cell a Newton iteration on the grain surface potential psi (from the cell's psi_old when
+ * non-zero, else from the global initial guess), then the grain charges zd, the ion and electron
+ * densities, and with `resistivities` the Ohmic, Hall and ambipolar conductivities and
+ * resistivities and the dust Hall factors.  Parameters are the reference's ionisation_commons
+ * values and the setup's units. */
+typedef struct shark_marchand_params {
+  double zeta;             /* x: cosmic-ray ionisation rate (s^-1) */
+  double mu_ions;          /* ion mass (m_H) */
+  double stickeff_el;      /* electron sticking coefficient */
+  double epsilon_ionis;    /* Newton tolerance on |f(psi)| */
+  int32_t nitermax_ionis;  /* Newton iteration cap */
+  int32_t resistivities;   /* res_Marchand */
+  double B_0_lee;          /* B at n_H = 1e4 cm^-3 (G) */
+  double rhograin;         /* grain material density (g cm^-3) */
+  double gamma;            /* adiabatic index (stopping time in the resistivities) */
+  double unit_l, unit_d, unit_m, unit_v, unit_nh;
+} shark_marchand_params;
+
+/* On the GPU over ncells cells and nbins species (host arrays, column-major (ncells, nbins) as the
+ * reference's): rho, rhod, T (barotrop), cs (cs_eos(T), code units), sdust, mdust (code units);
+ * psi_old (in/out).  Outputs: zd (ncells, nbins), ni, ne, and when resistivities is set
+ * cond = eta_o | eta_h | eta_a | sigma_o | sigma_p | sigma_h (6 x ncells) and gamma_d
+ * (ncells, nbins); cond and gamma_d may be NULL. */
+int shark_charge_marchand(const shark_marchand_params* p, int64_t ncells, int nbins, const double* rho,
+                          const double* rhod, const double* T, const double* cs, const double* sdust,
+                          const double* mdust, double* psi_old, double* zd, double* ni, double* ne,
+                          double* cond, double* gamma_d);
+
 int shark_growth_charge(shark_growth* g, double zeta, double mu_ions, int electrons, double* zmean,
                         double* xion);
 /* Per-cell growth intervals (ncells, >= 0) used instead of run's dt; NULL returns to dt. */

@@ -13,7 +13,8 @@
  *
  * Parameter vector layout (FP32): for each layer l: W_l (n_out_l x n_in_l, row-major), then
  * b_l (n_out_l).  Data are row-major (rows = samples).
- * Loss: L = 1/(n n_out) sum_{rows, outputs} (y - t)^2.
+ * Loss: L = 1/(n n_out) sum_{rows, outputs} w (y - t)^2, with per-element weights w (1 unless
+ * given, row-major like t).
  */
 #ifndef SHARK_MLP_H
 #define SHARK_MLP_H
@@ -52,8 +53,12 @@ int shark_mlp_forward_layered(shark_mlp* m, const float* x, int64_t n, float* y)
 /* Loss and its gradient w.r.t. every parameter on one batch (n <= max_batch). */
 int shark_mlp_grads(shark_mlp* m, const float* x, const float* t, int64_t n, float* loss,
                     float* grads);
-/* Training set kept on the device (copied in); replaces any earlier set. */
+int shark_mlp_grads_weighted(shark_mlp* m, const float* x, const float* t, const float* w, int64_t n,
+                             float* loss, float* grads);
+/* Training set kept on the device (copied in); replaces any earlier set.  The weighted form adds
+ * per-element loss weights (n x n_out; NULL: all 1). */
 int shark_mlp_set_data(shark_mlp* m, const float* x, const float* t, int64_t n);
+int shark_mlp_set_data_weighted(shark_mlp* m, const float* x, const float* t, const float* w, int64_t n);
 /* Adam over the device training set: `epochs` passes in batches of `batch` rows (<= max_batch)
  * in a seeded random order, learning rate lr.  loss_out[e] = mean batch loss of epoch e. */
 int shark_mlp_train(shark_mlp* m, int epochs, int batch, float lr, uint64_t seed, float* loss_out);

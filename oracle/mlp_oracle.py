@@ -48,14 +48,15 @@ def forward(layers, x, bf16_ops=True, keep=False):
     return (a, zs, ins) if keep else a
 
 
-def loss_grads(layers, x, t, bf16_ops=True):
-    """Loss 1/(n n_out) sum (y - t)^2 and [(dW, db)] per layer."""
+def loss_grads(layers, x, t, bf16_ops=True, w=None):
+    """Loss 1/(n n_out) sum w (y - t)^2 (w = 1 when None) and [(dW, db)] per layer."""
     r = bf16 if bf16_ops else _id
     y, zs, ins = forward(layers, x, bf16_ops, keep=True)
     n, no = y.shape
     d = y - _id(t)
-    loss = float((d * d).sum() / (n * no))
-    dz = 2.0 * d / (n * no)
+    wt = np.ones_like(d) if w is None else _id(w)
+    loss = float((wt * d * d).sum() / (n * no))
+    dz = 2.0 * wt * d / (n * no)
     grads = [None] * len(layers)
     for l in range(len(layers) - 1, -1, -1):
         W, _ = layers[l]

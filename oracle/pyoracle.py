@@ -237,3 +237,25 @@ def charge_disk(params, tables, sigma, cs, r, rhod, zeta=5e-17, mu_ions=25.0, el
                                 _ptr(_c(r)), _ptr(_c(rhod)), zeta, mu_ions, int(electrons), maxiter,
                                 _ptr(zd), _ptr(zm), _ptr(xi))
     return (zm, xi, zd) if per_bin else (zm, xi)
+
+
+def charge_marchand(p, rho, rhod, T, cs, sdust, mdust, psi_old=None):
+    """Marchand grain charging (ionisation.f90:4-214) on the CPU; p: shark_amd.growth.MarchandParams.
+    Arrays (nbins, ncells) / (ncells); returns the same dict as shark_amd.growth.charge_marchand."""
+    L = _glib()
+    nb, nc = rhod.shape
+    psi = np.zeros(nc) if psi_old is None else np.array(psi_old, dtype=np.float64)
+    zd = np.zeros((nb, nc))
+    ni = np.zeros(nc)
+    ne = np.zeros(nc)
+    cond = np.zeros((6, nc))
+    gd = np.zeros((nb, nc))
+    L.marchand_oracle_charge.argtypes = [C.c_void_p, C.c_int64, C.c_int] + [_P] * 12
+    L.marchand_oracle_charge(C.byref(p), nc, nb, _ptr(_c(rho)), _ptr(_c(rhod)), _ptr(_c(T)), _ptr(_c(cs)),
+                             _ptr(_c(sdust)), _ptr(_c(mdust)), _ptr(psi), _ptr(zd), _ptr(ni), _ptr(ne),
+                             _ptr(cond), _ptr(gd))
+    out = dict(zd=zd, ni=ni, ne=ne, psi=psi)
+    if p.resistivities:
+        out.update(eta_o=cond[0], eta_h=cond[1], eta_a=cond[2], sigma_o=cond[3], sigma_p=cond[4],
+                   sigma_h=cond[5], gamma_d=gd)
+    return out

@@ -8,7 +8,9 @@
 !   <out>/final.bin  : u_prim after nsteps
 !   <out>/dt.bin     : dt of every step
 !   <out>/meta.txt   : sizes and scalars the restatement needs
-! usage: shark_ref <namelist> <nsteps> <outdir>
+!   ./output_NNNNN/  : with a fourth argument iout, the reference's own output(iout)
+!                      (src/commons/write_output.f90:2) of the final state, after ctoprim
+! usage: shark_ref <namelist> <nsteps> <outdir> [iout]
 program ref_harness
   use parameters
   use commons
@@ -20,7 +22,7 @@ program ref_harness
   use setup_parameters, only: box_l, rho_init, omega_shear, q_shear, hoverr, eta_stream, rad0
 #endif
   implicit none
-  integer :: nsteps, istep, ilun
+  integer :: nsteps, istep, ilun, iout
   character(len=256) :: arg, outdir
   real(dp), allocatable :: dts(:)
 
@@ -80,6 +82,13 @@ program ref_harness
   write(ilun) dts(1:nsteps)
   close(ilun)
 
+  if (command_argument_count() >= 4) then
+     call getarg(4, arg)
+     read(arg, *) iout
+     call ctoprim
+     call output(iout)
+  end if
+
   open(ilun, file=trim(outdir)//'/meta.txt', status='replace')
   write(ilun, '(A,I12)') 'ncells ', ncells
   write(ilun, '(A,I12)') 'nvar ', nvar
@@ -107,6 +116,9 @@ program ref_harness
   write(ilun, '(A,ES25.17)') 'unit_d ', unit_d
   write(ilun, '(A,ES25.17)') 'unit_l ', unit_l
   write(ilun, '(A,ES25.17)') 'unit_t ', unit_t
+  write(ilun, '(A,ES25.17)') 'unit_v ', unit_v
+  write(ilun, '(A,ES25.17)') 'unit_p ', unit_P
+  write(ilun, '(A,ES25.17)') 'unit_dcol ', unit_dcol
 #ifdef REF_DISK
   write(ilun, '(A,ES25.17)') 'mstar ', mstar
   write(ilun, '(A,ES25.17)') 'r_relax ', r_relax

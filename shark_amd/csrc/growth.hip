@@ -718,6 +718,152 @@ __global__ void k_senv(GArgs a, double* Tout) {
   Tout[cc] = csp * csp * (a.mu_gas * kMH) / (a.gamma * kKB);
 }
 
+// ------------------------------------------------------------------ Marchand grain charging
+// `charge` of src/chemistry/ionisation.f90:4-214 (Marchand et al. 2021), one thread per cell, in the
+// reference's operand order with flang's lowering of the powers (x**2., x**3. as products, x**4.0
+// and non-integral exponents through pow, default-REAL literals rounded to single precision).
+struct MArgs {
+  int64_t nc;
+  int nb, nitermax, res;
+  double thetai, psi0, x, mu_ions, eps_ionis, B0, rhograin, gamma, unit_l, unit_d, unit_m, unit_v, unit_nh;
+  const double *rho, *rhod, *T, *cs, *sdust, *mdust;
+  double *tau, *alpha, *nk;        // per (bin, cell) scratch
+  double *psi, *zd, *ni, *ne, *cond, *gd;
+};
+
+constexpr double kCLight = 3e10;
+
+__global__ void k_mcharge_pre(MArgs a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.nc * a.nb) return;
+  const int64_t i = j % a.nc;
+  const double T = a.T[i];
+  const double lg = a.sdust[j] * a.unit_l;
+  const double tau = lg * kKB * T / (kEel * kEel);
+  a.tau[j] = tau;
+  a.alpha[j] = sqrt(8.0 / (kPi * tau));
+  a.nk[j] = a.rhod[j] * a.unit_d / (a.mdust[j] * a.unit_m);
+}
+
+__global__ void k_mcharge(MArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.nc) return;
+  const int nb = a.nb;
+  const int64_t nc = a.nc;
+  const double epsone = 0.99999, th = a.thetai, x = a.x;
+  const double T = a.T[i];
+  const double sigmav_ie = (2e-7) / sqrt(T / 300.0);
+  const double vi = sqrt(8.0 * kKB * T / (kPi * a.mu_ions * kMH));
+  const double nH = a.rho[i] * a.unit_nh;
+  auto lg2 = [&](int k) {
+    const double l = a.sdust[k * nc + i] * a.unit_l;
+    return l * l;
+  };
+  double conv = 1e4, psi = a.psi0;
+  int niter = 0, lowT = 0;
+  if (a.psi[i] != 0.0) psi = a.psi[i];
+  while (conv > a.eps_ionis && niter < a.nitermax) {
+    const double eps = (1.0 - psi) / th * exp(-psi);
+    const double et = eps * th;
+    double n_i = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0, s6 = 0.0, s7 = 0.0;
+    for (int k = 0; k < nb; ++k) {
+      const double tau = a.tau[k * nc + i], al = a.alpha[k * nc + i], nk = a.nk[k * nc + i];
+      n_i = n_i + -(psi * tau + (1.0 - et * et) / (1.0 + et * al + et * et)) * nk / (1.0 - eps);
+    }
+    for (int k = 0; k < nb; ++k) {
+      const double tau = a.tau[k * nc + i], al = a.alpha[k * nc + i], nk = a.nk[k * nc + i];
+      s2 = s2 + nk * kPi * lg2(k) * ((1.0 - psi) + (2.0 / tau) * (et * et + et) / (1.0 + et * al + et * et));
+    }
+    const double fpsi = sigmav_ie * eps * (n_i * n_i) / (x * nH) + n_i * vi / (x * nH) * s2 - 1.0;
+    const double B = -(eps * (2.0 - psi) / (1.0 - psi));
+    for (int k = 0; k < nb; ++k) {
+      const double al = a.alpha[k * nc + i], nk = a.nk[k * nc + i];
+      const double d0 = 1.0 + et * al + et * et;
+      s3 = s3 + nk * (-pow(et, 4.0) + (4.0 * (th * th) - (th * th) * th * al) * (eps * eps) +
+                      (2.0 * th * al - 4.0 * (th * th)) * eps + 1.0 - th * al) / (d0 * d0);
+    }
+    for (int k = 0; k < nb; ++k) s4 = s4 + a.nk[k * nc + i] * a.tau[k * nc + i];
+    const double dni = -(1.0 / ((1.0 - eps) * (1.0 - eps)) * B * s3) - (psi / (1.0 - eps) * B + 1.0) * s4 / (1.0 - eps);
+    for (int k = 0; k < nb; ++k) {
+      const double tau = a.tau[k * nc + i], al = a.alpha[k * nc + i], nk = a.nk[k * nc + i];
+      s5 = s5 + nk * kPi * lg2(k) / (tau * (1.0 + et * al + et * et));
+    }
+    for (int k = 0; k < nb; ++k) {
+      const double tau = a.tau[k * nc + i], al = a.alpha[k * nc + i], nk = a.nk[k * nc + i];
+      const double d0 = 1.0 + et * al + et * et;
+      s6 = s6 + nk * kPi * lg2(k) * (2.0 * eps * (th * th) + th * al) / (tau * (d0 * d0));
+    }
+    for (int k = 0; k < nb; ++k) s7 = s7 + a.nk[k * nc + i] * kPi * lg2(k);
+    const double dfdpsi = sigmav_ie * n_i / (x * nH) * (n_i * B + 2.0 * eps * dni) +
+                          2.0 * n_i * vi / (x * nH) * (et * et + et) *
+                              ((dni / n_i + B * (2.0 * et + 1.0) / (eps * (1.0 + et))) * s5 + B * s6) +
+                          n_i * vi / (x * nH) * (dni / n_i * (1.0 - psi) - 1.0) * s7;
+    psi = psi - fpsi / dfdpsi;
+    if (psi < a.psi0) {
+      psi = a.psi0 * (double)0.9999999f;
+      lowT = 1;
+    } else if (psi > 0.0) {
+      psi = -1e-5;
+    }
+    conv = fabs(fpsi);
+    niter = niter + 1;
+  }
+  double eps = (1.0 - psi) / th * exp(-psi);
+  if (lowT > 0) eps = epsone;
+  double n_i = 0.0;
+  for (int k = 0; k < nb; ++k) {
+    const double tau = a.tau[k * nc + i], al = a.alpha[k * nc + i], nk = a.nk[k * nc + i];
+    const double z = psi * tau + (1.0 - (eps * eps) * (th * th)) / (1.0 + eps * th * al + (eps * eps) * (th * th));
+    a.zd[k * nc + i] = z;
+    n_i = n_i - z * nk / (1.0 - eps);
+    if (lowT > 0) n_i = sqrt(eps * x * nH / sigmav_ie);
+  }
+  const double ne = eps * n_i;
+  a.ni[i] = n_i;
+  a.ne[i] = ne;
+  a.psi[i] = lowT > 0 ? a.psi0 : psi;
+  if (!a.res) return;
+  const double Bg = a.B0 * sqrt(a.rho[i] * a.unit_nh / 1e4);
+  const double mu_i = 2.0 * kMH * a.mu_ions * kMH / (2.0 * kMH + a.mu_ions * kMH);
+  const double mu_e = 2.0 * kMH * kMel / (kMel + 2.0 * kMH);
+  const double vrms_i = sqrt(8.0 * kKB * T / (kPi * mu_i)) * 1e-5;
+  const double vrms_el = sqrt(8.0 * kKB * T / (kPi * mu_e)) * 1e-5;
+  const double sigmav_el = 3.16e-11 * pow(vrms_el, (double)1.3f);
+  const double sigmav_ions = 2.4e-9 * pow(vrms_i, (double)0.6f);
+  const double t_sel = 1.0 / (double)1.16f * ((kMel + 2.0 * kMH) / (2.0 * kMH)) / sigmav_el / nH;
+  const double t_sions = 1.0 / (double)1.14f * ((a.mu_ions * kMH + 2.0 * kMH) / (2.0 * kMH)) / sigmav_ions / nH;
+  const double s_el = ne * (kEel * kEel) * t_sel / kMel;
+  const double s_ions = n_i * (kEel * kEel) * t_sions / (a.mu_ions * kMH);
+  const double o_el = -(kEel * Bg / kCLight / kMel);
+  const double o_ions = kEel * Bg / kCLight / (a.mu_ions * kMH);
+  double sum_sd = 0.0, sum_p = 0.0, sum_h = 0.0;
+  for (int k = 0; k < nb; ++k) {
+    const double tsd = sqrt(kPi * a.gamma / 8.0) * (a.rhograin) * (a.sdust[k * nc + i] * a.unit_l) /
+                       (a.rho[i] * a.unit_d * a.cs[i] * a.unit_v);
+    const double z = a.zd[k * nc + i], nk = a.nk[k * nc + i];
+    const double mk = a.mdust[k * nc + i] * a.unit_m;
+    const double sdk = nk * ((z * kEel) * (z * kEel)) * tsd / mk;
+    const double odk = z * kEel * Bg / kCLight / mk;
+    sum_sd = sum_sd + sdk;
+    sum_p = sum_p + sdk / (1.0 + (odk * tsd) * (odk * tsd));
+    sum_h = sum_h + sdk * (odk * tsd) / (1.0 + (odk * tsd) * (odk * tsd));
+    if (a.gd) a.gd[k * nc + i] = tsd * odk;
+  }
+  const double so = sum_sd + s_ions + s_el;
+  const double sp = s_ions / (1.0 + (o_ions * t_sions) * (o_ions * t_sions)) +
+                    s_el / (1.0 + (o_el * t_sel) * (o_el * t_sel)) + sum_p;
+  const double sh = -(s_ions * (o_ions * t_sions) / (1.0 + (o_ions * t_sions) * (o_ions * t_sions))) -
+                    s_el * (o_el * t_sel) / (1.0 + (o_el * t_sel) * (o_el * t_sel)) - sum_h;
+  if (a.cond) {
+    a.cond[0 * nc + i] = 1.0 / so;
+    a.cond[1 * nc + i] = sh / (sp * 2. + sh * sh);   // the reference's sigma_p*2. (ionisation.f90:197)
+    a.cond[2 * nc + i] = sp / (sp * sp + sh * sh) - 1.0 / so;
+    a.cond[3 * nc + i] = so;
+    a.cond[4 * nc + i] = sp;
+    a.cond[5 * nc + i] = sh;
+  }
+}
+
 // Ion fraction of a cell for the learned step's features.  The same charge equation as k_gcharge
 // with its terms regrouped (k_e = C_e exp(A y), k_i = C_i (1 - A y), A = e^2 / (s k T):
 // f = Q (1/k_i - 1/k_e) - y, f' = Q (A/k_e + A C_i/k_i^2) - 1, one exp per iteration), each bin's
@@ -1580,6 +1726,72 @@ int shark_growth_download(shark_growth* g, double* rhod, int64_t* nsub) {
     GCHK(hipMemcpy(n.data(), g->dnit, sizeof(int) * g->ncells, hipMemcpyDeviceToHost));
     for (int64_t c = 0; c < g->ncells; ++c) nsub[c] = n[c];
   }
+  return 0;
+}
+
+int shark_charge_marchand(const shark_marchand_params* p, int64_t nc, int nb, const double* rho,
+                          const double* rhod, const double* T, const double* cs, const double* sdust,
+                          const double* mdust, double* psi_old, double* zd, double* ni, double* ne,
+                          double* cond, double* gamma_d) {
+  if (!p || !rho || !rhod || !T || !cs || !sdust || !mdust || !psi_old || !zd || !ni || !ne)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_charge_marchand: null argument");
+  if (nc < 1 || nb < 1 || nb > MAXB) return gfail(SHARK_GROWTH_EINVAL, "shark_charge_marchand: need ncells >= 1 and 1 <= nbins <= 64");
+  if (!(p->zeta > 0.0) || !(p->mu_ions > 0.0) || !(p->stickeff_el > 0.0) || p->nitermax_ionis < 1)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_charge_marchand: zeta, mu_ions, stickeff_el and nitermax_ionis must be positive");
+  // the global initial guess (ionisation.f90:34-52), on the host
+  const double epsone = 0.99999;
+  const double thetai = p->stickeff_el * sqrt(p->mu_ions * kMH / kMel);
+  double conv = 1e4, psi0 = -2.5;
+  for (int it = 0; conv > 1e-6 && it < 1000; ++it) {
+    double psi = psi0;
+    const double fpsi = (1.0 - psi) / (epsone * thetai) * exp(-psi) - 1.0;
+    const double dfdpsi = -(exp(-psi) / (epsone * thetai)) - (1.0 - psi) / (epsone * thetai) * exp(-psi);
+    psi = psi - fpsi / dfdpsi;
+    conv = fabs(fpsi);
+    psi0 = psi;
+  }
+  MArgs a;
+  memset(&a, 0, sizeof(a));
+  a.nc = nc; a.nb = nb; a.nitermax = p->nitermax_ionis; a.res = p->resistivities;
+  a.thetai = thetai; a.psi0 = psi0; a.x = p->zeta; a.mu_ions = p->mu_ions; a.eps_ionis = p->epsilon_ionis;
+  a.B0 = p->B_0_lee; a.rhograin = p->rhograin; a.gamma = p->gamma;
+  a.unit_l = p->unit_l; a.unit_d = p->unit_d; a.unit_m = p->unit_m; a.unit_v = p->unit_v; a.unit_nh = p->unit_nh;
+  const size_t n1 = (size_t)nc * 8, nn = (size_t)nc * nb * 8;
+  // one device block: rho T cs psi ni ne | cond(6) | rhod sdust mdust zd gd tau alpha nk
+  double* d = nullptr;
+  const size_t total = 6 * n1 + 6 * n1 + 8 * nn;
+  GCHK(hipMalloc(&d, total));
+  char* b = (char*)d;
+  auto take = [&](size_t bytes) { double* q = (double*)b; b += bytes; return q; };
+  double *drho = take(n1), *dT = take(n1), *dcs = take(n1), *dpsi = take(n1), *dni = take(n1), *dne = take(n1);
+  double* dcond = take(6 * n1);
+  double *drhod = take(nn), *dsd = take(nn), *dmd = take(nn), *dzd = take(nn), *dgd = take(nn);
+  a.tau = take(nn); a.alpha = take(nn); a.nk = take(nn);
+  hipError_t e = hipSuccess;
+  auto up = [&](double* dst, const double* src, size_t bytes) {
+    if (e == hipSuccess) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  up(drho, rho, n1); up(dT, T, n1); up(dcs, cs, n1); up(dpsi, psi_old, n1);
+  up(drhod, rhod, nn); up(dsd, sdust, nn); up(dmd, mdust, nn);
+  a.rho = drho; a.T = dT; a.cs = dcs; a.rhod = drhod; a.sdust = dsd; a.mdust = dmd;
+  a.psi = dpsi; a.zd = dzd; a.ni = dni; a.ne = dne; a.cond = dcond; a.gd = dgd;
+  if (e == hipSuccess) {
+    const int64_t nj = nc * nb;
+    hipLaunchKernelGGL(k_mcharge_pre, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0, 0, a);
+    hipLaunchKernelGGL(k_mcharge, dim3((unsigned)((nc + 63) / 64)), dim3(64), 0, 0, a);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  auto down = [&](double* dst, const double* src, size_t bytes) {
+    if (dst && e == hipSuccess) e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  };
+  down(psi_old, dpsi, n1); down(zd, dzd, nn); down(ni, dni, n1); down(ne, dne, n1);
+  if (p->resistivities) {
+    down(cond, dcond, 6 * n1);
+    down(gamma_d, dgd, nn);
+  }
+  (void)hipFree(d);
+  GCHK(e);
   return 0;
 }
 

@@ -146,15 +146,17 @@ __global__ __launch_bounds__(256) void k_colsum(const float* X, int M, int N, in
 }
 
 // dY = 2 (Y - T) / (n n_out); per-block partial sums of (Y - T)^2 / (n n_out)
-__global__ __launch_bounds__(256) void k_lossgrad(const float* Y, const float* T, int64_t total,
-                                                  float scale, float* dY, float* partial) {
+// loss w (y - t)^2 * scale and its gradient; W == nullptr: w = 1
+__global__ __launch_bounds__(256) void k_lossgrad(const float* Y, const float* T, const float* W,
+                                                  int64_t total, float scale, float* dY, float* partial) {
   __shared__ float red[256];
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   float e2 = 0.f;
   if (i < total) {
     const float d = Y[i] - T[i];
-    dY[i] = 2.f * d * scale;
-    e2 = d * d * scale;
+    const float ws = W ? W[i] * scale : scale;
+    dY[i] = 2.f * d * ws;
+    e2 = d * d * ws;
   }
   red[threadIdx.x] = e2;
   __syncthreads();
@@ -369,6 +371,8 @@ struct shark_mlp {
   int64_t part_cap = 0;
   float* hloss = nullptr;              // pinned
   float *dX = nullptr, *dT = nullptr;  // training set
+  float* dW = nullptr;                 // training-set loss weights [ndata x n_out] (optional)
+  float* dw = nullptr;                 // batch loss weights [max_batch x n_out]
   int* didx = nullptr;
   int64_t ndata = 0;
   int64_t adam_t = 0;
@@ -393,7 +397,7 @@ static int mfail(const char* msg) { return shark::fail_msg(1, msg); }
 static void free_mlp(shark_mlp* m) {
   if (!m) return;
   float* bufs[] = {m->dp, m->dg, m->dm, m->dv, m->dx, m->dt, m->dy, m->ddy, m->dda, m->ddb, m->dpart,
-                   m->dX, m->dT};
+                   m->dX, m->dT, m->dW, m->dw};
   for (float* b : bufs)
     if (b) (void)hipFree(b);
   for (float* b : m->Z)
@@ -553,7 +557,7 @@ int mlp_n_out(const shark_mlp* m) { return m->cfg.n_out; }
 }  // namespace shark
 
 // gradients of the loss on rows already in m->dx / m->dt (n rows); returns loss via hloss[0]
-static int grads_device(shark_mlp* m, int64_t n) {
+static int grads_device(shark_mlp* m, int64_t n, bool weighted) {
   hipStream_t s = m->stream;
   const int L = m->L;
   int rc = shark::mlp_forward_device(m, m->dx, n, m->dy, s);
@@ -561,8 +565,8 @@ static int grads_device(shark_mlp* m, int64_t n) {
   const int no = m->cfg.n_out;
   const int64_t total = n * no;
   const int nblk = (int)((total + 255) / 256);
-  hipLaunchKernelGGL(k_lossgrad, dim3(nblk), dim3(256), 0, s, m->dy, m->dt, total,
-                     1.0f / (float)total, m->ddy, m->dpart);
+  hipLaunchKernelGGL(k_lossgrad, dim3(nblk), dim3(256), 0, s, m->dy, m->dt, weighted ? m->dw : nullptr,
+                     total, 1.0f / (float)total, m->ddy, m->dpart);
   MCHK(hipGetLastError());
   // reduce the loss partials in a fixed order on the device into hloss (pinned, mapped by copy)
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1), 0, s, m->dpart, nblk, (int64_t)1, m->dpart + nblk);
@@ -652,6 +656,7 @@ int shark_mlp_create(const shark_mlp_cfg* cfg, shark_mlp** out) {
   ALLOC(m->dt, B * cfg->n_out);
   ALLOC(m->dy, B * cfg->n_out);
   ALLOC(m->ddy, B * cfg->n_out);
+  ALLOC(m->dw, B * cfg->n_out);
   ALLOC(m->dda, B * cfg->n_hidden);
   ALLOC(m->ddb, B * cfg->n_hidden);
   ALLOC(m->dpart, m->part_cap);
@@ -719,12 +724,18 @@ int shark_mlp_forward_layered(shark_mlp* m, const float* x, int64_t n, float* y)
 }
 
 int shark_mlp_grads(shark_mlp* m, const float* x, const float* t, int64_t n, float* loss, float* grads) {
+  return shark_mlp_grads_weighted(m, x, t, nullptr, n, loss, grads);
+}
+
+int shark_mlp_grads_weighted(shark_mlp* m, const float* x, const float* t, const float* w, int64_t n,
+                             float* loss, float* grads) {
   if (!m || !x || !t) return mfail("shark_mlp_grads: null argument");
   if (n < 1 || n > m->cfg.max_batch) return mfail("shark_mlp_grads: n must lie in [1, max_batch]");
   hipStream_t s = m->stream;
   MCHK(hipMemcpyAsync(m->dx, x, sizeof(float) * n * m->cfg.n_in, hipMemcpyHostToDevice, s));
   MCHK(hipMemcpyAsync(m->dt, t, sizeof(float) * n * m->cfg.n_out, hipMemcpyHostToDevice, s));
-  int rc = grads_device(m, n);
+  if (w) MCHK(hipMemcpyAsync(m->dw, w, sizeof(float) * n * m->cfg.n_out, hipMemcpyHostToDevice, s));
+  int rc = grads_device(m, n, w != nullptr);
   if (rc) return rc;
   MCHK(hipStreamSynchronize(s));
   if (loss) *loss = m->hloss[0];
@@ -733,19 +744,26 @@ int shark_mlp_grads(shark_mlp* m, const float* x, const float* t, int64_t n, flo
 }
 
 int shark_mlp_set_data(shark_mlp* m, const float* x, const float* t, int64_t n) {
+  return shark_mlp_set_data_weighted(m, x, t, nullptr, n);
+}
+
+int shark_mlp_set_data_weighted(shark_mlp* m, const float* x, const float* t, const float* w, int64_t n) {
   if (!m || !x || !t || n < 1) return mfail("shark_mlp_set_data: null argument or empty set");
   if (n > (int64_t)1 << 30) return mfail("shark_mlp_set_data: at most 2^30 rows");
   if (m->dX) (void)hipFree(m->dX);
   if (m->dT) (void)hipFree(m->dT);
+  if (m->dW) (void)hipFree(m->dW);
   if (m->didx) (void)hipFree(m->didx);
-  m->dX = m->dT = nullptr;
+  m->dX = m->dT = m->dW = nullptr;
   m->didx = nullptr;
   m->ndata = 0;
   MCHK(hipMalloc(&m->dX, sizeof(float) * n * m->cfg.n_in));
   MCHK(hipMalloc(&m->dT, sizeof(float) * n * m->cfg.n_out));
+  if (w) MCHK(hipMalloc(&m->dW, sizeof(float) * n * m->cfg.n_out));
   MCHK(hipMalloc(&m->didx, sizeof(int) * n));
   MCHK(hipMemcpy(m->dX, x, sizeof(float) * n * m->cfg.n_in, hipMemcpyHostToDevice));
   MCHK(hipMemcpy(m->dT, t, sizeof(float) * n * m->cfg.n_out, hipMemcpyHostToDevice));
+  if (w) MCHK(hipMemcpy(m->dW, w, sizeof(float) * n * m->cfg.n_out, hipMemcpyHostToDevice));
   m->ndata = n;
   return 0;
 }
@@ -778,8 +796,11 @@ int shark_mlp_train(shark_mlp* m, int epochs, int batch, float lr, uint64_t seed
                          m->didx + b0, rows, ni, m->dx);
       hipLaunchKernelGGL(k_gather, dim3((unsigned)((rows * no + 255) / 256)), dim3(256), 0, s, m->dT,
                          m->didx + b0, rows, no, m->dt);
+      if (m->dW)
+        hipLaunchKernelGGL(k_gather, dim3((unsigned)((rows * no + 255) / 256)), dim3(256), 0, s, m->dW,
+                           m->didx + b0, rows, no, m->dw);
       MCHK(hipGetLastError());
-      int rc = grads_device(m, rows);
+      int rc = grads_device(m, rows, m->dW != nullptr);
       if (rc) return rc;
       ++m->adam_t;
       const float c1 = 1.f - powf(b1, (float)m->adam_t), c2 = 1.f - powf(b2, (float)m->adam_t);

@@ -3,6 +3,8 @@
 // bracketed by HIP events when timing is on).
 #include <hip/hip_runtime.h>
 
+#include <errno.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -14,6 +16,8 @@
 #include "../../include/shark_gpu.h"
 #include "growth.h"
 #include "../../include/shark_growth.h"
+#include "../../include/shark_output.h"
+#include "output.h"
 #include "physics.h"
 #include "step_kernels.h"
 
@@ -71,6 +75,7 @@ struct shark_ctx {
   bool timing = false;
   double kt_ms[KT_N] = {0, 0, 0, 0, 0};
   int64_t kt_n[KT_N] = {0, 0, 0, 0, 0};
+  std::vector<double> hpos, hrc;    // position(ncells,ndim) and radii_c(ncells) for outputs
   shark_growth* growth = nullptr;   // attached growth state (dust_growth inside each step)
   double* dRold = nullptr;          // dust densities before growth, for the velocity-keeping momenta
   int growth_mode = 0;
@@ -217,6 +222,9 @@ extern "C" int shark_gpu_set_geometry(shark_ctx* c, const double* dx, const doub
   }
   HIPCHK(hipMemcpyAsync(c->dgeom, col.data(), col.size() * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  c->hrc.assign(radii_c, radii_c + nc);
+  if (position) c->hpos.assign(position, position + (size_t)nc * a.ndim);
+  else c->hpos.clear();
   c->geom_set = true;
   return SHARK_OK;
 }
@@ -655,6 +663,103 @@ extern "C" int shark_gpu_autotune(shark_ctx* c, int reps, shark_tune* chosen) {
     chosen->hydro_ms = bh;
     chosen->post_ms = bp;
   }
+  return SHARK_OK;
+}
+
+// ------------------------------------------------------------------ outputs and restart dumps
+extern "C" int shark_gpu_write_output(shark_ctx* c, const shark_output_info* info, int iout) {
+  if (!c) return fail(SHARK_EINVAL, "null ctx");
+  if (!c->uploaded) return fail(SHARK_ESTATE, "upload before writing an output");
+  std::string err;
+  if (!shark::output_info_ok(info, &err)) return fail(SHARK_EINVAL, "%s", err.c_str());
+  if (c->hpos.empty()) return fail(SHARK_ESTATE, "outputs need the position array at shark_gpu_set_geometry");
+  const StepArgs& a = c->a;
+  const size_t nc = a.nc;
+  std::vector<double> q(nc * a.nvar);
+  int rc = shark_gpu_download(c, nullptr, q.data());
+  if (rc) return rc;
+  std::vector<double> sd;
+  if (a.nd > 0) {
+    sd.resize(nc * a.nd);
+    if (a.sdust_uniform) {
+      for (int k = 0; k < a.nd; ++k) std::fill(sd.begin() + (size_t)k * nc, sd.begin() + (size_t)(k + 1) * nc, a.sdust_k[k]);
+    } else if (a.sdust) {
+      HIPCHK(hipMemcpy(sd.data(), a.sdust, sd.size() * 8, hipMemcpyDeviceToHost));
+    } else {
+      return fail(SHARK_ESTATE, "grain sizes were not uploaded");
+    }
+  }
+  shark::OutputGrid g;
+  g.nx = a.nx;
+  g.ny = a.ny;
+  g.ndim = a.ndim;
+  g.nd = a.nd;
+  g.nxm = a.nxm;
+  g.geom = a.geom;
+  g.nc = a.nc;
+  if (shark::write_output_dir(info, iout, g, q.data(), a.nd > 0 ? sd.data() : nullptr, c->hpos.data(),
+                              c->hrc.data(), &err))
+    return fail(SHARK_EIO, "%s", err.c_str());
+  return SHARK_OK;
+}
+
+namespace {
+constexpr char kDumpMagic[8] = {'S', 'H', 'A', 'R', 'K', 'S', 'T', '1'};
+struct DumpHead {
+  char magic[8];
+  int32_t nx, ny, ndust, nvar, geom, pad;
+  int64_t ncells;
+};
+}  // namespace
+
+extern "C" int shark_gpu_save_state(shark_ctx* c, const char* path) {
+  if (!c || !path) return fail(SHARK_EINVAL, "null argument");
+  if (!c->uploaded) return fail(SHARK_ESTATE, "upload before saving a dump");
+  const StepArgs& a = c->a;
+  const size_t n = (size_t)a.nc * a.nvar;
+  std::vector<double> u(n);
+  HIPCHK(hipMemcpyAsync(u.data(), c->dU, n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hsc, c->dsc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  DumpHead h;
+  memset(&h, 0, sizeof(h));
+  memcpy(h.magic, kDumpMagic, 8);
+  h.nx = a.nx; h.ny = a.ny; h.ndust = a.nd; h.nvar = a.nvar; h.geom = a.geom;
+  h.ncells = a.nc;
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(SHARK_EIO, "cannot open %s: %s", path, strerror(errno));
+  bool ok = fwrite(&h, sizeof(h), 1, f) == 1 && fwrite(c->hsc, sizeof(DevScalars), 1, f) == 1 &&
+            fwrite(u.data(), 8, n, f) == n;
+  ok = (fclose(f) == 0) && ok;
+  if (!ok) return fail(SHARK_EIO, "short write to %s", path);
+  return SHARK_OK;
+}
+
+extern "C" int shark_gpu_load_state(shark_ctx* c, const char* path) {
+  if (!c || !path) return fail(SHARK_EINVAL, "null argument");
+  if (!c->uploaded) return fail(SHARK_ESTATE, "upload the setup's fields before loading a dump");
+  const StepArgs& a = c->a;
+  const size_t n = (size_t)a.nc * a.nvar;
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(SHARK_EIO, "cannot open %s: %s", path, strerror(errno));
+  DumpHead h;
+  DevScalars sc;
+  std::vector<double> u(n);
+  bool ok = fread(&h, sizeof(h), 1, f) == 1 && memcmp(h.magic, kDumpMagic, 8) == 0;
+  if (ok && (h.nx != a.nx || h.ny != a.ny || h.ndust != a.nd || h.nvar != a.nvar || h.geom != a.geom ||
+             h.ncells != a.nc)) {
+    fclose(f);
+    return fail(SHARK_EINVAL, "%s holds a %dx%d grid with %d species (geometry %d), not this configuration",
+                path, h.nx, h.ny, h.ndust, h.geom);
+  }
+  ok = ok && fread(&sc, sizeof(sc), 1, f) == 1 && fread(u.data(), 8, n, f) == n;
+  fclose(f);
+  if (!ok) return fail(SHARK_EIO, "%s is not a complete dump of this library", path);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpyAsync(c->dU, u.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+  *c->hsc = sc;
+  HIPCHK(hipMemcpyAsync(c->dsc, c->hsc, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return SHARK_OK;
 }
 

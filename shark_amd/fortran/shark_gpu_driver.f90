@@ -18,6 +18,10 @@ program shark_gpu_driver
   type(shark_cfg), target :: cfg
   type(c_ptr) :: ctx
   type(shark_tune) :: tune
+  type(shark_output_info) :: oinfo
+  character(len=512) :: odir
+  character(kind=c_char, len=513), target :: odir_c
+  integer :: ostat
   character(len=512) :: mode, arg, outfile
   integer :: nx, ncells, ndim, nvar, i, u_out
   integer(c_int64_t) :: nsteps, steps_done
@@ -59,6 +63,17 @@ program shark_gpu_driver
        tune%post_tw, ' xcd_order=', tune%xcd_order
   call shark_check(shark_gpu_run(ctx, nsteps, tend, steps_done, time_out), 'shark_gpu_run')
   call shark_check(shark_gpu_download(ctx, c_loc(u), c_null_ptr), 'shark_gpu_download')
+  ! the reference's output(1) of the final state (code units: the dusty wave's units are all 1)
+  call get_environment_variable('SHARK_OUTPUT_DIR', odir, status=ostat)
+  if (ostat == 0 .and. len_trim(odir) > 0) then
+     odir_c = trim(odir)//c_null_char
+     oinfo%version = 1; oinfo%gridspace = 0; oinfo%mhd = 0; oinfo%turb = 0
+     oinfo%ndustpscal = 0; oinfo%charging = 0; oinfo%time = time_out
+     oinfo%unit_t = 1d0; oinfo%unit_d = 1d0; oinfo%unit_l = 1d0; oinfo%unit_v = 1d0
+     oinfo%unit_p = 1d0; oinfo%unit_dcol = 1d0
+     oinfo%base_dir = c_loc(odir_c)
+     call shark_check(shark_gpu_write_output(ctx, oinfo, 1_c_int), 'shark_gpu_write_output')
+  end if
   call shark_check(shark_gpu_destroy(ctx), 'shark_gpu_destroy')
 
   open(newunit=u_out, file=trim(outfile), form='unformatted', access='stream', status='replace')

@@ -39,6 +39,14 @@ module shark_gpu_mod
      real(c_double) :: reserved(6)
   end type shark_tune
 
+  ! same layout as struct shark_output_info (include/shark_output.h)
+  type, bind(C) :: shark_output_info
+     integer(c_int32_t) :: version, gridspace, mhd, turb, ndustpscal, charging
+     real(c_double) :: time
+     real(c_double) :: unit_t, unit_d, unit_l, unit_v, unit_p, unit_dcol
+     type(c_ptr) :: base_dir
+  end type shark_output_info
+
   interface
      integer(c_int) function shark_gpu_create(cfg, ctx) bind(C, name="shark_gpu_create")
        import :: c_int, c_ptr, shark_cfg
@@ -85,6 +93,24 @@ module shark_gpu_mod
        integer(c_int), value :: reps
        type(shark_tune), intent(out) :: chosen
      end function shark_gpu_autotune
+     ! output(iout) of the reference (write_output.f90:2), from the device state
+     integer(c_int) function shark_gpu_write_output(ctx, info, iout) bind(C, name="shark_gpu_write_output")
+       import :: c_int, c_ptr, shark_output_info
+       type(c_ptr), value :: ctx
+       type(shark_output_info), intent(in) :: info
+       integer(c_int), value :: iout
+     end function shark_gpu_write_output
+     ! restart dumps (path: NUL-terminated)
+     integer(c_int) function shark_gpu_save_state(ctx, path) bind(C, name="shark_gpu_save_state")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+     end function shark_gpu_save_state
+     integer(c_int) function shark_gpu_load_state(ctx, path) bind(C, name="shark_gpu_load_state")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+     end function shark_gpu_load_state
      integer(c_int) function shark_gpu_destroy(ctx) bind(C, name="shark_gpu_destroy")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

@@ -24,18 +24,63 @@ EXPORTS = [
     "shark_gpu_kernel_time", "shark_gpu_set_timing", "shark_gpu_algorithmic_bytes",
     "shark_gpu_destroy", "shark_gpu_last_error", "shark_gpu_version",
     "shark_gpu_get_tune", "shark_gpu_set_tune", "shark_gpu_autotune",
+    "shark_output_write", "shark_gpu_write_output", "shark_gpu_save_state", "shark_gpu_load_state",
     "shark_growth_create", "shark_growth_run", "shark_growth_download", "shark_growth_upload",
     "shark_growth_destroy", "shark_growth_kernel_time", "shark_growth_charge",
     "shark_growth_set_cell_dt", "shark_growth_features", "shark_growth_set_surrogate",
-    "shark_growth_surrogate_stats", "shark_gpu_attach_growth",
+    "shark_growth_surrogate_stats", "shark_gpu_attach_growth", "shark_charge_marchand",
     "shark_mlp_create", "shark_mlp_num_params", "shark_mlp_set_params", "shark_mlp_get_params",
-    "shark_mlp_forward", "shark_mlp_forward_layered", "shark_mlp_grads", "shark_mlp_set_data", "shark_mlp_train",
+    "shark_mlp_forward", "shark_mlp_forward_layered", "shark_mlp_grads", "shark_mlp_grads_weighted",
+    "shark_mlp_set_data_weighted", "shark_mlp_set_data", "shark_mlp_train",
     "shark_mlp_kernel_time", "shark_mlp_destroy",
 ]
 
 
 class SharkGPUError(RuntimeError):
     pass
+
+
+class ShARKOutputInfo(C.Structure):
+    """struct shark_output_info: what info.dat records besides the grid."""
+    _fields_ = [("version", C.c_int32), ("gridspace", C.c_int32), ("mhd", C.c_int32), ("turb", C.c_int32),
+                ("ndustpscal", C.c_int32), ("charging", C.c_int32), ("time", C.c_double),
+                ("unit_t", C.c_double), ("unit_d", C.c_double), ("unit_l", C.c_double),
+                ("unit_v", C.c_double), ("unit_p", C.c_double), ("unit_dcol", C.c_double),
+                ("base_dir", C.c_char_p)]
+
+
+# unit sets of the reference setups (src/setups/<SETUP>/units.f90); disk_polar in cgs from
+# src/commons/phys_const.f90 (au, msun, grav)
+def units_of(setup):
+    if setup == "disk_polar":
+        au, msun, grav = 1.5e13, 2e33, 6.67e-8
+        unit_l = au
+        unit_d = msun / au ** 3
+        unit_dcol = msun / au ** 2
+        unit_t = 1.0 / np.sqrt(grav * msun / unit_l ** 3)
+        unit_v = unit_l / unit_t
+        return dict(unit_t=unit_t, unit_d=unit_d, unit_l=unit_l, unit_v=unit_v, unit_p=unit_v ** 2,
+                    unit_dcol=unit_dcol)
+    return dict(unit_t=1.0, unit_d=1.0, unit_l=1.0, unit_v=1.0, unit_p=1.0, unit_dcol=1.0)
+
+
+def output_info(time, units, gridspace=0, base_dir=None):
+    info = ShARKOutputInfo()
+    info.version = 1
+    info.gridspace = int(gridspace)
+    info.time = float(time)
+    for k, v in units.items():
+        setattr(info, k, float(v))
+    info.base_dir = None if base_dir is None else os.fsencode(base_dir)
+    return info
+
+
+def output_write(info, iout, cfg, q, sdust, position, radii_c):
+    """shark_output_write: the reference's output directory from host arrays (no GPU needed)."""
+    lib = load()
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) if x is not None else None
+            for x in (q, sdust, position, radii_c)]
+    check(lib.shark_output_write(C.byref(info), int(iout), C.byref(cfg), *[_ptr(x) for x in arrs]))
 
 
 class ShARKTune(C.Structure):
@@ -83,6 +128,10 @@ def load():
     lib.shark_gpu_algorithmic_bytes.restype = C.c_int64
     lib.shark_gpu_algorithmic_bytes.argtypes = [C.c_void_p]
     lib.shark_gpu_destroy.argtypes = [C.c_void_p]
+    lib.shark_output_write.argtypes = [C.POINTER(ShARKOutputInfo), C.c_int, C.c_void_p, _P, _P, _P, _P]
+    lib.shark_gpu_write_output.argtypes = [C.c_void_p, C.POINTER(ShARKOutputInfo), C.c_int]
+    lib.shark_gpu_save_state.argtypes = [C.c_void_p, C.c_char_p]
+    lib.shark_gpu_load_state.argtypes = [C.c_void_p, C.c_char_p]
     lib.shark_gpu_get_tune.argtypes = [C.c_void_p, C.POINTER(ShARKTune)]
     lib.shark_gpu_set_tune.argtypes = [C.c_void_p, C.POINTER(ShARKTune)]
     lib.shark_gpu_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(ShARKTune)]
@@ -160,6 +209,17 @@ class Step:
         n = C.c_int64()
         check(self.L.shark_gpu_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def write_output(self, iout, units, gridspace=0, base_dir=None):
+        """The reference's output_NNNNN/ directory of the current state (shark_gpu_write_output)."""
+        info = output_info(self.time, units, gridspace, base_dir)
+        check(self.L.shark_gpu_write_output(self.h, C.byref(info), int(iout)))
+
+    def save_state(self, path):
+        check(self.L.shark_gpu_save_state(self.h, os.fsencode(path)))
+
+    def load_state(self, path):
+        check(self.L.shark_gpu_load_state(self.h, os.fsencode(path)))
 
     def get_tune(self):
         t = ShARKTune()

@@ -42,6 +42,53 @@ class GrowthParams(C.Structure):
     ]
 
 
+class MarchandParams(C.Structure):
+    """struct shark_marchand_params (include/shark_growth.h)."""
+    _fields_ = [
+        ("zeta", C.c_double), ("mu_ions", C.c_double), ("stickeff_el", C.c_double),
+        ("epsilon_ionis", C.c_double), ("nitermax_ionis", C.c_int32), ("resistivities", C.c_int32),
+        ("B_0_lee", C.c_double), ("rhograin", C.c_double), ("gamma", C.c_double),
+        ("unit_l", C.c_double), ("unit_d", C.c_double), ("unit_m", C.c_double), ("unit_v", C.c_double),
+        ("unit_nh", C.c_double),
+    ]
+
+
+def marchand_params(**kw):
+    """Defaults of &IONISATION_PARAMS (src/chemistry/ionisation_commons.f90:71-77), overridden by kw."""
+    p = MarchandParams()
+    p.zeta, p.mu_ions, p.stickeff_el, p.epsilon_ionis = 5e-17, 25.0, 0.5, 1e-6
+    p.nitermax_ionis, p.resistivities = 1000, 1
+    p.B_0_lee, p.rhograin, p.gamma = 3e-5, 2.3, 1.66667
+    p.unit_l = p.unit_d = p.unit_m = p.unit_v = p.unit_nh = 1.0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def charge_marchand(p, rho, rhod, T, cs, sdust, mdust, psi_old=None):
+    """shark_charge_marchand on the GPU: arrays (nbins, ncells) / (ncells).  Returns a dict with
+    zd, ni, ne, psi (the new psi_old) and, with p.resistivities, eta_o, eta_h, eta_a, sigma_o,
+    sigma_p, sigma_h, gamma_d."""
+    L = gpu.load()
+    L.shark_charge_marchand.argtypes = [C.c_void_p, C.c_int64, C.c_int] + [_P] * 12
+    nb, nc = rhod.shape
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    rho, rhod, T, cs, sdust, mdust = map(f, (rho, rhod, T, cs, sdust, mdust))
+    psi = np.zeros(nc) if psi_old is None else f(psi_old).copy()
+    zd = np.zeros((nb, nc))
+    ni = np.zeros(nc)
+    ne = np.zeros(nc)
+    cond = np.zeros((6, nc))
+    gd = np.zeros((nb, nc))
+    gpu.check(L.shark_charge_marchand(C.byref(p), nc, nb, *[gpu._ptr(a) for a in (
+        rho, rhod, T, cs, sdust, mdust, psi, zd, ni, ne, cond, gd)]))
+    out = dict(zd=zd, ni=ni, ne=ne, psi=psi)
+    if p.resistivities:
+        out.update(eta_o=cond[0], eta_h=cond[1], eta_a=cond[2], sigma_o=cond[3], sigma_p=cond[4],
+                   sigma_h=cond[5], gamma_d=gd)
+    return out
+
+
 def make_params(**kw):
     """Defaults of &DUST_PARAMS (src/dust/dust_commons.f90:85-143), overridden by kw."""
     p = GrowthParams()

@@ -78,6 +78,8 @@ def _lib():
         lib.shark_mlp_forward_layered.argtypes = [C.c_void_p, _P32, C.c_int64, _P32]
         lib.shark_mlp_grads.argtypes = [C.c_void_p, _P32, _P32, C.c_int64, _P32, _P32]
         lib.shark_mlp_set_data.argtypes = [C.c_void_p, _P32, _P32, C.c_int64]
+        lib.shark_mlp_grads_weighted.argtypes = [C.c_void_p, _P32, _P32, _P32, C.c_int64, _P32, _P32]
+        lib.shark_mlp_set_data_weighted.argtypes = [C.c_void_p, _P32, _P32, _P32, C.c_int64]
         lib.shark_mlp_train.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_uint64, _P32]
         lib.shark_mlp_kernel_time.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         lib.shark_mlp_destroy.argtypes = [C.c_void_p]
@@ -126,18 +128,24 @@ class MLP:
             y[i:i + mb] = yi
         return y
 
-    def grads(self, x, t):
+    def grads(self, x, t, w=None):
+        """Loss and parameter gradient on one batch; w: per-element loss weights (n x n_out)."""
         x, t = _f32(x), _f32(t)
         loss = C.c_float()
         g = np.empty(self.np, dtype=np.float32)
-        gpu.check(self.L.shark_mlp_grads(self.h, x.ctypes.data_as(_P32), t.ctypes.data_as(_P32),
-                                         x.shape[0], C.byref(loss), g.ctypes.data_as(_P32)))
+        keep = None if w is None else _f32(w)
+        wp = None if keep is None else keep.ctypes.data_as(_P32)
+        gpu.check(self.L.shark_mlp_grads_weighted(self.h, x.ctypes.data_as(_P32), t.ctypes.data_as(_P32), wp,
+                                                  x.shape[0], C.byref(loss), g.ctypes.data_as(_P32)))
         return loss.value, g
 
-    def set_data(self, x, t):
+    def set_data(self, x, t, w=None):
+        """Training set on the device; w: per-element loss weights (n x n_out) or None."""
         x, t = _f32(x), _f32(t)
-        gpu.check(self.L.shark_mlp_set_data(self.h, x.ctypes.data_as(_P32), t.ctypes.data_as(_P32),
-                                            x.shape[0]))
+        keep = None if w is None else _f32(w)
+        wp = None if keep is None else keep.ctypes.data_as(_P32)
+        gpu.check(self.L.shark_mlp_set_data_weighted(self.h, x.ctypes.data_as(_P32), t.ctypes.data_as(_P32),
+                                                     wp, x.shape[0]))
 
     def train(self, epochs, batch, lr, seed=0):
         out = np.zeros(max(epochs, 1), dtype=np.float32)

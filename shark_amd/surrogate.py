@@ -44,8 +44,23 @@ def sample_states(params, tables_sdust, nr, nphi, seed, age_max=1e-2, dt_lo=1e-5
     sh1 = s1 / s1.sum(axis=0)
     sh2 = s2 / s2.sum(axis=0)
     Y = np.concatenate([(np.log(sh2) - np.log(sh1)).T, np.log10(-zm)[:, None]], axis=1).astype(np.float32)
-    state = dict(rhod=s1, sigma=sigma, cs=cs, r=r, dt=dt, rhod_next=s2, nsub=nsub)
+    state = dict(rhod=s1, sigma=sigma, cs=cs, r=r, dt=dt, rhod_next=s2, nsub=nsub,
+                 share_next=sh2.T.astype(np.float32))
     return X, Y, state
+
+
+def mass_weights(share_next, norm, n_in, floor=1e-6):
+    """Per-element loss weights that make the squared error of the standardised outputs a
+    mass-weighted error of the log-shares: bin k of a sample gets (share_k + floor) * s_k^2, with
+    s_k the output's standard deviation (the change of log-share per standard unit), normalised to
+    mean 1 over the bins; the charge output keeps weight 1."""
+    nb = share_next.shape[1]
+    so = norm[2 * n_in + nb + 1:][:nb].astype(np.float64)
+    w = np.empty((share_next.shape[0], nb + 1), dtype=np.float32)
+    wb = (share_next.astype(np.float64) + floor) * (so * so)[None, :]
+    w[:, :nb] = wb / wb.mean()
+    w[:, nb] = 1.0
+    return w
 
 
 def norm_of(X, Y):
@@ -64,13 +79,14 @@ def standardise(X, Y, norm):
 
 
 def train(X, Y, norm, hidden=256, layers=4, epochs=(30, 20, 10), lrs=(2e-3, 5e-4, 1e-4),
-          batch=4096, seed=0, max_batch=65536, progress_every=5, verbose=False):
-    """Adam on the GPU; returns (MLP, loss history)."""
+          batch=4096, seed=0, max_batch=65536, progress_every=5, verbose=False, W=None):
+    """Adam on the GPU (W: per-element loss weights or None); returns (MLP, loss history)."""
     Xs, Ys = standardise(X, Y, norm)
     cfg = M.make_cfg(n_in=X.shape[1], n_hidden=hidden, n_layers=layers, n_out=Y.shape[1],
                      max_batch=max_batch)
     net = M.MLP(cfg, M.init_params(cfg, seed))
-    net.set_data(Xs, Ys)
+    net.set_data(Xs, Ys, W)
+    del Xs, Ys
     hist = []
     for i, (e, lr) in enumerate(zip(epochs, lrs)):
         done = 0

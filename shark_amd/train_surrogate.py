@@ -46,27 +46,34 @@ def main(argv=None):
     ap.add_argument("--eval-cells", type=int, default=8192)
     ap.add_argument("--out", default=os.path.join(S.DATA, "surrogate_disk64.npz"))
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--weighted", action="store_true",
+                    help="mass-weighted loss (surrogate.mass_weights) instead of plain MSE")
     a = ap.parse_args(argv)
 
     p = G.disk_params(nbins=64, max_substeps=1000000)
     sd = tables_sdust(p)
     t0 = time.time()
-    Xs, Ys = [], []
+    Xs, Ys, Ss = [], [], []
     for c in range(a.chunks):
         X, Y, st = S.sample_states(p, sd, a.nr, a.nphi, seed=a.seed + c, age_max=a.age_max,
                                    dt_lo=a.dt_lo, dt_hi=a.dt_hi)
         Xs.append(X)
         Ys.append(Y)
+        if a.weighted:
+            Ss.append(st["share_next"])
         print(f"chunk {c}: {X.shape[0]} samples, sub-steps {st['nsub'].min()}..{st['nsub'].max()} "
               f"({time.time() - t0:.0f} s)", flush=True)
     X = np.concatenate(Xs)
     Y = np.concatenate(Ys)
     del Xs, Ys
     norm = S.norm_of(X, Y)
+    W = S.mass_weights(np.concatenate(Ss), norm, X.shape[1]) if a.weighted else None
+    del Ss
     t1 = time.time()
     epochs = [int(e) for e in a.epochs.split(",")]
     lrs = [float(v) for v in a.lrs.split(",")]
-    net, hist = S.train(X, Y, norm, epochs=epochs, lrs=lrs, batch=a.batch, seed=a.seed, verbose=True)
+    net, hist = S.train(X, Y, norm, epochs=epochs, lrs=lrs, batch=a.batch, seed=a.seed, verbose=True, W=W)
+    del W
     t_train = time.time() - t1
     print(f"trained on {X.shape[0]} samples in {t_train:.0f} s: loss {hist[0]:.3e} -> {hist[-1]:.3e}",
           flush=True)
@@ -92,7 +99,8 @@ def main(argv=None):
     base = S.mass_l1(st["rhod"], st["rhod_next"])
     meta = dict(samples=a.chunks * a.nr * a.nphi, chunks=a.chunks,
                 nr=a.nr, nphi=a.nphi, dt_range=[a.dt_lo, a.dt_hi], age_max=a.age_max,
-                epochs=epochs, lrs=lrs, batch=a.batch, seed=a.seed, loss_first=float(hist[0]),
+                epochs=epochs, lrs=lrs, batch=a.batch, seed=a.seed, weighted=bool(a.weighted),
+                loss_first=float(hist[0]),
                 loss_last=float(hist[-1]), train_seconds=t_train,
                 eval=dict(cells=n, l1_mean=float(err.mean()), l1_p50=float(np.median(err)),
                           l1_p99=float(np.quantile(err, 0.99)), l1_max=float(err.max()),

@@ -165,12 +165,57 @@ def run_charge():
     return m
 
 
+MARCHAND_NAME = "marchand_Collapse_1D_8_n16"
+MARCHAND_EXE = os.path.join(REPO, "oracle", "_ref", MARCHAND_NAME, "shark_ref")
+MARCHAND_OUT = ("zd", "ni", "ne", "psi", "eta_o", "eta_h", "eta_a", "sigma_o", "sigma_p", "sigma_h",
+                "gamma_d")
+
+
+def run_marchand():
+    """ref_marchand.npz: the reference's Marchand charge with resistivities
+    (src/chemistry/ionisation.f90:4-214) on the rescaled Collapse_1D cells of
+    oracle/ref/ref_marchand_harness.f90, called twice (cold start, then from psi_old)."""
+    if not os.path.exists(MARCHAND_EXE):
+        env = dict(os.environ, REF_NAME=MARCHAND_NAME, REF_HARNESS="ref_marchand_harness.f90")
+        subprocess.run([BUILD, "Collapse_1D", "8", "1", "16", "1", "0", "1", "0"], check=True,
+                       env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    nml = BASE.replace("growth              = .true.", "growth              = .false.")
+    nml = nml.replace("fragmentation       = .true.", "fragmentation       = .false.")
+    with tempfile.TemporaryDirectory() as td:
+        open(os.path.join(td, "c.nml"), "w").write(nml)
+        subprocess.run([MARCHAND_EXE, "c.nml", "."], cwd=td, check=True,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"), stdout=subprocess.DEVNULL)
+        m = _meta(os.path.join(td, "meta.txt"))
+        aux = np.fromfile(os.path.join(td, "aux.bin"))
+    nc, nd = m["ncells"], m["ndust"]
+    o = [0]
+
+    def take(n, shape=None):
+        a = aux[o[0]:o[0] + n]
+        o[0] += n
+        return a.reshape(shape) if shape else a
+
+    arrs = dict(u_rho=take(nc), rhod=take(nc * nd, (nd, nc)), T=take(nc), cs=take(nc),
+                sdust=take(nc * nd, (nd, nc)), mdust=take(nc * nd, (nd, nc)))
+    for call in (1, 2):
+        for k in MARCHAND_OUT:
+            shape = (nd, nc) if k in ("zd", "gamma_d") else None
+            arrs[f"{k}_{call}"] = take(nc * nd if shape else nc, shape)
+    assert o[0] == aux.size
+    m.update(namelist=nml)
+    np.savez_compressed(os.path.join(HERE, "ref_marchand.npz"), meta=json.dumps(m), **arrs)
+    return m
+
+
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(CASES) + ["charge"]
+    names = sys.argv[1:] or list(CASES) + ["charge", "marchand"]
     for n in names:
         if n == "charge":
             m = run_charge()
             print("charge cells", m["ncells"], "bins", m["ndust"])
+        elif n == "marchand":
+            m = run_marchand()
+            print("marchand cells", m["ncells"], "bins", m["ndust"])
             continue
         f, dt = CASES[n]
         m = run_case(n, f, dt)

@@ -10,7 +10,7 @@ from shark_amd import cfg as C
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 REF_CASES = sorted(f[4:-4] for f in os.listdir(GOLDEN)
                    if f.startswith("ref_") and f.endswith(".npz")
-                   and not f.startswith(("ref_growth_", "ref_charge")))
+                   and not f.startswith(("ref_growth_", "ref_charge", "ref_output_", "ref_marchand")))
 
 
 def _K(nml):

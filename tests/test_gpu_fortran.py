@@ -19,7 +19,8 @@ def run_driver(args, tmp_path):
     if not os.path.exists(DRIVER):
         pytest.fail("Fortran driver not built (make -C shark_amd/fortran)")
     out = str(tmp_path / "result.bin")
-    r = subprocess.run([DRIVER] + args + [out], capture_output=True, text=True, timeout=120)
+    env = dict(os.environ, SHARK_OUTPUT_DIR=str(tmp_path))
+    r = subprocess.run([DRIVER] + args + [out], capture_output=True, text=True, timeout=120, env=env)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stderr
     return statefile.read_result(out)
@@ -38,6 +39,14 @@ def test_fortran_dustywave_matches_python_host_and_oracle(tmp_path):
     assert np.array_equal(u, up)
     assert np.array_equal(u, o.u())
     assert time == pytest.approx(o.time, rel=0, abs=0)
+    # the driver's output(1) through the Fortran binding: rho and the dust velocity over active cells
+    act = np.zeros(u.shape[1], bool)
+    act[2:-2] = True
+    rho = np.fromfile(str(tmp_path / "output_00001" / "rho"))
+    vd = np.fromfile(str(tmp_path / "output_00001" / "vd"))
+    assert np.array_equal(rho, np.maximum(u[0, act], cfg.smallr))
+    assert np.array_equal(vd, u[6, act] / u[5, act])
+    assert "time        =" in open(str(tmp_path / "output_00001" / "info.dat")).read()
 
 
 def test_fortran_state_file_disk_matches_python_host(tmp_path):

@@ -102,3 +102,29 @@ def test_fused_inference_equals_layered_forward(dims, n):
     assert np.array_equal(yf, yl)
     assert np.array_equal(yf2, yl2)
     assert not np.array_equal(yf2, yf)
+
+
+def test_weighted_gradients_match_oracle():
+    """Per-element loss weights (the learned step's mass weighting) in the loss and its gradient."""
+    dims, n = (69, 256, 4, 65), 512
+    cfg = M.make_cfg(n_in=dims[0], n_hidden=dims[1], n_layers=dims[2], n_out=dims[3], max_batch=1024)
+    p = net(cfg, seed=7)
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((n, dims[0])).astype(np.float32)
+    t = rng.standard_normal((n, dims[3])).astype(np.float32)
+    w = rng.uniform(0.0, 3.0, (n, dims[3])).astype(np.float32)
+    m = M.MLP(cfg, p)
+    try:
+        loss, g = m.grads(x, t, w)
+        loss1, _ = m.grads(x, t)
+    finally:
+        m.close()
+    layers = [(W.astype(np.float64), b.astype(np.float64)) for W, b in M.unpack(cfg, p)]
+    lref, gref = MO.loss_grads(layers, x, t, w=w)
+    lref1, _ = MO.loss_grads(layers, x, t)
+    assert loss == pytest.approx(lref, rel=2e-3) and loss1 == pytest.approx(lref1, rel=2e-3)
+    assert abs(loss - loss1) > 1e-3 * loss1
+    for l, ((dW, db), (rW, rb)) in enumerate(zip(M.unpack(cfg, g), gref)):
+        for a, r in ((dW, rW), (db, rb)):
+            err = np.linalg.norm(a - r) / np.linalg.norm(r)
+            assert err <= 1e-3, (l, err)

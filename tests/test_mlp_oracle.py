@@ -1,6 +1,7 @@
 """The MLP oracle's backward pass against central finite differences (FP64, no rounding), and its
 BF16 operand rounding against known values.  No reference counterpart exists."""
 import numpy as np
+import pytest
 
 import mlp_oracle as M
 
@@ -11,12 +12,14 @@ def small_net(seed=0, dims=(5, 7, 7, 3)):
             for di, do in zip(dims[:-1], dims[1:])]
 
 
-def test_gradients_match_finite_differences():
+@pytest.mark.parametrize("weighted", [False, True])
+def test_gradients_match_finite_differences(weighted):
     layers = small_net()
     rng = np.random.default_rng(1)
     x = rng.standard_normal((9, 5))
     t = rng.standard_normal((9, 3))
-    _, g = M.loss_grads(layers, x, t, bf16_ops=False)
+    w = rng.uniform(0.0, 2.0, (9, 3)) if weighted else None
+    _, g = M.loss_grads(layers, x, t, bf16_ops=False, w=w)
     h = 1e-6
     for l, (W, b) in enumerate(layers):
         for arr, ga in ((W, g[l][0]), (b, g[l][1])):
@@ -25,9 +28,9 @@ def test_gradients_match_finite_differences():
             for i in range(0, flat.size, 3):
                 old = flat[i]
                 flat[i] = old + h
-                lp = M.loss_grads(layers, x, t, bf16_ops=False)[0]
+                lp = M.loss_grads(layers, x, t, bf16_ops=False, w=w)[0]
                 flat[i] = old - h
-                lm = M.loss_grads(layers, x, t, bf16_ops=False)[0]
+                lm = M.loss_grads(layers, x, t, bf16_ops=False, w=w)[0]
                 flat[i] = old
                 fd = (lp - lm) / (2 * h)
                 assert abs(fd - gflat[i]) <= 1e-7 * max(1.0, abs(fd)), (l, i, fd, gflat[i])
