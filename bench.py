@@ -12,9 +12,10 @@ Launch shapes: before the warm-up the library's autotuner (shark_gpu_autotune) t
 candidate hydro strip heights, post segment widths and block orders on the workload's own state
 and keeps the fastest; the state is restored, and the chosen shape is reported under "tune".
 
-Timed region: W warm-up steps, then exactly K steps with the step kernels bracketed by HIP
-events on the library's stream (per-kernel average durations feed the roofline), host
-synchronised on both sides.  One JSON line is printed.
+Timed region: W warm-up steps, then exactly K steps replaying the step's hipGraph (one graph per
+step), host synchronised on both sides; then up to 50 more steps with every kernel bracketed by
+HIP events on the library's stream, whose average durations feed the roofline.  `--events` times
+the event-bracketed launches instead.  One JSON line is printed.
 
 cpu_baseline: the CPU restatement of the same step (oracle/liboracle.so, OpenMP over the
 host cores) on the same workload for a bounded number of steps.
@@ -173,8 +174,9 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="disk")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--graph", action="store_true", help="time the hipGraph replay path instead "
-                    "of per-kernel event-bracketed launches")
+    ap.add_argument("--graph", action="store_true", help="(default) time the hipGraph replay")
+    ap.add_argument("--events", action="store_true", help="time per-kernel event-bracketed launches "
+                    "instead of the hipGraph replay")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-growth", action="store_true", help="skip the growth timing block")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
@@ -210,12 +212,19 @@ def main():
         g.attach_growth(gr, G.SURROGATE)
     g.run(args.warmup)
 
-    g.set_timing(not args.graph)
+    # the timed steps replay the step's hipGraph (the production path); per-kernel times come
+    # from a second, event-bracketed pass. A growth step inside the step is launched directly.
+    use_graph = not learned and not args.events
+    g.set_timing(not use_graph)
     t0 = time.perf_counter()
     n, _ = g.run(args.steps)
     wall = time.perf_counter() - t0
     assert n == args.steps, (n, args.steps)
+    if use_graph:
+        g.set_timing(True)
+        g.run(min(args.steps, 50))
     kt = {k: g.kernel_time(k)[0] for k in ("bc", "hydro", "post", "dt") + (("growth",) if learned else ())}
+    g.set_timing(False)
     learned_info = None
     if learned:
         nfb, raw = gr.surrogate_stats()
@@ -225,8 +234,12 @@ def main():
     nv = 5 + 4 * cfg.ndust
     bytes_hydro = cells * 8 * (2 * nv + 1)            # read u_prim + cs, write u_prim
     bytes_post = cells * 16 * nv if cfg.fargo else 0     # FARGO's second full pass
+    if learned:
+        # the post pass runs in two parts around the growth call: each reads and writes u_prim,
+        # and the dust densities before growth are written by the first and read by the second
+        bytes_post = cells * 8 * (4 * nv + 2 * cfg.ndust)
     step_alg = algorithmic_bytes(cfg)
-    if args.graph or not cfg.fargo:
+    if not cfg.fargo:
         # without FARGO the algorithmic traffic is one pass; rate the whole step against it
         dom, dom_ms, dom_bytes = "step", wall / args.steps * 1e3, step_alg
     else:
@@ -237,7 +250,7 @@ def main():
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": profiled_traffic(args.workload, dom),
-            "traffic_source": "profiles/disk_1024x2048_n16_hbm_traffic.md",
+            "traffic_source": "profiles/disk_1024x2048_n16_hbm_traffic.md" if args.workload == "disk" else None,
             "algorithmic_bytes": dom_bytes, "avg_ms": dom_ms}
     if learned and kt["growth"] > dom_ms:
         # the learned step dominates: rate its network products against the dense BF16 peak
@@ -268,6 +281,7 @@ def main():
         "roofline": roof,
         "kernels_ms": kt,
         "tune": tune,
+        "timed_path": "hipGraph replay, one graph per step" if use_graph else "direct launches",
         "step_roofline": {"algorithmic_bytes": step_alg,
                           "achieved_GBs": step_alg / (wall / args.steps) / 1e9,
                           "frac": step_alg / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},

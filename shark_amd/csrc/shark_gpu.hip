@@ -247,11 +247,14 @@ extern "C" int shark_gpu_upload(shark_ctx* c, const double* u, const double* cs,
   drop_graph(c);
   HIPCHK(hipMemcpyAsync(c->dU, u, nc * a.nvar * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->dcs, cs, nc * 8, hipMemcpyHostToDevice, c->stream));
-  // grain sizes: one value per species when uniform over the grid (every reference setup)
+  // grain sizes: one value per species when uniform over the grid (every reference setup); kept
+  // for the outputs also where the stopping-time law does not read them
   a.sdust_uniform = 1;
   a.sdust = nullptr;
-  if (a.nd > 0 && cf.tstop_kind != SHARK_TSTOP_DUSTYWAVE) {
-    if (!sdust) return fail(SHARK_EINVAL, "sdust is required by this stopping-time law");
+  for (int k = 0; k < SHARK_MAX_DUST; ++k) a.sdust_k[k] = 0.0;
+  if (a.nd > 0 && !sdust && cf.tstop_kind != SHARK_TSTOP_DUSTYWAVE)
+    return fail(SHARK_EINVAL, "sdust is required by this stopping-time law");
+  if (a.nd > 0 && sdust) {
     for (int k = 0; k < a.nd; ++k) {
       const double s0 = sdust[(size_t)k * nc + (a.ndim == 2 ? 2 + 2 * (size_t)a.nxm : 2)];
       a.sdust_k[k] = s0;

@@ -132,7 +132,8 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
     const int k = fluid - 1;
     p[0] = 5 + k; p[1] = 5 + a.nd + k; p[2] = 5 + 2 * a.nd + k; p[3] = 5 + 3 * a.nd + k; p[4] = p[3];
   }
-  const bool need_cs = GAS || a.solverdust == 3;
+  // sound speed: the gas, and the HLLC dust solver; known at compile time when SOLV is
+  const bool need_cs = GAS || (SOLV < 0 ? a.solverdust == 3 : SOLV == 3);
   const int ix = 1 + 60 * g + lane;
   const int ixc = ix < a.nxm ? ix : a.nxm;
   const int ixl = ixc > 1 ? ixc - 1 : 1;
