@@ -131,14 +131,15 @@ int64_t shark_gpu_algorithmic_bytes(const shark_cfg* cfg);
 
 /* Launch shapes of the step kernels.  No counterpart in the reference (its loops have no launch
  * shape).  Every shape gives bit-identical results; only the speed differs.
- *   strip_h   rows per hydro work item (one wavefront marches strip_h rows of a 60-column
- *             group; 2 extra rows are re-read at each end), 1..4096
+ *   strip_h   rows per dust hydro work item (one wavefront marches strip_h rows of a 60-column
+ *             group of one species; 2 extra rows are re-read at each end), 1..4096
+ *   strip_gas the same for the gas launch, 1..4096
  *   post_tw   cells per post row segment: 16 or 32 up to 16 species, 8 or 16 for 64 species
  *   xcd_order 1: blocks are renumbered so each of the 8 XCDs takes a contiguous band of work
  *             (neighbouring rows share one L2); 0: hardware order */
 typedef struct shark_tune {
-  int32_t strip_h, post_tw, xcd_order;
-  int32_t reserved_i[5];
+  int32_t strip_h, post_tw, xcd_order, strip_gas;
+  int32_t reserved_i[4];
   double hydro_ms, post_ms;   /* autotune: kernel times of the chosen shape (ms per step) */
   double reserved[6];
 } shark_tune;
@@ -148,8 +149,9 @@ int shark_gpu_get_tune(shark_ctx* ctx, shark_tune* t);
 int shark_gpu_set_tune(shark_ctx* ctx, const shark_tune* t);
 
 /* The autotuner: times the step kernels of this context on its own uploaded state over a set of
- * candidate shapes (strip heights 4..256, both segment widths, XCD order on and off; coordinate
- * search, `reps` timed steps per candidate, median kernel time) and keeps the fastest.  The
+ * candidate shapes (gas and dust strip heights 4..256, both segment widths, XCD order on and
+ * off; coordinate search, `reps` timed steps per candidate, median kernel time) and keeps the
+ * fastest.  The
  * state, time and dt are restored afterwards, so the run continues exactly as without tuning.
  * Needs device memory for one copy of u_prim and of the post-hydro state. */
 int shark_gpu_autotune(shark_ctx* ctx, int reps, shark_tune* chosen);

@@ -155,8 +155,10 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   for (int k = 0; k < SHARK_MAX_DUST; ++k) a.K[k] = cfg->K[k];
   // launch shapes (shark_gpu_set_tune / shark_gpu_autotune change them)
   a.strip_h = 32;
+  a.strip_g = 32;
   a.ngroups = (a.nx + 59) / 60;
   a.nstrips = a.ndim == 2 ? (a.ny + a.strip_h - 1) / a.strip_h : 1;
+  a.nstrips_g = a.ndim == 2 ? (a.ny + a.strip_g - 1) / a.strip_g : 1;
   a.post_tw = shark::post_tw_default(a.nd);
   a.xcd_order = 1;
 
@@ -514,10 +516,12 @@ extern "C" int shark_gpu_attach_growth(shark_ctx* c, shark_growth* g, int mode) 
 }
 
 // ------------------------------------------------------------------ launch shapes and the tuner
-static void apply_shape(shark_ctx* c, int strip_h, int tw, int xcd) {
+static void apply_shape(shark_ctx* c, int strip_h, int strip_g, int tw, int xcd) {
   StepArgs& a = c->a;
   a.strip_h = strip_h;
   a.nstrips = a.ndim == 2 ? (a.ny + strip_h - 1) / strip_h : 1;
+  a.strip_g = strip_g;
+  a.nstrips_g = a.ndim == 2 ? (a.ny + strip_g - 1) / strip_g : 1;
   a.post_tw = tw;
   a.xcd_order = xcd;
   drop_graph(c);   // the captured step holds the old shape
@@ -527,6 +531,7 @@ extern "C" int shark_gpu_get_tune(shark_ctx* c, shark_tune* t) {
   if (!c || !t) return fail(SHARK_EINVAL, "null argument");
   memset(t, 0, sizeof(*t));
   t->strip_h = c->a.strip_h;
+  t->strip_gas = c->a.strip_g;
   t->post_tw = c->a.post_tw;
   t->xcd_order = c->a.xcd_order;
   return SHARK_OK;
@@ -535,11 +540,13 @@ extern "C" int shark_gpu_get_tune(shark_ctx* c, shark_tune* t) {
 extern "C" int shark_gpu_set_tune(shark_ctx* c, const shark_tune* t) {
   if (!c || !t) return fail(SHARK_EINVAL, "null argument");
   if (t->strip_h < 1 || t->strip_h > 4096) return fail(SHARK_EINVAL, "strip_h %d outside 1..4096", t->strip_h);
+  if (t->strip_gas < 1 || t->strip_gas > 4096)
+    return fail(SHARK_EINVAL, "strip_gas %d outside 1..4096", t->strip_gas);
   if (!shark::post_tw_ok(c->a.nd, t->post_tw))
     return fail(SHARK_EINVAL, "post_tw %d not built for %d species", t->post_tw, c->a.nd);
   if (t->xcd_order != 0 && t->xcd_order != 1) return fail(SHARK_EINVAL, "xcd_order must be 0 or 1");
   HIPCHK(hipStreamSynchronize(c->stream));
-  apply_shape(c, t->strip_h, t->post_tw, t->xcd_order);
+  apply_shape(c, t->strip_h, t->strip_gas, t->post_tw, t->xcd_order);
   return SHARK_OK;
 }
 
@@ -568,38 +575,44 @@ static int restore_state(shark_ctx* c, const TuneSave& sv, size_t nbytes) {
   return SHARK_OK;
 }
 
-// Median hydro and post times (ms) of `reps` steps of the current shape, each run from the saved
-// state after one untimed step.
-static int time_shape(shark_ctx* c, const TuneSave& sv, size_t nbytes, int reps, double* hyd, double* post) {
+// Median gas-hydro, dust-hydro and post times (ms) of `reps` steps of the current shape, each run
+// from the saved state after one untimed step.
+static int time_shape(shark_ctx* c, const TuneSave& sv, size_t nbytes, int reps, double* tg, double* td,
+                      double* post) {
   int rc = restore_state(c, sv, nbytes);
   if (rc) return rc;
   shark::launch_arm(c->a, INFINITY, c->stream);
-  std::vector<hipEvent_t> ev((size_t)3 * (reps + 1));
+  std::vector<hipEvent_t> ev((size_t)4 * (reps + 1));
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   for (int i = 0; i <= reps; ++i) {
-    hipEvent_t* e = &ev[(size_t)3 * i];
+    hipEvent_t* e = &ev[(size_t)4 * i];
     shark::launch_bc(c->a, c->stream);
     HIPCHK(hipEventRecord(e[0], c->stream));
-    shark::launch_hydro(c->a, c->stream);
+    shark::launch_hydro_fluid(c->a, true, c->stream);
     HIPCHK(hipEventRecord(e[1], c->stream));
-    shark::launch_post(c->a, c->stream);
+    shark::launch_hydro_fluid(c->a, false, c->stream);
     HIPCHK(hipEventRecord(e[2], c->stream));
+    shark::launch_post(c->a, c->stream);
+    HIPCHK(hipEventRecord(e[3], c->stream));
     shark::launch_finalize(c->a, c->stream);
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
-  std::vector<float> th, tp;
+  std::vector<float> t0, t1, t2;
   for (int i = 1; i <= reps; ++i) {   // step 0 is the untimed warm-up
-    hipEvent_t* e = &ev[(size_t)3 * i];
-    float a = 0.f, b = 0.f;
+    hipEvent_t* e = &ev[(size_t)4 * i];
+    float a = 0.f, b = 0.f, d = 0.f;
     HIPCHK(hipEventElapsedTime(&a, e[0], e[1]));
     HIPCHK(hipEventElapsedTime(&b, e[1], e[2]));
-    th.push_back(a);
-    tp.push_back(b);
+    HIPCHK(hipEventElapsedTime(&d, e[2], e[3]));
+    t0.push_back(a);
+    t1.push_back(b);
+    t2.push_back(d);
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
-  *hyd = median(th);
-  *post = median(tp);
+  *tg = median(t0);
+  *td = median(t1);
+  *post = median(t2);
   return SHARK_OK;
 }
 
@@ -618,18 +631,20 @@ extern "C" int shark_gpu_autotune(shark_ctx* c, int reps, shark_tune* chosen) {
   HIPCHK(hipMemcpyAsync(sv.B, c->dB, nbytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(sv.sc, c->dsc, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream));
 
-  int best_h = a.strip_h, best_tw = a.post_tw, best_x = a.xcd_order;
-  double th = 0, tp = 0, bh = 0, bp = 0;
+  int best_h = a.strip_h, best_g = a.strip_g, best_tw = a.post_tw, best_x = a.xcd_order;
+  double tg = 0, td = 0, tp = 0, bh = 0, bp = 0;
   int rc = 0;
-  // 1. strip height by hydro time (2D only: a 1D run has one strip)
+  // 1. strip heights (2D only: a 1D run has one strip): dust by dust-hydro time, gas by gas-hydro
+  //    time (the gas launch has one work item per strip and column group, so it wants short strips)
   if (a.ndim == 2) {
     static const int kStrip[] = {4, 8, 16, 24, 32, 48, 64, 96, 128, 256};
-    double bt = INFINITY;
+    double btd = INFINITY, btg = INFINITY;
     for (int h : kStrip) {
       if (h > a.ny && h != kStrip[0]) break;
-      apply_shape(c, h, best_tw, best_x);
-      if ((rc = time_shape(c, sv, nbytes, reps, &th, &tp))) break;
-      if (th < bt) { bt = th; best_h = h; }
+      apply_shape(c, h, h, best_tw, best_x);
+      if ((rc = time_shape(c, sv, nbytes, reps, &tg, &td, &tp))) break;
+      if (a.nd > 0 && td < btd) { btd = td; best_h = h; }
+      if (tg < btg) { btg = tg; best_g = h; }
     }
   }
   // 2. post segment width by post time
@@ -638,8 +653,8 @@ extern "C" int shark_gpu_autotune(shark_ctx* c, int reps, shark_tune* chosen) {
     const int ntw = shark::post_tw_candidates(a.nd, tws);
     double bt = INFINITY;
     for (int k = 0; k < ntw && !rc; ++k) {
-      apply_shape(c, best_h, tws[k], best_x);
-      if ((rc = time_shape(c, sv, nbytes, reps, &th, &tp))) break;
+      apply_shape(c, best_h, best_g, tws[k], best_x);
+      if ((rc = time_shape(c, sv, nbytes, reps, &tg, &td, &tp))) break;
       if (tp < bt) { bt = tp; best_tw = tws[k]; }
     }
   }
@@ -647,20 +662,21 @@ extern "C" int shark_gpu_autotune(shark_ctx* c, int reps, shark_tune* chosen) {
   if (!rc) {
     double bt = INFINITY;
     for (int x = 1; x >= 0 && !rc; --x) {
-      apply_shape(c, best_h, best_tw, x);
-      if ((rc = time_shape(c, sv, nbytes, reps, &th, &tp))) break;
-      if (th + tp < bt) { bt = th + tp; best_x = x; bh = th; bp = tp; }
+      apply_shape(c, best_h, best_g, best_tw, x);
+      if ((rc = time_shape(c, sv, nbytes, reps, &tg, &td, &tp))) break;
+      if (tg + td + tp < bt) { bt = tg + td + tp; best_x = x; bh = tg + td; bp = tp; }
     }
   }
   // the run continues from the state it had before tuning, with the chosen shape
   const int rr = restore_state(c, sv, nbytes);
-  apply_shape(c, best_h, best_tw, best_x);
+  apply_shape(c, best_h, best_g, best_tw, best_x);
   HIPCHK(hipStreamSynchronize(c->stream));
   if (rc) return rc;
   if (rr) return rr;
   if (chosen) {
     memset(chosen, 0, sizeof(*chosen));
     chosen->strip_h = best_h;
+    chosen->strip_gas = best_g;
     chosen->post_tw = best_tw;
     chosen->xcd_order = best_x;
     chosen->hydro_ms = bh;

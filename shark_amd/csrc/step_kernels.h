@@ -42,8 +42,9 @@ struct StepArgs {
   int nx, ny, nd, nxm, nym, nc, nvar, ndim;
   int geom, solver, solverdust, slope, force_kick, force_kind, tstop_kind, inloop_kind;
   int bc_kind, drag, back_reaction, fargo, disk_bc_mode, sdust_uniform;
-  int strip_h;          // rows per hydro work item
-  int ngroups, nstrips; // hydro decomposition
+  int strip_h;          // rows per dust hydro work item
+  int ngroups, nstrips; // hydro decomposition (dust)
+  int strip_g, nstrips_g;  // rows per gas hydro work item, and the gas strips
   int post_tw;          // cells per post row segment (post_tw_ok)
   int xcd_order;        // 1: XCD-aware block order in the hydro and post launches
   double gamma, cfl, smallr, dust_ratio_min;
@@ -62,6 +63,7 @@ int post_tw_candidates(int nd, int* out);   // returns the count (<= 4)
 
 void launch_bc(const StepArgs& a, hipStream_t s);
 void launch_hydro(const StepArgs& a, hipStream_t s);
+void launch_hydro_fluid(const StepArgs& a, bool gas, hipStream_t s);   // the gas or the dust launch
 void launch_post(const StepArgs& a, hipStream_t s);
 // the post pass split around a growth step: part 1 remap + drag, part 2 in-loop hook + Courant
 void launch_post_part(const StepArgs& a, int part, hipStream_t s);

@@ -155,8 +155,9 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
 
   int y0 = 1, y1 = 1;
   if (NDIM == 2) {
-    y0 = 3 + strip * a.strip_h;
-    y1 = y0 + a.strip_h - 1;
+    const int sh = GAS ? a.strip_g : a.strip_h;
+    y0 = 3 + strip * sh;
+    y1 = y0 + sh - 1;
     if (y1 > a.ny + 2) y1 = a.ny + 2;
   }
 
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(256, (GAS ? 2 : 1)) void k_hydro(StepArgs a) {
   const unsigned nb = gridDim.x;
   if (a.xcd_order && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
   const long item = (long)b * 4 + wave;
-  const long G = a.ngroups, S = a.nstrips, F = GAS ? 1 : a.nd;
+  const long G = a.ngroups, S = GAS ? a.nstrips_g : a.nstrips, F = GAS ? 1 : a.nd;
   if (item >= G * S * F) return;
   const int g = (int)(item % G);
   const long t = item / G;
@@ -951,7 +952,7 @@ void launch_bc(const StepArgs& a, hipStream_t s) {
 
 template <int NDIM, int GEOM, bool GAS, int SLOPE>
 static void hydro_launch1(const StepArgs& a, hipStream_t s) {
-  const long items = (long)a.ngroups * a.nstrips * (GAS ? 1 : a.nd);
+  const long items = (long)a.ngroups * (GAS ? a.nstrips_g : a.nstrips) * (GAS ? 1 : a.nd);
   const dim3 grid((unsigned)((items + 3) / 4));
   const bool fast = GAS ? a.solver == 1 : a.solverdust == 0;   // HLL gas / Huang-Bai dust
   if (fast) hipLaunchKernelGGL((k_hydro<NDIM, GEOM, GAS, SLOPE, GAS ? 1 : 0>), grid, dim3(256), 0, s, a);
@@ -978,6 +979,10 @@ static void hydro_launch_geom(const StepArgs& a, hipStream_t s) {
 void launch_hydro(const StepArgs& a, hipStream_t s) {
   hydro_launch_geom<true>(a, s);
   if (a.nd > 0) hydro_launch_geom<false>(a, s);
+}
+void launch_hydro_fluid(const StepArgs& a, bool gas, hipStream_t s) {
+  if (gas) hydro_launch_geom<true>(a, s);
+  else if (a.nd > 0) hydro_launch_geom<false>(a, s);
 }
 
 template <int ND, int PART>

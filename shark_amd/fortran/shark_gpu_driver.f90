@@ -59,8 +59,8 @@ program shark_gpu_driver
        opt(has_cond, cond), opt1(has_fargo, fargo_v)), 'shark_gpu_upload')
   ! launch shapes picked on this state (the state is restored, so the run is unchanged)
   call shark_check(shark_gpu_autotune(ctx, 3_c_int, tune), 'shark_gpu_autotune')
-  print '(A,I0,A,I0,A,I0)', 'shark_gpu_driver: tuned strip_h=', tune%strip_h, ' post_tw=', &
-       tune%post_tw, ' xcd_order=', tune%xcd_order
+  print '(A,I0,A,I0,A,I0,A,I0)', 'shark_gpu_driver: tuned strip_h=', tune%strip_h, ' strip_gas=', &
+       tune%strip_gas, ' post_tw=', tune%post_tw, ' xcd_order=', tune%xcd_order
   call shark_check(shark_gpu_run(ctx, nsteps, tend, steps_done, time_out), 'shark_gpu_run')
   call shark_check(shark_gpu_download(ctx, c_loc(u), c_null_ptr), 'shark_gpu_download')
   ! the reference's output(1) of the final state (code units: the dusty wave's units are all 1)

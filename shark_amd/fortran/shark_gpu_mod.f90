@@ -33,8 +33,8 @@ module shark_gpu_mod
 
   ! same layout as struct shark_tune (launch shapes of the step kernels)
   type, bind(C) :: shark_tune
-     integer(c_int32_t) :: strip_h, post_tw, xcd_order
-     integer(c_int32_t) :: reserved_i(5)
+     integer(c_int32_t) :: strip_h, post_tw, xcd_order, strip_gas
+     integer(c_int32_t) :: reserved_i(4)
      real(c_double) :: hydro_ms, post_ms
      real(c_double) :: reserved(6)
   end type shark_tune

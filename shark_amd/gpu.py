@@ -86,12 +86,12 @@ def output_write(info, iout, cfg, q, sdust, position, radii_c):
 class ShARKTune(C.Structure):
     """struct shark_tune: launch shapes of the step kernels."""
     _fields_ = [("strip_h", C.c_int32), ("post_tw", C.c_int32), ("xcd_order", C.c_int32),
-                ("reserved_i", C.c_int32 * 5), ("hydro_ms", C.c_double), ("post_ms", C.c_double),
-                ("reserved", C.c_double * 6)]
+                ("strip_gas", C.c_int32), ("reserved_i", C.c_int32 * 4), ("hydro_ms", C.c_double),
+                ("post_ms", C.c_double), ("reserved", C.c_double * 6)]
 
     def as_dict(self):
-        return dict(strip_h=self.strip_h, post_tw=self.post_tw, xcd_order=self.xcd_order,
-                    hydro_ms=self.hydro_ms, post_ms=self.post_ms)
+        return dict(strip_h=self.strip_h, strip_gas=self.strip_gas, post_tw=self.post_tw,
+                    xcd_order=self.xcd_order, hydro_ms=self.hydro_ms, post_ms=self.post_ms)
 
 
 _LIB = None
@@ -226,11 +226,13 @@ class Step:
         check(self.L.shark_gpu_get_tune(self.h, C.byref(t)))
         return t.as_dict()
 
-    def set_tune(self, strip_h=None, post_tw=None, xcd_order=None):
+    def set_tune(self, strip_h=None, post_tw=None, xcd_order=None, strip_gas=None):
         t = ShARKTune()
         check(self.L.shark_gpu_get_tune(self.h, C.byref(t)))
         if strip_h is not None:
             t.strip_h = int(strip_h)
+        if strip_gas is not None:
+            t.strip_gas = int(strip_gas)
         if post_tw is not None:
             t.post_tw = int(post_tw)
         if xcd_order is not None:

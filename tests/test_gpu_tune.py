@@ -42,12 +42,12 @@ def test_every_shape_is_bit_identical(which):
     dto = np.array([o.step() for _ in range(n)])
     m = active_mask(cfg)
     uo = o.u()
-    for strip in (4, 13, 32, 200):
+    for strip, sg in ((4, 8), (13, 13), (32, 4), (200, 200)):
         for tw in tws:
             for xcd in (0, 1):
-                dts, u = _run(case, n, strip_h=strip, post_tw=tw, xcd_order=xcd)
-                assert np.array_equal(dts, dto), (strip, tw, xcd)
-                assert np.array_equal(u[:, m], uo[:, m]), (strip, tw, xcd)
+                dts, u = _run(case, n, strip_h=strip, strip_gas=sg, post_tw=tw, xcd_order=xcd)
+                assert np.array_equal(dts, dto), (strip, sg, tw, xcd)
+                assert np.array_equal(u[:, m], uo[:, m]), (strip, sg, tw, xcd)
 
 
 def test_autotune_restores_the_run():
@@ -57,7 +57,7 @@ def test_autotune_restores_the_run():
     try:
         t = g.autotune(reps=3)
         print("chosen", t, flush=True)
-        assert t["strip_h"] >= 1 and t["post_tw"] in (16, 32) and t["xcd_order"] in (0, 1)
+        assert t["strip_h"] >= 1 and t["strip_gas"] >= 1 and t["post_tw"] in (16, 32) and t["xcd_order"] in (0, 1)
         assert t["hydro_ms"] > 0 and t["post_ms"] > 0
         assert g.get_tune()["strip_h"] == t["strip_h"]
         assert g.time == 0.0
@@ -75,7 +75,7 @@ def test_autotune_one_dimensional():
     g = gpu.Step(*case)
     try:
         t = g.autotune(reps=2)
-        assert t["strip_h"] == 32          # a 1D run has one strip; the height is not swept
+        assert t["strip_h"] == 32 and t["strip_gas"] == 32   # a 1D run has one strip: not swept
         dts = np.array([g.step() for _ in range(20)])
         u = g.download()
     finally:
@@ -90,6 +90,8 @@ def test_set_tune_rejects_unbuilt_shapes():
             g.set_tune(post_tw=8)
         with pytest.raises(gpu.SharkGPUError, match="strip_h"):
             g.set_tune(strip_h=0)
+        with pytest.raises(gpu.SharkGPUError, match="strip_gas"):
+            g.set_tune(strip_gas=5000)
         with pytest.raises(gpu.SharkGPUError, match="xcd_order"):
             g.set_tune(xcd_order=2)
         assert g.get_tune()["post_tw"] == 32   # unchanged after the refusals
