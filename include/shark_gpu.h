@@ -48,7 +48,8 @@ enum { SHARK_DUST_HUANG_BAI = 0, SHARK_DUST_LLF = 1, SHARK_DUST_HLL = 2, SHARK_D
 /* per-step setup hooks, resolved at link time in the reference (bin/Makefile:67) */
 enum { SHARK_BC_PERIODIC = 0,   /* src/grid/boundary_types.f90:149-186 (DUSTYWAVE) */
        SHARK_BC_SHEAR_SI = 1,   /* src/setups/SI/boundaries.f90:10-85 */
-       SHARK_BC_DISK = 2 };     /* src/setups/disk_polar/boundaries.f90:10-62 */
+       SHARK_BC_DISK = 2,       /* src/setups/disk_polar/boundaries.f90:10-62 */
+       SHARK_BC_SOD = 3 };      /* src/setups/SOD/boundaries.f90:10-80 */
 enum { SHARK_TSTOP_DUSTYWAVE = 0,  /* src/setups/DUSTYWAVE/setup.f90:217 */
        SHARK_TSTOP_SI = 1,         /* src/setups/SI/setup.f90:392 */
        SHARK_TSTOP_EPSTEIN_DISK = 2 }; /* src/setups/disk_polar/setup.f90:436 */
@@ -67,11 +68,11 @@ typedef struct shark_cfg {
   int32_t nx, ny, ndust;    /* active cells per direction (ny=1 means 1D), dust species */
   int32_t nghost;           /* must be 2 (bin/Makefile:8) */
   int32_t geom, solver, solverdust, slope_type;
-  int32_t iso_cs;           /* must be 1: isothermal gas (adiabatic path not built) */
+  int32_t iso_cs;           /* 1: isothermal (setup's cs); 0: adiabatic (P from E, gamma), gas only */
   int32_t force_kick, fargo, drag, dust_back_reaction, growth, viscosity;
   int32_t bc_kind, tstop_kind, force_kind, inloop_kind;
   int32_t disk_bc_mode;     /* 0: reference indexing (boundaries.f90:45-46), 1: per-row copy */
-  int32_t pad0;
+  int32_t shock_dir;        /* SOD: direction_shock (0: along x, 1: along y) */
   double gamma, cfl, smallr, smallc, smallp, dust_ratio_min;
   /* DUSTYWAVE */
   double K[SHARK_MAX_DUST];
@@ -80,7 +81,9 @@ typedef struct shark_cfg {
   double rhograin;          /* grain material density in code units (rhograin/unit_d) */
   /* disk_polar */
   double mstar, r_relax, n_rel, smooth_r;
-  double reserved[16];
+  /* SOD (src/setups/SOD/setup_commons.f90): left and right states of the shock tube */
+  double rho_l, rho_r, p_l, p_r;
+  double reserved[12];
 } shark_cfg;
 
 typedef struct shark_ctx shark_ctx;

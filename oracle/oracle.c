@@ -228,17 +228,59 @@ static void bc_disk(oracle* o) {
     }
 }
 
+/* Sod shock tube, src/setups/SOD/boundaries.f90:10-80: the ghost layers across the shock
+ * direction hold the initial left/right states (E = P/(gamma-1), at rest), the other direction is
+ * periodic; y layers first (2D, active columns), then x layers over every row */
+static void sod_state(oracle* o, int c, double rho, double P) {
+  U(o->irho, c) = rho;
+  U(o->iP, c) = P / (o->cfg.gamma - 1.0);
+  U(o->ivx, c) = 0.0;
+  U(o->ivy, c) = 0.0;
+  U(o->ivz, c) = 0.0;
+}
+
+static void bc_sod(oracle* o) {
+  const shark_cfg* p = &o->cfg;
+  if (o->ndim == 2)
+    for (int ix = o->fa; ix <= o->la; ++ix)
+      for (int iy = 1; iy <= 2; ++iy) {
+        int bl = cidx(o, ix, iy), br = cidx(o, ix, o->nym + 1 - iy);
+        int al = cidx(o, ix, o->lay - 2 + iy), ar = cidx(o, ix, o->fay + 2 - iy);
+        if (p->shock_dir == 1) {
+          sod_state(o, bl, p->rho_l, p->p_l);
+          sod_state(o, br, p->rho_r, p->p_r);
+        } else {
+          for (int v = 0; v < o->nvar; ++v) { U(v, bl) = U(v, al); U(v, br) = U(v, ar); }
+        }
+      }
+  for (int ix = 1; ix <= 2; ++ix)
+    for (int iy = 1; iy <= o->nym; ++iy) {
+      int bl = cidx(o, ix, iy), br = cidx(o, o->nxm + 1 - ix, iy);
+      int al = cidx(o, o->la - 2 + ix, iy), ar = cidx(o, o->fa + 2 - ix, iy);
+      if (p->shock_dir == 0) {
+        sod_state(o, bl, p->rho_l, p->p_l);
+        sod_state(o, br, p->rho_r, p->p_r);
+      } else {
+        for (int v = 0; v < o->nvar; ++v) { U(v, bl) = U(v, al); U(v, br) = U(v, ar); }
+      }
+    }
+}
+
 static void apply_boundaries(oracle* o) {
   switch (o->cfg.bc_kind) {
     case SHARK_BC_PERIODIC: bc_periodic(o); break;
     case SHARK_BC_SHEAR_SI: bc_shear_si(o); break;
     case SHARK_BC_DISK: bc_disk(o); break;
+    case SHARK_BC_SOD: bc_sod(o); break;
   }
 }
 
 /* ---------------------------------------------------------------- ctoprim */
-/* src/solver/solve.f90:195-253, isothermal branch (:224-226) */
+/* src/solver/solve.f90:195-253: isothermal (:224-226, P = rho cs^2 with the setup's cs) or
+ * adiabatic (:218-222, P from the total energy, cs = sqrt(gamma P / rho) written back) */
 static void ctoprim(oracle* o) {
+  const int iso = o->cfg.iso_cs == 1;
+  const double gamma = o->cfg.gamma;
   #pragma omp parallel for schedule(static)
   for (int c = 0; c < o->nc; ++c) {
     double rho = U(o->irho, c);
@@ -246,7 +288,15 @@ static void ctoprim(oracle* o) {
     Q(o->ivx, c) = U(o->ivx, c) / rho;
     Q(o->ivy, c) = U(o->ivy, c) / rho;
     Q(o->ivz, c) = U(o->ivz, c) / rho;
-    Q(o->iP, c) = rho * (o->cs[c] * o->cs[c]);
+    if (iso) {
+      Q(o->iP, c) = rho * (o->cs[c] * o->cs[c]);
+    } else {
+      const double vx = U(o->ivx, c) / rho, vy = U(o->ivy, c) / rho, vz = U(o->ivz, c) / rho;
+      const double ekin = HALF * rho * (vx * vx) + HALF * rho * (vy * vy) + HALF * rho * (vz * vz);
+      const double P = fmax_f((gamma - 1.0) * (U(o->iP, c) - ekin - 0.0), o->cfg.smallp);
+      Q(o->iP, c) = P;
+      o->cs[c] = sqrt(gamma * P / Q(o->irho, c));
+    }
     for (int k = 0; k < o->nd; ++k) {
       double rd = U(IRHOD(k), c);
       Q(IRHOD(k), c) = rd;
@@ -434,26 +484,33 @@ static void predictor(oracle* o) {
     double dry = 0, dpy = 0, duy = 0, dvy = 0, dwy = 0;
     double r_rho = Q(o->irho, c), u = Q(o->ivx, c), dux = DQ(o->ivx, c, 0), drx = DQ(o->irho, c, 0);
     double w = Q(o->ivz, c), dwx = DQ(o->ivz, c, 0), v = Q(o->ivy, c), dvx = DQ(o->ivy, c, 0);
+    const int iso = p->iso_cs == 1;
     double cs2 = o->cs[c] * o->cs[c];
-    double P = Q(o->irho, c) * cs2;               /* iso_cs==1, :125-128 */
-    double dpx = DQ(o->irho, c, 0) * cs2;
+    double P = Q(o->iP, c), dpx = DQ(o->iP, c, 0);   /* :123-124 */
+    if (iso) {                                     /* :125-128 */
+      P = Q(o->irho, c) * cs2;
+      dpx = DQ(o->irho, c, 0) * cs2;
+    }
     if (o->ndim == 2) {
       duy = DQ(o->ivx, c, 1); dry = DQ(o->irho, c, 1); dvy = DQ(o->ivy, c, 1);
-      dwy = DQ(o->ivz, c, 1); dpy = DQ(o->irho, c, 1) * cs2;
+      dwy = DQ(o->ivz, c, 1); dpy = DQ(o->iP, c, 1);
+      if (iso) dpy = DQ(o->irho, c, 1) * cs2;
     }
     if (p->force_kick) {
       u = u + (FORCE(c, 0) * HALF) * dt;
       v = v + (FORCE(c, 1) * HALF) * dt;
       w = w + (FORCE(c, 2) * HALF) * dt;
     }
-    double sr0, su0, sv0, sw0;
+    double sr0, su0, sv0, sw0, sp0;
     if (p->geom == SHARK_GEOM_POLAR) {          /* :214-220 (overwrites the :196 block) */
       sr0 = ((-(u * drx)) - v * dry - (dux + dvy) * r_rho) - (r_rho * u) / rp;
+      sp0 = ((-(u * dpx)) - v * dpy - (dux + dvy) * gamma * P) - gamma * P * u / rp;
       su0 = ((-(u * dux)) - v * duy - dpx / r_rho) + (v * v) / rp;
       sv0 = ((-(u * dvx)) - v * dvy - dpy / r_rho) - (u * v) / rp;
       sw0 = (-(u * dwx)) - v * dwy;
     } else {                                     /* :196-202 */
       sr0 = (-(u * drx)) - v * dry - (dux + dvy) * r_rho;
+      sp0 = (-(u * dpx)) - v * dpy - (dux + dvy) * gamma * P;
       su0 = (-(u * dux)) - v * duy - dpx / r_rho;
       sv0 = (-(u * dvx)) - v * dvy - dpy / r_rho;
       sw0 = (-(u * dwx)) - v * dwy;
@@ -467,6 +524,10 @@ static void predictor(oracle* o) {
     QP(o->ivx, c, 0) = u + HALF * dt * su0 - HALF * dux * dxl;
     QP(o->ivy, c, 0) = v + HALF * dt * sv0 - HALF * dvx * dxl;
     QP(o->ivz, c, 0) = w + HALF * dt * sw0 - HALF * dwx * dxl;
+    if (!iso) {                                  /* :261, :266 */
+      QM(o->iP, c, 0) = fmax_f(P + HALF * dt * sp0 + HALF * dpx * dxl, p->smallp);
+      QP(o->iP, c, 0) = fmax_f(P + HALF * dt * sp0 - HALF * dpx * dxl, p->smallp);
+    }
     if (o->ndim == 2) {                          /* :271-282 */
       double dyl = rp * DX(c, 1);
       QM(o->irho, c, 1) = r_rho + HALF * dt * sr0 + HALF * dry * dyl;
@@ -477,8 +538,11 @@ static void predictor(oracle* o) {
       QP(o->ivx, c, 1) = u + HALF * dt * su0 - HALF * duy * dyl;
       QP(o->ivy, c, 1) = v + HALF * dt * sv0 - HALF * dvy * dyl;
       QP(o->ivz, c, 1) = w + HALF * dt * sw0 - HALF * dwy * dyl;
+      if (!iso) {                                /* :275, :280 */
+        QM(o->iP, c, 1) = fmax_f(P + HALF * dt * sp0 + HALF * dpy * dyl, p->smallp);
+        QP(o->iP, c, 1) = fmax_f(P + HALF * dt * sp0 - HALF * dpy * dyl, p->smallp);
+      }
     }
-    (void)P; (void)gamma;
     /* dust, :359-510 */
     for (int k = 0; k < o->nd; ++k) {
       int ir_ = IRHOD(k), iu = IVDX(k), iv = IVDY(k), iw = IVDZ(k);
@@ -520,10 +584,11 @@ static void predictor(oracle* o) {
       }
     }
     /* isothermal face pressure, :606-611 */
-    for (int d = 0; d < o->ndim; ++d) {
-      QP(o->iP, c, d) = cs2 * QP(o->irho, c, d);
-      QM(o->iP, c, d) = cs2 * QM(o->irho, c, d);
-    }
+    if (iso)
+      for (int d = 0; d < o->ndim; ++d) {
+        QP(o->iP, c, d) = cs2 * QP(o->irho, c, d);
+        QM(o->iP, c, d) = cs2 * QM(o->irho, c, d);
+      }
   }
 }
 
@@ -668,6 +733,10 @@ static void add_delta_u(oracle* o) {
       int il = (d == 0) ? cidx(o, ix - 1, iy) : cidx(o, ix, iy - 1);
       for (int v = 0; v < nv; ++v) { ql[v] = QM(v, il, d); qr[v] = QP(v, c, d); flx[v] = 0.0; }
       double csl = o->cs[il], csr = o->cs[c];
+      if (p->iso_cs != 1) {                      /* :679-680, :709-710 */
+        csl = sqrt(p->gamma * ql[o->iP] / ql[o->irho]);
+        csr = sqrt(p->gamma * qr[o->iP] / qr[o->irho]);
+      }
       int vn = (d == 0) ? o->ivx : o->ivy, vt = (d == 0) ? o->ivy : o->ivx;
       oracle_gas_flux(p->solver, p->gamma, ql, qr, flx, csl, csr, vn, vt);
       for (int k = 0; k < o->nd; ++k) {

@@ -21,6 +21,9 @@ program ref_harness
 #ifdef REF_SI
   use setup_parameters, only: box_l, rho_init, omega_shear, q_shear, hoverr, eta_stream, rad0
 #endif
+#ifdef REF_SOD
+  use setup_parameters, only: rho_l, rho_r, P_l, P_r, direction_shock
+#endif
   implicit none
   integer :: nsteps, istep, ilun, iout
   character(len=256) :: arg, outdir
@@ -134,6 +137,13 @@ program ref_harness
   write(ilun, '(A,ES25.17)') 'hoverr ', hoverr
   write(ilun, '(A,ES25.17)') 'eta_stream ', eta_stream
   write(ilun, '(A,ES25.17)') 'rad0 ', rad0
+#endif
+#ifdef REF_SOD
+  write(ilun, '(A,ES25.17)') 'rho_l ', rho_l
+  write(ilun, '(A,ES25.17)') 'rho_r ', rho_r
+  write(ilun, '(A,ES25.17)') 'p_l ', P_l
+  write(ilun, '(A,ES25.17)') 'p_r ', P_r
+  write(ilun, '(A,I12)') 'shock_dir ', direction_shock
 #endif
   close(ilun)
 end program ref_harness

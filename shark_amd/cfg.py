@@ -13,7 +13,7 @@ SHARK_MAX_DUST = 64
 GEOM_CARTESIAN, GEOM_POLAR = 0, 2
 SOLVER_LLF, SOLVER_HLL, SOLVER_HLLC = 0, 1, 2
 DUST_HUANG_BAI, DUST_LLF, DUST_HLL, DUST_HLLC = 0, 1, 2, 3
-BC_PERIODIC, BC_SHEAR_SI, BC_DISK = 0, 1, 2
+BC_PERIODIC, BC_SHEAR_SI, BC_DISK, BC_SOD = 0, 1, 2, 3
 TSTOP_DUSTYWAVE, TSTOP_SI, TSTOP_EPSTEIN_DISK = 0, 1, 2
 FORCE_NONE, FORCE_SI, FORCE_DISK = 0, 1, 2
 INLOOP_NONE, INLOOP_SI, INLOOP_DISK = 0, 1, 2
@@ -27,7 +27,7 @@ class ShARKCfg(C.Structure):
         ("force_kick", C.c_int32), ("fargo", C.c_int32), ("drag", C.c_int32),
         ("dust_back_reaction", C.c_int32), ("growth", C.c_int32), ("viscosity", C.c_int32),
         ("bc_kind", C.c_int32), ("tstop_kind", C.c_int32), ("force_kind", C.c_int32),
-        ("inloop_kind", C.c_int32), ("disk_bc_mode", C.c_int32), ("pad0", C.c_int32),
+        ("inloop_kind", C.c_int32), ("disk_bc_mode", C.c_int32), ("shock_dir", C.c_int32),
         ("gamma", C.c_double), ("cfl", C.c_double), ("smallr", C.c_double),
         ("smallc", C.c_double), ("smallp", C.c_double), ("dust_ratio_min", C.c_double),
         ("K", C.c_double * SHARK_MAX_DUST),
@@ -35,7 +35,8 @@ class ShARKCfg(C.Structure):
         ("rad0", C.c_double), ("eta_stream", C.c_double), ("rho_init", C.c_double),
         ("hoverr", C.c_double), ("rhograin", C.c_double),
         ("mstar", C.c_double), ("r_relax", C.c_double), ("n_rel", C.c_double),
-        ("smooth_r", C.c_double), ("reserved", C.c_double * 16),
+        ("smooth_r", C.c_double), ("rho_l", C.c_double), ("rho_r", C.c_double),
+        ("p_l", C.c_double), ("p_r", C.c_double), ("reserved", C.c_double * 12),
     ]
 
 

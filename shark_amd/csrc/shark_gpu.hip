@@ -101,7 +101,11 @@ static int validate(const shark_cfg* c) {
   if (c->ndust < 0 || c->ndust > SHARK_MAX_DUST) return fail(SHARK_EINVAL, "ndust=%d out of range", c->ndust);
   if (!shark::nd_supported(c->ndust))
     return fail(SHARK_EUNSUP, "ndust=%d: hydro step is instantiated for ndust in {0,1,2,3,4,8,16}", c->ndust);
-  if (c->iso_cs != 1) return fail(SHARK_EUNSUP, "iso_cs=%d: only the isothermal gas path is built", c->iso_cs);
+  if (c->iso_cs != 1 && c->ndust > 0)
+    return fail(SHARK_EUNSUP, "iso_cs=%d with dust: the adiabatic path is built for gas only", c->iso_cs);
+  if (c->bc_kind == SHARK_BC_SOD && (c->ndust > 0 || c->shock_dir < 0 || c->shock_dir > 1 ||
+                                     (c->shock_dir == 1 && c->ny < 4)))
+    return fail(SHARK_EUNSUP, "Sod boundaries need gas only and shock_dir 0 (x) or 1 (y, 2D)");
   if (c->geom != SHARK_GEOM_CARTESIAN && c->geom != SHARK_GEOM_POLAR)
     return fail(SHARK_EUNSUP, "geom=%d: GEOM 0 and 2 are built", c->geom);
   if (c->geom == SHARK_GEOM_POLAR && c->ny < 4) return fail(SHARK_EUNSUP, "polar geometry needs a 2D grid");
@@ -112,7 +116,7 @@ static int validate(const shark_cfg* c) {
   if (c->fargo && c->ny < 4) return fail(SHARK_EUNSUP, "FARGO needs a 2D grid");
   if (c->force_kick && !c->drag && c->ndust > 0)
     return fail(SHARK_EUNSUP, "explicit drag in the force (drag=.false. with force_kick) is not built");
-  if (c->bc_kind < 0 || c->bc_kind > 2 || c->tstop_kind < 0 || c->tstop_kind > 2 ||
+  if (c->bc_kind < 0 || c->bc_kind > 3 || c->tstop_kind < 0 || c->tstop_kind > 2 ||
       c->force_kind < 0 || c->force_kind > 2 || c->inloop_kind < 0 || c->inloop_kind > 2)
     return fail(SHARK_EINVAL, "unknown setup hook kind");
   if (c->bc_kind == SHARK_BC_DISK && c->ny < 4) return fail(SHARK_EUNSUP, "disk boundaries need a 2D grid");
@@ -148,6 +152,10 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   a.gamma = cfg->gamma; a.cfl = cfg->cfl; a.smallr = cfg->smallr; a.dust_ratio_min = cfg->dust_ratio_min;
   a.q_shear = cfg->q_shear; a.omega_shear = cfg->omega_shear; a.box_l = cfg->box_l; a.rad0 = cfg->rad0;
   a.eta_stream = cfg->eta_stream; a.rho_init = cfg->rho_init; a.hoverr = cfg->hoverr;
+  a.rho_l = cfg->rho_l; a.rho_r = cfg->rho_r; a.p_l = cfg->p_l; a.p_r = cfg->p_r;
+  a.shock_dir = cfg->shock_dir;
+  a.iso = cfg->iso_cs == 1;
+  a.smallp = cfg->smallp;
   a.eps_k1 = sqrt((3.141592653589793238462643383279 * cfg->gamma) / 8.0) * cfg->rhograin;
   a.s2pi = sqrt(2.0 * 3.141592653589793238462643383279);
   a.rhograin = cfg->rhograin; a.mstar = cfg->mstar; a.r_relax = cfg->r_relax; a.n_rel = cfg->n_rel;

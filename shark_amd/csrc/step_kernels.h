@@ -52,6 +52,10 @@ struct StepArgs {
   double mstar, r_relax, n_rel, smooth_r;
   double K[64];
   double sdust_k[64];   // per-species grain size when uniform over cells
+  double rho_l, rho_r, p_l, p_r;   // Sod tube states
+  int shock_dir;        // Sod tube direction
+  int iso;              // 1: isothermal (setup's cs); 0: adiabatic (P from E, cs = sqrt(gamma P / rho))
+  double smallp;
   double eps_k1;        // sqrt(pi gamma / 8) * rhograin: the Epstein prefactor (disk tstop)
   double s2pi;          // sqrt(2 pi)
 };

@@ -59,10 +59,24 @@ __global__ void k_bc(StepArgs a) {
   const size_t nc = a.nc;
   auto ysrc = [&](int iy) { return iy <= 2 ? a.ny + iy : iy - a.ny; };
   auto is_grow = [&](int iy) { return a.ndim == 2 && (iy <= 2 || iy >= a.ny + 3); };
+  // Sod tube (SOD/boundaries.f90:10-80): the ghost layers across the shock hold the initial
+  // left / right state at rest with E = P/(gamma-1)
+  auto sod_state = [&](int dst, bool left) {
+    a.U[0 * nc + dst] = left ? a.rho_l : a.rho_r;
+    a.U[4 * nc + dst] = (left ? a.p_l : a.p_r) / (a.gamma - 1.0);
+    a.U[1 * nc + dst] = 0.0;
+    a.U[2 * nc + dst] = 0.0;
+    a.U[3 * nc + dst] = 0.0;
+  };
+  const bool sod = a.bc_kind == 3;
   if (t < n1) {
     const int r = t / a.nx, ix = 3 + t % a.nx;
     const int iy = (r == 0) ? 1 : (r == 1) ? 2 : (r == 2) ? a.nym : a.nym - 1;
     const int dst = cell_of(a, ix, iy), src = cell_of(a, ix, ysrc(iy));
+    if (sod && a.shock_dir == 1) {
+      sod_state(dst, iy <= 2);
+      return;
+    }
     for (int v = 0; v < a.nvar; ++v) a.U[v * nc + dst] = a.U[v * nc + src];
     return;
   }
@@ -80,6 +94,14 @@ __global__ void k_bc(StepArgs a) {
     sx = left ? 3 : a.nx + 2;
   } else {
     sx = left ? a.nx + ix : ix - a.nx;
+  }
+  if (sod && a.shock_dir == 0) {
+    sod_state(cell_of(a, ix, iy), left);
+    return;
+  }
+  if (sod && is_grow(sy)) {   // the periodic source is a y-ghost holding the fixed state
+    sod_state(cell_of(a, ix, iy), sy <= 2);
+    return;
   }
   if (is_grow(sy)) sy = ysrc(sy);  // the source itself is a y-ghost filled first
   const int dst = cell_of(a, ix, iy), src = cell_of(a, sx, sy);
@@ -110,21 +132,23 @@ __global__ void k_bc(StepArgs a) {
 // Per-lane state of one loaded row of one fluid.
 struct RowF {
   double u[5];   // conservative (gas: rho, mx, my, mz, E; dust: rhod, mdx, mdy, mdz)
-  double q[4];   // primitive rho, vx, vy, vz (ctoprim, solve.f90:211-232)
+  double q[5];   // primitive rho, vx, vy, vz, and P for adiabatic gas (ctoprim, solve.f90:211-232)
   double cs;
 };
 struct FaceSet {
-  double m[4], p[4];  // qm (state at the right face) and qp (left face) of rho, vx, vy, vz
+  double m[5], p[5];  // qm (state at the right face) and qp (left face) of rho, vx, vy, vz [, P]
 };
 
 DEV double shfl_up1(double x) { return __shfl_up(x, 1); }
 DEV double shfl_dn1(double x) { return __shfl_down(x, 1); }
 
 // SLOPE: limiter (1..3); SOLV: compile-time Riemann solver (gas 0..2, dust 0..3) or -1 to
-// read it from the arguments.
-template <int NDIM, int GEOM, bool GAS, int SLOPE, int SOLV>
+// read it from the arguments.  ADI: adiabatic gas (P from the total energy, its own slopes,
+// Hancock source and face states, sound speeds from the face states; iso_cs < 1).
+template <int NDIM, int GEOM, bool GAS, int SLOPE, int SOLV, bool ADI>
 __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int lane) {
   constexpr int NU = GAS ? 5 : 4;
+  constexpr int NQ = ADI ? 5 : 4;   // primitives carried through slopes and faces
   const size_t nc = a.nc;
   int p[5];
   if (GAS) { p[0] = 0; p[1] = 1; p[2] = 2; p[3] = 3; p[4] = 4; }
@@ -166,7 +190,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
     const int cell = (NDIM == 1) ? ixc - 1 : (ixc - 1) + (j - 1) * a.nxm;
 #pragma unroll
     for (int c = 0; c < NU; ++c) r.u[c] = a.U[p[c] * nc + cell];
-    r.cs = need_cs ? a.cs[cell] : 0.0;
+    r.cs = (need_cs && !ADI) ? a.cs[cell] : 0.0;
   };
   auto convert_row = [&](RowF& r) {
     const double yr = 1.0 / r.u[0];
@@ -174,10 +198,17 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
     r.q[1] = exdiv(r.u[1], r.u[0], yr);
     r.q[2] = exdiv(r.u[2], r.u[0], yr);
     r.q[3] = exdiv(r.u[3], r.u[0], yr);
+    if constexpr (ADI) {
+      // solve.f90:214-220: P = max((gamma-1)(E - ekin), smallp), cs = sqrt(gamma P / rho)
+      const double ekin = kHalf * r.u[0] * (r.q[1] * r.q[1]) + kHalf * r.u[0] * (r.q[2] * r.q[2]) +
+                          kHalf * r.u[0] * (r.q[3] * r.q[3]);
+      r.q[4] = fmaxf_((a.gamma - 1.0) * (r.u[4] - ekin - 0.0), a.smallp);
+      r.cs = sqrt(a.gamma * r.q[4] / r.q[0]);
+    }
   };
 
   // face states of a row (predictor, godunov.f90:89-611); dqy = 0 in 1D
-  auto faces = [&](const RowF& r, const double dqx[4], const double dqy[4], int iy, FaceSet& fx,
+  auto faces = [&](const RowF& r, const double dqx[5], const double dqy[5], int iy, FaceSet& fx,
                    FaceSet& fy) {
     double f1 = 0.0, f2 = 0.0, f3 = 0.0;
     if (a.force_kick && col_active && (NDIM == 1 || (iy >= 3 && iy <= a.ny + 2)))
@@ -186,16 +217,24 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
     double u = r.q[1], v = r.q[2], w = r.q[3];
     const double drx = dqx[0], dux = dqx[1], dvx = dqx[2], dwx = dqx[3];
     const double dry = dqy[0], duy = dqy[1], dvy = dqy[2], dwy = dqy[3];
-    double sr0, su0, sv0, sw0;
+    double sr0, su0, sv0, sw0, sp0 = 0.0, P = 0.0, dpx = 0.0, dpy = 0.0;
     if (a.force_kick) {
       u = u + (f1 * kHalf) * dt;
       v = v + (f2 * kHalf) * dt;
       w = w + (f3 * kHalf) * dt;
     }
     if (GAS) {
-      const double cs2 = r.cs * r.cs;
-      const double dpx = dqx[0] * cs2;
-      const double dpy = (NDIM == 2) ? dqy[0] * cs2 : 0.0;
+      if constexpr (ADI) {   // godunov.f90:123-124, 131-133
+        P = r.q[4];
+        dpx = dqx[4];
+        dpy = (NDIM == 2) ? dqy[4] : 0.0;
+        if (GEOM == 2) sp0 = ((-(u * dpx)) - v * dpy - (dux + dvy) * a.gamma * P) - exdiv(a.gamma * P * u, rp, yrp);
+        else sp0 = (-(u * dpx)) - v * dpy - (dux + dvy) * a.gamma * P;
+      } else {               // :125-128
+        const double cs2 = r.cs * r.cs;
+        dpx = dqx[0] * cs2;
+        dpy = (NDIM == 2) ? dqy[0] * cs2 : 0.0;
+      }
       if (GEOM == 2) {
         sr0 = ((-(u * drx)) - v * dry - (dux + dvy) * r_rho) - exdiv(r_rho * u, rp, yrp);
         su0 = ((-(u * dux)) - v * duy - dpx / r_rho) + exdiv(v * v, rp, yrp);
@@ -224,25 +263,43 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
     fx.m[1] = b1 + kHalf * dux * dx1; fx.p[1] = b1 - kHalf * dux * dx1;
     fx.m[2] = b2 + kHalf * dvx * dx1; fx.p[2] = b2 - kHalf * dvx * dx1;
     fx.m[3] = b3 + kHalf * dwx * dx1; fx.p[3] = b3 - kHalf * dwx * dx1;
+    if constexpr (ADI) {   // :261, :266
+      const double b4 = P + hdt * sp0;
+      fx.m[4] = fmaxf_(b4 + kHalf * dpx * dx1, a.smallp);
+      fx.p[4] = fmaxf_(b4 - kHalf * dpx * dx1, a.smallp);
+    }
     if (NDIM == 2) {
       const double dyl = rp * dx2;
       fy.m[0] = b0 + kHalf * dry * dyl; fy.p[0] = b0 - kHalf * dry * dyl;
       fy.m[1] = b1 + kHalf * duy * dyl; fy.p[1] = b1 - kHalf * duy * dyl;
       fy.m[2] = b2 + kHalf * dvy * dyl; fy.p[2] = b2 - kHalf * dvy * dyl;
       fy.m[3] = b3 + kHalf * dwy * dyl; fy.p[3] = b3 - kHalf * dwy * dyl;
+      if constexpr (ADI) {   // :275, :280
+        const double b4 = P + hdt * sp0;
+        fy.m[4] = fmaxf_(b4 + kHalf * dpy * dyl, a.smallp);
+        fy.p[4] = fmaxf_(b4 - kHalf * dpy * dyl, a.smallp);
+      }
     }
   };
 
   // Riemann flux between left state L(q) and right state R(q); dir 0: x (normal vx),
   // dir 1: y (normal vy). Returns conservative components in plane order.
-  auto riemann = [&](const double L[4], double csl, const double R[4], double csr, int dir,
+  auto riemann = [&](const double L[5], double csl, const double R[5], double csr, int dir,
                      double F[5]) {
+    if constexpr (ADI) {   // sound speeds of the face states, godunov.f90:679-680, 709-710
+      csl = sqrt(a.gamma * L[4] / L[0]);
+      csr = sqrt(a.gamma * R[4] / R[0]);
+    }
     Face fl, fr;
     fl.r = L[0]; fr.r = R[0];
     fl.un = dir == 0 ? L[1] : L[2]; fl.ut = dir == 0 ? L[2] : L[1]; fl.w = L[3];
     fr.un = dir == 0 ? R[1] : R[2]; fr.ut = dir == 0 ? R[2] : R[1]; fr.w = R[3];
     Flux5 f;
-    if (GAS) {
+    if (GAS && ADI) {
+      fl.p = L[4];
+      fr.p = R[4];
+      f = gas_flux(SOLV >= 0 ? SOLV : a.solver, a.gamma, fl, fr, csl, csr);
+    } else if (GAS) {
       fl.p = (csl * csl) * L[0];   // isothermal face pressure, godunov.f90:606-611
       fr.p = (csr * csr) * R[0];
       f = gas_flux(SOLV >= 0 ? SOLV : a.solver, a.gamma, fl, fr, csl, csr);
@@ -260,8 +317,8 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
   // ---- march
   RowF R0, R1, R2;
   FaceSet fx1, fy1;
-  double sLy_prev[4] = {0, 0, 0, 0};
-  double fym_prev[4] = {0, 0, 0, 0};
+  double sLy_prev[5] = {0, 0, 0, 0, 0};
+  double fym_prev[5] = {0, 0, 0, 0, 0};
   double cs_prev = 0.0;
   double Fx_own_prev[5] = {0, 0, 0, 0, 0}, Fx_rt_prev[5] = {0, 0, 0, 0, 0};
   double Fy_bot[5] = {0, 0, 0, 0, 0};
@@ -285,18 +342,18 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
     // in 1D the single row is row "R1" of the loop body below
     const RowF& Rm = (NDIM == 2) ? R1 : R2;
     const int m = (NDIM == 2) ? j - 1 : 1;   // row whose face states are built now
-    double sLy_new[4] = {0, 0, 0, 0};
+    double sLy_new[5] = {0, 0, 0, 0, 0};
     if (NDIM == 2 && j >= y0 - 1) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) sLy_new[c] = exdiv(2.0 * (R2.q[c] - R1.q[c]), dsy, ysy);
+      for (int c = 0; c < NQ; ++c) sLy_new[c] = exdiv(2.0 * (R2.q[c] - R1.q[c]), dsy, ysy);
     }
     const bool have_faces = (NDIM == 1) || (j >= y0);
     double Fx_own[5] = {0, 0, 0, 0, 0}, Fx_rt[5] = {0, 0, 0, 0, 0};
     double Fy_top[5] = {0, 0, 0, 0, 0};
     if (have_faces) {
-      double dqx[4], dqy[4];
+      double dqx[5] = {0, 0, 0, 0, 0}, dqy[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 0; c < NQ; ++c) {
         const double ql = shfl_up1(Rm.q[c]);
         const double sL = exdiv(2.0 * (Rm.q[c] - ql), dsx, ysx);
         const double sR = shfl_dn1(sL);
@@ -306,9 +363,9 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
       faces(Rm, dqx, dqy, m, fx1, fy1);
       // x fluxes of row m, needed when m is an owned row
       if (NDIM == 1 || (m >= y0 && m <= y1)) {
-        double L[4];
+        double L[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-        for (int c = 0; c < 4; ++c) L[c] = shfl_up1(fx1.m[c]);
+        for (int c = 0; c < NQ; ++c) L[c] = shfl_up1(fx1.m[c]);
         const double csl = shfl_up1(Rm.cs);
         riemann(L, csl, fx1.p, Rm.cs, 0, Fx_own);
 #pragma unroll
@@ -351,7 +408,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
       if (GEOM == 2) {
         const double r = rp;
         if (GAS) {
-          const double P = Rf.u[0] * (Rf.cs * Rf.cs);
+          const double P = ADI ? Rf.q[4] : Rf.u[0] * (Rf.cs * Rf.cs);
           S[1] = 0.0 + exdiv(dt * (Rf.q[0] * (Rf.q[2] * Rf.q[2]) + P), r, yrp);
           S[2] = 0.0 - dt * exdiv(Rf.q[0] * Rf.q[2] * Rf.q[1], r, yrp);
         } else {
@@ -370,7 +427,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
     // rotate
     if (NDIM == 2) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) { sLy_prev[c] = sLy_new[c]; fym_prev[c] = fy1.m[c]; }
+      for (int c = 0; c < NQ; ++c) { sLy_prev[c] = sLy_new[c]; fym_prev[c] = fy1.m[c]; }
       cs_prev = R1.cs;
 #pragma unroll
       for (int c = 0; c < 5; ++c) { Fx_own_prev[c] = Fx_own[c]; Fx_rt_prev[c] = Fx_rt[c]; Fy_bot[c] = Fy_top[c]; }
@@ -380,7 +437,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
 
 // the gas kernel is allowed two waves per SIMD (a few registers spill; measured faster on the
 // 1024 x 2048 disk), the dust kernel keeps its full register budget
-template <int NDIM, int GEOM, bool GAS, int SLOPE, int SOLV>
+template <int NDIM, int GEOM, bool GAS, int SLOPE, int SOLV, bool ADI>
 __global__ __launch_bounds__(256, (GAS ? 2 : 1)) void k_hydro(StepArgs a) {
   if (!a.sc->active) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -396,11 +453,21 @@ __global__ __launch_bounds__(256, (GAS ? 2 : 1)) void k_hydro(StepArgs a) {
   const long t = item / G;
   const int strip = (int)(t % S);
   const int fluid = GAS ? 0 : 1 + (int)(t / S);
-  hydro_fluid<NDIM, GEOM, GAS, SLOPE, SOLV>(a, fluid, g, strip, lane);
+  hydro_fluid<NDIM, GEOM, GAS, SLOPE, SOLV, ADI>(a, fluid, g, strip, lane);
 }
 
 // ------------------------------------------------------------------ post
 // Courant candidate of one active cell (courant.f90:68-158), from its conservative state.
+// Sound speed of a cell from its conservative gas state: the setup's cs when isothermal, else
+// ctoprim's sqrt(gamma P / max(rho, smallr)) with P = max((gamma-1)(E - ekin), smallp)
+// (solve.f90:214-221), from the velocities u/rho.
+DEV double cs_cell(const StepArgs& a, double rho, double E, double vx, double vy, double vz, int cell) {
+  if (a.iso) return a.cs[cell];
+  const double ekin = kHalf * rho * (vx * vx) + kHalf * rho * (vy * vy) + kHalf * rho * (vz * vz);
+  const double P = fmaxf_((a.gamma - 1.0) * (E - ekin - 0.0), a.smallp);
+  return sqrt(a.gamma * P / fmaxf_(rho, a.smallr));
+}
+
 template <int ND>
 DEV double courant_cell(const StepArgs& a, const double* un, int ix, int cell) {
   double dxx;
@@ -409,7 +476,7 @@ DEV double courant_cell(const StepArgs& a, const double* un, int ix, int cell) {
   else dxx = fminf_(a.g.dx1[ix - 1], a.g.dx2[ix - 1]);
   const double rho = un[0];
   const double vx = un[1] / rho, vy = un[2] / rho, vz = un[3] / rho;
-  const double cs = a.cs[cell];
+  const double cs = cs_cell(a, rho, un[4], vx, vy, vz, cell);
   double vv = (fabs(vx) + fabs(vy)) + fabs(vz);
   double vmax = cs + vv;
   double vdx[ND > 0 ? ND : 1], vdz[ND > 0 ? ND : 1];
@@ -826,7 +893,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
       const double vx = exdiv(un[1][c], rho, yr), vy = exdiv(un[2][c], rho, yr),
                    vz = exdiv(un[3][c], rho, yr);
       const double vv = (fabs(vx) + fabs(vy)) + fabs(vz);
-      vmax = a.cs[cell] + vv;
+      vmax = cs_cell(a, rho, un[4][c], vx, vy, vz, cell) + vv;
       if (ND == 0) vvlast[c] = vv;
       if (a.force_kick) {
         cell_force(a, ixs, vx, vz, true, f1, f2, f3);
@@ -934,7 +1001,13 @@ __global__ void k_ctoprim(StepArgs a, double* q) {
   q[nc + c] = a.U[nc + c] / rho;
   q[2 * nc + c] = a.U[2 * nc + c] / rho;
   q[3 * nc + c] = a.U[3 * nc + c] / rho;
-  q[4 * nc + c] = rho * (a.cs[c] * a.cs[c]);
+  if (a.iso) {
+    q[4 * nc + c] = rho * (a.cs[c] * a.cs[c]);
+  } else {
+    const double vx = a.U[nc + c] / rho, vy = a.U[2 * nc + c] / rho, vz = a.U[3 * nc + c] / rho;
+    const double ekin = kHalf * rho * (vx * vx) + kHalf * rho * (vy * vy) + kHalf * rho * (vz * vz);
+    q[4 * nc + c] = fmaxf_((a.gamma - 1.0) * (a.U[4 * nc + c] - ekin - 0.0), a.smallp);
+  }
   for (int k = 0; k < a.nd; ++k) {
     const double rd = a.U[(5 + k) * nc + c];
     q[(5 + k) * nc + c] = rd;
@@ -955,8 +1028,15 @@ static void hydro_launch1(const StepArgs& a, hipStream_t s) {
   const long items = (long)a.ngroups * (GAS ? a.nstrips_g : a.nstrips) * (GAS ? 1 : a.nd);
   const dim3 grid((unsigned)((items + 3) / 4));
   const bool fast = GAS ? a.solver == 1 : a.solverdust == 0;   // HLL gas / Huang-Bai dust
-  if (fast) hipLaunchKernelGGL((k_hydro<NDIM, GEOM, GAS, SLOPE, GAS ? 1 : 0>), grid, dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_hydro<NDIM, GEOM, GAS, SLOPE, -1>), grid, dim3(256), 0, s, a);
+  if constexpr (GAS) {
+    if (!a.iso) {   // adiabatic gas
+      if (fast) hipLaunchKernelGGL((k_hydro<NDIM, GEOM, true, SLOPE, 1, true>), grid, dim3(256), 0, s, a);
+      else hipLaunchKernelGGL((k_hydro<NDIM, GEOM, true, SLOPE, -1, true>), grid, dim3(256), 0, s, a);
+      return;
+    }
+  }
+  if (fast) hipLaunchKernelGGL((k_hydro<NDIM, GEOM, GAS, SLOPE, GAS ? 1 : 0, false>), grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_hydro<NDIM, GEOM, GAS, SLOPE, -1, false>), grid, dim3(256), 0, s, a);
 }
 
 template <int NDIM, int GEOM, bool GAS>

@@ -15,6 +15,7 @@ module shark_gpu_mod
   integer(c_int32_t), parameter :: SHARK_SOLVER_LLF = 0, SHARK_SOLVER_HLL = 1, SHARK_SOLVER_HLLC = 2
   integer(c_int32_t), parameter :: SHARK_DUST_HUANG_BAI = 0
   integer(c_int32_t), parameter :: SHARK_BC_PERIODIC = 0, SHARK_BC_SHEAR_SI = 1, SHARK_BC_DISK = 2
+  integer(c_int32_t), parameter :: SHARK_BC_SOD = 3
   integer(c_int32_t), parameter :: SHARK_TSTOP_DUSTYWAVE = 0, SHARK_TSTOP_SI = 1, SHARK_TSTOP_EPSTEIN_DISK = 2
   integer(c_int32_t), parameter :: SHARK_FORCE_NONE = 0, SHARK_INLOOP_NONE = 0
 
@@ -22,13 +23,14 @@ module shark_gpu_mod
   type, bind(C) :: shark_cfg
      integer(c_int32_t) :: version, nx, ny, ndust, nghost, geom, solver, solverdust, slope_type
      integer(c_int32_t) :: iso_cs, force_kick, fargo, drag, dust_back_reaction, growth, viscosity
-     integer(c_int32_t) :: bc_kind, tstop_kind, force_kind, inloop_kind, disk_bc_mode, pad0
+     integer(c_int32_t) :: bc_kind, tstop_kind, force_kind, inloop_kind, disk_bc_mode, shock_dir
      real(c_double) :: gamma, cfl, smallr, smallc, smallp, dust_ratio_min
      real(c_double) :: K(SHARK_MAX_DUST)
      real(c_double) :: q_shear, omega_shear, box_l, rad0, eta_stream, rho_init, hoverr
      real(c_double) :: rhograin
      real(c_double) :: mstar, r_relax, n_rel, smooth_r
-     real(c_double) :: reserved(16)
+     real(c_double) :: rho_l, rho_r, p_l, p_r
+     real(c_double) :: reserved(12)
   end type shark_cfg
 
   ! same layout as struct shark_tune (launch shapes of the step kernels)
@@ -148,13 +150,14 @@ contains
     c%slope_type = 1; c%iso_cs = 1; c%force_kick = 0; c%fargo = 0; c%drag = 1
     c%dust_back_reaction = 1; c%growth = 0; c%viscosity = 0
     c%bc_kind = SHARK_BC_PERIODIC; c%tstop_kind = SHARK_TSTOP_DUSTYWAVE
-    c%force_kind = SHARK_FORCE_NONE; c%inloop_kind = SHARK_INLOOP_NONE; c%disk_bc_mode = 0; c%pad0 = 0
+    c%force_kind = SHARK_FORCE_NONE; c%inloop_kind = SHARK_INLOOP_NONE; c%disk_bc_mode = 0; c%shock_dir = 0
     c%gamma = 1.66667d0; c%cfl = 0.5d0; c%smallr = 1d-27; c%smallc = 1d-27
     c%smallp = 1d-27 * 1d-27**2; c%dust_ratio_min = 1d-11
     c%K = 0.0d0
     c%q_shear = 0.0d0; c%omega_shear = 0.0d0; c%box_l = 0.0d0; c%rad0 = 0.0d0
     c%eta_stream = 0.0d0; c%rho_init = 0.0d0; c%hoverr = 0.0d0; c%rhograin = 0.0d0
     c%mstar = 0.0d0; c%r_relax = 0.0d0; c%n_rel = 0.0d0; c%smooth_r = 0.0d0
+    c%rho_l = 0.0d0; c%rho_r = 0.0d0; c%p_l = 0.0d0; c%p_r = 0.0d0
     c%reserved = 0.0d0
   end subroutine shark_cfg_defaults
 end module shark_gpu_mod

@@ -96,6 +96,27 @@ def primtoc(q, nd, gamma):
 
 
 # --------------------------------------------------------------------------- setups
+def sod(nx=1024, ny=1, shock_dir=0, cfl=0.8, slope_type=3, solver=C.SOLVER_HLL, gamma=1.4,
+        rho_l=1.0, rho_r=0.125, p_l=1.0, p_r=0.1, box_l=1.0, box_l_y=1.0):
+    """Sod shock tube (SOD setup.f90:1-43, boundaries.f90:10-80): adiabatic gas (iso_cs < 1), left
+    state where position(:, shock_dir+1) <= 1/2, right state beyond, at rest; the ghost layers
+    across the shock keep the two states.  Defaults: the reference's test-suite tube
+    (test/test-suite/Sod-tube/sod.nml, 1024 cells)."""
+    cfg = C.make_cfg(nx=nx, ny=ny, ndust=0, iso_cs=-1, cfl=cfl, slope_type=slope_type, solver=solver,
+                     gamma=gamma, bc_kind=C.BC_SOD, tstop_kind=C.TSTOP_DUSTYWAVE, force_kind=C.FORCE_NONE,
+                     inloop_kind=C.INLOOP_NONE, shock_dir=shock_dir, rho_l=rho_l, rho_r=rho_r, p_l=p_l,
+                     p_r=p_r)
+    g = grid_cartesian_1d(nx, box_l) if ny == 1 else grid_cartesian_2d(nx, ny, box_l, box_l_y)
+    pos = g["position"][shock_dir]
+    right = pos > 0.5
+    nc = pos.size
+    q = np.zeros((5, nc))
+    q[0] = np.where(right, rho_r, rho_l)
+    q[4] = np.where(right, p_r, p_l)
+    u = primtoc(q, 0, gamma)
+    return cfg, g, dict(u=u, cs=np.zeros(nc), sdust=None, condinit=None, fargo_velocity=None)
+
+
 def dustywave(nx=512, K=10.0, dust2gas=1.0, vx0=1e-4, drho=0.0, cs0=1.0, rho0=1.0, box_l=1.0,
               kx=1.0, cfl=0.5, slope_type=1, solver=C.SOLVER_HLL, solverdust=C.DUST_HUANG_BAI,
               gamma=1.6666667):

@@ -111,6 +111,25 @@ charging=.false.
 /
 """
 
+SOD = """&GRID_PARAMS
+freq_out=100000
+CFL = {cfl}
+tend = {tend}
+gamma = 1.4d0
+slope_type = {slope}
+/
+&SETUP_PARAMS
+direction_shock = {dir}
+/
+&DUST_PARAMS
+/
+&GRAVITY_PARAMS
+/
+&IONISATION_PARAMS
+charging=.false.
+/
+"""
+
 # name: (setup, nx, ny, ndust, geom, gridspace, solver, solverdust, namelist, nsteps)
 CASES = {
     "dustywave_k50": ("DUSTYWAVE", 128, 1, 1, 0, 0, 1, 0,
@@ -129,6 +148,11 @@ CASES = {
     "si_2d_s3_hllc": ("SI", 24, 24, 2, 0, 0, 2, 0, SI.format(slope=3), 25),
     "disk_2d": ("disk_polar", 24, 32, 2, 2, 0, 1, 0, DISK.format(slope=1, r_relax="0.3d0"), 25),
     "disk_2d_relax_s2": ("disk_polar", 24, 32, 2, 2, 0, 1, 0, DISK.format(slope=2, r_relax="3.0d0"), 25),
+    "sod_hll_s3": ("SOD", 128, 1, 0, 0, 0, 1, 0, SOD.format(cfl=0.8, tend=0.245, slope=3, dir=0), -1),
+    "sod_hllc_s1": ("SOD", 128, 1, 0, 0, 0, 2, 0, SOD.format(cfl=0.8, tend=0.245, slope=1, dir=0), -1),
+    "sod_llf_s2": ("SOD", 96, 1, 0, 0, 0, 0, 0, SOD.format(cfl=0.5, tend=0.2, slope=2, dir=0), -1),
+    "sod_2d_y_hll": ("SOD", 16, 64, 0, 0, 0, 1, 0, SOD.format(cfl=0.5, tend="1d10", slope=1, dir=1), 40),
+    "sod_2d_x_hllc": ("SOD", 48, 8, 0, 0, 0, 2, 0, SOD.format(cfl=0.5, tend="1d10", slope=3, dir=0), 30),
 }
 
 

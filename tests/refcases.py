@@ -22,7 +22,7 @@ def load_ref(name):
     z = np.load(os.path.join(GOLDEN, f"ref_{name}.npz"))
     m = json.loads(str(z["meta"]))
     kw = dict(nx=m["nx"], ny=m["ny"], ndust=m["ndust"], geom=m["geom"], solver=m["solver"],
-              solverdust=m["solverdust"], slope_type=m["slope_type"], iso_cs=1,
+              solverdust=m["solverdust"], slope_type=m["slope_type"], iso_cs=m["iso_cs"],
               force_kick=int(m["force_kick"]), fargo=int(m["fargo"]), drag=int(m.get("drag", 1)),
               dust_back_reaction=int(m.get("dust_back_reaction", 1)), gamma=m["gamma"],
               cfl=m["CFL"], smallr=m["smallr"], smallc=m["smallc"], smallp=m["smallp"],
@@ -41,6 +41,10 @@ def load_ref(name):
                   inloop_kind=C.INLOOP_DISK, mstar=m["mstar"], r_relax=m["r_relax"],
                   n_rel=m["n_rel"], smooth_r=m["smooth_r"], hoverr=m["hoverr"],
                   rhograin=m["rhograin"] / m["unit_d"], disk_bc_mode=0)
+    elif setup == "SOD":
+        kw.update(bc_kind=C.BC_SOD, tstop_kind=C.TSTOP_DUSTYWAVE, force_kind=C.FORCE_NONE,
+                  inloop_kind=C.INLOOP_NONE, rho_l=m["rho_l"], rho_r=m["rho_r"], p_l=m["p_l"],
+                  p_r=m["p_r"], shock_dir=m["shock_dir"])
     cfg = C.make_cfg(**kw)
     geom = dict(dx=z["dx"], surf=z["surf"], vol=z["vol"], radii_c=z["radii_c"], position=z["position"])
     state = dict(u=z["ic"], cs=z["cs"], sdust=z["sdust"] if "sdust" in z else None,

@@ -1,0 +1,56 @@
+"""GPU adiabatic path (iso_cs < 1) on the Sod shock tube: the reference's test-suite tube
+(test/test-suite/Sod-tube: 1024 cells, CFL 0.8, slope_type 3, tend 0.245) equals the oracle bit for
+bit at every step, and both lie within the L1 bounds of the exact solution; 2D tubes along x and y
+and each Riemann solver equal the oracle bit for bit."""
+import numpy as np
+import pytest
+
+from shark_amd import cfg as C
+from shark_amd import setups
+from refcases import active_mask
+from test_sod import L1_BOUND, l1_errors
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(cfg, geom, st, tend, nmax=100000):
+    import pyoracle
+    from shark_amd import gpu
+    o = pyoracle.Oracle(cfg, geom, st)
+    n_o = o.run(nmax, tend)
+    g = gpu.Step(cfg, geom, st)
+    n_g, t_g = g.run(nmax, tend)
+    u = g.download()
+    g.close()
+    assert n_g == n_o and t_g == o.time, (n_g, n_o, t_g, o.time)
+    m = active_mask(cfg)
+    return u[:, m], o.u()[:, m]
+
+
+def test_gpu_sod_tube_matches_oracle_and_exact_solution():
+    cfg, geom, st = setups.sod()
+    print("sod 1024 cells to t=0.245", flush=True)
+    u, ref = run_both(cfg, geom, st, 0.245)
+    np.testing.assert_array_equal(u, ref)
+    err = l1_errors(np.pad(u, ((0, 0), (2, 2))), cfg.nx)
+    for k, b in L1_BOUND.items():
+        assert err[k] < b, (k, err[k])
+
+
+@pytest.mark.parametrize("solver", [C.SOLVER_LLF, C.SOLVER_HLL, C.SOLVER_HLLC])
+@pytest.mark.parametrize("slope", [1, 2, 3])
+def test_gpu_sod_solvers_and_slopes(solver, slope):
+    cfg, geom, st = setups.sod(nx=256, solver=solver, slope_type=slope)
+    print(f"sod 256 cells solver={solver} slope={slope}", flush=True)
+    u, ref = run_both(cfg, geom, st, 0.2)
+    np.testing.assert_array_equal(u, ref)
+
+
+@pytest.mark.parametrize("direction", [0, 1])
+def test_gpu_sod_2d(direction):
+    nx, ny = (384, 96) if direction == 0 else (96, 384)
+    cfg, geom, st = setups.sod(nx=nx, ny=ny, shock_dir=direction, solver=C.SOLVER_HLLC,
+                               box_l=nx / 384, box_l_y=ny / 384)
+    print(f"sod 2D {nx}x{ny} along {'xy'[direction]}", flush=True)
+    u, ref = run_both(cfg, geom, st, 0.05)
+    np.testing.assert_array_equal(u, ref)
