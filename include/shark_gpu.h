@@ -52,7 +52,9 @@ enum { SHARK_BC_PERIODIC = 0,   /* src/grid/boundary_types.f90:149-186 (DUSTYWAV
        SHARK_BC_SOD = 3 };      /* src/setups/SOD/boundaries.f90:10-80 */
 enum { SHARK_TSTOP_DUSTYWAVE = 0,  /* src/setups/DUSTYWAVE/setup.f90:217 */
        SHARK_TSTOP_SI = 1,         /* src/setups/SI/setup.f90:392 */
-       SHARK_TSTOP_EPSTEIN_DISK = 2 }; /* src/setups/disk_polar/setup.f90:436 */
+       SHARK_TSTOP_EPSTEIN_DISK = 2, /* src/setups/disk_polar/setup.f90:436 */
+       SHARK_TSTOP_EPSTEIN = 3 };    /* src/setups/KH/setup.f90:238, SOD/setup.f90:182: uniform medium,
+                                        sqrt(pi gamma/8) rhograin s / (rho cs), rhograin in code units */
 enum { SHARK_FORCE_NONE = 0,
        SHARK_FORCE_SI = 1,          /* src/setups/SI/setup.f90:335-368 */
        SHARK_FORCE_DISK = 2 };      /* src/setups/disk_polar/setup.f90:360-403 */

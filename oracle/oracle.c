@@ -356,7 +356,8 @@ static void distribution_dust_refresh(oracle* o) {
       o->eps_dust[(size_t)k * o->nc + c] = Q(IRHOD(k), c) / Q(o->irho, c);
 }
 
-/* compute_tstop: DUSTYWAVE setup.f90:212-221, SI :389-397, disk_polar :425-451 */
+/* compute_tstop: DUSTYWAVE setup.f90:212-221, SI :389-397, disk_polar :425-451, uniform Epstein
+ * (KH setup.f90:218-236, SOD :162-190) with the cs of this step's ctoprim */
 static void compute_tstop(oracle* o) {
   const shark_cfg* p = &o->cfg;
   #pragma omp parallel for schedule(static)
@@ -368,6 +369,8 @@ static void compute_tstop(oracle* o) {
         ts = ((Q(IRHOD(k), c) * Q(o->irho, c)) / p->K[k]) / Q(o->irho, c);
       } else if (p->tstop_kind == SHARK_TSTOP_SI) {
         ts = ((p->rhograin * SDUST(c, k)) / p->rho_init) / ((p->rad0 * p->omega_shear) * p->hoverr);
+      } else if (p->tstop_kind == SHARK_TSTOP_EPSTEIN) {
+        ts = (sqrt((PI * p->gamma) / 8.0) * p->rhograin) * SDUST(c, k) / (Q(o->irho, c) * o->cs[c]);
       } else {
         double H = p->hoverr * o->radii_c[c];
         ts = (sqrt((PI * p->gamma) / 8.0) * p->rhograin) * SDUST(c, k) /
@@ -768,7 +771,8 @@ static void add_delta_u(oracle* o) {
 }
 
 /* ---------------------------------------------------------------- sources */
-/* Source_terms, src/solver/source_terms.f90:1-224 (GEOM 0/2, isothermal, no viscosity) */
+/* Source_terms, src/solver/source_terms.f90:1-224 (GEOM 0/2, no viscosity): polar terms and, for
+ * adiabatic gas with dust back-reaction, the drag heating of :88-95 */
 static void source_terms(oracle* o) {
   const shark_cfg* p = &o->cfg;
   const double dt = o->dt;
@@ -786,6 +790,19 @@ static void source_terms(oracle* o) {
       for (int k = 0; k < o->nd; ++k) {
         S[IVDX(k) * nc + c] = S[IVDX(k) * nc + c] + dt * (Q(IRHOD(k), c) * (Q(IVDY(k), c) * Q(IVDY(k), c)) / r);
         S[IVDY(k) * nc + c] = S[IVDY(k) * nc + c] - dt * (Q(IRHOD(k), c) * Q(IVDY(k), c) * Q(IVDX(k), c) / r);
+      }
+    }
+  }
+  if (o->nd > 0 && p->dust_back_reaction && p->iso_cs < 1) {
+    #pragma omp parallel for schedule(static)
+    for (int c = 0; c < o->nc; ++c) {
+      if (!o->active[c]) continue;
+      double* S = o->S_U + (size_t)o->iP * o->nc + c;
+      const double vx = Q(o->ivx, c), vy = Q(o->ivy, c), vz = Q(o->ivz, c);
+      for (int k = 0; k < o->nd; ++k) {
+        const double ts = TSTOP(c, k);
+        *S = ((*S + (dt * (vx * (vx - Q(IVDX(k), c)))) / ts) + (dt * (vy * (vy - Q(IVDY(k), c)))) / ts) +
+             (dt * (vz * (vz - Q(IVDZ(k), c)))) / ts;
       }
     }
   }

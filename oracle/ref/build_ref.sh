@@ -54,6 +54,7 @@ HFLAGS=""
 [ "$SETUP" = "disk_polar" ] && HFLAGS="-DREF_DISK"
 [ "$SETUP" = "SI" ] && HFLAGS="-DREF_SI"
 [ "$SETUP" = "SOD" ] && HFLAGS="-DREF_SOD"
+[ "$SETUP" = "KH" ] && HFLAGS="-DREF_KH"
 HARNESS=${REF_HARNESS:-ref_harness.f90}
 $FC $FFLAGS $HFLAGS -c "$HERE/$HARNESS" -o "$OUT/obj/ref_harness.o"
 $FC -fopenmp $OBJS "$OUT/obj/ref_harness.o" -o "$OUT/shark_ref"

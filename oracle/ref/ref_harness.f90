@@ -24,6 +24,9 @@ program ref_harness
 #ifdef REF_SOD
   use setup_parameters, only: rho_l, rho_r, P_l, P_r, direction_shock
 #endif
+#ifdef REF_KH
+  use setup_parameters, only: box_l, box_l_y, rho_dense, rho_diffuse, T_dense, Mach, vy0, kx
+#endif
   implicit none
   integer :: nsteps, istep, ilun, iout
   character(len=256) :: arg, outdir
@@ -144,6 +147,20 @@ program ref_harness
   write(ilun, '(A,ES25.17)') 'p_l ', P_l
   write(ilun, '(A,ES25.17)') 'p_r ', P_r
   write(ilun, '(A,I12)') 'shock_dir ', direction_shock
+#endif
+#ifdef REF_KH
+  write(ilun, '(A,ES25.17)') 'box_l ', box_l
+  write(ilun, '(A,ES25.17)') 'box_l_y ', box_l_y
+  write(ilun, '(A,ES25.17)') 'rho_dense ', rho_dense
+  write(ilun, '(A,ES25.17)') 'rho_diffuse ', rho_diffuse
+  write(ilun, '(A,ES25.17)') 'T_dense ', T_dense
+  write(ilun, '(A,ES25.17)') 'Mach ', Mach
+  write(ilun, '(A,ES25.17)') 'vy0 ', vy0
+  write(ilun, '(A,ES25.17)') 'kx ', kx
+  write(ilun, '(A,ES25.17)') 'tend ', tend
+#if NDUST>0
+  write(ilun, '(A,ES25.17)') 'scut ', scut
+#endif
 #endif
   close(ilun)
 end program ref_harness

@@ -14,7 +14,7 @@ GEOM_CARTESIAN, GEOM_POLAR = 0, 2
 SOLVER_LLF, SOLVER_HLL, SOLVER_HLLC = 0, 1, 2
 DUST_HUANG_BAI, DUST_LLF, DUST_HLL, DUST_HLLC = 0, 1, 2, 3
 BC_PERIODIC, BC_SHEAR_SI, BC_DISK, BC_SOD = 0, 1, 2, 3
-TSTOP_DUSTYWAVE, TSTOP_SI, TSTOP_EPSTEIN_DISK = 0, 1, 2
+TSTOP_DUSTYWAVE, TSTOP_SI, TSTOP_EPSTEIN_DISK, TSTOP_EPSTEIN = 0, 1, 2, 3
 FORCE_NONE, FORCE_SI, FORCE_DISK = 0, 1, 2
 INLOOP_NONE, INLOOP_SI, INLOOP_DISK = 0, 1, 2
 

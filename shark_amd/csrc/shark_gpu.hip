@@ -78,6 +78,7 @@ struct shark_ctx {
   std::vector<double> hpos, hrc;    // position(ncells,ndim) and radii_c(ncells) for outputs
   shark_growth* growth = nullptr;   // attached growth state (dust_growth inside each step)
   double* dRold = nullptr;          // dust densities before growth, for the velocity-keeping momenta
+  double* dfcs = nullptr;           // gas face sound speeds for the HLLC dust solver (adiabatic gas)
   int growth_mode = 0;
 };
 
@@ -101,8 +102,10 @@ static int validate(const shark_cfg* c) {
   if (c->ndust < 0 || c->ndust > SHARK_MAX_DUST) return fail(SHARK_EINVAL, "ndust=%d out of range", c->ndust);
   if (!shark::nd_supported(c->ndust))
     return fail(SHARK_EUNSUP, "ndust=%d: hydro step is instantiated for ndust in {0,1,2,3,4,8,16}", c->ndust);
-  if (c->iso_cs != 1 && c->ndust > 0)
-    return fail(SHARK_EUNSUP, "iso_cs=%d with dust: the adiabatic path is built for gas only", c->iso_cs);
+  if (c->iso_cs > 1)
+    return fail(SHARK_EINVAL, "iso_cs=%d: 1 is isothermal, below 1 adiabatic", c->iso_cs);
+  if (c->iso_cs != 1 && (c->geom != SHARK_GEOM_CARTESIAN || c->fargo))
+    return fail(SHARK_EUNSUP, "iso_cs=%d: adiabatic gas is built on cartesian grids without FARGO", c->iso_cs);
   if (c->bc_kind == SHARK_BC_SOD && (c->ndust > 0 || c->shock_dir < 0 || c->shock_dir > 1 ||
                                      (c->shock_dir == 1 && c->ny < 4)))
     return fail(SHARK_EUNSUP, "Sod boundaries need gas only and shock_dir 0 (x) or 1 (y, 2D)");
@@ -116,7 +119,7 @@ static int validate(const shark_cfg* c) {
   if (c->fargo && c->ny < 4) return fail(SHARK_EUNSUP, "FARGO needs a 2D grid");
   if (c->force_kick && !c->drag && c->ndust > 0)
     return fail(SHARK_EUNSUP, "explicit drag in the force (drag=.false. with force_kick) is not built");
-  if (c->bc_kind < 0 || c->bc_kind > 3 || c->tstop_kind < 0 || c->tstop_kind > 2 ||
+  if (c->bc_kind < 0 || c->bc_kind > 3 || c->tstop_kind < 0 || c->tstop_kind > 3 ||
       c->force_kind < 0 || c->force_kind > 2 || c->inloop_kind < 0 || c->inloop_kind > 2)
     return fail(SHARK_EINVAL, "unknown setup hook kind");
   if (c->bc_kind == SHARK_BC_DISK && c->ny < 4) return fail(SHARK_EUNSUP, "disk boundaries need a 2D grid");
@@ -182,6 +185,16 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
     shark_gpu_destroy(c);
     return rc;
   }
+  if (cfg->iso_cs != 1 && a.nd > 0 && cfg->solverdust == 3) {
+    // godunov.f90:676-680, 706-712: the dust HLLC solver reads the gas face sound speeds
+    if ((e = hipMalloc(&c->dfcs, (size_t)4 * a.nc * 8)) != hipSuccess) {
+      rc = fail(e == hipErrorOutOfMemory ? SHARK_ENOMEM : SHARK_EHIP, "allocation failed: %s", hipGetErrorString(e));
+      shark_gpu_destroy(c);
+      return rc;
+    }
+    (void)hipMemsetAsync(c->dfcs, 0, (size_t)4 * a.nc * 8, c->stream);
+  }
+  a.fcs = c->dfcs;
   (void)hipMemsetAsync(c->dU, 0, nbytes, c->stream);
   (void)hipMemsetAsync(c->dB, 0, nbytes, c->stream);
   memset(c->hsc, 0, sizeof(DevScalars));
@@ -797,6 +810,7 @@ extern "C" int shark_gpu_destroy(shark_ctx* c) {
   // an attached growth state belongs to the caller (shark_growth_destroy after this call)
   (void)hipFree(c->dU); (void)hipFree(c->dB); (void)hipFree(c->dcs); (void)hipFree(c->dgeom); (void)hipFree(c->dsc);
   if (c->dRold) (void)hipFree(c->dRold);
+  if (c->dfcs) (void)hipFree(c->dfcs);
   if (c->dsdust) (void)hipFree(c->dsdust);
   if (c->dcond) (void)hipFree(c->dcond);
   if (c->dfargo) (void)hipFree(c->dfargo);

@@ -38,6 +38,10 @@ struct StepArgs {
   const double* condinit;   // (nvar, ncells), disk in-loop hook
   const double* fargo_vk;   // fargo_velocity per column
   double* Rold;             // (ndust, ncells) dust densities before growth (growth in the step)
+  // (4, ncells) gas face sound speeds for the HLLC dust solver with adiabatic gas, written by the
+  // gas hydro pass: [0]/[1] left/right state of the x face on the cell's left, [2]/[3] of the y
+  // face below it; nullptr when the dust solver does not read them
+  double* fcs;
   DevScalars* sc;
   int nx, ny, nd, nxm, nym, nc, nvar, ndim;
   int geom, solver, solverdust, slope, force_kick, force_kind, tstop_kind, inloop_kind;

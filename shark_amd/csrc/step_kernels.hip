@@ -142,6 +142,9 @@ struct FaceSet {
 DEV double shfl_up1(double x) { return __shfl_up(x, 1); }
 DEV double shfl_dn1(double x) { return __shfl_down(x, 1); }
 
+template <int ND>
+DEV double tstop_of(const StepArgs& a, int k, int cell, int ix, double rho_q, double rhod_q, double cs_q);
+
 // SLOPE: limiter (1..3); SOLV: compile-time Riemann solver (gas 0..2, dust 0..3) or -1 to
 // read it from the arguments.  ADI: adiabatic gas (P from the total energy, its own slopes,
 // Hancock source and face states, sound speeds from the face states; iso_cs < 1).
@@ -367,12 +370,37 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
 #pragma unroll
         for (int c = 0; c < NQ; ++c) L[c] = shfl_up1(fx1.m[c]);
         const double csl = shfl_up1(Rm.cs);
-        riemann(L, csl, fx1.p, Rm.cs, 0, Fx_own);
+        const int fcell = (NDIM == 1) ? ixc - 1 : (ixc - 1) + (m - 1) * a.nxm;
+        if (!GAS && a.fcs) {   // adiabatic gas: the gas pass's face sound speeds
+          riemann(L, a.fcs[fcell], fx1.p, a.fcs[nc + fcell], 0, Fx_own);
+        } else {
+          riemann(L, csl, fx1.p, Rm.cs, 0, Fx_own);
+        }
+        if constexpr (GAS && ADI) {
+          // the face on the left of every owned column and of the ghost column nx+3
+          if (a.fcs && (own_col || ix == a.nx + 3)) {
+            a.fcs[fcell] = sqrt(a.gamma * L[4] / L[0]);
+            a.fcs[nc + fcell] = sqrt(a.gamma * fx1.p[4] / fx1.p[0]);
+          }
+        }
 #pragma unroll
         for (int c = 0; c < 5; ++c) Fx_rt[c] = shfl_dn1(Fx_own[c]);
       }
       // y flux through the face between rows m-1 and m
-      if (NDIM == 2 && m - 1 >= y0 - 1 && m - 1 <= y1) riemann(fym_prev, cs_prev, fy1.p, Rm.cs, 1, Fy_top);
+      if (NDIM == 2 && m - 1 >= y0 - 1 && m - 1 <= y1) {
+        const int fcell = (ixc - 1) + (m - 1) * a.nxm;
+        if (!GAS && a.fcs) {
+          riemann(fym_prev, a.fcs[2 * nc + fcell], fy1.p, a.fcs[3 * nc + fcell], 1, Fy_top);
+        } else {
+          riemann(fym_prev, cs_prev, fy1.p, Rm.cs, 1, Fy_top);
+        }
+        if constexpr (GAS && ADI) {
+          if (a.fcs && own_col) {
+            a.fcs[2 * nc + fcell] = sqrt(a.gamma * fym_prev[4] / fym_prev[0]);
+            a.fcs[3 * nc + fcell] = sqrt(a.gamma * fy1.p[4] / fy1.p[0]);
+          }
+        }
+      }
     }
     // finalize row m-1 (2D) or the single row (1D)
     const int fr = (NDIM == 2) ? m - 1 : 1;
@@ -414,6 +442,25 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
         } else {
           S[1] = 0.0 + dt * exdiv(Rf.q[0] * (Rf.q[2] * Rf.q[2]), r, yrp);
           S[2] = 0.0 - dt * exdiv(Rf.q[0] * Rf.q[2] * Rf.q[1], r, yrp);
+        }
+      }
+      if constexpr (GAS && ADI) {
+        // drag heating of adiabatic gas (source_terms.f90:88-95), species in order, from the
+        // start-of-step primitives and stopping times
+        if (a.nd > 0 && a.back_reaction) {
+          const double vx = Rf.q[1], vy = Rf.q[2], vz = Rf.q[3];
+          const int nd = a.nd;
+          double s4 = 0.0;
+          for (int k = 0; k < nd; ++k) {
+            const double rd = a.U[(5 + k) * nc + cell];
+            const double vdx = a.U[(5 + nd + k) * nc + cell] / rd;
+            const double vdy = a.U[(5 + 2 * nd + k) * nc + cell] / rd;
+            const double vdz = a.U[(5 + 3 * nd + k) * nc + cell] / rd;
+            const double ts = tstop_of<1>(a, k, cell, ixc, Rf.q[0], rd, Rf.cs);
+            s4 = ((s4 + (dt * (vx * (vx - vdx))) / ts) + (dt * (vy * (vy - vdy))) / ts) +
+                 (dt * (vz * (vz - vdz))) / ts;
+          }
+          S[4] = s4;
         }
       }
 #pragma unroll
@@ -466,6 +513,15 @@ DEV double cs_cell(const StepArgs& a, double rho, double E, double vx, double vy
   const double ekin = kHalf * rho * (vx * vx) + kHalf * rho * (vy * vy) + kHalf * rho * (vz * vz);
   const double P = fmaxf_((a.gamma - 1.0) * (E - ekin - 0.0), a.smallp);
   return sqrt(a.gamma * P / fmaxf_(rho, a.smallr));
+}
+
+// ctoprim's sound speed of a cell's start-of-step state (a.U)
+DEV double start_cs(const StepArgs& a, int cell) {
+  if (a.iso) return a.cs[cell];
+  const size_t nc = a.nc;
+  const double rho = a.U[cell];
+  return cs_cell(a, rho, a.U[4 * nc + cell], a.U[nc + cell] / rho, a.U[2 * nc + cell] / rho,
+                 a.U[3 * nc + cell] / rho, cell);
 }
 
 template <int ND>
@@ -529,12 +585,15 @@ DEV void block_atomic_min(DevScalars* sc, double d) {
   }
 }
 
+// Stopping time of species k from the start-of-step primitives; cs_q is that state's sound speed
+// (read only by the uniform Epstein law, KH setup.f90:238 / SOD setup.f90:182)
 template <int ND>
-DEV double tstop_of(const StepArgs& a, int k, int cell, int ix, double rho_q, double rhod_q) {
+DEV double tstop_of(const StepArgs& a, int k, int cell, int ix, double rho_q, double rhod_q, double cs_q) {
   if (a.tstop_kind == 0) return ((rhod_q * rho_q) / a.K[k]) / rho_q;  // DUSTYWAVE setup.f90:217
   const double s = a.sdust_uniform ? a.sdust_k[k] : a.sdust[(size_t)k * a.nc + cell];
   if (a.tstop_kind == 1)  // SI setup.f90:392
     return ((a.rhograin * s) / a.rho_init) / ((a.rad0 * a.omega_shear) * a.hoverr);
+  if (a.tstop_kind == 3) return ((sqrt((kPi * a.gamma) / 8.0) * a.rhograin) * s) / (rho_q * cs_q);
   const double H = a.hoverr * a.g.rc[ix - 1];  // disk_polar setup.f90:436
   return (sqrt((kPi * a.gamma) / 8.0) * a.rhograin) * s /
          ((rho_q / (sqrt(2.0 * kPi) * H)) * a.cs[cell]);
@@ -595,11 +654,12 @@ __global__ __launch_bounds__(256) void k_post(StepArgs a) {
       // dust_drag(1.0d0), dust_drag.f90:21-73, with the stopping times of the step
       // (compute_tstop from the start-of-step primitives)
       const double rho_q = fmaxf_(a.U[cell], a.smallr);
+      const double cs_q = a.tstop_kind == 3 ? start_cs(a, cell) : 0.0;
       double ak[ND > 0 ? ND : 1];
       double rhon = un[0], pnx = un[1], pny = un[2], pnz = un[3];
 #pragma unroll
       for (int k = 0; k < ND; ++k) {
-        const double ts = tstop_of<ND>(a, k, cell, ix, rho_q, a.U[(5 + k) * nc + cell]);
+        const double ts = tstop_of<ND>(a, k, cell, ix, rho_q, a.U[(5 + k) * nc + cell], cs_q);
         ak[k] = (1.0 * dt) / ts;
         const double f = ak[k] / (1.0 + ak[k]);
         pnx = pnx + f * un[5 + ND + k];
@@ -791,9 +851,10 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
         gS[k][c] = 1.0 / (1.0 + ak);
       }
     } else {
+      const double cs_q = a.tstop_kind == 3 ? start_cs(a, cell) : 0.0;
       for (int k = s; k < ND; k += NS) {
         const double rq = (a.tstop_kind == 0) ? a.U[(5 + k) * nc + cell] : 0.0;
-        const double ak = (1.0 * dt) / tstop_of<ND>(a, k, cell, ixs, rho_q, rq);
+        const double ak = (1.0 * dt) / tstop_of<ND>(a, k, cell, ixs, rho_q, rq, cs_q);
         akS[k][c] = ak;
         gS[k][c] = 1.0 / (1.0 + ak);
       }

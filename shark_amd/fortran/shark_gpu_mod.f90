@@ -16,7 +16,8 @@ module shark_gpu_mod
   integer(c_int32_t), parameter :: SHARK_DUST_HUANG_BAI = 0
   integer(c_int32_t), parameter :: SHARK_BC_PERIODIC = 0, SHARK_BC_SHEAR_SI = 1, SHARK_BC_DISK = 2
   integer(c_int32_t), parameter :: SHARK_BC_SOD = 3
-  integer(c_int32_t), parameter :: SHARK_TSTOP_DUSTYWAVE = 0, SHARK_TSTOP_SI = 1, SHARK_TSTOP_EPSTEIN_DISK = 2
+  integer(c_int32_t), parameter :: SHARK_TSTOP_DUSTYWAVE = 0, SHARK_TSTOP_SI = 1, SHARK_TSTOP_EPSTEIN_DISK = 2, &
+       SHARK_TSTOP_EPSTEIN = 3
   integer(c_int32_t), parameter :: SHARK_FORCE_NONE = 0, SHARK_INLOOP_NONE = 0
 
   ! same layout as struct shark_cfg

@@ -117,6 +117,114 @@ def sod(nx=1024, ny=1, shock_dir=0, cfl=0.8, slope_type=3, solver=C.SOLVER_HLL, 
     return cfg, g, dict(u=u, cs=np.zeros(nc), sdust=None, condinit=None, fargo_velocity=None)
 
 
+# src/commons/phys_const.f90:8-12
+MH, KB, GRAV, PC = 1.67e-24, 1.38e-16, 6.67e-8, 3e18
+
+
+def _powi(a, b):
+    """a**b for an integer b, by binary powering as flang lowers it."""
+    recip = b < 0
+    b = abs(b)
+    r = 1.0
+    while True:
+        if b & 1:
+            r *= a
+        b //= 2
+        if b == 0:
+            break
+        a *= a
+    return 1.0 / r if recip else r
+
+
+def units_kh(mu_gas=2.31):
+    """The KH setup's units module (src/setups/KH/units.f90)."""
+    unit_l = PC
+    unit_d = mu_gas * MH
+    unit_t = 1.0 / math.sqrt(GRAV * unit_d)
+    unit_v = unit_l / unit_t
+    return dict(unit_l=unit_l, unit_d=unit_d, unit_t=unit_t, unit_v=unit_v,
+                unit_p=unit_d * (unit_v * unit_v), unit_m=_powi(unit_l, 3) * unit_d,
+                unit_dcol=unit_d * unit_l)
+
+
+def mrn_sizes_and_ratios(ndust, units, smin=1e-7, smax=1e-2, scutmin=5e-7, scut=250e-7, mrn=3.5,
+                         rhograin=2.7, dust2gas=0.01, ice_mantle=8.7e-7, dust_ratio_min=1e-11):
+    """Grain sizes (code units) and dust-to-gas ratios of distribution_dust(.true.) with the MRN
+    law (src/dust/dust_init.f90:11-103, mrn_distri :164-197; kernel_type 0): logarithmic bins
+    from smin to smax, the power law normalised over the bins between scutmin and scut, other
+    bins at dust_ratio_min, each size sqrt(a- a+) plus the ice mantle."""
+    ul, um = units["unit_l"], units["unit_m"]
+    f43 = _f32(4.0 / 3.0)
+    zeta = math.pow(smax / smin, 1.0 / ndust)
+    eta = zeta * zeta * zeta
+    m_min = (((f43 * PI) * rhograin) * (smin * smin * smin)) / um
+    am = [(smin * _powi(zeta, k - 1)) / ul for k in range(1, ndust + 1)]
+    ap = [(smin * _powi(zeta, k)) / ul for k in range(1, ndust + 1)]
+    massmin = m_min * _powi(eta, 0)
+    mono1 = (((f43 * PI) * rhograin) * (scutmin * scutmin * scutmin)) / um
+    mono2 = (((f43 * PI) * rhograin) * (scut * scut * scut)) / um
+    icutmin = max(1, math.floor(math.log(mono1 / massmin) / math.log(eta) + 1))
+    icutmax = min(math.floor(math.log(mono2 / massmin) / math.log(eta) + 1), ndust)
+    e = 4.0 - mrn
+    eps = [0.0] * ndust
+    norm = 0.0
+    for k in range(icutmin, icutmax + 1):
+        t = math.pow(ap[k - 1], e) - math.pow(am[k - 1], e)
+        norm = norm + t
+        eps[k - 1] = t
+    eps = [max((dust2gas * x) / norm, dust_ratio_min) for x in eps]
+    sizes = [math.sqrt(am[k] * ap[k]) + ice_mantle / ul for k in range(ndust)]
+    return sizes, eps
+
+
+def kh(nx=256, ny=256, ndust=1, cfl=0.5, slope_type=3, solver=C.SOLVER_HLLC, solverdust=0,
+       gamma=1.666667, box_l=4.8481e-6, box_l_y=4.8481e-6, rho_dense=1e-18, rho_diffuse=1e-18,
+       T_dense=10.0, mach=0.25, vy0=0.1, kx=2.0, smin=5e-7, scutmin=1e-7, scut=250e-7, smax=0.1,
+       rhograin=2.3, dust2gas=0.01, mrn=3.5, mu_gas=2.31):
+    """Kelvin-Helmholtz shear layer with dust (src/setups/KH/setup.f90:1-52): adiabatic gas at
+    Mach/2 of the T_dense sound speed, streaming in +x outside |y| < box_l_y/4 and in -x inside,
+    with a vy0 cos(2 kx pi x / box_l) seed; dust at rest, one species at dust-to-gas 0.01 and size
+    scut, or the MRN distribution; Epstein stopping times with the step's sound speed
+    (:218-236); periodic boundaries; drag with heating of the gas.  Defaults: the reference's
+    test/KH/kh.nml (1 au box, cgs inputs, units of src/setups/KH/units.f90)."""
+    u = units_kh(mu_gas)
+    smallr = 1e-27 / u["unit_d"]
+    smallc = 1e-27 / u["unit_v"]
+    cfg = C.make_cfg(nx=nx, ny=ny, ndust=ndust, iso_cs=-1, cfl=cfl, slope_type=slope_type,
+                     solver=solver, solverdust=solverdust, gamma=gamma, bc_kind=C.BC_PERIODIC,
+                     tstop_kind=C.TSTOP_EPSTEIN, force_kind=C.FORCE_NONE, inloop_kind=C.INLOOP_NONE,
+                     rhograin=rhograin / u["unit_d"], smallr=smallr, smallc=smallc,
+                     smallp=smallr * (smallc * smallc))
+    g = grid_cartesian_2d(nx, ny, box_l, box_l_y)
+    nc = g["vol"].size
+    cs = math.sqrt(((gamma * KB) * T_dense) / (mu_gas * MH))
+    nv = 5 + 4 * ndust
+    q = np.zeros((nv, nc))
+    acos1 = math.acos(-1.0)
+    for c in range(nc):
+        xx = g["position"][0, c] - 0.5 * box_l
+        yy = g["position"][1, c] - 0.5 * box_l_y
+        rho = rho_dense / u["unit_d"]
+        vx = ((cs * mach) / u["unit_v"]) * 0.5
+        vy = (((vy0 * cs) * mach) * math.cos((((xx * 2.0) * kx) * acos1) / box_l)) / u["unit_v"]
+        if abs(yy / box_l_y) < 0.25:
+            rho = rho_diffuse / u["unit_d"]
+            vx = -vx
+        q[0, c], q[1, c], q[2, c] = rho, vx, vy
+        q[4, c] = (rho_dense * (cs * cs)) / u["unit_p"]
+    sd = np.zeros((ndust, nc))
+    if ndust == 1:
+        sizes, eps = [scut / u["unit_l"]], [_f32(0.01)]
+    elif ndust > 1:
+        sizes, eps = mrn_sizes_and_ratios(ndust, u, smin=smin, smax=smax, scutmin=scutmin, scut=scut,
+                                          mrn=mrn, rhograin=rhograin, dust2gas=dust2gas)
+    for k in range(ndust):
+        q[5 + k] = eps[k] * q[0]
+        sd[k] = sizes[k]
+    return cfg, g, dict(u=primtoc(q, ndust, gamma), cs=np.zeros(nc), sdust=sd if ndust else None,
+                        condinit=None, fargo_velocity=None)
+
+
 def dustywave(nx=512, K=10.0, dust2gas=1.0, vx0=1e-4, drho=0.0, cs0=1.0, rho0=1.0, box_l=1.0,
               kx=1.0, cfl=0.5, slope_type=1, solver=C.SOLVER_HLL, solverdust=C.DUST_HUANG_BAI,
               gamma=1.6666667):

@@ -130,6 +130,46 @@ charging=.false.
 /
 """
 
+# Kelvin-Helmholtz (setups/KH): adiabatic gas with dust, drag and drag heating; the parameters of
+# the reference's test/KH/kh.nml on a 1 au box, tend in Myr
+KH = """&GRID_PARAMS
+freq_out=100000
+CFL = {cfl}
+tend = 1d3
+gamma = 1.666667d0
+slope_type = {slope}
+force_kick = .false.
+/
+&SETUP_PARAMS
+box_l   = 4.8481e-6
+box_l_y = 4.8481e-6
+rho_dense    = 1.0d-18
+rho_diffuse  = {rho_diffuse}
+T_dense      = 10.0d0
+T_diffuse    = 10.0d0
+Mach         = 0.25d0
+vy0          = 1d-1
+kx           = 2.0d0
+/
+&DUST_PARAMS
+dust_distribution='mrn'
+smin     = 5d-7
+scutmin  = 1d-7
+scut     = {scut}
+smax     = 0.10d0
+rhograin = 2.3d0
+dust2gas = 0.01d0
+mrn      = 3.5d0
+dust_back_reaction = .true.
+growth             = .false.
+/
+&GRAVITY_PARAMS
+/
+&IONISATION_PARAMS
+charging=.false.
+/
+"""
+
 # name: (setup, nx, ny, ndust, geom, gridspace, solver, solverdust, namelist, nsteps)
 CASES = {
     "dustywave_k50": ("DUSTYWAVE", 128, 1, 1, 0, 0, 1, 0,
@@ -153,6 +193,12 @@ CASES = {
     "sod_llf_s2": ("SOD", 96, 1, 0, 0, 0, 0, 0, SOD.format(cfl=0.5, tend=0.2, slope=2, dir=0), -1),
     "sod_2d_y_hll": ("SOD", 16, 64, 0, 0, 0, 1, 0, SOD.format(cfl=0.5, tend="1d10", slope=1, dir=1), 40),
     "sod_2d_x_hllc": ("SOD", 48, 8, 0, 0, 0, 2, 0, SOD.format(cfl=0.5, tend="1d10", slope=3, dir=0), 30),
+    "kh_n1_hllc": ("KH", 32, 32, 1, 0, 0, 2, 0,
+                   KH.format(cfl=0.5, slope=3, rho_diffuse="0.5d-18", scut="250d-7"), 40),
+    "kh_n4_hll": ("KH", 24, 32, 4, 0, 0, 1, 2,
+                  KH.format(cfl=0.4, slope=1, rho_diffuse="1.0d-18", scut="250d-7"), 40),
+    "kh_n4_llf_dhllc": ("KH", 24, 32, 4, 0, 0, 0, 3,
+                        KH.format(cfl=0.5, slope=2, rho_diffuse="0.7d-18", scut="250d-7"), 30),
 }
 
 

@@ -45,6 +45,9 @@ def load_ref(name):
         kw.update(bc_kind=C.BC_SOD, tstop_kind=C.TSTOP_DUSTYWAVE, force_kind=C.FORCE_NONE,
                   inloop_kind=C.INLOOP_NONE, rho_l=m["rho_l"], rho_r=m["rho_r"], p_l=m["p_l"],
                   p_r=m["p_r"], shock_dir=m["shock_dir"])
+    elif setup == "KH":
+        kw.update(bc_kind=C.BC_PERIODIC, tstop_kind=C.TSTOP_EPSTEIN, force_kind=C.FORCE_NONE,
+                  inloop_kind=C.INLOOP_NONE, rhograin=m["rhograin"] / m["unit_d"])
     cfg = C.make_cfg(**kw)
     geom = dict(dx=z["dx"], surf=z["surf"], vol=z["vol"], radii_c=z["radii_c"], position=z["position"])
     state = dict(u=z["ic"], cs=z["cs"], sdust=z["sdust"] if "sdust" in z else None,
