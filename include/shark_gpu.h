@@ -49,18 +49,22 @@ enum { SHARK_DUST_HUANG_BAI = 0, SHARK_DUST_LLF = 1, SHARK_DUST_HLL = 2, SHARK_D
 enum { SHARK_BC_PERIODIC = 0,   /* src/grid/boundary_types.f90:149-186 (DUSTYWAVE) */
        SHARK_BC_SHEAR_SI = 1,   /* src/setups/SI/boundaries.f90:10-85 */
        SHARK_BC_DISK = 2,       /* src/setups/disk_polar/boundaries.f90:10-62 */
-       SHARK_BC_SOD = 3 };      /* src/setups/SOD/boundaries.f90:10-80 */
+       SHARK_BC_SOD = 3,        /* src/setups/SOD/boundaries.f90:10-80 */
+       SHARK_BC_ZERO_GRADIENT = 4 }; /* src/grid/boundary_types.f90:48-100 (cartesian disk) */
 enum { SHARK_TSTOP_DUSTYWAVE = 0,  /* src/setups/DUSTYWAVE/setup.f90:217 */
        SHARK_TSTOP_SI = 1,         /* src/setups/SI/setup.f90:392 */
        SHARK_TSTOP_EPSTEIN_DISK = 2, /* src/setups/disk_polar/setup.f90:436 */
-       SHARK_TSTOP_EPSTEIN = 3 };    /* src/setups/KH/setup.f90:238, SOD/setup.f90:182: uniform medium,
+       SHARK_TSTOP_EPSTEIN = 3,      /* src/setups/KH/setup.f90:238, SOD/setup.f90:182: uniform medium,
                                         sqrt(pi gamma/8) rhograin s / (rho cs), rhograin in code units */
+       SHARK_TSTOP_EPSTEIN_CDISK = 4 }; /* src/setups/disk/setup.f90:322-345 (cartesian disk) */
 enum { SHARK_FORCE_NONE = 0,
        SHARK_FORCE_SI = 1,          /* src/setups/SI/setup.f90:335-368 */
-       SHARK_FORCE_DISK = 2 };      /* src/setups/disk_polar/setup.f90:360-403 */
+       SHARK_FORCE_DISK = 2,        /* src/setups/disk_polar/setup.f90:360-403 */
+       SHARK_FORCE_CDISK = 3 };     /* src/setups/disk/setup.f90:265-318: softened star (+ planet) */
 enum { SHARK_INLOOP_NONE = 0,
        SHARK_INLOOP_SI = 1,         /* src/setups/SI/setup.f90:299-332 */
-       SHARK_INLOOP_DISK = 2 };     /* src/setups/disk_polar/setup.f90:206-284 */
+       SHARK_INLOOP_DISK = 2,       /* src/setups/disk_polar/setup.f90:206-284 */
+       SHARK_INLOOP_CDISK = 3 };    /* src/setups/disk/setup.f90:203-262: buffer zones reset */
 
 /* Configuration: the reference's cpp macro set (bin/Makefile:5-36) plus the run-time
  * namelist scalars the step reads (&GRID_PARAMS src/commons/read_params.f90:10,
@@ -70,7 +74,7 @@ typedef struct shark_cfg {
   int32_t nx, ny, ndust;    /* active cells per direction (ny=1 means 1D), dust species */
   int32_t nghost;           /* must be 2 (bin/Makefile:8) */
   int32_t geom, solver, solverdust, slope_type;
-  int32_t iso_cs;           /* 1: isothermal (setup's cs); 0: adiabatic (P from E, gamma), gas only */
+  int32_t iso_cs;           /* 1: isothermal (setup's cs); below 1: adiabatic (P from E, gamma) */
   int32_t force_kick, fargo, drag, dust_back_reaction, growth, viscosity;
   int32_t bc_kind, tstop_kind, force_kind, inloop_kind;
   int32_t disk_bc_mode;     /* 0: reference indexing (boundaries.f90:45-46), 1: per-row copy */
@@ -85,7 +89,12 @@ typedef struct shark_cfg {
   double mstar, r_relax, n_rel, smooth_r;
   /* SOD (src/setups/SOD/setup_commons.f90): left and right states of the shock tube */
   double rho_l, rho_r, p_l, p_r;
-  double reserved[12];
+  /* cartesian disk (src/setups/disk/setup_commons.f90, after setup's unit conversion; box_l,
+   * mstar, smooth_r, hoverr above); dust2gas of &DUST_PARAMS; alpha_visc sets
+   * eta_visc = alpha_visc H cs (setup.f90:60) when viscosity is on */
+  double box_l_y, mplanet, rplanet, sigma_r0, r0_disk, frac_r_disk, buffer_in, buffer_out;
+  double alpha_visc, dust2gas;
+  double reserved[2];
 } shark_cfg;
 
 typedef struct shark_ctx shark_ctx;

@@ -266,8 +266,33 @@ static void bc_sod(oracle* o) {
     }
 }
 
+/* zero_gradients, src/grid/boundary_types.f90:48-100 (cartesian disk).  In 2D the source cells
+ * are first_active_y / last_active_y (:70-71) and first_active / last_active (:88-89) taken as
+ * flat cell indices, i.e. cells of the first ghost row; kept as written, in the loop order */
+static void bc_zero_gradient(oracle* o) {
+  if (o->ndim == 1) {
+    for (int ix = 1; ix <= 2; ++ix)
+      for (int v = 0; v < o->nvar; ++v) {
+        U(v, ix - 1) = U(v, o->fa - 1);
+        U(v, o->nxm - ix) = U(v, o->la - 1);
+      }
+    return;
+  }
+  for (int ix = o->fa; ix <= o->la; ++ix)
+    for (int iy = 1; iy <= 2; ++iy) {
+      int bl = cidx(o, ix, iy), br = cidx(o, ix, o->nym + 1 - iy);
+      for (int v = 0; v < o->nvar; ++v) { U(v, bl) = U(v, o->fay - 1); U(v, br) = U(v, o->lay - 1); }
+    }
+  for (int ix = 1; ix <= 2; ++ix)
+    for (int iy = 1; iy <= o->nym; ++iy) {
+      int bl = cidx(o, ix, iy), br = cidx(o, o->nxm + 1 - ix, iy);
+      for (int v = 0; v < o->nvar; ++v) { U(v, bl) = U(v, o->fa - 1); U(v, br) = U(v, o->la - 1); }
+    }
+}
+
 static void apply_boundaries(oracle* o) {
   switch (o->cfg.bc_kind) {
+    case SHARK_BC_ZERO_GRADIENT: bc_zero_gradient(o); break;
     case SHARK_BC_PERIODIC: bc_periodic(o); break;
     case SHARK_BC_SHEAR_SI: bc_shear_si(o); break;
     case SHARK_BC_DISK: bc_disk(o); break;
@@ -308,6 +333,17 @@ static void ctoprim(oracle* o) {
 }
 
 /* ---------------------------------------------------------------- setup hooks */
+/* x**3 for the integer literal 3, as flang lowers it */
+static double cube_i(double x) { return x * (x * x); }
+
+/* cartesian disk: offsets from the box centre and the softened radius (disk/setup.f90:266-272) */
+static void cdisk_r(const oracle* o, int c, double* xx, double* yy, double* rr) {
+  const shark_cfg* p = &o->cfg;
+  *xx = POS(c, 0) - HALF * p->box_l;
+  *yy = POS(c, 1) - HALF * p->box_l_y;
+  *rr = sqrt((*xx * *xx + *yy * *yy) + p->smooth_r * p->smooth_r);
+}
+
 /* update_force_setup: SI src/setups/SI/setup.f90:335-368; disk :360-403 */
 static void update_force(oracle* o) {
   const shark_cfg* p = &o->cfg;
@@ -324,6 +360,25 @@ static void update_force(oracle* o) {
         FDUST(c, 0, k) = a - (2.0 * Om) * Q(IVDZ(k), c);
         FDUST(c, 1, k) = 0.0;
         FDUST(c, 2, k) = (2.0 * Om) * Q(IVDX(k), c);
+      }
+    } else if (p->force_kind == SHARK_FORCE_CDISK) {
+      /* disk/setup.f90:274-301: softened star, plus a planet on a circular orbit */
+      double xx, yy, rr;
+      cdisk_r(o, c, &xx, &yy, &rr);
+      double f1 = -(p->mstar * xx / cube_i(rr)), f2 = -(p->mstar * yy / cube_i(rr));
+      double g1 = f1, g2 = f2;
+      if (p->mplanet > 0.0) {
+        double theta_p = 2.0 * PI * fmod(o->time / (2.0 * PI / sqrt(p->mstar / cube_i(p->rplanet))), 1.0);
+        double xx_p = cos(theta_p) * p->rplanet, yy_p = sin(theta_p) * p->rplanet;
+        double rr_p = sqrt((xx_p * xx_p + yy_p * yy_p) + p->smooth_r * p->smooth_r);
+        f1 = f1 - p->mplanet * xx_p / cube_i(rr_p);
+        f2 = f2 - p->mplanet * yy_p / cube_i(rr_p);
+        g1 = g1 - p->mplanet * xx_p / cube_i(rr_p);
+        g2 = g2 - p->mplanet * yy_p / cube_i(rr_p);
+      }
+      FORCE(c, 0) = f1; FORCE(c, 1) = f2; FORCE(c, 2) = 0.0;
+      for (int k = 0; k < o->nd; ++k) {
+        FDUST(c, 0, k) = g1; FDUST(c, 1, k) = g2; FDUST(c, 2, k) = 0.0;
       }
     } else if (p->force_kind == SHARK_FORCE_DISK) {
       double r = o->radii_c[c];
@@ -371,6 +426,12 @@ static void compute_tstop(oracle* o) {
         ts = ((p->rhograin * SDUST(c, k)) / p->rho_init) / ((p->rad0 * p->omega_shear) * p->hoverr);
       } else if (p->tstop_kind == SHARK_TSTOP_EPSTEIN) {
         ts = (sqrt((PI * p->gamma) / 8.0) * p->rhograin) * SDUST(c, k) / (Q(o->irho, c) * o->cs[c]);
+      } else if (p->tstop_kind == SHARK_TSTOP_EPSTEIN_CDISK) {
+        double xx, yy, rr;
+        cdisk_r(o, c, &xx, &yy, &rr);
+        double H = p->hoverr * rr;
+        ts = (sqrt((PI * p->gamma) / 8.0) * p->rhograin) * SDUST(c, k) /
+             ((Q(o->irho, c) / sqrt(2.0 * PI * H)) * o->cs[c]);
       } else {
         double H = p->hoverr * o->radii_c[c];
         ts = (sqrt((PI * p->gamma) / 8.0) * p->rhograin) * SDUST(c, k) /
@@ -806,6 +867,28 @@ static void source_terms(oracle* o) {
       }
     }
   }
+  if (p->viscosity && p->geom == SHARK_GEOM_CARTESIAN && o->ndim == 2) {
+    /* :99-110, GEOM 0: eta_visc = alpha_visc H cs of the cartesian disk (disk/setup.f90:60) */
+    #pragma omp parallel for schedule(static)
+    for (int c = 0; c < o->nc; ++c) {
+      if (!o->active[c]) continue;
+      int ix = ixx(o, c), iy = iyy(o, c);
+      int xp = cidx(o, ix + 1, iy), xm = cidx(o, ix - 1, iy), yp = cidx(o, ix, iy + 1), ym = cidx(o, ix, iy - 1);
+      double xx, yy, rr;
+      cdisk_r(o, c, &xx, &yy, &rr);
+      double eta = (p->alpha_visc * (p->hoverr * rr)) * o->cs[c];
+      double d1 = DX(c, 0), d2 = DX(c, 1);
+      double* S = o->S_U;
+      size_t nc = o->nc;
+      double lxu = ((Q(o->ivx, xp) + Q(o->ivx, xm)) - 2.0 * Q(o->ivx, c)) / (d1 * d1);
+      S[o->ivx * nc + c] = S[o->ivx * nc + c] + Q(o->irho, c) * dt * eta * lxu;
+      double lyu = ((Q(o->ivx, yp) + Q(o->ivx, ym)) - 2.0 * Q(o->ivx, c)) / (d2 * d2);
+      S[o->ivx * nc + c] = S[o->ivx * nc + c] + Q(o->irho, c) * dt * eta * lyu;
+      double lxv = ((Q(o->ivy, xp) + Q(o->ivy, xm)) - 2.0 * Q(o->ivy, c)) / (d1 * d1);
+      double lyv = ((Q(o->ivy, yp) + Q(o->ivy, ym)) - 2.0 * Q(o->ivy, c)) / (d2 * d2);
+      S[o->ivy * nc + c] = S[o->ivy * nc + c] + Q(o->irho, c) * dt * eta * (lxv + lyv);
+    }
+  }
   /* :221 u_prim = u_prim + S_U over the whole array */
   #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < n; ++i) o->u[i] = o->u[i] + o->S_U[i];
@@ -911,6 +994,35 @@ static void setup_inloop(oracle* o) {
         u = U(IVDZ(k), c) / U(IRHOD(k), c); v = U(IVDX(k), c) / U(IRHOD(k), c); d = U(IRHOD(k), c);
         U(IVDZ(k), c) = d * (u * (1. - Ohmdt * Ohmdt) + 2. * v * Ohmdt + AA * (Ohmdt * Ohmdt)) / (1. + Ohmdt * Ohmdt);
         U(IVDX(k), c) = d * (v * (1. - Ohmdt * Ohmdt) - 2. * u * Ohmdt + AA * Ohmdt) / (1. + Ohmdt * Ohmdt);
+      }
+    }
+  } else if (p->inloop_kind == SHARK_INLOOP_CDISK) {
+    /* disk/setup.f90:203-262: cells of the inner and outer buffers reset to the initial disk
+     * (rate = 0), every expression kept as written */
+    const double rate = 0.0, R_out_disk = p->frac_r_disk * p->box_l;
+    #pragma omp parallel for schedule(static)
+    for (int c = 0; c < o->nc; ++c) {
+      if (!o->active[c]) continue;
+      double xx, yy, rr;
+      cdisk_r(o, c, &xx, &yy, &rr);
+      double r_cyl = sqrt(xx * xx + yy * yy);
+      double x_soft = xx * rr / r_cyl, y_soft = yy * rr / r_cyl;
+      if (!(rr < p->buffer_in * p->box_l || rr > p->buffer_out * p->box_l)) continue;
+      double H = p->hoverr * rr;
+      double Omega = sqrt(p->mstar / cube_i(rr));
+      double cs0 = H * Omega;
+      double rho_ana = p->sigma_r0 * (p->r0_disk / rr);
+      if (rr > R_out_disk) rho_ana = p->sigma_r0 * (p->r0_disk / R_out_disk) / 100.0;
+      double v_ana = -(Omega * rho_ana);
+      U(o->irho, c) = rho_ana + (U(o->irho, c) - rho_ana) * rate;
+      U(o->ivx, c) = -(y_soft * v_ana) + (U(o->ivx, c) + yy * v_ana) * rate;
+      U(o->ivy, c) = x_soft * v_ana + (U(o->ivy, c) - xx * v_ana) * rate;
+      U(o->iP, c) = (rho_ana + (U(o->irho, c) - rho_ana) * rate) * (cs0 * cs0);
+      for (int k = 0; k < o->nd; ++k) {
+        double dust_to = p->dust2gas;
+        U(IRHOD(k), c) = dust_to * rho_ana + (U(IRHOD(k), c) - rho_ana * dust_to) * rate;
+        U(IVDX(k), c) = -(yy * v_ana * dust_to) + (U(IVDX(k), c) + dust_to * yy * v_ana) * rate;
+        U(IVDY(k), c) = xx * v_ana * dust_to + (U(IVDY(k), c) - dust_to * xx * v_ana) * rate;
       }
     }
   } else if (p->inloop_kind == SHARK_INLOOP_DISK) {

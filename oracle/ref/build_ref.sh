@@ -55,6 +55,7 @@ HFLAGS=""
 [ "$SETUP" = "SI" ] && HFLAGS="-DREF_SI"
 [ "$SETUP" = "SOD" ] && HFLAGS="-DREF_SOD"
 [ "$SETUP" = "KH" ] && HFLAGS="-DREF_KH"
+[ "$SETUP" = "disk" ] && HFLAGS="-DREF_CDISK"
 HARNESS=${REF_HARNESS:-ref_harness.f90}
 $FC $FFLAGS $HFLAGS -c "$HERE/$HARNESS" -o "$OUT/obj/ref_harness.o"
 $FC -fopenmp $OBJS "$OUT/obj/ref_harness.o" -o "$OUT/shark_ref"

@@ -27,6 +27,10 @@ program ref_harness
 #ifdef REF_KH
   use setup_parameters, only: box_l, box_l_y, rho_dense, rho_diffuse, T_dense, Mach, vy0, kx
 #endif
+#ifdef REF_CDISK
+  use setup_parameters, only: box_l, box_l_y, sigma_R0, R0_disk, Mstar, Mplanet, Rplanet, &
+       frac_R_disk, buffer_in, buffer_out, HoverR, smooth_r, alpha_visc
+#endif
   implicit none
   integer :: nsteps, istep, ilun, iout
   character(len=256) :: arg, outdir
@@ -147,6 +151,28 @@ program ref_harness
   write(ilun, '(A,ES25.17)') 'p_l ', P_l
   write(ilun, '(A,ES25.17)') 'p_r ', P_r
   write(ilun, '(A,I12)') 'shock_dir ', direction_shock
+#endif
+#ifdef REF_CDISK
+  write(ilun, '(A,ES25.17)') 'box_l ', box_l
+  write(ilun, '(A,ES25.17)') 'box_l_y ', box_l_y
+  write(ilun, '(A,ES25.17)') 'sigma_R0 ', sigma_R0
+  write(ilun, '(A,ES25.17)') 'R0_disk ', R0_disk
+  write(ilun, '(A,ES25.17)') 'mstar ', Mstar
+  write(ilun, '(A,ES25.17)') 'mplanet ', Mplanet
+  write(ilun, '(A,ES25.17)') 'rplanet ', Rplanet
+  write(ilun, '(A,ES25.17)') 'frac_R_disk ', frac_R_disk
+  write(ilun, '(A,ES25.17)') 'buffer_in ', buffer_in
+  write(ilun, '(A,ES25.17)') 'buffer_out ', buffer_out
+  write(ilun, '(A,ES25.17)') 'hoverr ', HoverR
+  write(ilun, '(A,ES25.17)') 'smooth_r ', smooth_r
+  write(ilun, '(A,ES25.17)') 'alpha_visc ', alpha_visc
+  write(ilun, '(A,L2)') 'viscosity ', viscosity
+#if NDUST>0
+  write(ilun, '(A,ES25.17)') 'dust2gas ', dust2gas
+#endif
+  open(32, file=trim(outdir)//'/eta_visc.bin', form='unformatted', access='stream', status='replace')
+  write(32) eta_visc
+  close(32)
 #endif
 #ifdef REF_KH
   write(ilun, '(A,ES25.17)') 'box_l ', box_l

@@ -13,10 +13,10 @@ SHARK_MAX_DUST = 64
 GEOM_CARTESIAN, GEOM_POLAR = 0, 2
 SOLVER_LLF, SOLVER_HLL, SOLVER_HLLC = 0, 1, 2
 DUST_HUANG_BAI, DUST_LLF, DUST_HLL, DUST_HLLC = 0, 1, 2, 3
-BC_PERIODIC, BC_SHEAR_SI, BC_DISK, BC_SOD = 0, 1, 2, 3
-TSTOP_DUSTYWAVE, TSTOP_SI, TSTOP_EPSTEIN_DISK, TSTOP_EPSTEIN = 0, 1, 2, 3
-FORCE_NONE, FORCE_SI, FORCE_DISK = 0, 1, 2
-INLOOP_NONE, INLOOP_SI, INLOOP_DISK = 0, 1, 2
+BC_PERIODIC, BC_SHEAR_SI, BC_DISK, BC_SOD, BC_ZERO_GRADIENT = 0, 1, 2, 3, 4
+TSTOP_DUSTYWAVE, TSTOP_SI, TSTOP_EPSTEIN_DISK, TSTOP_EPSTEIN, TSTOP_EPSTEIN_CDISK = 0, 1, 2, 3, 4
+FORCE_NONE, FORCE_SI, FORCE_DISK, FORCE_CDISK = 0, 1, 2, 3
+INLOOP_NONE, INLOOP_SI, INLOOP_DISK, INLOOP_CDISK = 0, 1, 2, 3
 
 
 class ShARKCfg(C.Structure):
@@ -36,7 +36,11 @@ class ShARKCfg(C.Structure):
         ("hoverr", C.c_double), ("rhograin", C.c_double),
         ("mstar", C.c_double), ("r_relax", C.c_double), ("n_rel", C.c_double),
         ("smooth_r", C.c_double), ("rho_l", C.c_double), ("rho_r", C.c_double),
-        ("p_l", C.c_double), ("p_r", C.c_double), ("reserved", C.c_double * 12),
+        ("p_l", C.c_double), ("p_r", C.c_double),
+        ("box_l_y", C.c_double), ("mplanet", C.c_double), ("rplanet", C.c_double),
+        ("sigma_r0", C.c_double), ("r0_disk", C.c_double), ("frac_r_disk", C.c_double),
+        ("buffer_in", C.c_double), ("buffer_out", C.c_double), ("alpha_visc", C.c_double),
+        ("dust2gas", C.c_double), ("reserved", C.c_double * 2),
     ]
 
 

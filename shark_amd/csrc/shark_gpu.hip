@@ -115,12 +115,17 @@ static int validate(const shark_cfg* c) {
   if (c->solver < 0 || c->solver > 2) return fail(SHARK_EINVAL, "solver=%d", c->solver);
   if (c->solverdust < 0 || c->solverdust > 3) return fail(SHARK_EINVAL, "solverdust=%d", c->solverdust);
   if (c->slope_type < 1 || c->slope_type > 3) return fail(SHARK_EINVAL, "unknown slope limiter %d", c->slope_type);
-  if (c->viscosity) return fail(SHARK_EUNSUP, "viscosity source terms are not built");
+  const bool cdisk = c->force_kind == SHARK_FORCE_CDISK || c->inloop_kind == SHARK_INLOOP_CDISK ||
+                     c->tstop_kind == SHARK_TSTOP_EPSTEIN_CDISK;
+  if (cdisk && (c->geom != SHARK_GEOM_CARTESIAN || c->ny < 4 || c->iso_cs != 1))
+    return fail(SHARK_EUNSUP, "the cartesian disk hooks need a 2D cartesian grid and isothermal gas");
+  if (c->viscosity && !(c->inloop_kind == SHARK_INLOOP_CDISK && c->iso_cs == 1 && c->ny >= 4))
+    return fail(SHARK_EUNSUP, "viscosity is built for the isothermal cartesian disk (eta_visc = alpha_visc H cs)");
   if (c->fargo && c->ny < 4) return fail(SHARK_EUNSUP, "FARGO needs a 2D grid");
   if (c->force_kick && !c->drag && c->ndust > 0)
     return fail(SHARK_EUNSUP, "explicit drag in the force (drag=.false. with force_kick) is not built");
-  if (c->bc_kind < 0 || c->bc_kind > 3 || c->tstop_kind < 0 || c->tstop_kind > 3 ||
-      c->force_kind < 0 || c->force_kind > 2 || c->inloop_kind < 0 || c->inloop_kind > 2)
+  if (c->bc_kind < 0 || c->bc_kind > 4 || c->tstop_kind < 0 || c->tstop_kind > 4 ||
+      c->force_kind < 0 || c->force_kind > 3 || c->inloop_kind < 0 || c->inloop_kind > 3)
     return fail(SHARK_EINVAL, "unknown setup hook kind");
   if (c->bc_kind == SHARK_BC_DISK && c->ny < 4) return fail(SHARK_EUNSUP, "disk boundaries need a 2D grid");
   if (c->cfl <= 0) return fail(SHARK_EINVAL, "CFL must be positive");
@@ -163,6 +168,10 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   a.s2pi = sqrt(2.0 * 3.141592653589793238462643383279);
   a.rhograin = cfg->rhograin; a.mstar = cfg->mstar; a.r_relax = cfg->r_relax; a.n_rel = cfg->n_rel;
   a.smooth_r = cfg->smooth_r;
+  a.box_l_y = cfg->box_l_y; a.mplanet = cfg->mplanet; a.rplanet = cfg->rplanet; a.sigma_r0 = cfg->sigma_r0;
+  a.r0_disk = cfg->r0_disk; a.frac_r_disk = cfg->frac_r_disk; a.buffer_in = cfg->buffer_in;
+  a.buffer_out = cfg->buffer_out; a.alpha_visc = cfg->alpha_visc; a.dust2gas = cfg->dust2gas;
+  a.viscosity = cfg->viscosity;
   for (int k = 0; k < SHARK_MAX_DUST; ++k) a.K[k] = cfg->K[k];
   // launch shapes (shark_gpu_set_tune / shark_gpu_autotune change them)
   a.strip_h = 32;
@@ -178,7 +187,7 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipMalloc(&c->dU, nbytes)) != hipSuccess || (e = hipMalloc(&c->dB, nbytes)) != hipSuccess ||
       (e = hipMalloc(&c->dcs, (size_t)a.nc * 8)) != hipSuccess ||
-      (e = hipMalloc(&c->dgeom, (size_t)7 * a.nxm * 8)) != hipSuccess ||
+      (e = hipMalloc(&c->dgeom, ((size_t)7 * a.nxm + a.nym) * 8)) != hipSuccess ||
       (e = hipMalloc(&c->dsc, sizeof(DevScalars))) != hipSuccess ||
       (e = hipHostMalloc(&c->hsc, sizeof(DevScalars))) != hipSuccess) {
     rc = fail(e == hipErrorOutOfMemory ? SHARK_ENOMEM : SHARK_EHIP, "allocation failed: %s", hipGetErrorString(e));
@@ -203,7 +212,7 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   a.U = c->dU; a.B = c->dB; a.cs = c->dcs; a.sc = c->dsc;
   const double* gp = c->dgeom;
   a.g.dx1 = gp; a.g.dx2 = gp + a.nxm; a.g.surf1 = gp + 2 * a.nxm; a.g.surf2 = gp + 3 * a.nxm;
-  a.g.vol = gp + 4 * a.nxm; a.g.rc = gp + 5 * a.nxm; a.g.pos1 = gp + 6 * a.nxm;
+  a.g.vol = gp + 4 * a.nxm; a.g.rc = gp + 5 * a.nxm; a.g.pos1 = gp + 6 * a.nxm; a.g.pos2 = gp + 7 * a.nxm;
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     rc = fail(SHARK_EHIP, "stream sync failed");
     shark_gpu_destroy(c);
@@ -219,7 +228,7 @@ extern "C" int shark_gpu_set_geometry(shark_ctx* c, const double* dx, const doub
   if (!dx || !surf || !vol || !radii_c) return fail(SHARK_EINVAL, "null geometry array");
   const StepArgs& a = c->a;
   const int nxm = a.nxm, nym = a.nym, nc = a.nc;
-  std::vector<double> col((size_t)7 * nxm, 0.0);
+  std::vector<double> col((size_t)7 * nxm + nym, 0.0);
   auto at = [&](const double* arr, int plane, int ix, int iy) {
     return arr[(size_t)plane * nc + (ix - 1) + (size_t)(iy - 1) * nxm];
   };
@@ -237,12 +246,21 @@ extern "C" int shark_gpu_set_geometry(shark_ctx* c, const double* dx, const doub
       bool ok = dx[ci] == col[ix - 1] && surf[ci] == col[2 * nxm + ix - 1] && vol[ci] == col[4 * nxm + ix - 1] &&
                 radii_c[ci] == col[5 * nxm + ix - 1];
       if (a.ndim == 2) ok = ok && dx[nc + ci] == col[nxm + ix - 1] && surf[nc + ci] == col[3 * nxm + ix - 1];
-      // position(:,1) is read only by the shearing-box force and in-loop hooks
-      const bool need_pos = c->cfg.force_kind == SHARK_FORCE_SI || c->cfg.inloop_kind == SHARK_INLOOP_SI;
+      // position is read only by the shearing-box and cartesian-disk hooks
+      const bool cdisk = c->cfg.force_kind == SHARK_FORCE_CDISK || c->cfg.inloop_kind == SHARK_INLOOP_CDISK ||
+                         c->cfg.tstop_kind == SHARK_TSTOP_EPSTEIN_CDISK || c->cfg.viscosity;
+      const bool need_pos = c->cfg.force_kind == SHARK_FORCE_SI || c->cfg.inloop_kind == SHARK_INLOOP_SI || cdisk;
       if (position && need_pos) ok = ok && position[ci] == col[6 * nxm + ix - 1];
+      if (position && cdisk && a.ndim == 2) ok = ok && position[nc + ci] == position[nc + (size_t)(iy - 1) * nxm];
       if (!ok) return fail(SHARK_EUNSUP, "geometry varies along y at column %d: only separable grids are built", ix);
     }
   }
+  if (position && a.ndim == 2)
+    for (int iy = 1; iy <= nym; ++iy) col[7 * (size_t)nxm + iy - 1] = position[nc + (size_t)(iy - 1) * nxm];
+  const bool need_pos = c->cfg.force_kind == SHARK_FORCE_SI || c->cfg.inloop_kind == SHARK_INLOOP_SI ||
+                        c->cfg.force_kind == SHARK_FORCE_CDISK || c->cfg.inloop_kind == SHARK_INLOOP_CDISK ||
+                        c->cfg.tstop_kind == SHARK_TSTOP_EPSTEIN_CDISK || c->cfg.viscosity;
+  if (!position && need_pos) return fail(SHARK_EINVAL, "position is required by this setup's hooks");
   HIPCHK(hipMemcpyAsync(c->dgeom, col.data(), col.size() * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->hrc.assign(radii_c, radii_c + nc);

@@ -15,6 +15,9 @@ struct DevScalars {
   long long nstep;      // completed steps
   int active;           // 0 once time >= tend: every kernel returns immediately
   int pad;
+  // cartesian disk planet (disk/setup.f90:292-301), Mplanet x_p / r_p^3 and Mplanet y_p / r_p^3
+  // at the step's time [0..1] and at time + dt [2..3] (the next step's Courant candidate)
+  double planet[4];
 };
 
 // Geometry of a separable grid (every reference grid is: gridinit builds dx, surf, vol and
@@ -27,6 +30,7 @@ struct ColGeom {
   const double* vol;
   const double* rc;     // radii_c
   const double* pos1;   // position(:,1)
+  const double* pos2;   // position(:,2) per row iy (cartesian 2D: separable too)
 };
 
 struct StepArgs {
@@ -62,6 +66,11 @@ struct StepArgs {
   double smallp;
   double eps_k1;        // sqrt(pi gamma / 8) * rhograin: the Epstein prefactor (disk tstop)
   double s2pi;          // sqrt(2 pi)
+  // cartesian disk (src/setups/disk): second box length, planet, initial disk, buffer zones,
+  // alpha viscosity (eta_visc = alpha_visc H cs), dust-to-gas ratio of the buffers
+  double box_l_y, mplanet, rplanet, sigma_r0, r0_disk, frac_r_disk, buffer_in, buffer_out;
+  double alpha_visc, dust2gas;
+  int viscosity;
 };
 
 // launch shapes: every shape gives the same results bit for bit, only the speed differs

@@ -1,12 +1,14 @@
 // step_kernels.hip — the per-step kernels of SHARK's Godunov step on gfx950.
 //
 // One step (src/solver/solve.f90:10-181) runs as four launches on one stream:
-//   k_bc       ghost fill of the state (apply_boundaries)
+//   k_bc       ghost fill of the state (apply_boundaries); k_bc_zg for the zero-gradient
+//              boundaries, whose source cells are themselves ghosts
 //   k_hydro    per fluid: ctoprim + force + kick + predictor + Riemann fluxes + update +
-//              polar sources, fused; a wavefront owns 60 columns x a strip of rows and
-//              marches the strip row by row with a 3-row register window; x neighbours are
-//              exchanged with cross-lane shuffles.  qm/qp/flux/dq never reach HBM.
-//   k_post     per cell, all fluids in registers: FARGO remap, implicit drag + floor,
+//              sources (polar, drag heating, viscosity), fused; a wavefront owns 60 columns x
+//              a strip of rows and marches the strip row by row with a 3-row register window;
+//              x neighbours are exchanged with cross-lane shuffles.  qm/qp/flux/dq never reach
+//              HBM.
+//   k_post2    per row segment, all fluids staged in LDS: FARGO remap, implicit drag + floor,
 //              in-loop hook, and the next step's Courant candidate (dt fused into the
 //              epilogue of the previous step; the min over cells is an integer atomic min
 //              on the ordered bit pattern, exact and order independent)
@@ -30,9 +32,27 @@ DEV bool is_active(const StepArgs& a, int ix, int iy) {
 
 // External accelerations of update_force_setup for an active cell (SI: setup.f90:346-353,
 // disk_polar: setup.f90:366-377).  qa/qb: gas (vx, vz) or dust (vdx, vdz) velocities.
-DEV void cell_force(const StepArgs& a, int ix, double qx, double qz, bool gas, double& f1,
-                    double& f2, double& f3) {
+// next: 0 for the step being taken, 1 for the next step (its Courant candidate), which selects
+// the planet's position in time.  CD: the cartesian-disk force is compiled in (the hydro
+// kernels of other grids leave it out, so it costs them no registers)
+template <bool CD = true>
+DEV void cell_force(const StepArgs& a, int ix, int iy, int next, double qx, double qz, bool gas,
+                    double& f1, double& f2, double& f3) {
   f1 = 0.0; f2 = 0.0; f3 = 0.0;
+  if (CD && a.force_kind == 3) {
+    // cartesian disk (disk/setup.f90:274-301): softened star, plus a planet on a circular orbit
+    // (its terms per step from k_planet); rr**3 with the integer literal 3 is rr*(rr*rr)
+    const double xx = a.g.pos1[ix - 1] - kHalf * a.box_l, yy = a.g.pos2[iy - 1] - kHalf * a.box_l_y;
+    const double rr = sqrt((xx * xx + yy * yy) + a.smooth_r * a.smooth_r);
+    const double r3 = rr * (rr * rr);
+    f1 = -(a.mstar * xx / r3);
+    f2 = -(a.mstar * yy / r3);
+    if (a.mplanet > 0.0) {
+      f1 = f1 - a.sc->planet[2 * next];
+      f2 = f2 - a.sc->planet[2 * next + 1];
+    }
+    return;
+  }
   if (a.force_kind == 1) {
     const double Om = a.omega_shear;
     double t = ((2.0 * a.q_shear) * (Om * Om)) * (a.g.pos1[ix - 1] - kHalf * a.box_l);
@@ -142,11 +162,12 @@ struct FaceSet {
 DEV double shfl_up1(double x) { return __shfl_up(x, 1); }
 DEV double shfl_dn1(double x) { return __shfl_down(x, 1); }
 
-template <int ND>
+template <int ND, bool CD = true>
 DEV double tstop_of(const StepArgs& a, int k, int cell, int ix, double rho_q, double rhod_q, double cs_q);
 
-// SLOPE: limiter (1..3); SOLV: compile-time Riemann solver (gas 0..2, dust 0..3) or -1 to
-// read it from the arguments.  ADI: adiabatic gas (P from the total energy, its own slopes,
+// GEOM: 0 cartesian, 2 polar, 3 cartesian with the cartesian disk's point-mass force and
+// viscosity compiled in.  SLOPE: limiter (1..3); SOLV: compile-time Riemann solver (gas 0..2,
+// dust 0..3) or -1 to read it from the arguments.  ADI: adiabatic gas (P from the total energy, its own slopes,
 // Hancock source and face states, sound speeds from the face states; iso_cs < 1).
 template <int NDIM, int GEOM, bool GAS, int SLOPE, int SOLV, bool ADI>
 __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int lane) {
@@ -215,7 +236,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
                    FaceSet& fy) {
     double f1 = 0.0, f2 = 0.0, f3 = 0.0;
     if (a.force_kick && col_active && (NDIM == 1 || (iy >= 3 && iy <= a.ny + 2)))
-      cell_force(a, ix, r.q[1], r.q[3], GAS, f1, f2, f3);
+      cell_force<GEOM == 3>(a, ix, iy, 0, r.q[1], r.q[3], GAS, f1, f2, f3);
     const double r_rho = r.q[0];
     double u = r.q[1], v = r.q[2], w = r.q[3];
     const double drx = dqx[0], dux = dqx[1], dvx = dqx[2], dwx = dqx[3];
@@ -325,6 +346,10 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
   double cs_prev = 0.0;
   double Fx_own_prev[5] = {0, 0, 0, 0, 0}, Fx_rt_prev[5] = {0, 0, 0, 0, 0};
   double Fy_bot[5] = {0, 0, 0, 0, 0};
+  // viscosity (cartesian disk): velocities of the row below the one being finalized
+  constexpr bool VISC = GAS && GEOM == 3 && NDIM == 2 && !ADI;
+  const bool visc = VISC && a.viscosity;
+  double vbx = 0.0, vby = 0.0;
 
   const int jstart = (NDIM == 2) ? y0 - 2 : 1;
   const int jend = (NDIM == 2) ? y1 + 2 : 1;
@@ -335,6 +360,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
   for (int i = 0; i < PF; ++i)
     if (jstart + i <= jend) fetch_row(jstart + i, pend[i]);
   for (int j = jstart; j <= jend; ++j) {
+    if (visc && j >= jstart + 3) { vbx = R0.q[1]; vby = R0.q[2]; }   // row j-3
     R0 = R1;
     R1 = R2;
     R2 = pend[0];
@@ -407,6 +433,12 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
     const RowF& Rf = (NDIM == 2) ? R0 : R2;
     const double* Fxo = (NDIM == 2) ? Fx_own_prev : Fx_own;
     const double* Fxr = (NDIM == 2) ? Fx_rt_prev : Fx_rt;
+    // x neighbours of the row's velocities for the viscous Laplacian (all lanes shuffle)
+    double vlx = 0.0, vrx = 0.0, vly = 0.0, vry = 0.0;
+    if (visc) {
+      vlx = shfl_up1(Rf.q[1]); vrx = shfl_dn1(Rf.q[1]);
+      vly = shfl_up1(Rf.q[2]); vry = shfl_dn1(Rf.q[2]);
+    }
     if (own_col && ((NDIM == 1) || (have_faces && fr >= y0 && fr <= y1))) {
       const int cell = (NDIM == 1) ? ixc - 1 : (ixc - 1) + (fr - 1) * a.nxm;
       double uk[5];
@@ -414,7 +446,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
       for (int c = 0; c < NU; ++c) uk[c] = Rf.u[c];
       if (a.force_kick) {  // kick(1.0d0), force_kicks.f90:12-39
         double f1, f2, f3;
-        cell_force(a, ix, Rf.q[1], Rf.q[3], GAS, f1, f2, f3);
+        cell_force<GEOM == 3>(a, ix, fr, 0, Rf.q[1], Rf.q[3], GAS, f1, f2, f3);
         const double rho = Rf.u[0];
         if (GAS) {
           const double eold = ((kHalf * (uk[1] * uk[1])) / rho + (kHalf * (uk[2] * uk[2])) / rho) +
@@ -461,6 +493,24 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
                  (dt * (vz * (vz - vdz))) / ts;
           }
           S[4] = s4;
+        }
+      }
+      if constexpr (VISC) {
+        // viscosity (source_terms.f90:99-110, GEOM 0) from the start-of-step primitives, with
+        // eta_visc = alpha_visc H cs (disk/setup.f90:60); rows fr-1 and fr+1 are vb* and R1
+        if (visc) {
+          const double xx = a.g.pos1[ixc - 1] - kHalf * a.box_l, yy = a.g.pos2[fr - 1] - kHalf * a.box_l_y;
+          const double rr = sqrt((xx * xx + yy * yy) + a.smooth_r * a.smooth_r);
+          const double eta = (a.alpha_visc * (a.hoverr * rr)) * Rf.cs;
+          const double vx = Rf.q[1], vy = Rf.q[2];
+          const double d1 = dx1 * dx1, d2 = dx2 * dx2;
+          const double lxu = ((vrx + vlx) - 2.0 * vx) / d1;
+          const double lyu = ((R1.q[1] + vbx) - 2.0 * vx) / d2;
+          const double lxv = ((vry + vly) - 2.0 * vy) / d1;
+          const double lyv = ((R1.q[2] + vby) - 2.0 * vy) / d2;
+          const double k = Rf.q[0] * dt * eta;
+          S[1] = (0.0 + k * lxu) + k * lyu;
+          S[2] = 0.0 + k * (lxv + lyv);
         }
       }
 #pragma unroll
@@ -524,8 +574,9 @@ DEV double start_cs(const StepArgs& a, int cell) {
                  a.U[3 * nc + cell] / rho, cell);
 }
 
+// next: 1 when the candidate is for the step after the one being taken (cell_force)
 template <int ND>
-DEV double courant_cell(const StepArgs& a, const double* un, int ix, int cell) {
+DEV double courant_cell(const StepArgs& a, const double* un, int ix, int iy, int next, int cell) {
   double dxx;
   if (a.ndim == 1) dxx = a.g.dx1[ix - 1];
   else if (a.geom == 2) dxx = fminf_(a.g.dx1[ix - 1], a.g.rc[ix - 1] * a.g.dx2[ix - 1]);
@@ -548,11 +599,11 @@ DEV double courant_cell(const StepArgs& a, const double* un, int ix, int cell) {
   double dt = (a.cfl * dxx) / fabs(vmax);
   if (a.force_kick) {
     double f1, f2, f3;
-    cell_force(a, ix, vx, vz, true, f1, f2, f3);
+    cell_force(a, ix, iy, next, vx, vz, true, f1, f2, f3);
     double fm = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
 #pragma unroll
     for (int k = 0; k < ND; ++k) {
-      cell_force(a, ix, vdx[k], vdz[k], false, f1, f2, f3);
+      cell_force(a, ix, iy, next, vdx[k], vdz[k], false, f1, f2, f3);
       fm = fmaxf_(fm, sqrt((f1 * f1 + f2 * f2) + f3 * f3));
     }
     if (vv != 0.0) {
@@ -586,157 +637,30 @@ DEV void block_atomic_min(DevScalars* sc, double d) {
 }
 
 // Stopping time of species k from the start-of-step primitives; cs_q is that state's sound speed
-// (read only by the uniform Epstein law, KH setup.f90:238 / SOD setup.f90:182)
-template <int ND>
+// (read only by the uniform Epstein law, KH setup.f90:238 / SOD setup.f90:182).  CD: the
+// cartesian disk's law is compiled in (left out of the FARGO kernels, which never need it)
+template <int ND, bool CD>
 DEV double tstop_of(const StepArgs& a, int k, int cell, int ix, double rho_q, double rhod_q, double cs_q) {
   if (a.tstop_kind == 0) return ((rhod_q * rho_q) / a.K[k]) / rho_q;  // DUSTYWAVE setup.f90:217
   const double s = a.sdust_uniform ? a.sdust_k[k] : a.sdust[(size_t)k * a.nc + cell];
   if (a.tstop_kind == 1)  // SI setup.f90:392
     return ((a.rhograin * s) / a.rho_init) / ((a.rad0 * a.omega_shear) * a.hoverr);
   if (a.tstop_kind == 3) return ((sqrt((kPi * a.gamma) / 8.0) * a.rhograin) * s) / (rho_q * cs_q);
+  if (CD && a.tstop_kind == 4) {   // cartesian disk, disk/setup.f90:333-341
+    const int iy = cell / a.nxm + 1;
+    const double xx = a.g.pos1[ix - 1] - kHalf * a.box_l, yy = a.g.pos2[iy - 1] - kHalf * a.box_l_y;
+    const double H = a.hoverr * sqrt((xx * xx + yy * yy) + a.smooth_r * a.smooth_r);
+    return ((sqrt((kPi * a.gamma) / 8.0) * a.rhograin) * s) / ((rho_q / sqrt(2.0 * kPi * H)) * a.cs[cell]);
+  }
   const double H = a.hoverr * a.g.rc[ix - 1];  // disk_polar setup.f90:436
   return (sqrt((kPi * a.gamma) / 8.0) * a.rhograin) * s /
          ((rho_q / (sqrt(2.0 * kPi) * H)) * a.cs[cell]);
 }
 
-template <int ND, bool FARGO, int NDIM>
-__global__ __launch_bounds__(256) void k_post(StepArgs a) {
-  if (!a.sc->active) return;
-  constexpr int NV = 5 + 4 * ND;
-  int ix, iy;
-  if (NDIM == 2) {
-    ix = 3 + blockIdx.x * 64 + (threadIdx.x & 63);
-    iy = 3 + blockIdx.y * 4 + (threadIdx.x >> 6);
-  } else {
-    ix = 3 + blockIdx.x * 256 + threadIdx.x;
-    iy = 1;
-  }
-  const bool valid = ix <= a.nx + 2 && (NDIM == 1 || iy <= a.ny + 2);
-  double dtc = 2e44;
-  if (valid) {
-    const size_t nc = a.nc;
-    const int cell = cell_of(a, ix, iy);
-    const double dt = a.sc->dt;
-    double un[NV];
-    if (FARGO) {
-      // fargo_scheme.f90:44-74 (one = 1.0d0, FP64 eps, periodic slopes; rows wrap mod ny)
-      const double rp = (a.geom == 2) ? a.g.rc[ix - 1] : 1.0;
-      const double vk = a.fargo_vk[ix - 1];
-      const double dy = a.g.dx2[ix - 1] * rp;
-      const double sh = fabs(vk * dt);
-      const int nshft = (int)(sh / dy);
-      const double eps = fmod(sh, dy) / dy;
-      const int ny = a.ny;
-      int j0 = (vk > 0.0) ? iy - nshft - 1 : iy + nshft;
-      j0 = ((j0 - 3) % ny + ny) % ny + 3;
-      const int j1 = (j0 - 3 + 1) % ny + 3;
-      const int jm = (j0 - 3 + ny - 1) % ny + 3;
-      const int jp = (j1 - 3 + 1) % ny + 3;
-      const int cm = cell_of(a, ix, jm), c0 = cell_of(a, ix, j0), c1 = cell_of(a, ix, j1),
-                cp = cell_of(a, ix, jp);
-      const double h2 = (kHalf - eps) * (kHalf - eps);
-      const double w0 = (vk > 0.0) ? eps : (1.0 - eps), w1 = (vk > 0.0) ? (1.0 - eps) : eps;
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        const double* Bv = a.B + v * nc;
-        const double um = Bv[cm], u0 = Bv[c0], u1 = Bv[c1], up = Bv[cp];
-        const double d0 = slope_limit(a.slope, u0 - um, u1 - u0);
-        const double d1 = slope_limit(a.slope, u1 - u0, up - u1);
-        double x = u0 * w0 + u1 * w1;
-        x = x + kHalf * d0 * (0.25 - h2) + kHalf * d1 * (h2 - 0.25);
-        un[v] = x;
-      }
-    } else {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) un[v] = a.B[v * nc + cell];
-    }
-    if (ND > 0 && a.drag) {
-      // dust_drag(1.0d0), dust_drag.f90:21-73, with the stopping times of the step
-      // (compute_tstop from the start-of-step primitives)
-      const double rho_q = fmaxf_(a.U[cell], a.smallr);
-      const double cs_q = a.tstop_kind == 3 ? start_cs(a, cell) : 0.0;
-      double ak[ND > 0 ? ND : 1];
-      double rhon = un[0], pnx = un[1], pny = un[2], pnz = un[3];
-#pragma unroll
-      for (int k = 0; k < ND; ++k) {
-        const double ts = tstop_of<ND>(a, k, cell, ix, rho_q, a.U[(5 + k) * nc + cell], cs_q);
-        ak[k] = (1.0 * dt) / ts;
-        const double f = ak[k] / (1.0 + ak[k]);
-        pnx = pnx + f * un[5 + ND + k];
-        pny = pny + f * un[5 + 2 * ND + k];
-        pnz = pnz + f * un[5 + 3 * ND + k];
-        rhon = rhon + f * un[5 + k];
-      }
-#pragma unroll
-      for (int k = 0; k < ND; ++k) {
-        const double f = ak[k] / (1.0 + ak[k]);
-        const double rd = un[5 + k];
-        un[5 + ND + k] = un[5 + ND + k] / (1.0 + ak[k]) + f * pnx / rhon * rd;
-        un[5 + 2 * ND + k] = un[5 + 2 * ND + k] / (1.0 + ak[k]) + f * pny / rhon * rd;
-        un[5 + 3 * ND + k] = un[5 + 3 * ND + k] / (1.0 + ak[k]) + f * pnz / rhon * rd;
-      }
-      if (a.back_reaction) {
-        un[1] = pnx / rhon * un[0];
-        un[2] = pny / rhon * un[0];
-        un[3] = pnz / rhon * un[0];
-      }
-#pragma unroll
-      for (int k = 0; k < ND; ++k) un[5 + k] = fmaxf_(un[0] * a.dust_ratio_min, un[5 + k]);
-    }
-    if (a.inloop_kind == 2) {
-      // disk_polar setup_inloop, setup.f90:226-281
-      const double r = a.g.rc[ix - 1];
-      if (r < a.r_relax) {
-        const double t_rel = a.n_rel * 2.0 * kPi / (sqrt(a.mstar / (r * r * r))) -
-                             a.n_rel * 2.0 * kPi / (sqrt(a.mstar / (a.smooth_r * a.smooth_r * a.smooth_r)));
-        const double e = exp(-dt / t_rel);
-        const double* ci = a.condinit;
-#pragma unroll
-        for (int f = 0; f <= ND; ++f) {
-          const int ir = f == 0 ? 0 : 5 + f - 1;
-          const int iu = f == 0 ? 1 : 5 + ND + f - 1;
-          const int iv = f == 0 ? 2 : 5 + 2 * ND + f - 1;
-          const double rho_i = ci[ir * nc + cell], vx_i = ci[iu * nc + cell] / rho_i,
-                       vy_i = ci[iv * nc + cell] / rho_i;
-          const double rho_o = un[ir], vx_o = un[iu] / rho_o, vy_o = un[iv] / rho_o;
-          const double rho_n = (rho_i) * (1.0 - e) + rho_o * e;
-          const double vx_n = (vx_i) * (1.0 - e) + vx_o * e;
-          const double vy_n = (vy_i) * (1.0 - e) + vy_o * e;
-          un[ir] = rho_n; un[iu] = rho_n * vx_n; un[iv] = rho_n * vy_n;
-        }
-      }
-      if (ix == 3) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) un[v] = a.condinit[v * nc + cell];
-      }
-    } else if (a.inloop_kind == 1 && !a.force_kick) {
-      // SI Crank-Nicolson rotation, SI/setup.f90:309-331
-      const double Om = a.omega_shear, O = Om * dt;
-      const double x0 = a.g.pos1[ix - 1] - kHalf * a.box_l;
-      double AA = (2.0 * a.q_shear) * Om * x0 - (2.0 * a.rad0) * a.eta_stream;
-      double u = un[3] / un[0], v = un[1] / un[0], d = un[0];
-      un[3] = d * (u * (1. - O * O) + 2. * v * O + AA * (O * O)) / (1. + O * O);
-      un[1] = d * (v * (1. - O * O) - 2. * u * O + AA * O) / (1. + O * O);
-      AA = (2.0 * a.q_shear) * Om * x0;
-#pragma unroll
-      for (int k = 0; k < ND; ++k) {
-        const int ir = 5 + k, ix_ = 5 + ND + k, iz = 5 + 3 * ND + k;
-        u = un[iz] / un[ir]; v = un[ix_] / un[ir]; d = un[ir];
-        un[iz] = d * (u * (1. - O * O) + 2. * v * O + AA * (O * O)) / (1. + O * O);
-        un[ix_] = d * (v * (1. - O * O) - 2. * u * O + AA * O) / (1. + O * O);
-      }
-    }
-#pragma unroll
-    for (int v = 0; v < NV; ++v) a.U[v * nc + cell] = un[v];
-    dtc = courant_cell<ND>(a, un, ix, cell);
-  }
-  block_atomic_min(a.sc, dtc);
-}
-
 // k_post2: the post pass with all fluids of a row segment of TW cells staged in LDS and the
 // species spread over the NS = 256/TW thread slots of the block.  The drag sums run in
-// species order in one thread per (cell, component), so results equal k_post / the
-// reference bit for bit; loads, divisions and the Courant work are shared by all slots.
+// species order in one thread per (cell, component), so results equal the reference bit for
+// bit; loads, divisions and the Courant work are shared by all slots.
 // PART 0: the whole post pass.  With a growth step between drag and the in-loop hook
 // (solve.f90:110-152) the pass is split: PART 1 remaps and drags (writes U, no Courant candidate),
 // PART 2 reloads U after growth, applies the in-loop hook and computes the Courant candidate.
@@ -854,7 +778,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
       const double cs_q = a.tstop_kind == 3 ? start_cs(a, cell) : 0.0;
       for (int k = s; k < ND; k += NS) {
         const double rq = (a.tstop_kind == 0) ? a.U[(5 + k) * nc + cell] : 0.0;
-        const double ak = (1.0 * dt) / tstop_of<ND>(a, k, cell, ixs, rho_q, rq, cs_q);
+        const double ak = (1.0 * dt) / tstop_of<ND, !FARGO>(a, k, cell, ixs, rho_q, rq, cs_q);
         akS[k][c] = ak;
         gS[k][c] = 1.0 / (1.0 + ak);
       }
@@ -921,6 +845,35 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
     if (ix == 3)
       for (int v = s; v < NV; v += NS) un[v][c] = a.condinit[v * nc + cell];
     __syncthreads();
+  } else if (!FARGO && a.inloop_kind == 3) {
+    // cartesian disk buffers (disk/setup.f90:203-262): reset to the initial disk (rate = 0)
+    const double xx = a.g.pos1[ixs - 1] - kHalf * a.box_l, yy = a.g.pos2[iy - 1] - kHalf * a.box_l_y;
+    const double rr = sqrt((xx * xx + yy * yy) + a.smooth_r * a.smooth_r);
+    if (rr < a.buffer_in * a.box_l || rr > a.buffer_out * a.box_l) {
+      const double r_cyl = sqrt(xx * xx + yy * yy);
+      const double x_soft = xx * rr / r_cyl, y_soft = yy * rr / r_cyl;
+      const double Omega = sqrt(a.mstar / (rr * (rr * rr)));
+      const double cs0 = (a.hoverr * rr) * Omega;
+      const double R_out_disk = a.frac_r_disk * a.box_l, rate = 0.0;
+      double rho_ana = a.sigma_r0 * (a.r0_disk / rr);
+      if (rr > R_out_disk) rho_ana = a.sigma_r0 * (a.r0_disk / R_out_disk) / 100.0;
+      const double v_ana = -(Omega * rho_ana);
+      for (int f = s; f <= ND; f += NS) {
+        if (f == 0) {
+          un[0][c] = rho_ana + (un[0][c] - rho_ana) * rate;
+          un[1][c] = -(y_soft * v_ana) + (un[1][c] + yy * v_ana) * rate;
+          un[2][c] = x_soft * v_ana + (un[2][c] - xx * v_ana) * rate;
+          un[4][c] = (rho_ana + (un[0][c] - rho_ana) * rate) * (cs0 * cs0);
+        } else {
+          const int k = f - 1;
+          const double dust_to = a.dust2gas;
+          un[5 + k][c] = dust_to * rho_ana + (un[5 + k][c] - rho_ana * dust_to) * rate;
+          un[5 + ND + k][c] = -(yy * v_ana * dust_to) + (un[5 + ND + k][c] + dust_to * yy * v_ana) * rate;
+          un[5 + 2 * ND + k][c] = xx * v_ana * dust_to + (un[5 + 2 * ND + k][c] - dust_to * xx * v_ana) * rate;
+        }
+      }
+    }
+    __syncthreads();
   } else if (a.inloop_kind == 1 && !a.force_kick) {
     const double Om = a.omega_shear, O = Om * dt;
     const double x0 = a.g.pos1[ixs - 1] - kHalf * a.box_l;
@@ -957,7 +910,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
       vmax = cs_cell(a, rho, un[4][c], vx, vy, vz, cell) + vv;
       if (ND == 0) vvlast[c] = vv;
       if (a.force_kick) {
-        cell_force(a, ixs, vx, vz, true, f1, f2, f3);
+        cell_force<!FARGO>(a, ixs, iy, 1, vx, vz, true, f1, f2, f3);
         fm = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
       }
       any = true;
@@ -965,7 +918,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
     // the disk's dust acceleration does not depend on the species' velocity: one evaluation
     double fk_disk = 0.0;
     if (a.force_kick && a.force_kind == 2 && s < ND) {
-      cell_force(a, ixs, 0.0, 0.0, false, f1, f2, f3);
+      cell_force<!FARGO>(a, ixs, iy, 1, 0.0, 0.0, false, f1, f2, f3);
       fk_disk = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
     }
     for (int k = s; k < ND; k += NS) {
@@ -977,7 +930,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
       if (a.force_kick) {
         double fk = fk_disk;
         if (a.force_kind != 2) {
-          cell_force(a, ixs, vdx, vdz, false, f1, f2, f3);
+          cell_force<!FARGO>(a, ixs, iy, 1, vdx, vdz, false, f1, f2, f3);
           fk = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
         }
         fm = any ? fmaxf_(fm, fk) : fk;
@@ -1027,7 +980,7 @@ __global__ __launch_bounds__(256) void k_courant(StepArgs a) {
     double un[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) un[v] = a.U[(size_t)v * a.nc + cell];
-    dtc = courant_cell<ND>(a, un, ix, cell);
+    dtc = courant_cell<ND>(a, un, ix, iy, 0, cell);
   }
   block_atomic_min(a.sc, dtc);
 }
@@ -1079,7 +1032,85 @@ __global__ void k_ctoprim(StepArgs a, double* q) {
 }
 
 // ------------------------------------------------------------------ launchers
+// zero_gradients (src/grid/boundary_types.f90:48-100).  In 2D its source cells are
+// first_active_y / last_active_y and first_active / last_active taken as flat cell indices
+// (:70-71, :88-89): cells 3, ny+2 and nx+2 of the first ghost row, themselves written by the
+// loops.  In the reference's loop order every ghost cell receives cell 3's value (cell 3 copies
+// onto itself, cell nx+2 receives it in the first loop before the second loop reads it), except
+// the upper ghost rows of the first loop, which receive cell ny+2's value as it was before the
+// first loop reached it and cell 3's value after.  One block stages both source values in LDS
+// before any ghost cell is written.
+__global__ __launch_bounds__(1024) void k_bc_zg(StepArgs a) {
+  if (!a.sc->active) return;
+  const size_t nc = a.nc;
+  const int nv = a.nvar;
+  if (a.ndim == 1) {
+    for (int v = threadIdx.x; v < nv; v += blockDim.x) {
+      const double l = a.U[v * nc + 2], r = a.U[v * nc + a.nx + 1];
+      a.U[v * nc + 0] = l;
+      a.U[v * nc + 1] = l;
+      a.U[v * nc + a.nxm - 1] = r;
+      a.U[v * nc + a.nxm - 2] = r;
+    }
+    return;
+  }
+  __shared__ double s3[5 + 4 * 64], fo[5 + 4 * 64];
+  const int f = a.ny + 1;   // flat last_active_y, 0-based
+  for (int v = threadIdx.x; v < nv; v += blockDim.x) {
+    s3[v] = a.U[v * nc + 2];
+    fo[v] = a.U[v * nc + f];
+  }
+  __syncthreads();
+  const int fx = f % a.nxm + 1, fy = f / a.nxm + 1;
+  const bool f_written = fy <= 2 && fx >= 3 && fx <= a.nx + 2;   // a target of the first loop
+  const int nA = 2 * a.nx, nB = 2 * a.nym;
+  for (int t = threadIdx.x; t < nA + nB; t += blockDim.x) {
+    if (t < nA) {
+      const int ix = 3 + t / 2, iy = 1 + t % 2;
+      const int bl = cell_of(a, ix, iy), br = cell_of(a, ix, a.nym + 1 - iy);
+      const bool late = f_written && (ix > fx || (ix == fx && iy >= fy));
+      for (int v = 0; v < nv; ++v) {
+        a.U[v * nc + bl] = s3[v];
+        a.U[v * nc + br] = late ? s3[v] : fo[v];
+      }
+    } else {
+      const int u = t - nA, ix = 1 + u / a.nym, iy = 1 + u % a.nym;
+      const int bl = cell_of(a, ix, iy), br = cell_of(a, a.nxm + 1 - ix, iy);
+      for (int v = 0; v < nv; ++v) {
+        a.U[v * nc + bl] = s3[v];
+        a.U[v * nc + br] = s3[v];
+      }
+    }
+  }
+}
+
+// The planet of the cartesian disk at the step's time and at time + dt (the time loop's
+// time = time + dt, shark.f90:107), once per step: theta_p = 2 pi mod(t / T_p, 1)
+// (disk/setup.f90:292-301).
+__global__ void k_planet(StepArgs a) {
+  DevScalars* sc = a.sc;
+  const double period = 2.0 * kPi / sqrt(a.mstar / (a.rplanet * (a.rplanet * a.rplanet)));
+  for (int w = 0; w < 2; ++w) {
+    const double t = w == 0 ? sc->time : sc->time + sc->dt;
+    const double theta_p = 2.0 * kPi * fmod(t / period, 1.0);
+    const double xx_p = cos(theta_p) * a.rplanet, yy_p = sin(theta_p) * a.rplanet;
+    const double rr_p = sqrt((xx_p * xx_p + yy_p * yy_p) + a.smooth_r * a.smooth_r);
+    const double rp3 = rr_p * (rr_p * rr_p);
+    sc->planet[2 * w] = a.mplanet * xx_p / rp3;
+    sc->planet[2 * w + 1] = a.mplanet * yy_p / rp3;
+  }
+}
+
+static void launch_planet(const StepArgs& a, hipStream_t s) {
+  if (a.force_kind == 3 && a.mplanet > 0.0) hipLaunchKernelGGL(k_planet, dim3(1), dim3(1), 0, s, a);
+}
+
 void launch_bc(const StepArgs& a, hipStream_t s) {
+  launch_planet(a, s);
+  if (a.bc_kind == 4) {
+    hipLaunchKernelGGL(k_bc_zg, dim3(1), dim3(1024), 0, s, a);
+    return;
+  }
   const int n = (a.ndim == 2 ? 4 * a.nx : 0) + 4 * a.nym;
   hipLaunchKernelGGL(k_bc, dim3((n + 255) / 256), dim3(256), 0, s, a);
 }
@@ -1089,7 +1120,7 @@ static void hydro_launch1(const StepArgs& a, hipStream_t s) {
   const long items = (long)a.ngroups * (GAS ? a.nstrips_g : a.nstrips) * (GAS ? 1 : a.nd);
   const dim3 grid((unsigned)((items + 3) / 4));
   const bool fast = GAS ? a.solver == 1 : a.solverdust == 0;   // HLL gas / Huang-Bai dust
-  if constexpr (GAS) {
+  if constexpr (GAS && GEOM != 3) {
     if (!a.iso) {   // adiabatic gas
       if (fast) hipLaunchKernelGGL((k_hydro<NDIM, GEOM, true, SLOPE, 1, true>), grid, dim3(256), 0, s, a);
       else hipLaunchKernelGGL((k_hydro<NDIM, GEOM, true, SLOPE, -1, true>), grid, dim3(256), 0, s, a);
@@ -1113,6 +1144,7 @@ template <bool GAS>
 static void hydro_launch_geom(const StepArgs& a, hipStream_t s) {
   if (a.ndim == 1) hydro_launch_slope<1, 0, GAS>(a, s);
   else if (a.geom == 2) hydro_launch_slope<2, 2, GAS>(a, s);
+  else if (a.force_kind == 3 || a.viscosity) hydro_launch_slope<2, 3, GAS>(a, s);   // cartesian disk
   else hydro_launch_slope<2, 0, GAS>(a, s);
 }
 
@@ -1196,6 +1228,7 @@ void launch_post_part(const StepArgs& a, int part, hipStream_t s) {
   else { SHARK_ND_SWITCH(post2_nd) }
 }
 void launch_courant_init(const StepArgs& a, hipStream_t s) {
+  launch_planet(a, s);
   SHARK_ND_SWITCH(courant_nd)
   hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, s, a.sc);
 }

@@ -15,10 +15,11 @@ module shark_gpu_mod
   integer(c_int32_t), parameter :: SHARK_SOLVER_LLF = 0, SHARK_SOLVER_HLL = 1, SHARK_SOLVER_HLLC = 2
   integer(c_int32_t), parameter :: SHARK_DUST_HUANG_BAI = 0
   integer(c_int32_t), parameter :: SHARK_BC_PERIODIC = 0, SHARK_BC_SHEAR_SI = 1, SHARK_BC_DISK = 2
-  integer(c_int32_t), parameter :: SHARK_BC_SOD = 3
+  integer(c_int32_t), parameter :: SHARK_BC_SOD = 3, SHARK_BC_ZERO_GRADIENT = 4
   integer(c_int32_t), parameter :: SHARK_TSTOP_DUSTYWAVE = 0, SHARK_TSTOP_SI = 1, SHARK_TSTOP_EPSTEIN_DISK = 2, &
        SHARK_TSTOP_EPSTEIN = 3
   integer(c_int32_t), parameter :: SHARK_FORCE_NONE = 0, SHARK_INLOOP_NONE = 0
+  integer(c_int32_t), parameter :: SHARK_TSTOP_EPSTEIN_CDISK = 4, SHARK_FORCE_CDISK = 3, SHARK_INLOOP_CDISK = 3
 
   ! same layout as struct shark_cfg
   type, bind(C) :: shark_cfg
@@ -31,7 +32,9 @@ module shark_gpu_mod
      real(c_double) :: rhograin
      real(c_double) :: mstar, r_relax, n_rel, smooth_r
      real(c_double) :: rho_l, rho_r, p_l, p_r
-     real(c_double) :: reserved(12)
+     real(c_double) :: box_l_y, mplanet, rplanet, sigma_r0, r0_disk, frac_r_disk, buffer_in, buffer_out
+     real(c_double) :: alpha_visc, dust2gas
+     real(c_double) :: reserved(2)
   end type shark_cfg
 
   ! same layout as struct shark_tune (launch shapes of the step kernels)
@@ -159,6 +162,9 @@ contains
     c%eta_stream = 0.0d0; c%rho_init = 0.0d0; c%hoverr = 0.0d0; c%rhograin = 0.0d0
     c%mstar = 0.0d0; c%r_relax = 0.0d0; c%n_rel = 0.0d0; c%smooth_r = 0.0d0
     c%rho_l = 0.0d0; c%rho_r = 0.0d0; c%p_l = 0.0d0; c%p_r = 0.0d0
+    c%box_l_y = 0.0d0; c%mplanet = 0.0d0; c%rplanet = 0.0d0; c%sigma_r0 = 0.0d0; c%r0_disk = 0.0d0
+    c%frac_r_disk = 0.0d0; c%buffer_in = 0.0d0; c%buffer_out = 0.0d0; c%alpha_visc = 0.0d0
+    c%dust2gas = 0.0d0
     c%reserved = 0.0d0
   end subroutine shark_cfg_defaults
 end module shark_gpu_mod

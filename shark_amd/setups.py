@@ -225,6 +225,86 @@ def kh(nx=256, ny=256, ndust=1, cfl=0.5, slope_type=3, solver=C.SOLVER_HLLC, sol
                         condinit=None, fargo_velocity=None)
 
 
+AU, MSUN = 1.5e13, 2e33   # src/commons/phys_const.f90:11,13
+
+
+def units_cdisk():
+    """The cartesian disk's units module (src/setups/disk/units.f90)."""
+    unit_l = AU
+    unit_d = MSUN / _powi(AU, 3)
+    unit_t = 1.0 / math.sqrt(GRAV * unit_d)
+    unit_v = unit_l / unit_t
+    return dict(unit_l=unit_l, unit_d=unit_d, unit_dcol=MSUN / _powi(AU, 2), unit_t=unit_t, unit_m=MSUN,
+                unit_v=unit_v, unit_p=unit_v * unit_v)
+
+
+def cdisk(nx=256, ny=256, ndust=1, cfl=0.5, slope_type=1, solver=C.SOLVER_HLL, solverdust=0, box_l=400.0,
+          sigma_r0=1700.0, r0_disk=1.0, mstar=1.0, mplanet=0.0, rplanet=20.0, hoverr=0.05, smooth_r=0.5,
+          alpha_visc=1e-2, frac_r_disk=0.1, buffer_in=0.1e-2, buffer_out=0.25, viscosity=1, smin=1e-7,
+          smax=1e-2, scutmin=5e-7, scut=1e-4, rhograin=2.3, dust2gas=0.01, gamma=1.66667):
+    """Cartesian disk (src/setups/disk/setup.f90:1-70): a softened point mass (plus an optional
+    planet on a circular orbit) at the centre of a square box, surface density
+    sigma_R0 (R0/r) out to frac_R_disk box_l and a floor of 1/100 of its edge value beyond,
+    Keplerian rotation with the softened radius r = sqrt(x^2 + y^2 + smooth_r^2), isothermal
+    cs = HoverR sqrt(Mstar / r), dust at dust2gas with the gas velocity and the MRN bin sizes;
+    zero-gradient boundaries, the inner and outer buffers reset to this state every step, alpha
+    viscosity eta = alpha_visc H cs.  Inputs in the namelist's units (au, g/cm^2, solar
+    masses), converted as the setup does; defaults: the setup module's, except HoverR taken as
+    the double 0.05."""
+    u = units_cdisk()
+    sigma_r0 = sigma_r0 / u["unit_dcol"]
+    r0_disk = r0_disk * AU / u["unit_l"]
+    rplanet = rplanet * AU / u["unit_l"]
+    smooth_r = smooth_r * AU / u["unit_l"]
+    mstar = mstar * MSUN / u["unit_m"]
+    mplanet = mplanet * MSUN / u["unit_m"]
+    box_l = box_l * AU / u["unit_l"]
+    box_l_y = box_l
+    r_out = frac_r_disk * box_l
+    cfg = C.make_cfg(nx=nx, ny=ny, ndust=ndust, iso_cs=1, cfl=cfl, slope_type=slope_type, solver=solver,
+                     solverdust=solverdust, gamma=gamma, force_kick=1, viscosity=int(viscosity),
+                     bc_kind=C.BC_ZERO_GRADIENT, tstop_kind=C.TSTOP_EPSTEIN_CDISK, force_kind=C.FORCE_CDISK,
+                     inloop_kind=C.INLOOP_CDISK, box_l=box_l, box_l_y=box_l_y, sigma_r0=sigma_r0,
+                     r0_disk=r0_disk, mstar=mstar, mplanet=mplanet, rplanet=rplanet, hoverr=hoverr,
+                     smooth_r=smooth_r, alpha_visc=alpha_visc, frac_r_disk=frac_r_disk, buffer_in=buffer_in,
+                     buffer_out=buffer_out, dust2gas=dust2gas, rhograin=rhograin / u["unit_d"],
+                     smallr=1e-27 / u["unit_d"], smallc=1e-27 / u["unit_v"],
+                     smallp=(1e-27 / u["unit_d"]) * ((1e-27 / u["unit_v"]) * (1e-27 / u["unit_v"])))
+    g = grid_cartesian_2d(nx, ny, box_l, box_l_y)
+    nc = g["vol"].size
+    nv = 5 + 4 * ndust
+    q = np.zeros((nv, nc))
+    cs = np.zeros(nc)
+    eta = np.zeros(nc)
+    for c in range(nc):
+        xx = g["position"][0, c] - 0.5 * box_l
+        yy = g["position"][1, c] - 0.5 * box_l_y
+        r_cyl = math.sqrt(xx * xx + yy * yy)
+        rr = math.sqrt((xx * xx + yy * yy) + smooth_r * smooth_r)
+        x_soft, y_soft = xx * rr / r_cyl, yy * rr / r_cyl
+        H = hoverr * rr
+        cs0 = hoverr * math.sqrt(mstar / rr)
+        om = math.sqrt(mstar / _powi(rr, 3))
+        rho = sigma_r0 * (r0_disk / rr)
+        if rr > r_out:
+            rho = sigma_r0 * (r0_disk / r_out) / 100.0
+        q[0, c], q[1, c], q[2, c] = rho, -(om * y_soft), om * x_soft
+        q[4, c] = rho * (cs0 * cs0)
+        cs[c] = cs0
+        for k in range(ndust):
+            q[5 + k, c] = dust2gas * rho
+            q[5 + ndust + k, c] = -(om * y_soft)
+            q[5 + 2 * ndust + k, c] = om * x_soft
+        eta[c] = (alpha_visc * H) * cs0
+    sd = None
+    if ndust:
+        sizes, _ = mrn_sizes_and_ratios(ndust, u, smin=smin, smax=smax, scutmin=scutmin, scut=scut,
+                                        rhograin=rhograin, dust2gas=dust2gas)
+        sd = np.repeat(np.asarray(sizes)[:, None], nc, axis=1)
+    return cfg, g, dict(u=primtoc(q, ndust, gamma), cs=cs, sdust=sd, condinit=None, fargo_velocity=None,
+                        eta_visc=eta)
+
+
 def dustywave(nx=512, K=10.0, dust2gas=1.0, vx0=1e-4, drho=0.0, cs0=1.0, rho0=1.0, box_l=1.0,
               kx=1.0, cfl=0.5, slope_type=1, solver=C.SOLVER_HLL, solverdust=C.DUST_HUANG_BAI,
               gamma=1.6666667):

@@ -170,6 +170,41 @@ charging=.false.
 /
 """
 
+# Cartesian disk (setups/disk): softened point-mass gravity, zero-gradient boundaries, buffer
+# zones reset every step, alpha viscosity; box and disk in au, Mstar in solar masses
+CDISK = """&GRID_PARAMS
+freq_out=100000
+CFL = {cfl}
+tend = 1d10
+slope_type = {slope}
+force_kick = .true.
+viscosity = {visc}
+/
+&SETUP_PARAMS
+box_l = 400.0d0
+sigma_R0 = 1700.0d0
+R0_disk = 1.0d0
+Mstar = 1.0d0
+HoverR = 0.05d0
+smooth_r = {smooth}
+alpha_visc = {alpha}
+Mplanet = {mplanet}
+frac_R_disk = 0.1d0
+buffer_in = 0.1d-2
+buffer_out = 0.25d0
+/
+&DUST_PARAMS
+scut = 1.0d-4
+dust2gas = 0.01d0
+rhograin = 2.3d0
+/
+&GRAVITY_PARAMS
+/
+&IONISATION_PARAMS
+charging=.false.
+/
+"""
+
 # name: (setup, nx, ny, ndust, geom, gridspace, solver, solverdust, namelist, nsteps)
 CASES = {
     "dustywave_k50": ("DUSTYWAVE", 128, 1, 1, 0, 0, 1, 0,
@@ -199,6 +234,15 @@ CASES = {
                   KH.format(cfl=0.4, slope=1, rho_diffuse="1.0d-18", scut="250d-7"), 40),
     "kh_n4_llf_dhllc": ("KH", 24, 32, 4, 0, 0, 0, 3,
                         KH.format(cfl=0.5, slope=2, rho_diffuse="0.7d-18", scut="250d-7"), 30),
+    "cdisk_visc_n1": ("disk", 32, 32, 1, 0, 0, 1, 0,
+                      CDISK.format(cfl=0.5, slope=1, visc=".true.", smooth="0.5d0", alpha="1d-2", mplanet="0.0d0"), 30),
+    "cdisk_n2_s3": ("disk", 40, 32, 2, 0, 0, 2, 2,
+                    CDISK.format(cfl=0.4, slope=3, visc=".false.", smooth="2.0d0", alpha="1d-2", mplanet="0.0d0"), 30),
+    "cdisk_visc_gas": ("disk", 32, 24, 0, 0, 0, 1, 0,
+                       CDISK.format(cfl=0.5, slope=2, visc=".true.", smooth="0.5d0", alpha="5d-2", mplanet="0.0d0"), 30),
+    "cdisk_planet_n1": ("disk", 32, 32, 1, 0, 0, 1, 1,
+                        CDISK.format(cfl=0.5, slope=1, visc=".true.", smooth="1.0d0", alpha="1d-2",
+                                     mplanet="3.0d-3"), 40),
 }
 
 
@@ -237,6 +281,8 @@ def run_case(name, spec, force_build=False):
         final = np.fromfile(os.path.join(td, "final.bin")).reshape(nv, nc)
         dts = np.fromfile(os.path.join(td, "dt.bin"))
         aux = np.fromfile(os.path.join(td, "aux.bin"))
+        evp = os.path.join(td, "eta_visc.bin")
+        eta_visc = np.fromfile(evp) if os.path.exists(evp) else None
     off = 0
 
     def take(n, shape=None):
@@ -246,6 +292,8 @@ def run_case(name, spec, force_build=False):
         return a.reshape(shape) if shape else a
 
     arrs = dict(ic=ic, final=final, dt=dts)
+    if eta_visc is not None:
+        arrs["eta_visc"] = eta_visc
     arrs["cs"] = take(nc)
     arrs["dx"] = take(nc * ndim, (ndim, nc))
     arrs["surf"] = take(nc * ndim, (ndim, nc))

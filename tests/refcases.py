@@ -45,6 +45,14 @@ def load_ref(name):
         kw.update(bc_kind=C.BC_SOD, tstop_kind=C.TSTOP_DUSTYWAVE, force_kind=C.FORCE_NONE,
                   inloop_kind=C.INLOOP_NONE, rho_l=m["rho_l"], rho_r=m["rho_r"], p_l=m["p_l"],
                   p_r=m["p_r"], shock_dir=m["shock_dir"])
+    elif setup == "disk":
+        kw.update(bc_kind=C.BC_ZERO_GRADIENT, tstop_kind=C.TSTOP_EPSTEIN_CDISK, force_kind=C.FORCE_CDISK,
+                  inloop_kind=C.INLOOP_CDISK, viscosity=int(m["viscosity"]), box_l=m["box_l"],
+                  box_l_y=m["box_l_y"], sigma_r0=m["sigma_R0"], r0_disk=m["R0_disk"], mstar=m["mstar"],
+                  mplanet=m["mplanet"], rplanet=m["rplanet"], frac_r_disk=m["frac_R_disk"],
+                  buffer_in=m["buffer_in"], buffer_out=m["buffer_out"], hoverr=m["hoverr"],
+                  smooth_r=m["smooth_r"], alpha_visc=m["alpha_visc"], dust2gas=m.get("dust2gas", 0.0),
+                  rhograin=m.get("rhograin", 0.0) / m["unit_d"])
     elif setup == "KH":
         kw.update(bc_kind=C.BC_PERIODIC, tstop_kind=C.TSTOP_EPSTEIN, force_kind=C.FORCE_NONE,
                   inloop_kind=C.INLOOP_NONE, rhograin=m["rhograin"] / m["unit_d"])
