@@ -56,6 +56,7 @@ HFLAGS=""
 [ "$SETUP" = "SOD" ] && HFLAGS="-DREF_SOD"
 [ "$SETUP" = "KH" ] && HFLAGS="-DREF_KH"
 [ "$SETUP" = "disk" ] && HFLAGS="-DREF_CDISK"
+[ "$SETUP" = "Collapse_1D" ] && HFLAGS="-DREF_COLLAPSE"
 HARNESS=${REF_HARNESS:-ref_harness.f90}
 $FC $FFLAGS $HFLAGS -c "$HERE/$HARNESS" -o "$OUT/obj/ref_harness.o"
 $FC -fopenmp $OBJS "$OUT/obj/ref_harness.o" -o "$OUT/shark_ref"

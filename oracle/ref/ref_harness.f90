@@ -31,6 +31,9 @@ program ref_harness
   use setup_parameters, only: box_l, box_l_y, sigma_R0, R0_disk, Mstar, Mplanet, Rplanet, &
        frac_R_disk, buffer_in, buffer_out, HoverR, smooth_r, alpha_visc
 #endif
+#ifdef REF_COLLAPSE
+  use setup_parameters, only: M_cloud, alpha_cloud, T0_cloud, mu_gas
+#endif
   implicit none
   integer :: nsteps, istep, ilun, iout
   character(len=256) :: arg, outdir
@@ -172,6 +175,24 @@ program ref_harness
 #endif
   open(32, file=trim(outdir)//'/eta_visc.bin', form='unformatted', access='stream', status='replace')
   write(32) eta_visc
+  close(32)
+#endif
+#ifdef REF_COLLAPSE
+  write(ilun, '(A,ES25.17)') 'M_cloud ', M_cloud
+  write(ilun, '(A,ES25.17)') 'alpha_cloud ', alpha_cloud
+  write(ilun, '(A,ES25.17)') 't0_cloud ', T0_cloud
+  write(ilun, '(A,ES25.17)') 'mu_gas ', mu_gas
+  write(ilun, '(A,ES25.17)') 'unit_nh ', unit_nH
+  write(ilun, '(A,ES25.17)') 'unit_m ', unit_m
+  write(ilun, '(A,ES25.17)') 'l_soft ', l_soft
+  write(ilun, '(A,ES25.17)') 'm_cent ', M_cent
+  write(ilun, '(A,ES25.17)') 'rin ', rin
+  write(ilun, '(A,I12)') 'non_standard_eos ', non_standard_eos
+#if NDUST>0
+  write(ilun, '(A,ES25.17)') 'rhograin_c ', rhograin
+#endif
+  open(32, file=trim(outdir)//'/dvol.bin', form='unformatted', access='stream', status='replace')
+  write(32) dvol
   close(32)
 #endif
 #ifdef REF_KH

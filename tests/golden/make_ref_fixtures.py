@@ -205,7 +205,32 @@ charging=.false.
 /
 """
 
-# name: (setup, nx, ny, ndust, geom, gridspace, solver, solverdust, namelist, nsteps)
+# Collapse_1D: 1D spherical log grid from rin (au) to the cloud radius, barotropic EOS, 1D
+# self-gravity (built with GRAVITY=1); the reference's test-suite collapse run
+COLLAPSE = """&GRID_PARAMS
+freq_out = 100000
+CFL = {cfl}
+tend = 1
+gamma = 1.66667d0
+slope_type = {slope}
+rin = 1d-1
+force_kick = .false.
+/
+&SETUP_PARAMS
+M_cloud = 1
+more_outputs = .false.
+/
+&DUST_PARAMS
+{dust}
+/
+&GRAVITY_PARAMS
+/
+&IONISATION_PARAMS
+charging=.false.
+/
+"""
+
+# name: (setup, nx, ny, ndust, geom, gridspace, solver, solverdust, namelist, nsteps[, extra -D])
 CASES = {
     "dustywave_k50": ("DUSTYWAVE", 128, 1, 1, 0, 0, 1, 0,
                       WAVE.format(cfl=0.8, tend=4.5, slope=2, vx0="1.0d-4", drho="1.0d-4", K="50d0"), -1),
@@ -245,6 +270,15 @@ CASES = {
                                      mplanet="3.0d-3"), 40),
 }
 
+# Runs of the reference that are not fixtures: its Collapse_1D step turns the first active cell into
+# NaN on the first step (gridinit_sphere1D, grid.f90:335-337, leaves radii_c = 0 in the inner
+# ghost cells, so the spherical predictor source -rho u / radii_c, godunov.f90:207, is 0/0 there and
+# the inner-face flux carries it in).  `python make_ref_fixtures.py --check-collapse` shows it.
+DEFECTIVE = {
+    "collapse_gas": ("Collapse_1D", 128, 1, 0, 1, 0, 1, 0, COLLAPSE.format(cfl=0.8, slope=1, dust=""), 1,
+                     "-DGRAVITY=1"),
+}
+
 
 def _meta(path):
     m = {}
@@ -262,13 +296,14 @@ def _meta(path):
 
 
 def run_case(name, spec, force_build=False):
-    setup, nx, ny, nd, geom, gs, solver, sdust, nml, nsteps = spec
-    bname = f"{setup}_{nx}x{ny}_n{nd}_g{geom}{gs}_s{solver}{sdust}"
+    setup, nx, ny, nd, geom, gs, solver, sdust, nml, nsteps = spec[:10]
+    extra = list(spec[10:])
+    bname = f"{setup}_{nx}x{ny}_n{nd}_g{geom}{gs}_s{solver}{sdust}" + ("_grav" if extra else "")
     exe = os.path.join(REPO, "oracle", "_ref", bname, "shark_ref")
     if force_build or not os.path.exists(exe):
         subprocess.run([BUILD, setup, str(nx), str(ny), str(nd), str(geom), str(gs), str(solver),
-                        str(sdust)], check=True, stdout=subprocess.DEVNULL,
-                       stderr=subprocess.DEVNULL)
+                        str(sdust)] + extra, check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL, env=dict(os.environ, REF_NAME=bname))
     with tempfile.TemporaryDirectory() as td:
         nmlp = os.path.join(td, "run.nml")
         open(nmlp, "w").write(nml)
@@ -283,6 +318,8 @@ def run_case(name, spec, force_build=False):
         aux = np.fromfile(os.path.join(td, "aux.bin"))
         evp = os.path.join(td, "eta_visc.bin")
         eta_visc = np.fromfile(evp) if os.path.exists(evp) else None
+        dvp = os.path.join(td, "dvol.bin")
+        dvol = np.fromfile(dvp) if os.path.exists(dvp) else None
     off = 0
 
     def take(n, shape=None):
@@ -294,6 +331,8 @@ def run_case(name, spec, force_build=False):
     arrs = dict(ic=ic, final=final, dt=dts)
     if eta_visc is not None:
         arrs["eta_visc"] = eta_visc
+    if dvol is not None:
+        arrs["dvol"] = dvol
     arrs["cs"] = take(nc)
     arrs["dx"] = take(nc * ndim, (ndim, nc))
     arrs["surf"] = take(nc * ndim, (ndim, nc))
@@ -315,7 +354,33 @@ def run_case(name, spec, force_build=False):
     return m
 
 
+def check_collapse():
+    """Runs the Collapse_1D case for one step into a scratch directory and reports the NaN cells."""
+    name = "collapse_gas"
+    spec = DEFECTIVE[name]
+    setup, nx, ny, nd, geom, gs, solver, sdust, nml, nsteps = spec[:10]
+    bname = f"{setup}_{nx}x{ny}_n{nd}_g{geom}{gs}_s{solver}{sdust}_grav"
+    exe = os.path.join(REPO, "oracle", "_ref", bname, "shark_ref")
+    if not os.path.exists(exe):
+        subprocess.run([BUILD, setup, str(nx), str(ny), str(nd), str(geom), str(gs), str(solver), str(sdust)]
+                       + list(spec[10:]), check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                       env=dict(os.environ, REF_NAME=bname))
+    with tempfile.TemporaryDirectory() as td:
+        open(os.path.join(td, "run.nml"), "w").write(nml)
+        subprocess.run([exe, os.path.join(td, "run.nml"), "1", td], check=True, cwd=td,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"), stdout=subprocess.DEVNULL)
+        m = _meta(os.path.join(td, "meta.txt"))
+        u = np.fromfile(os.path.join(td, "final.bin")).reshape(m["nvar"], m["ncells"])
+        aux = np.fromfile(os.path.join(td, "aux.bin"))
+        radii_c = aux[4 * m["ncells"]:5 * m["ncells"]]
+    bad = np.flatnonzero(~np.isfinite(u).all(axis=0))
+    print("cells with NaN after one step (1-based):", (bad + 1).tolist(), "radii_c of cells 1-3:", radii_c[:3])
+
+
 def main(argv):
+    if argv[1:] == ["--check-collapse"]:
+        check_collapse()
+        return
     names = argv[1:] or list(CASES)
     for n in names:
         m = run_case(n, CASES[n])
