@@ -889,6 +889,25 @@ static void source_terms(oracle* o) {
       S[o->ivy * nc + c] = S[o->ivy * nc + c] + Q(o->irho, c) * dt * eta * (lxv + lyv);
     }
   }
+  if (p->viscosity && p->geom == SHARK_GEOM_POLAR) {
+    /* :111-116, GEOM 2: radial Laplacians only; eta_visc = alpha_visc H cs with H = HoverR r
+     * (disk_polar/setup.f90:29-30, 67) */
+    #pragma omp parallel for schedule(static)
+    for (int c = 0; c < o->nc; ++c) {
+      if (!o->active[c]) continue;
+      int ix = ixx(o, c), iy = iyy(o, c);
+      int xp = cidx(o, ix + 1, iy), xm = cidx(o, ix - 1, iy);
+      double r = o->radii_c[c], d1 = DX(c, 0);
+      double eta = (p->alpha_visc * (p->hoverr * r)) * o->cs[c];
+      double* S = o->S_U;
+      size_t nc = o->nc;
+      double su = Q(o->ivx, xp) + Q(o->ivx, xm), sv = Q(o->ivy, xp) + Q(o->ivy, xm);
+      double lxu = (su - 2.0 * Q(o->ivx, c)) / (d1 * d1) + su / r / d1 / 2.0 - Q(o->ivx, c) / r / r;
+      S[o->ivx * nc + c] = S[o->ivx * nc + c] + Q(o->irho, c) * dt * eta * lxu;
+      double lxv = (sv - 2.0 * Q(o->ivy, c)) / (d1 * d1) - Q(o->ivy, c) / (r * r) + sv / r / d1 / 2.0;
+      S[o->ivy * nc + c] = S[o->ivy * nc + c] + Q(o->irho, c) * dt * eta * lxv;
+    }
+  }
   /* :221 u_prim = u_prim + S_U over the whole array */
   #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < n; ++i) o->u[i] = o->u[i] + o->S_U[i];

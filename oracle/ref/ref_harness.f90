@@ -16,7 +16,7 @@ program ref_harness
   use commons
   use units
 #ifdef REF_DISK
-  use setup_parameters, only: uprim_condinit, mstar, r_relax, n_rel, smooth_r, hoverr
+  use setup_parameters, only: uprim_condinit, mstar, r_relax, n_rel, smooth_r, hoverr, alpha_visc
 #endif
 #ifdef REF_SI
   use setup_parameters, only: box_l, rho_init, omega_shear, q_shear, hoverr, eta_stream, rad0
@@ -138,6 +138,11 @@ program ref_harness
   write(ilun, '(A,ES25.17)') 'n_rel ', n_rel
   write(ilun, '(A,ES25.17)') 'smooth_r ', smooth_r
   write(ilun, '(A,ES25.17)') 'hoverr ', hoverr
+  write(ilun, '(A,ES25.17)') 'alpha_visc ', alpha_visc
+  write(ilun, '(A,L2)') 'viscosity ', viscosity
+  open(32, file=trim(outdir)//'/eta_visc.bin', form='unformatted', access='stream', status='replace')
+  write(32) eta_visc
+  close(32)
 #endif
 #ifdef REF_SI
   write(ilun, '(A,ES25.17)') 'box_l ', box_l

@@ -119,8 +119,10 @@ static int validate(const shark_cfg* c) {
                      c->tstop_kind == SHARK_TSTOP_EPSTEIN_CDISK;
   if (cdisk && (c->geom != SHARK_GEOM_CARTESIAN || c->ny < 4 || c->iso_cs != 1))
     return fail(SHARK_EUNSUP, "the cartesian disk hooks need a 2D cartesian grid and isothermal gas");
-  if (c->viscosity && !(c->inloop_kind == SHARK_INLOOP_CDISK && c->iso_cs == 1 && c->ny >= 4))
-    return fail(SHARK_EUNSUP, "viscosity is built for the isothermal cartesian disk (eta_visc = alpha_visc H cs)");
+  if (c->viscosity && !(c->iso_cs == 1 && c->ny >= 4 &&
+                        ((c->geom == SHARK_GEOM_CARTESIAN && c->inloop_kind == SHARK_INLOOP_CDISK) ||
+                         (c->geom == SHARK_GEOM_POLAR && c->inloop_kind == SHARK_INLOOP_DISK))))
+    return fail(SHARK_EUNSUP, "viscosity is built for the isothermal cartesian and polar disks (eta_visc = alpha_visc H cs)");
   if (c->fargo && c->ny < 4) return fail(SHARK_EUNSUP, "FARGO needs a 2D grid");
   if (c->force_kick && !c->drag && c->ndust > 0)
     return fail(SHARK_EUNSUP, "explicit drag in the force (drag=.false. with force_kick) is not built");
@@ -248,7 +250,8 @@ extern "C" int shark_gpu_set_geometry(shark_ctx* c, const double* dx, const doub
       if (a.ndim == 2) ok = ok && dx[nc + ci] == col[nxm + ix - 1] && surf[nc + ci] == col[3 * nxm + ix - 1];
       // position is read only by the shearing-box and cartesian-disk hooks
       const bool cdisk = c->cfg.force_kind == SHARK_FORCE_CDISK || c->cfg.inloop_kind == SHARK_INLOOP_CDISK ||
-                         c->cfg.tstop_kind == SHARK_TSTOP_EPSTEIN_CDISK || c->cfg.viscosity;
+                         c->cfg.tstop_kind == SHARK_TSTOP_EPSTEIN_CDISK ||
+                         (c->cfg.viscosity && c->cfg.geom == SHARK_GEOM_CARTESIAN);
       const bool need_pos = c->cfg.force_kind == SHARK_FORCE_SI || c->cfg.inloop_kind == SHARK_INLOOP_SI || cdisk;
       if (position && need_pos) ok = ok && position[ci] == col[6 * nxm + ix - 1];
       if (position && cdisk && a.ndim == 2) ok = ok && position[nc + ci] == position[nc + (size_t)(iy - 1) * nxm];
@@ -259,7 +262,8 @@ extern "C" int shark_gpu_set_geometry(shark_ctx* c, const double* dx, const doub
     for (int iy = 1; iy <= nym; ++iy) col[7 * (size_t)nxm + iy - 1] = position[nc + (size_t)(iy - 1) * nxm];
   const bool need_pos = c->cfg.force_kind == SHARK_FORCE_SI || c->cfg.inloop_kind == SHARK_INLOOP_SI ||
                         c->cfg.force_kind == SHARK_FORCE_CDISK || c->cfg.inloop_kind == SHARK_INLOOP_CDISK ||
-                        c->cfg.tstop_kind == SHARK_TSTOP_EPSTEIN_CDISK || c->cfg.viscosity;
+                        c->cfg.tstop_kind == SHARK_TSTOP_EPSTEIN_CDISK ||
+                        (c->cfg.viscosity && c->cfg.geom == SHARK_GEOM_CARTESIAN);
   if (!position && need_pos) return fail(SHARK_EINVAL, "position is required by this setup's hooks");
   HIPCHK(hipMemcpyAsync(c->dgeom, col.data(), col.size() * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

@@ -166,11 +166,12 @@ template <int ND, bool CD = true>
 DEV double tstop_of(const StepArgs& a, int k, int cell, int ix, double rho_q, double rhod_q, double cs_q);
 
 // GEOM: 0 cartesian, 2 polar, 3 cartesian with the cartesian disk's point-mass force and
-// viscosity compiled in.  SLOPE: limiter (1..3); SOLV: compile-time Riemann solver (gas 0..2,
+// viscosity compiled in, 4 polar with viscosity compiled in.  SLOPE: limiter (1..3); SOLV: compile-time Riemann solver (gas 0..2,
 // dust 0..3) or -1 to read it from the arguments.  ADI: adiabatic gas (P from the total energy, its own slopes,
 // Hancock source and face states, sound speeds from the face states; iso_cs < 1).
 template <int NDIM, int GEOM, bool GAS, int SLOPE, int SOLV, bool ADI>
 __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int lane) {
+  constexpr bool POLAR = GEOM == 2 || GEOM == 4;
   constexpr int NU = GAS ? 5 : 4;
   constexpr int NQ = ADI ? 5 : 4;   // primitives carried through slopes and faces
   const size_t nc = a.nc;
@@ -190,7 +191,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
   const double dt = a.sc->dt;
   constexpr int st = SLOPE;
   const double dx1 = a.g.dx1[ixc - 1], dx1m = a.g.dx1[ixl - 1];
-  const double rp = (GEOM == 2) ? a.g.rc[ixc - 1] : 1.0;
+  const double rp = (POLAR) ? a.g.rc[ixc - 1] : 1.0;
   const double dx2 = (NDIM == 2) ? a.g.dx2[ixc - 1] : 0.0;
   const double surf1 = a.g.surf1[ixc - 1], surf1r = a.g.surf1[ixr - 1];
   const double surf2 = (NDIM == 2) ? a.g.surf2[ixc - 1] : 0.0;
@@ -252,14 +253,14 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
         P = r.q[4];
         dpx = dqx[4];
         dpy = (NDIM == 2) ? dqy[4] : 0.0;
-        if (GEOM == 2) sp0 = ((-(u * dpx)) - v * dpy - (dux + dvy) * a.gamma * P) - exdiv(a.gamma * P * u, rp, yrp);
+        if (POLAR) sp0 = ((-(u * dpx)) - v * dpy - (dux + dvy) * a.gamma * P) - exdiv(a.gamma * P * u, rp, yrp);
         else sp0 = (-(u * dpx)) - v * dpy - (dux + dvy) * a.gamma * P;
       } else {               // :125-128
         const double cs2 = r.cs * r.cs;
         dpx = dqx[0] * cs2;
         dpy = (NDIM == 2) ? dqy[0] * cs2 : 0.0;
       }
-      if (GEOM == 2) {
+      if (POLAR) {
         sr0 = ((-(u * drx)) - v * dry - (dux + dvy) * r_rho) - exdiv(r_rho * u, rp, yrp);
         su0 = ((-(u * dux)) - v * duy - dpx / r_rho) + exdiv(v * v, rp, yrp);
         sv0 = ((-(u * dvx)) - v * dvy - dpy / r_rho) - exdiv(u * v, rp, yrp);
@@ -275,7 +276,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
       su0 = (-(u * dux)) - v * duy;
       sv0 = (-(u * dvx)) - v * dvy;
       sw0 = (-(u * dwx)) - v * dwy;
-      if (GEOM == 2) {
+      if (POLAR) {
         sr0 = sr0 - exdiv(r_rho * u, rp, yrp);
         su0 = su0 + exdiv(v * v, rp, yrp);
         sv0 = sv0 - exdiv(u * v, rp, yrp);
@@ -347,7 +348,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
   double Fx_own_prev[5] = {0, 0, 0, 0, 0}, Fx_rt_prev[5] = {0, 0, 0, 0, 0};
   double Fy_bot[5] = {0, 0, 0, 0, 0};
   // viscosity (cartesian disk): velocities of the row below the one being finalized
-  constexpr bool VISC = GAS && GEOM == 3 && NDIM == 2 && !ADI;
+  constexpr bool VISC = GAS && (GEOM == 3 || GEOM == 4) && NDIM == 2 && !ADI;
   const bool visc = VISC && a.viscosity;
   double vbx = 0.0, vby = 0.0;
 
@@ -360,7 +361,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
   for (int i = 0; i < PF; ++i)
     if (jstart + i <= jend) fetch_row(jstart + i, pend[i]);
   for (int j = jstart; j <= jend; ++j) {
-    if (visc && j >= jstart + 3) { vbx = R0.q[1]; vby = R0.q[2]; }   // row j-3
+    if (GEOM == 3 && visc && j >= jstart + 3) { vbx = R0.q[1]; vby = R0.q[2]; }   // row j-3
     R0 = R1;
     R1 = R2;
     R2 = pend[0];
@@ -465,7 +466,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
       }
       // sources (source_terms.f90:37-47), from the start-of-step primitives
       double S[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      if (GEOM == 2) {
+      if (POLAR) {
         const double r = rp;
         if (GAS) {
           const double P = ADI ? Rf.q[4] : Rf.u[0] * (Rf.cs * Rf.cs);
@@ -495,7 +496,24 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
           S[4] = s4;
         }
       }
-      if constexpr (VISC) {
+      if constexpr (VISC && GEOM == 4) {
+        // viscosity (source_terms.f90:111-116, GEOM 2): radial Laplacians from the start-of-step
+        // primitives, eta_visc = alpha_visc H cs with H = HoverR r (disk_polar/setup.f90:29-30, 67),
+        // added to the polar sources
+        if (visc) {
+          const double r = rp;
+          const double eta = (a.alpha_visc * (a.hoverr * r)) * Rf.cs;
+          const double vx = Rf.q[1], vy = Rf.q[2];
+          const double su = vrx + vlx, sv = vry + vly;
+          const double d1 = dx1 * dx1;
+          const double lxu = (su - 2.0 * vx) / d1 + su / r / dx1 / 2.0 - vx / r / r;
+          const double lxv = (sv - 2.0 * vy) / d1 - vy / (r * r) + sv / r / dx1 / 2.0;
+          const double k = Rf.q[0] * dt * eta;
+          S[1] = S[1] + k * lxu;
+          S[2] = S[2] + k * lxv;
+        }
+      }
+      if constexpr (VISC && GEOM == 3) {
         // viscosity (source_terms.f90:99-110, GEOM 0) from the start-of-step primitives, with
         // eta_visc = alpha_visc H cs (disk/setup.f90:60); rows fr-1 and fr+1 are vb* and R1
         if (visc) {
@@ -1120,8 +1138,8 @@ static void hydro_launch1(const StepArgs& a, hipStream_t s) {
   const long items = (long)a.ngroups * (GAS ? a.nstrips_g : a.nstrips) * (GAS ? 1 : a.nd);
   const dim3 grid((unsigned)((items + 3) / 4));
   const bool fast = GAS ? a.solver == 1 : a.solverdust == 0;   // HLL gas / Huang-Bai dust
-  if constexpr (GAS && GEOM != 3) {
-    if (!a.iso) {   // adiabatic gas
+  if constexpr (GAS && GEOM == 0) {
+    if (!a.iso) {   // adiabatic gas (cartesian grids only: shark_gpu_create rejects the others)
       if (fast) hipLaunchKernelGGL((k_hydro<NDIM, GEOM, true, SLOPE, 1, true>), grid, dim3(256), 0, s, a);
       else hipLaunchKernelGGL((k_hydro<NDIM, GEOM, true, SLOPE, -1, true>), grid, dim3(256), 0, s, a);
       return;
@@ -1143,6 +1161,7 @@ static void hydro_launch_slope(const StepArgs& a, hipStream_t s) {
 template <bool GAS>
 static void hydro_launch_geom(const StepArgs& a, hipStream_t s) {
   if (a.ndim == 1) hydro_launch_slope<1, 0, GAS>(a, s);
+  else if (a.geom == 2 && a.viscosity) hydro_launch_slope<2, 4, GAS>(a, s);   // viscous polar disk
   else if (a.geom == 2) hydro_launch_slope<2, 2, GAS>(a, s);
   else if (a.force_kind == 3 || a.viscosity) hydro_launch_slope<2, 3, GAS>(a, s);   // cartesian disk
   else hydro_launch_slope<2, 0, GAS>(a, s);

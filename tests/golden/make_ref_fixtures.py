@@ -89,8 +89,10 @@ tend = 1.0d10
 gamma=1.666667d0
 force_kick =.true.
 slope_type = {slope}
+viscosity = {visc}
 /
 &SETUP_PARAMS
+alpha_visc   = {alpha}
 box_l        = 2.1d0
 R_out_disk   = 1d30
 smooth_r     = 0.4d0
@@ -246,8 +248,12 @@ CASES = {
                         WAVE.format(cfl=0.5, tend="1d10", slope=3, vx0="3.0d-1", drho="1.0d-1", K="5d0"), 60),
     "si_2d": ("SI", 24, 24, 2, 0, 0, 1, 0, SI.format(slope=1), 25),
     "si_2d_s3_hllc": ("SI", 24, 24, 2, 0, 0, 2, 0, SI.format(slope=3), 25),
-    "disk_2d": ("disk_polar", 24, 32, 2, 2, 0, 1, 0, DISK.format(slope=1, r_relax="0.3d0"), 25),
-    "disk_2d_relax_s2": ("disk_polar", 24, 32, 2, 2, 0, 1, 0, DISK.format(slope=2, r_relax="3.0d0"), 25),
+    "disk_2d": ("disk_polar", 24, 32, 2, 2, 0, 1, 0, DISK.format(slope=1, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 25),
+    "disk_2d_relax_s2": ("disk_polar", 24, 32, 2, 2, 0, 1, 0, DISK.format(slope=2, r_relax="3.0d0", visc=".false.", alpha="1d-2"), 25),
+    "disk_2d_visc": ("disk_polar", 24, 32, 2, 2, 0, 1, 0,
+                     DISK.format(slope=1, r_relax="0.3d0", visc=".true.", alpha="5d-2"), 25),
+    "disk_2d_visc_gas": ("disk_polar", 32, 24, 0, 2, 0, 2, 0,
+                         DISK.format(slope=3, r_relax="0.3d0", visc=".true.", alpha="1d-1"), 25),
     "sod_hll_s3": ("SOD", 128, 1, 0, 0, 0, 1, 0, SOD.format(cfl=0.8, tend=0.245, slope=3, dir=0), -1),
     "sod_hllc_s1": ("SOD", 128, 1, 0, 0, 0, 2, 0, SOD.format(cfl=0.8, tend=0.245, slope=1, dir=0), -1),
     "sod_llf_s2": ("SOD", 96, 1, 0, 0, 0, 0, 0, SOD.format(cfl=0.5, tend=0.2, slope=2, dir=0), -1),

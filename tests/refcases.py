@@ -40,7 +40,8 @@ def load_ref(name):
         kw.update(bc_kind=C.BC_DISK, tstop_kind=C.TSTOP_EPSTEIN_DISK, force_kind=C.FORCE_DISK,
                   inloop_kind=C.INLOOP_DISK, mstar=m["mstar"], r_relax=m["r_relax"],
                   n_rel=m["n_rel"], smooth_r=m["smooth_r"], hoverr=m["hoverr"],
-                  rhograin=m["rhograin"] / m["unit_d"], disk_bc_mode=0)
+                  rhograin=m.get("rhograin", 0.0) / m["unit_d"], disk_bc_mode=0,
+                  viscosity=int(m.get("viscosity", False)), alpha_visc=m.get("alpha_visc", 0.0))
     elif setup == "SOD":
         kw.update(bc_kind=C.BC_SOD, tstop_kind=C.TSTOP_DUSTYWAVE, force_kind=C.FORCE_NONE,
                   inloop_kind=C.INLOOP_NONE, rho_l=m["rho_l"], rho_r=m["rho_r"], p_l=m["p_l"],

@@ -38,8 +38,9 @@ def test_setup_matches_reference_initial_state(case, kw):
         np.testing.assert_array_equal(s2["sdust"], state["sdust"])
 
 
-@pytest.mark.parametrize("case", ["cdisk_visc_n1", "cdisk_visc_gas"])
+@pytest.mark.parametrize("case", ["cdisk_visc_n1", "cdisk_visc_gas", "disk_2d_visc", "disk_2d_visc_gas"])
 def test_viscous_fixtures_pin_the_viscous_terms(case):
+    """Also the polar disk's (source_terms.f90:111-116)."""
     import pyoracle
     cfg, geom, state, final, dts, _ = load_ref(case)
     cfg.viscosity = 0
