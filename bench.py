@@ -23,10 +23,11 @@ host cores) on the same workload for a bounded number of steps.
 growth (second half of the metric, "growth-step speedup at fixed L1 err"): on the 512 x 512
 polar disk with 64 mass bins (MRN initial distribution), one growth call per mode over the
 same cells and interval: the plain FP64 coagulation (reference order) and the FP32 / BF16
-MFMA contractions, each with its sub-step count, time per cell-sub-step, speedup over FP64 and
-mass-weighted L1 error against it.  With trained weights in shark_amd/data, the learned step is
-timed against the plain FP64 step on cells aged by the plain solver, at intervals in the disk's
-hydro-step range, with its L1 error and fallback count.
+MFMA contractions, each with its sub-step count, time per cell-sub-step, speedup over FP64,
+mass-weighted L1 error against it and the relative L1 error of every bin's density field
+(l1_per_bin).  With trained weights in shark_amd/data, the learned step is timed against the
+plain FP64 step on cells aged by the plain solver, at intervals in the disk's hydro-step range,
+with its L1 errors (per cell and per bin) and fallback count.
 """
 import argparse
 import json
@@ -95,6 +96,14 @@ def profiled_traffic(workload, kernel):
     return vals.get(kernel)
 
 
+def per_bin_l1(a, ref):
+    """Relative L1 error of each bin's density field over the cells: sum_c |a_k - ref_k| / sum_c ref_k,
+    three significant digits."""
+    import numpy as np
+    e = np.abs(a - ref).sum(axis=1) / np.maximum(ref.sum(axis=1), 1e-300)
+    return [float(f"{x:.3g}") for x in e]
+
+
 def growth_block(steps_probe_dt=2e-8, surrogate_cells=16384, seed=99):
     """Growth timing and error per mode (see module docstring)."""
     import numpy as np
@@ -123,6 +132,7 @@ def growth_block(steps_probe_dt=2e-8, surrogate_cells=16384, seed=99):
                 "ms": ms, "substeps_max": int(nsub.max()), "substeps_mean": float(nsub.mean()),
                 "ns_per_cell_substep": ms * 1e6 / float(nsub.sum()),
                 "l1_vs_fp64_mean": float(err.mean()), "l1_vs_fp64_max": float(err.max()),
+                "l1_per_bin": per_bin_l1(out, ref),
                 "mass_residual_max": float(np.abs(out.sum(0) / rhod.sum(0) - 1).max())}
     finally:
         g.close()
@@ -161,6 +171,7 @@ def growth_block(steps_probe_dt=2e-8, surrogate_cells=16384, seed=99):
             "speedup_vs_fp64": ms_f / ms_s, "fp64_substeps_mean": float(fsub.mean()),
             "l1_vs_fp64_mean": float(err.mean()), "l1_vs_fp64_p99": float(np.quantile(err, 0.99)),
             "no_change_l1_mean": float(S.mass_l1(st["rhod"], fo).mean()),
+            "l1_per_bin": per_bin_l1(so, fo),
             "fallback_cells": int(nfb), "raw_mass_residual_mean": raw,
             "mass_residual_max": float(np.abs(so.sum(0) / st["rhod"].sum(0) - 1).max())}
     return out
