@@ -54,3 +54,31 @@ def test_gpu_sod_2d(direction):
     print(f"sod 2D {nx}x{ny} along {'xy'[direction]}", flush=True)
     u, ref = run_both(cfg, geom, st, 0.05)
     np.testing.assert_array_equal(u, ref)
+
+
+@pytest.mark.parametrize("slope,solver", [(1, C.SOLVER_LLF), (3, C.SOLVER_HLLC)])
+def test_gpu_sod2d_symmetries(slope, solver):
+    """The reference's Sod2d tests on the GPU: x and y tubes and their mirror images equal each
+    other (transposed / mirrored) and the oracle, bit for bit."""
+    from shark_amd import gpu
+    from test_sod import along_x, mirrored, tube
+    n, w = 512, 16
+    ref = None
+    for d in (0, 1):
+        for f in (False, True):
+            cfg, g, st = tube(d, f, n=n, w=w, slope=slope, solver=solver)
+            print(f"sod2d {'xy'[d]} flipped={f} slope={slope} solver={solver}", flush=True)
+            s = gpu.Step(cfg, g, st)
+            n_steps, t = s.run(100000, 0.245)
+            u = along_x(s.download(), d, n=n, w=w)
+            s.close()
+            u = mirrored(u) if f else u
+            if ref is None:
+                import pyoracle
+                o = pyoracle.Oracle(cfg, g, st)
+                o.run(100000, 0.245)
+                assert o.time == t
+                np.testing.assert_array_equal(u, along_x(o.u(), d, n=n, w=w))
+                ref = (t, u)
+            assert t == ref[0]
+            np.testing.assert_array_equal(u, ref[1])

@@ -101,3 +101,43 @@ def test_sod_2d_tube_is_uniform_across(direction):
     for row in range(1, u.shape[1]):
         np.testing.assert_array_equal(u[:, row, :], u[:, 0, :])
     assert np.abs(u[0, 0, :] - ref[0]).mean() < 5e-3
+
+
+def tube(direction, flip, n=128, w=8, slope=2, solver=C.SOLVER_HLL):
+    """The reference's Sod2d tests (test/Sod2d/sod_{x,y}[_up].nml): a tube along x or y, with the
+    dense gas on the left, or on the right (_up)."""
+    kw = dict(rho_l=0.125, rho_r=1.0, p_l=0.1, p_r=1.0) if flip else {}
+    if direction == 0:
+        return setups.sod(nx=n, ny=w, shock_dir=0, box_l=1.0, box_l_y=w / n, slope_type=slope, solver=solver, **kw)
+    return setups.sod(nx=w, ny=n, shock_dir=1, box_l=w / n, box_l_y=1.0, slope_type=slope, solver=solver, **kw)
+
+
+def along_x(u, direction, n=128, w=8):
+    """Active cells as (var, across, along) with the tube's velocity in slot 1."""
+    if direction == 0:
+        return u.reshape(5, w + 4, n + 4)[:, 2:w + 2, 2:n + 2]
+    return np.swapaxes(u.reshape(5, n + 4, w + 4)[:, 2:n + 2, 2:w + 2], 1, 2)[[0, 2, 1, 3, 4]]
+
+
+def mirrored(u):
+    m = u[:, :, ::-1].copy()
+    m[1] = -m[1]
+    return m
+
+
+@pytest.mark.parametrize("slope,solver", [(1, C.SOLVER_LLF), (2, C.SOLVER_HLL), (3, C.SOLVER_HLLC)])
+def test_sod2d_symmetries(slope, solver):
+    """Oracle: the tube along y is the tube along x transposed, and the tube with the dense gas on
+    the right is the mirror image (velocity reversed), bit for bit, to t = 0.245."""
+    import pyoracle
+    runs = {}
+    for d in (0, 1):
+        for f in (False, True):
+            cfg, g, st = tube(d, f, slope=slope, solver=solver)
+            o = pyoracle.Oracle(cfg, g, st)
+            o.run(100000, 0.245)
+            runs[d, f] = (o.time, along_x(o.u(), d))
+    t0, u0 = runs[0, False]
+    for (d, f), (t, u) in runs.items():
+        assert t == t0
+        np.testing.assert_array_equal(mirrored(u) if f else u, u0)
