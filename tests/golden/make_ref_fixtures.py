@@ -23,6 +23,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.abspath(os.path.join(HERE, "..", ".."))
 BUILD = os.path.join(REPO, "oracle", "ref", "build_ref.sh")
+REF = os.environ.get("SHARK_REFERENCE", "/root/reference")
 
 WAVE = """&GRID_PARAMS
 freq_out=100000
@@ -232,6 +233,19 @@ charging=.false.
 /
 """
 
+def suite_nml(rel, **subst):
+    """A namelist of the reference's own test suite (test/test-suite/<rel>), with the run length
+    and output cadence replaced so that a fixed number of steps is taken."""
+    text = open(os.path.join(REF, "test", "test-suite", rel)).read()
+    out = []
+    for line in text.splitlines():
+        key = line.split("=")[0].strip().lower()
+        if key in subst:
+            line = f"{line.split('=')[0]}= {subst[key]}"
+        out.append(line)
+    return "\n".join(out) + "\n"
+
+
 # name: (setup, nx, ny, ndust, geom, gridspace, solver, solverdust, namelist, nsteps[, extra -D])
 CASES = {
     "dustywave_k50": ("DUSTYWAVE", 128, 1, 1, 0, 0, 1, 0,
@@ -250,6 +264,9 @@ CASES = {
     "si_2d_s3_hllc": ("SI", 24, 24, 2, 0, 0, 2, 0, SI.format(slope=3), 25),
     "disk_2d": ("disk_polar", 24, 32, 2, 2, 0, 1, 0, DISK.format(slope=1, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 25),
     "disk_2d_relax_s2": ("disk_polar", 24, 32, 2, 2, 0, 1, 0, DISK.format(slope=2, r_relax="3.0d0", visc=".false.", alpha="1d-2"), 25),
+    "suite_disk_dust": ("disk_polar", 24, 32, 4, 2, 0, 1, 0,
+                        suite_nml("DISK_DUST/disk_dust.nml", freq_out="100000", tend="1d10"), 30),
+    "suite_si_ba": ("SI", 24, 24, 1, 0, 0, 1, 0, suite_nml("SI_BA/si.nml", freq_out="100000", tend="1d10"), 30),
     "disk_2d_visc": ("disk_polar", 24, 32, 2, 2, 0, 1, 0,
                      DISK.format(slope=1, r_relax="0.3d0", visc=".true.", alpha="5d-2"), 25),
     "disk_2d_visc_gas": ("disk_polar", 32, 24, 0, 2, 0, 2, 0,
