@@ -61,9 +61,9 @@ def units_of(setup):
         unit_v = unit_l / unit_t
         return dict(unit_t=unit_t, unit_d=unit_d, unit_l=unit_l, unit_v=unit_v, unit_p=unit_v ** 2,
                     unit_dcol=unit_dcol)
-    if setup == "KH":
-        from .setups import units_kh
-        u = units_kh()
+    if setup in ("KH", "disk"):
+        from .setups import units_cdisk, units_kh
+        u = units_kh() if setup == "KH" else units_cdisk()
         return {k: u[k] for k in ("unit_t", "unit_d", "unit_l", "unit_v", "unit_p", "unit_dcol")}
     return dict(unit_t=1.0, unit_d=1.0, unit_l=1.0, unit_v=1.0, unit_p=1.0, unit_dcol=1.0)
 

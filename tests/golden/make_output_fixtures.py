@@ -19,7 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 from make_ref_fixtures import BUILD, CASES, REPO, _meta  # noqa: E402
 
-OUT_CASES = {"disk_2d": 7, "si_2d": 12, "dustywave_dhll": 3}
+OUT_CASES = {"disk_2d": 7, "si_2d": 12, "dustywave_dhll": 3, "sod_hll_s3": 4, "kh_n1_hllc": 9,
+             "cdisk_visc_n1": 5}
 
 
 def run(name, iout):
@@ -48,7 +49,9 @@ def run(name, iout):
 
 
 def main():
-    for name, iout in OUT_CASES.items():
+    names = sys.argv[1:] or list(OUT_CASES)
+    for name in names:
+        iout = OUT_CASES[name]
         m = run(name, iout)
         print(name, m["files"])
 

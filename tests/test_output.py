@@ -21,13 +21,18 @@ from shark_amd import gpu
 OUT_CASES = sorted(f[11:-4] for f in os.listdir(GOLDEN) if f.startswith("ref_output_") and f.endswith(".npz"))
 
 
-def ctoprim(u, cs, nd, smallr):
-    """q from u_prim as the reference's isothermal ctoprim (solve.f90:210-239)."""
+def ctoprim(u, cs, nd, smallr, cfg=None):
+    """q from u_prim as the reference's ctoprim (solve.f90:210-239): isothermal, or with
+    iso_cs < 1 the pressure from the total energy."""
     q = np.empty_like(u)
     rho = u[0]
     q[0] = np.maximum(rho, smallr)
     q[1], q[2], q[3] = u[1] / rho, u[2] / rho, u[3] / rho
-    q[4] = rho * (cs * cs)
+    if cfg is not None and cfg.iso_cs < 1:
+        ekin = 0.5 * rho * (q[1] * q[1]) + 0.5 * rho * (q[2] * q[2]) + 0.5 * rho * (q[3] * q[3])
+        q[4] = np.maximum((cfg.gamma - 1.0) * (u[4] - ekin - 0.0), cfg.smallp)
+    else:
+        q[4] = rho * (cs * cs)
     for k in range(nd):
         rd = u[5 + k]
         q[5 + k] = rd
@@ -61,7 +66,7 @@ def compare_dir(d, files):
 def test_output_files_match_reference(case):
     cfg, geom, state, final, _, meta = load_ref(case)
     m, files = load_expected(case)
-    q = ctoprim(final, state["cs"], cfg.ndust, cfg.smallr)
+    q = ctoprim(final, state["cs"], cfg.ndust, cfg.smallr, cfg)
     info = gpu.output_info(m["time"], m["units"], m["gridspace"])
     with tempfile.TemporaryDirectory() as td:
         info.base_dir = os.fsencode(td)
@@ -71,7 +76,8 @@ def test_output_files_match_reference(case):
 
 def test_units_of_reference_setups():
     """The unit sets the Python host offers equal the reference's units modules bit for bit."""
-    for case, setup in (("disk_2d", "disk_polar"), ("si_2d", "SI"), ("dustywave_dhll", "DUSTYWAVE")):
+    for case, setup in (("disk_2d", "disk_polar"), ("si_2d", "SI"), ("dustywave_dhll", "DUSTYWAVE"),
+                        ("sod_hll_s3", "SOD"), ("kh_n1_hllc", "KH"), ("cdisk_visc_n1", "disk")):
         m, _ = load_expected(case)
         assert gpu.units_of(setup) == m["units"], case
 
