@@ -305,6 +305,30 @@ def cdisk(nx=256, ny=256, ndust=1, cfl=0.5, slope_type=1, solver=C.SOLVER_HLL, s
                         eta_visc=eta)
 
 
+def sedov(n=256, cfl=0.5, slope_type=3, solver=C.SOLVER_HLLC, gamma=1.4, box_l=1.0, rho_ext=1.0,
+          p_ext=1e-5, p_cen=1.0, r_expl=None):
+    """Sedov blast (src/setups/sedov/setup.f90:1-34): adiabatic gas at rest in a square periodic
+    box, pressure P_ext + P_cen within r_expl of the centre; only the active cells are set (the
+    ghosts are filled by the first step's boundaries).  r_expl defaults to the setup module's
+    default-real 0.1 (single precision)."""
+    r = _f32(0.1) if r_expl is None else r_expl
+    cfg = C.make_cfg(nx=n, ny=n, ndust=0, iso_cs=-1, cfl=cfl, slope_type=slope_type, solver=solver,
+                     gamma=gamma, bc_kind=C.BC_PERIODIC, tstop_kind=C.TSTOP_DUSTYWAVE, force_kind=C.FORCE_NONE,
+                     inloop_kind=C.INLOOP_NONE)
+    g = grid_cartesian_2d(n, n, box_l, box_l)
+    nm = n + 4
+    q = np.zeros((5, nm * nm))
+    ix, iy = _ix_iy(n, n)
+    act = (ix >= 3) & (ix <= n + 2) & (iy >= 3) & (iy <= n + 2)
+    xn = np.abs(g["position"][0] - 0.5 * box_l)
+    yn = np.abs(g["position"][1] - 0.5 * box_l)
+    q[0, act] = rho_ext
+    q[4, act] = p_ext
+    q[4, act & (xn * xn + yn * yn < r * r)] = p_ext + p_cen
+    return cfg, g, dict(u=primtoc(q, 0, gamma), cs=np.zeros(nm * nm), sdust=None, condinit=None,
+                        fargo_velocity=None)
+
+
 def dustywave(nx=512, K=10.0, dust2gas=1.0, vx0=1e-4, drho=0.0, cs0=1.0, rho0=1.0, box_l=1.0,
               kx=1.0, cfl=0.5, slope_type=1, solver=C.SOLVER_HLL, solverdust=C.DUST_HUANG_BAI,
               gamma=1.6666667):

@@ -233,6 +233,31 @@ charging=.false.
 /
 """
 
+# Sedov blast (setups/sedov): adiabatic gas at rest, the pressure raised by P_cen within r_expl of
+# the box centre, periodic boundaries
+SEDOV = """&GRID_PARAMS
+freq_out=100000
+CFL = {cfl}
+tend = 1d10
+gamma = 1.4d0
+slope_type = {slope}
+/
+&SETUP_PARAMS
+box_l = 1.0d0
+rho_ext = 1.0d0
+P_ext = 1d-5
+P_cen = 1.0d0
+/
+&DUST_PARAMS
+/
+&GRAVITY_PARAMS
+/
+&IONISATION_PARAMS
+charging=.false.
+/
+"""
+
+
 def suite_nml(rel, **subst):
     """A namelist of the reference's own test suite (test/test-suite/<rel>), with the run length
     and output cadence replaced so that a fixed number of steps is taken."""
@@ -264,6 +289,8 @@ CASES = {
     "si_2d_s3_hllc": ("SI", 24, 24, 2, 0, 0, 2, 0, SI.format(slope=3), 25),
     "disk_2d": ("disk_polar", 24, 32, 2, 2, 0, 1, 0, DISK.format(slope=1, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 25),
     "disk_2d_relax_s2": ("disk_polar", 24, 32, 2, 2, 0, 1, 0, DISK.format(slope=2, r_relax="3.0d0", visc=".false.", alpha="1d-2"), 25),
+    "sedov_hllc_s3": ("sedov", 32, 32, 0, 0, 0, 2, 0, SEDOV.format(cfl=0.5, slope=3), 40),
+    "sedov_llf_s1": ("sedov", 40, 40, 0, 0, 0, 0, 0, SEDOV.format(cfl=0.8, slope=1), 40),
     "suite_disk_dust": ("disk_polar", 24, 32, 4, 2, 0, 1, 0,
                         suite_nml("DISK_DUST/disk_dust.nml", freq_out="100000", tend="1d10"), 30),
     "suite_si_ba": ("SI", 24, 24, 1, 0, 0, 1, 0, suite_nml("SI_BA/si.nml", freq_out="100000", tend="1d10"), 30),

@@ -54,6 +54,9 @@ def load_ref(name):
                   buffer_in=m["buffer_in"], buffer_out=m["buffer_out"], hoverr=m["hoverr"],
                   smooth_r=m["smooth_r"], alpha_visc=m["alpha_visc"], dust2gas=m.get("dust2gas", 0.0),
                   rhograin=m.get("rhograin", 0.0) / m["unit_d"])
+    elif setup == "sedov":
+        kw.update(bc_kind=C.BC_PERIODIC, tstop_kind=C.TSTOP_DUSTYWAVE, force_kind=C.FORCE_NONE,
+                  inloop_kind=C.INLOOP_NONE)
     elif setup == "KH":
         kw.update(bc_kind=C.BC_PERIODIC, tstop_kind=C.TSTOP_EPSTEIN, force_kind=C.FORCE_NONE,
                   inloop_kind=C.INLOOP_NONE, rhograin=m["rhograin"] / m["unit_d"])
