@@ -78,9 +78,10 @@ def cpu_baseline(cfg, geom, state, zones_per_step, steps):
                       f"{dt:.1f} s, OpenMP over {cores} threads"}
 
 
-def profiled_traffic(workload, kernel):
+def profiled_traffic(workload, kernel, post_tw=None):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of the disk
-    workload (FETCH_SIZE/WRITE_SIZE runs scaled by the copy calibration), or None."""
+    workload (FETCH_SIZE/WRITE_SIZE runs scaled by the copy calibration), or None.  The post
+    kernel's traffic is the one of the row segment width the run was tuned to."""
     if workload != "disk":
         return None
     path = os.path.join(ROOT, "profiles", "disk_1024x2048_n16_hbm_traffic.md")
@@ -93,6 +94,8 @@ def profiled_traffic(workload, kernel):
             vals[parts[1]] = int(parts[2])
     if kernel == "hydro" and "hydro_gas" in vals and "hydro_dust" in vals:
         return vals["hydro_gas"] + vals["hydro_dust"]
+    if kernel == "post":
+        return vals.get(f"post_tw{post_tw}")
     return vals.get(kernel)
 
 
@@ -260,7 +263,7 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profiled_traffic(args.workload, dom),
+            "traffic": profiled_traffic(args.workload, dom, g.get_tune()["post_tw"]),
             "traffic_source": "profiles/disk_1024x2048_n16_hbm_traffic.md" if args.workload == "disk" else None,
             "algorithmic_bytes": dom_bytes, "avg_ms": dom_ms}
     if learned and kt["growth"] > dom_ms:
