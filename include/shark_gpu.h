@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define SHARK_CFG_VERSION 1
+#define SHARK_CFG_VERSION 2
 #define SHARK_MAX_DUST 64
 
 enum {
@@ -95,6 +95,13 @@ typedef struct shark_cfg {
   double box_l_y, mplanet, rplanet, sigma_r0, r0_disk, frac_r_disk, buffer_in, buffer_out;
   double alpha_visc, dust2gas;
   double reserved[2];
+  /* disk_polar accretion through the outer boundary (boundaries.f90:63-94): accretion = 1 sets
+   * the two outer ghost columns and the last active column to the inflow state of the accretion
+   * rate m_acc (code units: M_acc/(365.25*24*3600)*unit_t) at the local cs, with phi_mom
+   * (only 2, every row) and L_out; box_l above is the disk's radial extent */
+  int32_t accretion, reserved_i2;
+  double m_acc, phi_mom, l_out;
+  double reserved2[8];
 } shark_cfg;
 
 typedef struct shark_ctx shark_ctx;

@@ -200,7 +200,8 @@ static void bc_shear_si(oracle* o) {
     }
 }
 
-/* disk boundaries, src/setups/disk_polar/boundaries.f90:10-62 (accretion :64-94 not built) */
+/* disk boundaries, src/setups/disk_polar/boundaries.f90:10-94, with the accretion inflow of
+ * :63-94 through the two outer ghost columns and the last active column */
 static void bc_disk(oracle* o) {
   for (int ix = o->fa; ix <= o->la; ++ix)
     for (int iy = 1; iy <= 2; ++iy) {
@@ -224,6 +225,25 @@ static void bc_disk(oracle* o) {
       for (int k = 0; k < o->nd; ++k) {
         U(IVDX(k), br) = fmin_f(U(IVDX(k), br), 0.0);
         U(IVDX(k), bl) = fmax_f(U(IVDX(k), bl), 0.0);
+      }
+    }
+  if (!o->cfg.accretion) return;
+  const shark_cfg* p = &o->cfg;
+  for (int ix = 1; ix <= 3; ++ix)
+    for (int iy = o->fay; iy <= o->lay; ++iy) {
+      int br = cidx(o, o->nxm + 1 - ix, iy);
+      double phi = (((iy - o->fay) + HALF) * (2.0 * PI)) / (double)(float)p->ny;
+      if (!(phi <= p->phi_mom * PI)) continue;
+      double vrout = o->cs[cidx(o, o->la, iy)];
+      double sigma_out = p->m_acc / vrout / p->phi_mom / PI / p->box_l;
+      double vfac = sigma_out * p->l_out * sqrt(p->mstar / 100.0) * 100.0 / p->box_l;
+      U(o->irho, br) = sigma_out;
+      U(o->ivx, br) = -(sigma_out * vrout);
+      U(o->ivy, br) = vfac;
+      for (int k = 0; k < o->nd; ++k) {
+        U(IRHOD(k), br) = p->dust2gas * sigma_out;
+        U(IVDX(k), br) = -(p->dust2gas * sigma_out * vrout);
+        U(IVDY(k), br) = p->dust2gas * sigma_out * p->l_out * sqrt(p->mstar / 100.0) * 100.0 / p->box_l;
       }
     }
 }

@@ -16,7 +16,8 @@ program ref_harness
   use commons
   use units
 #ifdef REF_DISK
-  use setup_parameters, only: uprim_condinit, mstar, r_relax, n_rel, smooth_r, hoverr, alpha_visc
+  use setup_parameters, only: uprim_condinit, mstar, r_relax, n_rel, smooth_r, hoverr, alpha_visc, &
+       box_l, accretion, M_acc, phi_mom, L_out
 #endif
 #ifdef REF_SI
   use setup_parameters, only: box_l, rho_init, omega_shear, q_shear, hoverr, eta_stream, rad0
@@ -119,6 +120,7 @@ program ref_harness
   write(ilun, '(A,L2)') 'dust_back_reaction ', dust_back_reaction
   write(ilun, '(A,ES25.17)') 'dust_ratio_min ', dust_ratio_min
   write(ilun, '(A,ES25.17)') 'rhograin ', rhograin
+  write(ilun, '(A,ES25.17)') 'dust2gas ', dust2gas
 #endif
   write(ilun, '(A,ES25.17)') 'gamma ', gamma
   write(ilun, '(A,ES25.17)') 'CFL ', CFL
@@ -140,6 +142,11 @@ program ref_harness
   write(ilun, '(A,ES25.17)') 'hoverr ', hoverr
   write(ilun, '(A,ES25.17)') 'alpha_visc ', alpha_visc
   write(ilun, '(A,L2)') 'viscosity ', viscosity
+  write(ilun, '(A,ES25.17)') 'box_l ', box_l
+  write(ilun, '(A,L2)') 'accretion ', accretion
+  write(ilun, '(A,ES25.17)') 'M_acc ', M_acc
+  write(ilun, '(A,ES25.17)') 'phi_mom ', phi_mom
+  write(ilun, '(A,ES25.17)') 'L_out ', L_out
   open(32, file=trim(outdir)//'/eta_visc.bin', form='unformatted', access='stream', status='replace')
   write(32) eta_visc
   close(32)
@@ -175,9 +182,6 @@ program ref_harness
   write(ilun, '(A,ES25.17)') 'smooth_r ', smooth_r
   write(ilun, '(A,ES25.17)') 'alpha_visc ', alpha_visc
   write(ilun, '(A,L2)') 'viscosity ', viscosity
-#if NDUST>0
-  write(ilun, '(A,ES25.17)') 'dust2gas ', dust2gas
-#endif
   open(32, file=trim(outdir)//'/eta_visc.bin', form='unformatted', access='stream', status='replace')
   write(32) eta_visc
   close(32)

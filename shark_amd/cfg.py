@@ -7,7 +7,7 @@ namelist scalars the step reads (&GRID_PARAMS, src/commons/read_params.f90:10;
 import ctypes as C
 from dataclasses import dataclass
 
-SHARK_CFG_VERSION = 1
+SHARK_CFG_VERSION = 2
 SHARK_MAX_DUST = 64
 
 GEOM_CARTESIAN, GEOM_POLAR = 0, 2
@@ -41,6 +41,8 @@ class ShARKCfg(C.Structure):
         ("sigma_r0", C.c_double), ("r0_disk", C.c_double), ("frac_r_disk", C.c_double),
         ("buffer_in", C.c_double), ("buffer_out", C.c_double), ("alpha_visc", C.c_double),
         ("dust2gas", C.c_double), ("reserved", C.c_double * 2),
+        ("accretion", C.c_int32), ("reserved_i2", C.c_int32), ("m_acc", C.c_double),
+        ("phi_mom", C.c_double), ("l_out", C.c_double), ("reserved2", C.c_double * 8),
     ]
 
 
@@ -60,6 +62,8 @@ def make_cfg(**kw) -> ShARKCfg:
     c.dust_back_reaction = 1
     c.gamma = 1.66667
     c.cfl = 0.5
+    c.phi_mom = 2.0
+    c.l_out = 1.0
     c.smallr = 1e-27
     c.smallc = 1e-27
     c.smallp = 1e-27 * 1e-27 ** 2

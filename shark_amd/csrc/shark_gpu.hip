@@ -130,6 +130,8 @@ static int validate(const shark_cfg* c) {
       c->force_kind < 0 || c->force_kind > 3 || c->inloop_kind < 0 || c->inloop_kind > 3)
     return fail(SHARK_EINVAL, "unknown setup hook kind");
   if (c->bc_kind == SHARK_BC_DISK && c->ny < 4) return fail(SHARK_EUNSUP, "disk boundaries need a 2D grid");
+  if (c->accretion && (c->bc_kind != SHARK_BC_DISK || c->iso_cs != 1))
+    return fail(SHARK_EUNSUP, "accretion is built for the isothermal polar disk's boundaries");
   if (c->cfl <= 0) return fail(SHARK_EINVAL, "CFL must be positive");
   return SHARK_OK;
 }
@@ -174,6 +176,7 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   a.r0_disk = cfg->r0_disk; a.frac_r_disk = cfg->frac_r_disk; a.buffer_in = cfg->buffer_in;
   a.buffer_out = cfg->buffer_out; a.alpha_visc = cfg->alpha_visc; a.dust2gas = cfg->dust2gas;
   a.viscosity = cfg->viscosity;
+  a.accretion = cfg->accretion; a.m_acc = cfg->m_acc; a.phi_mom = cfg->phi_mom; a.l_out = cfg->l_out;
   for (int k = 0; k < SHARK_MAX_DUST; ++k) a.K[k] = cfg->K[k];
   // launch shapes (shark_gpu_set_tune / shark_gpu_autotune change them)
   a.strip_h = 32;

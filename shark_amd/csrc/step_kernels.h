@@ -71,6 +71,8 @@ struct StepArgs {
   double box_l_y, mplanet, rplanet, sigma_r0, r0_disk, frac_r_disk, buffer_in, buffer_out;
   double alpha_visc, dust2gas;
   int viscosity;
+  int accretion;        // polar disk inflow through the outer boundary (boundaries.f90:63-94)
+  double m_acc, phi_mom, l_out;
 };
 
 // launch shapes: every shape gives the same results bit for bit, only the speed differs

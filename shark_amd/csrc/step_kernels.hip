@@ -772,6 +772,25 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
         un[v][c] = x;
       }
     }
+    if (a.accretion && ix == a.nx + 2) {
+      // fargo_scheme applies the boundaries again after the remap (fargo_scheme.f90:80), and
+      // with accretion they reset the last active column to the inflow state (as k_bc_accrete)
+      const double phi = (((iy - 3) + kHalf) * (2.0 * kPi)) / (double)(float)a.ny;
+      if (phi <= a.phi_mom * kPi) {
+        const double vrout = a.cs[cell];
+        const double sigma_out = a.m_acc / vrout / a.phi_mom / kPi / a.box_l;
+        const double sq = sqrt(a.mstar / 100.0);
+        for (int v = s; v < NV; v += NS) {
+          const bool gas = v < 5;
+          const int f = gas ? v : (v - 5) / ND, k = gas ? 0 : (v - 5) % ND;   // dust: plane f of species k
+          const double rs = gas ? sigma_out : a.dust2gas * sigma_out;
+          if ((gas && v == 0) || (!gas && f == 0)) un[v][c] = rs;
+          else if ((gas && v == 1) || (!gas && f == 1)) un[v][c] = -(rs * vrout);
+          else if ((gas && v == 2) || (!gas && f == 2)) un[v][c] = rs * a.l_out * sq * 100.0 / a.box_l;
+          (void)k;
+        }
+      }
+    }
   } else {
 #pragma unroll
     for (int v = s; v < NV; v += NS) un[v][c] = a.B[v * nc + cell];
@@ -1123,6 +1142,34 @@ static void launch_planet(const StepArgs& a, hipStream_t s) {
   if (a.force_kind == 3 && a.mplanet > 0.0) hipLaunchKernelGGL(k_planet, dim3(1), dim3(1), 0, s, a);
 }
 
+// Accretion through the outer boundary of the polar disk (disk_polar/boundaries.f90:63-94),
+// after the ghost fill: the two outer ghost columns and the last active column of every row with
+// phi <= phi_mom pi take the inflow state of the accretion rate at the last active cell's cs.
+__global__ void k_bc_accrete(StepArgs a) {
+  if (!a.sc->active) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * a.ny) return;
+  const int ix = a.nxm - t / a.ny, iy = 3 + t % a.ny;
+  const double phi = (((iy - 3) + kHalf) * (2.0 * kPi)) / (double)(float)a.ny;
+  if (!(phi <= a.phi_mom * kPi)) return;
+  const size_t nc = a.nc;
+  const int br = cell_of(a, ix, iy);
+  const double vrout = a.cs[cell_of(a, a.nx + 2, iy)];
+  const double sigma_out = a.m_acc / vrout / a.phi_mom / kPi / a.box_l;
+  const double sq = sqrt(a.mstar / 100.0);
+  a.U[br] = sigma_out;
+  a.U[nc + br] = -(sigma_out * vrout);
+  a.U[2 * nc + br] = sigma_out * a.l_out * sq * 100.0 / a.box_l;
+  const int nd = a.nd;
+  for (int k = 0; k < nd; ++k) {
+    a.U[(5 + k) * nc + br] = a.dust2gas * sigma_out;
+    a.U[(5 + nd + k) * nc + br] = -(a.dust2gas * sigma_out * vrout);
+    a.U[(5 + 2 * nd + k) * nc + br] = a.dust2gas * sigma_out * a.l_out * sq * 100.0 / a.box_l;
+  }
+}
+
+static void launch_courant_now(const StepArgs& a, hipStream_t s);
+
 void launch_bc(const StepArgs& a, hipStream_t s) {
   launch_planet(a, s);
   if (a.bc_kind == 4) {
@@ -1131,6 +1178,13 @@ void launch_bc(const StepArgs& a, hipStream_t s) {
   }
   const int n = (a.ndim == 2 ? 4 * a.nx : 0) + 4 * a.nym;
   hipLaunchKernelGGL(k_bc, dim3((n + 255) / 256), dim3(256), 0, s, a);
+  if (a.accretion && a.bc_kind == 2) {
+    hipLaunchKernelGGL(k_bc_accrete, dim3((3 * a.ny + 255) / 256), dim3(256), 0, s, a);
+    // the inflow rewrites the last active column, so this step's dt is taken here, from the
+    // filled state, as courant does after apply_boundaries (solve.f90:29-56); the candidate the
+    // previous post pass left is replaced
+    launch_courant_now(a, s);
+  }
 }
 
 template <int NDIM, int GEOM, bool GAS, int SLOPE>
@@ -1246,10 +1300,14 @@ void launch_post_part(const StepArgs& a, int part, hipStream_t s) {
   if (part == 1) { SHARK_ND_SWITCH(post1_nd) }
   else { SHARK_ND_SWITCH(post2_nd) }
 }
-void launch_courant_init(const StepArgs& a, hipStream_t s) {
-  launch_planet(a, s);
+static void launch_courant_now(const StepArgs& a, hipStream_t s) {
   SHARK_ND_SWITCH(courant_nd)
   hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, s, a.sc);
+}
+
+void launch_courant_init(const StepArgs& a, hipStream_t s) {
+  launch_planet(a, s);
+  launch_courant_now(a, s);
 }
 void launch_set_dt(const StepArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, s, a.sc);

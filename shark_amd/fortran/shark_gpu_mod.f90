@@ -9,7 +9,7 @@ module shark_gpu_mod
   use iso_c_binding
   implicit none
 
-  integer(c_int32_t), parameter :: SHARK_CFG_VERSION = 1
+  integer(c_int32_t), parameter :: SHARK_CFG_VERSION = 2
   integer, parameter :: SHARK_MAX_DUST = 64
   integer(c_int32_t), parameter :: SHARK_GEOM_CARTESIAN = 0, SHARK_GEOM_POLAR = 2
   integer(c_int32_t), parameter :: SHARK_SOLVER_LLF = 0, SHARK_SOLVER_HLL = 1, SHARK_SOLVER_HLLC = 2
@@ -35,6 +35,9 @@ module shark_gpu_mod
      real(c_double) :: box_l_y, mplanet, rplanet, sigma_r0, r0_disk, frac_r_disk, buffer_in, buffer_out
      real(c_double) :: alpha_visc, dust2gas
      real(c_double) :: reserved(2)
+     integer(c_int32_t) :: accretion, reserved_i2
+     real(c_double) :: m_acc, phi_mom, l_out
+     real(c_double) :: reserved2(8)
   end type shark_cfg
 
   ! same layout as struct shark_tune (launch shapes of the step kernels)
@@ -166,5 +169,7 @@ contains
     c%frac_r_disk = 0.0d0; c%buffer_in = 0.0d0; c%buffer_out = 0.0d0; c%alpha_visc = 0.0d0
     c%dust2gas = 0.0d0
     c%reserved = 0.0d0
+    c%accretion = 0; c%reserved_i2 = 0; c%m_acc = 0.0d0; c%phi_mom = 2.0d0; c%l_out = 1.0d0
+    c%reserved2 = 0.0d0
   end subroutine shark_cfg_defaults
 end module shark_gpu_mod

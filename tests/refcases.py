@@ -41,7 +41,10 @@ def load_ref(name):
                   inloop_kind=C.INLOOP_DISK, mstar=m["mstar"], r_relax=m["r_relax"],
                   n_rel=m["n_rel"], smooth_r=m["smooth_r"], hoverr=m["hoverr"],
                   rhograin=m.get("rhograin", 0.0) / m["unit_d"], disk_bc_mode=0,
-                  viscosity=int(m.get("viscosity", False)), alpha_visc=m.get("alpha_visc", 0.0))
+                  viscosity=int(m.get("viscosity", False)), alpha_visc=m.get("alpha_visc", 0.0),
+                  box_l=m.get("box_l", 0.0), accretion=int(m.get("accretion", False)),
+                  m_acc=m.get("M_acc", 0.0) / (365.25 * 24.0 * 3600.0) * m["unit_t"],
+                  phi_mom=m.get("phi_mom", 2.0), l_out=m.get("L_out", 1.0), dust2gas=m.get("dust2gas", 0.0))
     elif setup == "SOD":
         kw.update(bc_kind=C.BC_SOD, tstop_kind=C.TSTOP_DUSTYWAVE, force_kind=C.FORCE_NONE,
                   inloop_kind=C.INLOOP_NONE, rho_l=m["rho_l"], rho_r=m["rho_r"], p_l=m["p_l"],
