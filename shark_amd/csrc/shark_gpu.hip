@@ -130,6 +130,8 @@ static int validate(const shark_cfg* c) {
       c->force_kind < 0 || c->force_kind > 3 || c->inloop_kind < 0 || c->inloop_kind > 3)
     return fail(SHARK_EINVAL, "unknown setup hook kind");
   if (c->bc_kind == SHARK_BC_DISK && c->ny < 4) return fail(SHARK_EUNSUP, "disk boundaries need a 2D grid");
+  if (c->force_kind == SHARK_FORCE_DISK && c->mplanet > 0.0)
+    return fail(SHARK_EUNSUP, "the polar disk's planet (disk_polar/setup.f90:323-340) is not built");
   if (c->accretion && (c->bc_kind != SHARK_BC_DISK || c->iso_cs != 1))
     return fail(SHARK_EUNSUP, "accretion is built for the isothermal polar disk's boundaries");
   if (c->cfl <= 0) return fail(SHARK_EINVAL, "CFL must be positive");
