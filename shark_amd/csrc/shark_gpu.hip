@@ -771,10 +771,10 @@ extern "C" int shark_gpu_write_output(shark_ctx* c, const shark_output_info* inf
 }
 
 namespace {
-constexpr char kDumpMagic[8] = {'S', 'H', 'A', 'R', 'K', 'S', 'T', '1'};
+constexpr char kDumpMagic[8] = {'S', 'H', 'A', 'R', 'K', 'S', 'T', '2'};
 struct DumpHead {
   char magic[8];
-  int32_t nx, ny, ndust, nvar, geom, pad;
+  int32_t nx, ny, ndust, nvar, geom, scalars_bytes;   // scalars_bytes: sizeof(DevScalars) written
   int64_t ncells;
 };
 }  // namespace
@@ -792,6 +792,7 @@ extern "C" int shark_gpu_save_state(shark_ctx* c, const char* path) {
   memset(&h, 0, sizeof(h));
   memcpy(h.magic, kDumpMagic, 8);
   h.nx = a.nx; h.ny = a.ny; h.ndust = a.nd; h.nvar = a.nvar; h.geom = a.geom;
+  h.scalars_bytes = (int32_t)sizeof(DevScalars);
   h.ncells = a.nc;
   FILE* f = fopen(path, "wb");
   if (!f) return fail(SHARK_EIO, "cannot open %s: %s", path, strerror(errno));
@@ -812,7 +813,8 @@ extern "C" int shark_gpu_load_state(shark_ctx* c, const char* path) {
   DumpHead h;
   DevScalars sc;
   std::vector<double> u(n);
-  bool ok = fread(&h, sizeof(h), 1, f) == 1 && memcmp(h.magic, kDumpMagic, 8) == 0;
+  bool ok = fread(&h, sizeof(h), 1, f) == 1 && memcmp(h.magic, kDumpMagic, 8) == 0 &&
+            h.scalars_bytes == (int32_t)sizeof(DevScalars);
   if (ok && (h.nx != a.nx || h.ny != a.ny || h.ndust != a.nd || h.nvar != a.nvar || h.geom != a.geom ||
              h.ncells != a.nc)) {
     fclose(f);
