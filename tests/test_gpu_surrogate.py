@@ -2,7 +2,7 @@
 test runs in seconds), then compared with the plain FP64 coagulation on held-out cells.
 
 The ablation: the same growth call with the network (mode SURROGATE) and without it (mode FP64,
-the plain coagulation that is also its fallback and its judge).  The network must be >= 10x
+the plain coagulation that is also its fallback and its reference).  The network must be >= 10x
 faster, beat the no-change prediction by a wide margin, conserve dust mass to 1e-12, and send the
 cells it cannot handle (outside its training envelope, or a failed mass residual) to the
 fallback."""
@@ -56,7 +56,7 @@ def test_surrogate_vs_plain_fp64(trained):
         ms_p = g.kernel_ms()
     finally:
         g.close()
-    assert np.array_equal(ref, st["rhod_next"])           # the judge is the plain step itself
+    assert np.array_equal(ref, st["rhod_next"])           # the reference is the plain step itself
     err = S.mass_l1(out, ref)
     base = S.mass_l1(st["rhod"], ref)                     # no-change prediction
     mass = np.abs(out.sum(axis=0) / st["rhod"].sum(axis=0) - 1)
