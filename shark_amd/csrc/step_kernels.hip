@@ -782,12 +782,11 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
         const double sq = sqrt(a.mstar / 100.0);
         for (int v = s; v < NV; v += NS) {
           const bool gas = v < 5;
-          const int f = gas ? v : (v - 5) / ND, k = gas ? 0 : (v - 5) % ND;   // dust: plane f of species k
+          const int f = gas ? v : (v - 5) / NDL;   // dust: plane f (density, x, y, z momentum)
           const double rs = gas ? sigma_out : a.dust2gas * sigma_out;
-          if ((gas && v == 0) || (!gas && f == 0)) un[v][c] = rs;
-          else if ((gas && v == 1) || (!gas && f == 1)) un[v][c] = -(rs * vrout);
-          else if ((gas && v == 2) || (!gas && f == 2)) un[v][c] = rs * a.l_out * sq * 100.0 / a.box_l;
-          (void)k;
+          if (f == 0) un[v][c] = rs;
+          else if (f == 1) un[v][c] = -(rs * vrout);
+          else if (f == 2) un[v][c] = rs * a.l_out * sq * 100.0 / a.box_l;
         }
       }
     }
