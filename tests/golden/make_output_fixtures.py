@@ -20,7 +20,7 @@ sys.path.insert(0, HERE)
 from make_ref_fixtures import BUILD, CASES, REPO, _meta  # noqa: E402
 
 OUT_CASES = {"disk_2d": 7, "si_2d": 12, "dustywave_dhll": 3, "sod_hll_s3": 4, "kh_n1_hllc": 9,
-             "cdisk_visc_n1": 5}
+             "cdisk_visc_n1": 5, "disk_2d_logr": 6}
 
 
 def run(name, iout):

@@ -298,6 +298,8 @@ CASES = {
     "suite_s71_accrete": ("disk_polar", 32, 24, 2, 2, 0, 1, 0,
                           suite_nml("DISK_S7-1/disk_accrete.nml", freq_out="100000", tend="1d10"), 30),
     "suite_si_ba": ("SI", 24, 24, 1, 0, 0, 1, 0, suite_nml("SI_BA/si.nml", freq_out="100000", tend="1d10"), 30),
+    "disk_2d_logr": ("disk_polar", 24, 32, 2, 2, 1, 1, 0,
+                     DISK.format(slope=3, r_relax="0.3d0", visc=".true.", alpha="1d-2"), 25),
     "disk_2d_visc": ("disk_polar", 24, 32, 2, 2, 0, 1, 0,
                      DISK.format(slope=1, r_relax="0.3d0", visc=".true.", alpha="5d-2"), 25),
     "disk_2d_visc_gas": ("disk_polar", 32, 24, 0, 2, 0, 2, 0,
