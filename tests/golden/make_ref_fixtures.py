@@ -261,7 +261,13 @@ charging=.false.
 def suite_nml(rel, **subst):
     """A namelist of the reference's own test suite (test/test-suite/<rel>), with the run length
     and output cadence replaced so that a fixed number of steps is taken."""
-    text = open(os.path.join(REF, "test", "test-suite", rel)).read()
+    return test_nml(os.path.join("test-suite", rel), **subst)
+
+
+def test_nml(rel, **subst):
+    """A namelist of the reference's test directory (test/<rel>), run length and output cadence
+    replaced as in suite_nml."""
+    text = open(os.path.join(REF, "test", rel)).read()
     out = []
     for line in text.splitlines():
         key = line.split("=")[0].strip().lower()
@@ -297,6 +303,13 @@ CASES = {
                               suite_nml("DISK_EQ/disk_accrete.nml", freq_out="100000", tend="1d10"), 30),
     "suite_s71_accrete": ("disk_polar", 32, 24, 2, 2, 0, 1, 0,
                           suite_nml("DISK_S7-1/disk_accrete.nml", freq_out="100000", tend="1d10"), 30),
+    "test_polar_disk": ("disk_polar", 24, 32, 2, 2, 0, 1, 0,
+                        test_nml("polar_disk/disk.nml", freq_out="100000", tend="1d10"), 30),
+    "test_polar_disk_s3": ("disk_polar", 32, 24, 1, 2, 0, 2, 0,
+                           test_nml("polar_disk/disk_test.nml", freq_out="100000", tend="1d10"), 30),
+    "test_cartesian_disk": ("disk", 32, 32, 2, 0, 0, 1, 0,
+                            test_nml("CARTESIAN_disk/disk.nml", freq_out="100000", tend="1d10"), 30),
+    "test_si": ("SI", 32, 32, 2, 0, 0, 1, 0, test_nml("SI/si.nml", freq_out="100000", tend="1d10"), 30),
     "suite_si_ba": ("SI", 24, 24, 1, 0, 0, 1, 0, suite_nml("SI_BA/si.nml", freq_out="100000", tend="1d10"), 30),
     "disk_2d_logr": ("disk_polar", 24, 32, 2, 2, 1, 1, 0,
                      DISK.format(slope=3, r_relax="0.3d0", visc=".true.", alpha="1d-2"), 25),
