@@ -310,6 +310,13 @@ CASES = {
     "test_cartesian_disk": ("disk", 32, 32, 2, 0, 0, 1, 0,
                             test_nml("CARTESIAN_disk/disk.nml", freq_out="100000", tend="1d10"), 30),
     "test_si": ("SI", 32, 32, 2, 0, 0, 1, 0, test_nml("SI/si.nml", freq_out="100000", tend="1d10"), 30),
+    "test_kh": ("KH", 32, 32, 4, 0, 0, 2, 0, test_nml("KH/kh.nml", freq_out="100000", tend="1d10"), 30),
+    "test_sod": ("SOD", 200, 1, 0, 0, 0, 1, 0, test_nml("Sod/sod.nml", freq_out="100000"), -1),
+    "test_sod2d_x_up": ("SOD", 64, 8, 0, 0, 0, 2, 0,
+                        test_nml("Sod2d/sod_x_up.nml", freq_out="100000", tend="1d10"), 40),
+    "test_sod2d_y": ("SOD", 8, 64, 0, 0, 0, 1, 0, test_nml("Sod2d/sod_y.nml", freq_out="100000", tend="1d10"), 40),
+    "test_dustywave": ("DUSTYWAVE", 128, 1, 1, 0, 0, 1, 0,
+                       test_nml("DUSTYWAVE/dustywave.nml", freq_out="100000"), -1),
     "suite_si_ba": ("SI", 24, 24, 1, 0, 0, 1, 0, suite_nml("SI_BA/si.nml", freq_out="100000", tend="1d10"), 30),
     "disk_2d_logr": ("disk_polar", 24, 32, 2, 2, 1, 1, 0,
                      DISK.format(slope=3, r_relax="0.3d0", visc=".true.", alpha="1d-2"), 25),
