@@ -317,6 +317,7 @@ CASES = {
     "test_sod2d_y": ("SOD", 8, 64, 0, 0, 0, 1, 0, test_nml("Sod2d/sod_y.nml", freq_out="100000", tend="1d10"), 40),
     "test_dustywave": ("DUSTYWAVE", 128, 1, 1, 0, 0, 1, 0,
                        test_nml("DUSTYWAVE/dustywave.nml", freq_out="100000"), -1),
+    "test_psi": ("SI", 32, 32, 8, 0, 0, 1, 0, test_nml("PSI/psi.nml", freq_out="100000", tend="1d10"), 30),
     "suite_si_ba": ("SI", 24, 24, 1, 0, 0, 1, 0, suite_nml("SI_BA/si.nml", freq_out="100000", tend="1d10"), 30),
     "disk_2d_logr": ("disk_polar", 24, 32, 2, 2, 1, 1, 0,
                      DISK.format(slope=3, r_relax="0.3d0", visc=".true.", alpha="1d-2"), 25),
