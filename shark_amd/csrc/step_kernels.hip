@@ -683,7 +683,7 @@ DEV double tstop_of(const StepArgs& a, int k, int cell, int ix, double rho_q, do
 // (solve.f90:110-152) the pass is split: PART 1 remaps and drags (writes U, no Courant candidate),
 // PART 2 reloads U after growth, applies the in-loop hook and computes the Courant candidate.
 template <int ND, bool FARGO, int NDIM, int TW, int PART>
-__global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
+__global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
   if (!a.sc->active) return;
   constexpr int NV = 5 + 4 * ND;
   constexpr int NS = 256 / TW;
@@ -750,26 +750,32 @@ __global__ __launch_bounds__(256, (TW == 32 ? 5 : 3)) void k_post2(StepArgs a) {
               cp = cell_of(a, ixs, jp);
     const double h2 = (kHalf - eps) * (kHalf - eps);
     const double w0 = (vk > 0.0) ? eps : (1.0 - eps), w1 = (vk > 0.0) ? (1.0 - eps) : eps;
-    // all loads of the slot's variables are issued before any is used
+    // the slot's variables are remapped CH at a time: the 4·CH loads of a chunk are issued
+    // before any is used, and the small chunk keeps the kernel at 68 VGPRs, so six blocks fit a
+    // CU (the LDS limit) where issuing every load at once needed 96 VGPRs and allowed five
     constexpr int NPS = (NV + NS - 1) / NS;
-    double um[NPS], u0[NPS], u1[NPS], up[NPS];
+    constexpr int CH = 3;
 #pragma unroll
-    for (int i = 0; i < NPS; ++i) {
-      const int v = s + i * NS;
-      if (v < NV) {
-        const double* Bv = a.B + v * nc;
-        um[i] = Bv[cm]; u0[i] = Bv[c0]; u1[i] = Bv[c1]; up[i] = Bv[cp];
+    for (int i0 = 0; i0 < NPS; i0 += CH) {
+      double um[CH], u0[CH], u1[CH], up[CH];
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        const int v = s + (i0 + j) * NS;
+        if (i0 + j < NPS && v < NV) {
+          const double* Bv = a.B + v * nc;
+          um[j] = Bv[cm]; u0[j] = Bv[c0]; u1[j] = Bv[c1]; up[j] = Bv[cp];
+        }
       }
-    }
 #pragma unroll
-    for (int i = 0; i < NPS; ++i) {
-      const int v = s + i * NS;
-      if (v < NV) {
-        const double d0 = slope_limit(a.slope, u0[i] - um[i], u1[i] - u0[i]);
-        const double d1 = slope_limit(a.slope, u1[i] - u0[i], up[i] - u1[i]);
-        double x = u0[i] * w0 + u1[i] * w1;
-        x = x + kHalf * d0 * (0.25 - h2) + kHalf * d1 * (h2 - 0.25);
-        un[v][c] = x;
+      for (int j = 0; j < CH; ++j) {
+        const int v = s + (i0 + j) * NS;
+        if (i0 + j < NPS && v < NV) {
+          const double d0 = slope_limit(a.slope, u0[j] - um[j], u1[j] - u0[j]);
+          const double d1 = slope_limit(a.slope, u1[j] - u0[j], up[j] - u1[j]);
+          double x = u0[j] * w0 + u1[j] * w1;
+          x = x + kHalf * d0 * (0.25 - h2) + kHalf * d1 * (h2 - 0.25);
+          un[v][c] = x;
+        }
       }
     }
     if (a.accretion && ix == a.nx + 2) {
