@@ -47,6 +47,8 @@ WORKLOADS = {
     "c5": ("disk_1024x2048_b64_learned_growth", lambda s: s.disk_growth(nr=1024, nphi=2048, nbins=64)),
 }
 MLP_DIMS = (69, 256, 256, 256, 65)   # the learned step's layer widths
+GAS_SOLVERS = {0: "llf", 1: "hll", 2: "hllc"}                  # SOLVER (bin/Makefile:10)
+DUST_SOLVERS = {0: "huang_bai", 1: "llf", 2: "hll", 3: "hllc"}  # SOLVERDUST
 
 
 def copy_probe(nbytes=1 << 30, reps=10):
@@ -61,7 +63,27 @@ def copy_probe(nbytes=1 << 30, reps=10):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}))
 
 
-def cpu_baseline(cfg, geom, state, zones_per_step, steps):
+# the reference's step time over the CPU restatement's on the same state and step count, from
+# oracle/ref/calibrate.py (flang-built reference, the development host's cores)
+CALIBRATION = {"disk": "disk_256x512_n16", "si": "si_256x256_n8", "dustywave": "dustywave_512_n1"}
+
+
+def calibration(key):
+    path = os.path.join(ROOT, "oracle", "ref", "calibration.json")
+    if key not in CALIBRATION or not os.path.exists(path):
+        return None
+    c = json.load(open(path))
+    row = c.get(CALIBRATION[key])
+    if not row:
+        return None
+    threads = {k: v for k, v in row.items() if k.isdigit()}
+    most = max(threads, key=int)
+    return {"config": CALIBRATION[key], "host": c.get("host"),
+            "reference_s_over_port_s": {k: round(v["ratio"], 3) for k, v in threads.items()},
+            "all_cores_ratio": threads[most]["ratio"], "all_cores_threads": int(most)}
+
+
+def cpu_baseline(cfg, geom, state, zones_per_step, steps, key=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -72,10 +94,17 @@ def cpu_baseline(cfg, geom, state, zones_per_step, steps):
         o.step()
     dt = time.perf_counter() - t0
     o.close()
-    return {"value": zones_per_step * steps / dt, "unit": "zone-updates/s", "cores": cores,
-            "kind": "port",
-            "sample": f"{steps} steps of the same workload after 1 untimed step, "
-                      f"{dt:.1f} s, OpenMP over {cores} threads"}
+    out = {"value": zones_per_step * steps / dt, "unit": "zone-updates/s", "cores": cores,
+           "kind": "port",
+           "sample": f"{steps} steps of the same workload after 1 untimed step, "
+                     f"{dt:.1f} s, OpenMP over {cores} threads"}
+    cal = calibration(key)
+    if cal:
+        # the flang-built reference takes `ratio` times the restatement's time on the same steps:
+        # its own rate on this host would be about value / ratio
+        out["calibration"] = cal
+        out["reference_equivalent_value"] = out["value"] / cal["all_cores_ratio"]
+    return out
 
 
 def profiled_traffic(workload, kernel, post_tw=None):
@@ -186,7 +215,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="disk")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", action="store_true", help="(default) time the hipGraph replay")
     ap.add_argument("--events", action="store_true", help="time per-kernel event-bracketed launches "
@@ -194,6 +223,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-growth", action="store_true", help="skip the growth timing block")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the shearing-box and learned-growth sub-blocks of the default line")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -204,18 +235,97 @@ def main():
         copy_probe()
         return
 
+    from shark_amd import setups
+
+    attainable = hbm_attainable()
+    res = time_workload(args.workload, args.steps, args.warmup, tune=not args.no_tune,
+                        events=args.events, attainable=attainable)
+    cfg, geom, state = res.pop("_problem")
+    learned = args.workload == "c5"
+    out = {
+        "metric": METRIC,
+        "value": res["value"],
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+    }
+    out.update({k: v for k, v in res.items() if k not in ("value", "ms_per_step")})
+    out["hbm_attainable"] = attainable
+    if learned:
+        out["dtype"] = "f64 step; learned growth bf16 x bf16 -> f32"
+        # the oracle's FP64 coagulation needs up to ~1700 sub-steps per cell at this dt: a CPU
+        # sample of the coupled step would not finish in a bounded time
+        out["cpu_baseline"] = None
+        out["cpu_baseline_note"] = "not run for the coupled learned-growth workload"
+    elif not args.no_cpu_baseline:
+        zones_per_step = res["zones_per_step"]
+        out["cpu_baseline"] = cpu_baseline(cfg, geom, state, zones_per_step, args.cpu_steps,
+                                           key=args.workload)
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    del cfg, geom, state
+    if args.workload == "disk" and not args.no_workloads:
+        # the other workloads of the metric, each timed the same way (the 2D shearing box and the
+        # 64-bin disk with the learned growth step inside every step)
+        out["workloads"] = {}
+        for key in ("si", "c5"):
+            r = time_workload(key, min(args.steps, 50), min(args.warmup, 5), tune=not args.no_tune,
+                              attainable=attainable)
+            r.pop("_problem")
+            out["workloads"][WORKLOADS[key][0]] = r
+    if not args.no_growth and not learned:
+        out["growth"] = growth_block()
+    print(json.dumps(out))
+
+
+def hbm_attainable(nbytes=1 << 30, reps=20):
+    """On-box HBM bandwidth a plain streaming kernel attains (torch's device copy, read and fill of
+    1 GiB FP64 buffers, timed with events): the figure the kernels' traffic is also rated against."""
+    import torch
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    res = {}
+    for name, fn, moved in (("copy", lambda: b.copy_(a), 2 * nbytes), ("read", lambda: a.sum(), nbytes),
+                            ("write", lambda: b.fill_(2.0), nbytes)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name + "_GBs"] = moved * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    res["GBs"] = max(res["copy_GBs"], 0.0)
+    res["source"] = "torch device copy (read + write) of 1 GiB, 20 reps, HIP events"
+    return res
+
+
+def time_workload(key, steps, warmup, tune=True, events=False, attainable=None):
+    """Time `steps` steps of a workload after `warmup` (graph replay unless a growth step is inside
+    the step or events=True), then per-kernel times from event-bracketed launches; returns the
+    measured fields of the JSON line."""
     from shark_amd import gpu, setups
     from shark_amd.cfg import algorithmic_bytes
 
-    name, make = WORKLOADS[args.workload]
-    learned = args.workload == "c5"
+    name, make = WORKLOADS[key]
+    learned = key == "c5"
     made = make(setups)
     cfg, geom, state = made[:3]
     cells = cfg.nx * (cfg.ny if cfg.ny > 1 else 1)
     fluids = 1 + cfg.ndust
     zones_per_step = cells * fluids
     g = gpu.Step(cfg, geom, state)
-    tune = None if args.no_tune else g.autotune(reps=5)
+    tn = g.autotune(reps=5) if tune else None
     gr = net = None
     if learned:
         from shark_amd import growth as G
@@ -224,19 +334,19 @@ def main():
         gr = G.Growth(made[3], (cfg.nx + 4) * (cfg.ny + 4))
         gr.set_surrogate(net, norm)
         g.attach_growth(gr, G.SURROGATE)
-    g.run(args.warmup)
+    g.run(warmup)
 
     # the timed steps replay the step's hipGraph (the production path); per-kernel times come
     # from a second, event-bracketed pass. A growth step inside the step is launched directly.
-    use_graph = not learned and not args.events
+    use_graph = not learned and not events
     g.set_timing(not use_graph)
     t0 = time.perf_counter()
-    n, _ = g.run(args.steps)
+    n, _ = g.run(steps)
     wall = time.perf_counter() - t0
-    assert n == args.steps, (n, args.steps)
+    assert n == steps, (n, steps)
     if use_graph:
         g.set_timing(True)
-        g.run(min(args.steps, 50))
+        g.run(min(steps, 50))
     kt = {k: g.kernel_time(k)[0] for k in ("bc", "hydro", "post", "dt") + (("growth",) if learned else ())}
     g.set_timing(False)
     learned_info = None
@@ -253,9 +363,11 @@ def main():
         # and the dust densities before growth are written by the first and read by the second
         bytes_post = cells * 8 * (4 * nv + 2 * cfg.ndust)
     step_alg = algorithmic_bytes(cfg)
+    ms_step = wall / steps * 1e3
+    att = attainable["GBs"] if attainable else None
     if not cfg.fargo:
         # without FARGO the algorithmic traffic is one pass; rate the whole step against it
-        dom, dom_ms, dom_bytes = "step", wall / args.steps * 1e3, step_alg
+        dom, dom_ms, dom_bytes = "step", ms_step, step_alg
     else:
         dom = max(("hydro", "post"), key=lambda k: kt[k])
         dom_ms = kt[dom]
@@ -263,61 +375,67 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profiled_traffic(args.workload, dom, g.get_tune()["post_tw"]),
-            "traffic_source": "profiles/disk_1024x2048_n16_hbm_traffic.md" if args.workload == "disk" else None,
+            "frac_of_attainable": achieved / att if att else None,
+            "traffic": profiled_traffic(key, dom, g.get_tune()["post_tw"]),
+            "traffic_source": "profiles/disk_1024x2048_n16_hbm_traffic.md" if key == "disk" else None,
             "algorithmic_bytes": dom_bytes, "avg_ms": dom_ms}
-    if learned and kt["growth"] > dom_ms:
-        # the learned step dominates: rate its network products against the dense BF16 peak
-        # (2 flops per multiply-add over every layer, every cell incl. ghosts)
-        ncell_all = (cfg.nx + 4) * (cfg.ny + 4)
-        flops = 2.0 * ncell_all * sum(a * b for a, b in zip(MLP_DIMS[:-1], MLP_DIMS[1:]))
-        tf = flops / (kt["growth"] * 1e-3) / 1e12
-        roof = {"bound": "mfma", "kernel": "growth", "achieved": tf, "peak": 2500.0,
-                "unit": "TFLOP/s", "frac": tf / 2500.0, "traffic": None,
-                "algorithmic_flops": flops, "avg_ms": kt["growth"]}
+    ncell_all = (cfg.nx + 4) * (cfg.ny + 4)
+    mlp = None
+    if learned:
+        mlp = mlp_roofline(ncell_all)
+        if kt["growth"] > dom_ms:
+            # the learned step dominates: rate its network products against the dense BF16 peak
+            # (2 flops per multiply-add over every layer, every cell incl. ghosts)
+            flops = 2.0 * ncell_all * sum(a * b for a, b in zip(MLP_DIMS[:-1], MLP_DIMS[1:]))
+            tf = flops / (kt["growth"] * 1e-3) / 1e12
+            roof = {"bound": "mfma", "kernel": "growth", "achieved": tf, "peak": 2500.0,
+                    "unit": "TFLOP/s", "frac": tf / 2500.0, "traffic": None,
+                    "algorithmic_flops": flops, "avg_ms": kt["growth"]}
+    step_gbs = step_alg / (wall / steps) / 1e9
     out = {
-        "metric": METRIC,
-        "value": zones_per_step * args.steps / wall,
-        "unit": "zone-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
+        "value": zones_per_step * steps / wall,
+        "ms_per_step": ms_step,
+        "zones_per_step": zones_per_step,
         "config": {"workload": name, "nx": cfg.nx, "ny": cfg.ny, "ndust": cfg.ndust,
                    "fluids": fluids, "geom": "polar" if cfg.geom == 2 else "cartesian",
-                   "fargo": bool(cfg.fargo), "drag": bool(cfg.drag), "solver": "hll",
-                   "solverdust": "huang_bai", "slope_type": cfg.slope_type},
+                   "fargo": bool(cfg.fargo), "drag": bool(cfg.drag),
+                   "solver": GAS_SOLVERS[cfg.solver], "solverdust": DUST_SOLVERS[cfg.solverdust],
+                   "slope_type": cfg.slope_type},
         "roofline": roof,
         "kernels_ms": kt,
-        "tune": tune,
+        "tune": tn,
         "timed_path": "hipGraph replay, one graph per step" if use_graph else "direct launches",
-        "step_roofline": {"algorithmic_bytes": step_alg,
-                          "achieved_GBs": step_alg / (wall / args.steps) / 1e9,
-                          "frac": step_alg / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},
+        "step_roofline": {"algorithmic_bytes": step_alg, "achieved_GBs": step_gbs,
+                          "frac": step_gbs / HBM_PEAK_GBS,
+                          "frac_of_attainable": step_gbs / att if att else None},
     }
     if learned:
         out["learned_growth"] = learned_info
-        out["dtype"] = "f64 step; learned growth bf16 x bf16 -> f32"
+        out["mlp_infer_roofline"] = mlp
     g.close()
     if gr is not None:
         gr.close()
         net.close()
-    if learned:
-        # the oracle's FP64 coagulation needs up to ~1700 sub-steps per cell at this dt: a CPU
-        # sample of the coupled step would not finish in a bounded time
-        out["cpu_baseline"] = None
-        out["cpu_baseline_note"] = "not run for the coupled learned-growth workload"
-    elif not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, geom, state, zones_per_step, args.cpu_steps)
-        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
-    if not args.no_growth and not learned:
-        out["growth"] = growth_block()
-    print(json.dumps(out))
+    out["_problem"] = (cfg, geom, state)
+    return out
+
+
+def mlp_roofline(ncells, seed=5):
+    """k_mlp_infer (the fused network of the learned step) alone over `ncells` feature rows, one
+    launch, against the dense BF16 MFMA peak."""
+    import numpy as np
+    from shark_amd import surrogate as S
+    net, _, _ = S.load(os.path.join(ROOT, "shark_amd", "data", "surrogate_disk64.npz"), max_batch=ncells)
+    try:
+        x = np.random.default_rng(seed).standard_normal((ncells, MLP_DIMS[0])).astype(np.float32)
+        net.forward(x)
+        ms = net.kernel_ms()
+    finally:
+        net.close()
+    flops = 2.0 * ncells * sum(a * b for a, b in zip(MLP_DIMS[:-1], MLP_DIMS[1:]))
+    tf = flops / (ms * 1e-3) / 1e12
+    return {"kernel": "k_mlp_infer", "rows": ncells, "ms": ms, "algorithmic_flops": flops,
+            "achieved_TFLOPs": tf, "peak_TFLOPs": 2500.0, "frac": tf / 2500.0}
 
 
 if __name__ == "__main__":

@@ -934,27 +934,34 @@ static void source_terms(oracle* o) {
 }
 
 /* ---------------------------------------------------------------- FARGO */
-/* fargo_scheme, src/solver/fargo_scheme.f90:2-81, with `one`=1.0, FP64 eps, periodic slopes */
-static void fargo_scheme(oracle* o) {
-  const double dt = o->dt, forth = 0.25;
-  const int st = o->cfg.slope_type, ny = o->ny;
-  apply_boundaries(o);
-  memcpy(o->uold, o->u, (size_t)o->nc * o->nvar * 8);
-  for (int ix = o->fa; ix <= o->la; ++ix)
-    for (int iy = o->fay; iy <= o->lay; ++iy) {
-      int c = cidx(o, ix, iy), il = cidx(o, ix, iy - 1), ir = cidx(o, ix, iy + 1);
-      for (int v = 0; v < o->nvar; ++v)
-        o->duprim[(size_t)v * o->nc + c] =
-            oracle_slope_limit(st, o->uold[(size_t)v * o->nc + c] - o->uold[(size_t)v * o->nc + il],
-                               o->uold[(size_t)v * o->nc + ir] - o->uold[(size_t)v * o->nc + c]);
-    }
+/* The remap of fargo_scheme (src/solver/fargo_scheme.f90:17-77), with `one`=1.0, FP64 eps,
+ * slopes for every active cell from its y neighbours, and shifts wrapped modulo ny.  uold is the
+ * state after apply_boundaries (its y-ghost rows hold the periodic copies the slopes read);
+ * the active cells of columns fa..la are written to u.  A cell's dy is dx(:,2)*radii_c on polar
+ * grids (radii == NULL: cartesian, radius 1), its shift velocity vk_cell (fargo_velocity). */
+static void fargo_remap_core(int nxm, int nc, int ny, int nvar, int fa, int la, int st, double dt,
+                             const double* dx2, const double* radii, const double* vk_cell,
+                             const double* uold, double* duprim, double* u) {
+  const double forth = 0.25;
+  const int fay = 3, lay = ny + 2;
+  /* :20-31 slopes (every active cell) */
   #pragma omp parallel for schedule(static)
-  for (int ix = o->fa; ix <= o->la; ++ix)
-    for (int iy = o->fay; iy <= o->lay; ++iy) {
-      int c = cidx(o, ix, iy);
-      double rp = (o->cfg.geom == SHARK_GEOM_POLAR) ? o->radii_c[c] : 1.0;
-      double vk = o->fargo_v[c];
-      double dy = DX(c, 1) * rp;
+  for (int ix = fa; ix <= la; ++ix)
+    for (int iy = fay; iy <= lay; ++iy) {
+      int c = (ix - 1) + (iy - 1) * nxm, il = c - nxm, ir = c + nxm;
+      for (int v = 0; v < nvar; ++v)
+        duprim[(size_t)v * nc + c] =
+            oracle_slope_limit(st, uold[(size_t)v * nc + c] - uold[(size_t)v * nc + il],
+                               uold[(size_t)v * nc + ir] - uold[(size_t)v * nc + c]);
+    }
+  /* :34-77 remap */
+  #pragma omp parallel for schedule(static)
+  for (int ix = fa; ix <= la; ++ix)
+    for (int iy = fay; iy <= lay; ++iy) {
+      int c = (ix - 1) + (iy - 1) * nxm;
+      double rp = radii ? radii[c] : 1.0;
+      double vk = vk_cell[c];
+      double dy = dx2[c] * rp;
       double sh = fabs(vk * dt);
       int nshft = (int)(sh / dy);
       double eps = fmod(sh, dy) / dy;
@@ -963,18 +970,50 @@ static void fargo_scheme(oracle* o) {
       if (vk > 0.0) j0 = iy - nshft - 1; else j0 = iy + nshft;
       j0 = ((j0 - 3) % ny + ny) % ny + 3;
       j1 = (j0 - 3 + 1) % ny + 3;
-      int a = cidx(o, ix, j0), b = cidx(o, ix, j1);
+      int a = (ix - 1) + (j0 - 1) * nxm, b = (ix - 1) + (j1 - 1) * nxm;
       double h2 = (HALF - eps) * (HALF - eps);
-      for (int v = 0; v < o->nvar; ++v) {
-        size_t pv = (size_t)v * o->nc;
+      for (int v = 0; v < nvar; ++v) {
+        size_t pv = (size_t)v * nc;
         double un;
-        if (vk > 0.0) un = o->uold[pv + a] * eps + o->uold[pv + b] * (1.0 - eps);
-        else un = o->uold[pv + a] * (1.0 - eps) + o->uold[pv + b] * eps;
-        un = un + HALF * o->duprim[pv + a] * (forth - h2) + HALF * o->duprim[pv + b] * (h2 - forth);
-        o->u[pv + c] = un;
+        if (vk > 0.0) un = uold[pv + a] * eps + uold[pv + b] * (1.0 - eps);
+        else un = uold[pv + a] * (1.0 - eps) + uold[pv + b] * eps;
+        un = un + HALF * duprim[pv + a] * (forth - h2) + HALF * duprim[pv + b] * (h2 - forth);
+        u[pv + c] = un;
       }
     }
+}
+
+/* fargo_scheme, src/solver/fargo_scheme.f90:2-81: boundaries, remap, boundaries */
+static void fargo_scheme(oracle* o) {
   apply_boundaries(o);
+  memcpy(o->uold, o->u, (size_t)o->nc * o->nvar * 8);
+  fargo_remap_core(o->nxm, o->nc, o->ny, o->nvar, o->fa, o->la, o->cfg.slope_type, o->dt,
+                   o->dx + o->nc, (o->cfg.geom == SHARK_GEOM_POLAR) ? o->radii_c : NULL, o->fargo_v,
+                   o->uold, o->duprim, o->u);
+  apply_boundaries(o);
+}
+
+/* The remap alone (the code fargo_scheme runs) on a 2D state of nvar planes of (nx+4) x (ny+4)
+ * cells: uin's y-ghost rows are taken as periodic copies of its active rows (as the disk's
+ * apply_boundaries leaves them), every cell of uout outside the active block is a copy of uin. */
+void oracle_fargo_remap(int nx, int ny, int nvar, int slope, double dt, const double* dx2,
+                        const double* radii_c, const double* fargo_v, const double* uin, double* uout) {
+  const int nxm = nx + 4, nym = ny + 4, nc = nxm * nym;
+  const size_t n = (size_t)nc * nvar;
+  double* uold = malloc(n * 8);
+  double* du = calloc(n, 8);
+  memcpy(uold, uin, n * 8);
+  for (int v = 0; v < nvar; ++v)
+    for (int ix = 1; ix <= nxm; ++ix)
+      for (int g = 1; g <= 2; ++g) {
+        double* p = uold + (size_t)v * nc + (ix - 1);
+        p[(size_t)(g - 1) * nxm] = p[(size_t)(ny + g - 1) * nxm];        /* rows 1,2 <- ny+1,ny+2 */
+        p[(size_t)(ny + 1 + g) * nxm] = p[(size_t)(1 + g) * nxm];        /* rows ny+3,ny+4 <- 3,4 */
+      }
+  memcpy(uout, uin, n * 8);
+  fargo_remap_core(nxm, nc, ny, nvar, 3, nx + 2, slope, dt, dx2, radii_c, fargo_v, uold, du, uout);
+  free(uold);
+  free(du);
 }
 
 /* ---------------------------------------------------------------- drag */

@@ -23,6 +23,8 @@ void oracle_get_aux(const oracle* o, double* tstop, double* force, double* force
 double oracle_time(const oracle* o);
 double oracle_dt(const oracle* o);
 double oracle_slope_limit(int type, double sl, double sr);
+void oracle_fargo_remap(int nx, int ny, int nvar, int slope, double dt, const double* dx2,
+                        const double* radii_c, const double* fargo_v, const double* uin, double* uout);
 void oracle_gas_flux(int solver, double gamma, const double* ql, const double* qr, double* flx,
                      double csl, double csr, int vn, int vt);
 void oracle_dust_flux(int solver, double gamma, const double ql[4], const double qr[4],

@@ -44,6 +44,8 @@ def load():
     lib.oracle_dt.argtypes = [C.c_void_p]
     lib.oracle_slope_limit.restype = C.c_double
     lib.oracle_slope_limit.argtypes = [C.c_int, C.c_double, C.c_double]
+    lib.oracle_fargo_remap.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _P, _P, _P,
+                                       _P, _P]
     lib.oracle_gas_flux.argtypes = [C.c_int, C.c_double, _P, _P, _P, C.c_double, C.c_double,
                                     C.c_int, C.c_int]
     lib.oracle_dust_flux.argtypes = [C.c_int, C.c_double, _P, _P, _P, C.c_double, C.c_double]
@@ -58,6 +60,23 @@ def lib():
     if _LIB is None:
         _LIB = load()
     return _LIB
+
+
+def fargo_remap(u, nx, ny, slope, dt, dx2, radii_c, fargo_v):
+    """The oracle's FARGO remap alone (the code its step runs, fargo_scheme.f90:17-77) on a 2D
+    state u of shape (nvar, (ny+4)*(nx+4)); dx2, radii_c and fargo_v are per cell (ncells,).
+    The y-ghost rows of u are taken as periodic copies; cells outside the active block are
+    returned unchanged."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    nvar = u.shape[0]
+    assert u.size == nvar * (nx + 4) * (ny + 4)
+    out = np.empty_like(u)
+    c = [np.ascontiguousarray(x, dtype=np.float64) for x in (dx2, radii_c, fargo_v)]
+    for x in c:
+        assert x.size == (nx + 4) * (ny + 4)
+    lib().oracle_fargo_remap(nx, ny, nvar, slope, float(dt), _ptr(c[0]), _ptr(c[1]), _ptr(c[2]),
+                             _ptr(u), _ptr(out))
+    return out
 
 
 class Oracle:

@@ -45,9 +45,14 @@ find_src() {
 }
 
 OBJS=""
+# REF_FARGO_FLAGS: extra flags for fargo_scheme.f90 alone, e.g. "-Done=1.0d0 -fdefault-real-8" to
+# give its undeclared `one` (implicit REAL(4), never assigned, :57,69) the value 1 and its
+# undeclared `eps` (:47) double precision (-fdefault-double-8 keeps 1.0d0 literals at kind 8)
 for o in $MODOBJ; do
   src=$(find_src "$o")
-  $FC $FFLAGS -c "$src" -o "$OUT/obj/$o.o"
+  OFLAGS=""
+  [ "$o" = "fargo_scheme" ] && OFLAGS="${REF_FARGO_FLAGS:-}"
+  $FC $FFLAGS $OFLAGS -c "$src" -o "$OUT/obj/$o.o"
   OBJS="$OBJS $OUT/obj/$o.o"
 done
 HFLAGS=""

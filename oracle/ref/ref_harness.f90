@@ -10,6 +10,12 @@
 !   <out>/meta.txt   : sizes and scalars the restatement needs
 !   ./output_NNNNN/  : with a fourth argument iout, the reference's own output(iout)
 !                      (src/commons/write_output.f90:2) of the final state, after ctoprim
+!   <out>/fargo_v.bin: with REF_FARGO_SHIFTS=<file> (one signed integer per column, nx_max of
+!                      them), the fargo_velocity of every step: before each step the harness
+!                      takes the step's dt as solve does (apply_boundaries, ctoprim,
+!                      update_force_setup, courant: solve.f90:20-75) and gives column ix the
+!                      velocity sign(s)*v with |v dt| = |s| dy exactly (dy = dx(:,2)*radii_c),
+!                      so that the remap shifts that ring by exactly |s| cells (eps = 0)
 ! usage: shark_ref <namelist> <nsteps> <outdir> [iout]
 program ref_harness
   use parameters
@@ -36,9 +42,12 @@ program ref_harness
   use setup_parameters, only: M_cloud, alpha_cloud, T0_cloud, mu_gas
 #endif
   implicit none
-  integer :: nsteps, istep, ilun, iout
-  character(len=256) :: arg, outdir
+  integer :: nsteps, istep, ilun, iout, fst
+  integer(8) :: clk0, clk1, clk_rate
+  character(len=256) :: arg, outdir, fpath
   real(dp), allocatable :: dts(:)
+  integer, allocatable :: shifts(:)
+  logical :: fargo_shifts
 
   call read_params
   call setup
@@ -71,9 +80,21 @@ program ref_harness
 #endif
   close(ilun)
 
+  call get_environment_variable('REF_FARGO_SHIFTS', fpath, status=fst)
+  fargo_shifts = fst == 0 .and. len_trim(fpath) > 0
+  if (fargo_shifts) then
+     allocate(shifts(nx_max))
+     open(33, file=trim(fpath), status='old')
+     read(33, *) shifts
+     close(33)
+     open(34, file=trim(outdir)//'/fargo_v.bin', form='unformatted', access='stream', status='replace')
+  end if
+
   time = 0.0d0
+  call system_clock(clk0, clk_rate)
   if (nsteps >= 0) then
      do istep = 1, nsteps
+        if (fargo_shifts) call integer_fargo_velocities
         call solve(.false., .false.)
         time = time + dt
         dts(istep) = dt
@@ -89,6 +110,8 @@ program ref_harness
      nsteps = istep
   end if
 
+  call system_clock(clk1)
+  if (fargo_shifts) close(34)
   open(ilun, file=trim(outdir)//'/final.bin', form='unformatted', access='stream', status='replace')
   write(ilun) u_prim
   close(ilun)
@@ -111,6 +134,7 @@ program ref_harness
   write(ilun, '(A,I12)') 'ndim ', ndim
   write(ilun, '(A,I12)') 'ndust ', ndust
   write(ilun, '(A,I12)') 'nsteps ', nsteps
+  write(ilun, '(A,ES25.17)') 'loop_seconds ', real(clk1 - clk0, dp) / real(clk_rate, dp)
   write(ilun, '(A,I12)') 'slope_type ', slope_type
   write(ilun, '(A,I12)') 'iso_cs ', iso_cs
   write(ilun, '(A,L2)') 'force_kick ', force_kick
@@ -219,4 +243,43 @@ program ref_harness
 #endif
 #endif
   close(ilun)
+
+contains
+
+  subroutine integer_fargo_velocities
+    integer, external :: icell
+    integer :: ix, iy, k, m, i
+    real(dp) :: dyl, v, sh, target, sgn
+    call apply_boundaries
+    call ctoprim
+    if (force_kick) call update_force_setup
+    call courant
+    do ix = 1, nx_max
+       m = abs(shifts(ix))
+       sgn = 1.0d0
+       if (shifts(ix) < 0) sgn = -1.0d0
+       i = icell(ix, nghost + 1)
+       dyl = dx(i, 2) * radii_c(i)
+       v = 0.0d0
+       if (m > 0) then
+          target = real(m, dp) * dyl
+          v = target / dt
+          do k = 1, 64
+             sh = abs(v * dt)
+             if (sh == target .and. int(sh / dyl) == m .and. mod(sh, dyl) == 0.0d0) exit
+             if (sh < target) then
+                v = nearest(v, 1.0d0)
+             else
+                v = nearest(v, -1.0d0)
+             end if
+          end do
+          if (k > 64) v = 0.0d0
+       end if
+       do iy = 1, ny_max
+          fargo_velocity(icell(ix, iy)) = sgn * v
+       end do
+    end do
+    write(34) fargo_velocity
+  end subroutine integer_fargo_velocities
+
 end program ref_harness

@@ -101,7 +101,7 @@ static int validate(const shark_cfg* c) {
   if (c->nghost != 2) return fail(SHARK_EUNSUP, "nghost=%d: only NGHOST=2 is built", c->nghost);
   if (c->ndust < 0 || c->ndust > SHARK_MAX_DUST) return fail(SHARK_EINVAL, "ndust=%d out of range", c->ndust);
   if (!shark::nd_supported(c->ndust))
-    return fail(SHARK_EUNSUP, "ndust=%d: hydro step is instantiated for ndust in {0,1,2,3,4,8,16}", c->ndust);
+    return fail(SHARK_EUNSUP, "ndust=%d: hydro step is instantiated for ndust in {0,1,2,3,4,8,16,64}", c->ndust);
   if (c->iso_cs > 1)
     return fail(SHARK_EINVAL, "iso_cs=%d: 1 is isothermal, below 1 adiabatic", c->iso_cs);
   if (c->iso_cs != 1 && (c->geom != SHARK_GEOM_CARTESIAN || c->fargo))
@@ -123,7 +123,8 @@ static int validate(const shark_cfg* c) {
                         ((c->geom == SHARK_GEOM_CARTESIAN && c->inloop_kind == SHARK_INLOOP_CDISK) ||
                          (c->geom == SHARK_GEOM_POLAR && c->inloop_kind == SHARK_INLOOP_DISK))))
     return fail(SHARK_EUNSUP, "viscosity is built for the isothermal cartesian and polar disks (eta_visc = alpha_visc H cs)");
-  if (c->fargo && c->ny < 4) return fail(SHARK_EUNSUP, "FARGO needs a 2D grid");
+  if (c->fargo && (c->ny < 4 || c->geom != SHARK_GEOM_POLAR))
+    return fail(SHARK_EUNSUP, "FARGO is built for 2D polar grids (rings of constant radius, the disk_polar setup)");
   if (c->force_kick && !c->drag && c->ndust > 0)
     return fail(SHARK_EUNSUP, "explicit drag in the force (drag=.false. with force_kick) is not built");
   if (c->bc_kind < 0 || c->bc_kind > 4 || c->tstop_kind < 0 || c->tstop_kind > 4 ||

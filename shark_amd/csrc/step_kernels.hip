@@ -632,6 +632,24 @@ DEV double courant_cell(const StepArgs& a, const double* un, int ix, int iy, int
   return dt;
 }
 
+// The inflow value of the polar disk's accretion boundary (disk_polar/boundaries.f90:63-94) for
+// plane v of the last active column in row iy, as k_bc_accrete writes it; false where the row
+// takes no inflow (phi > phi_mom pi) or the plane is not rewritten (vz, E and the dust vz).
+DEV bool accretion_inflow(const StepArgs& a, int v, int iy, double& out) {
+  const double phi = (((iy - 3) + kHalf) * (2.0 * kPi)) / (double)(float)a.ny;
+  if (!(phi <= a.phi_mom * kPi)) return false;
+  const bool gas = v < 5;
+  const int f = gas ? v : (v - 5) / (a.nd > 0 ? a.nd : 1);   // dust: density, x, y, z plane
+  if (f > 2) return false;
+  const double vrout = a.cs[cell_of(a, a.nx + 2, iy)];
+  const double sigma_out = a.m_acc / vrout / a.phi_mom / kPi / a.box_l;
+  const double rs = gas ? sigma_out : a.dust2gas * sigma_out;
+  if (f == 0) out = rs;
+  else if (f == 1) out = -(rs * vrout);
+  else out = rs * a.l_out * sqrt(a.mstar / 100.0) * 100.0 / a.box_l;
+  return true;
+}
+
 DEV unsigned long long wave_min_u64(unsigned long long v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) {
@@ -781,19 +799,9 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
     if (a.accretion && ix == a.nx + 2) {
       // fargo_scheme applies the boundaries again after the remap (fargo_scheme.f90:80), and
       // with accretion they reset the last active column to the inflow state (as k_bc_accrete)
-      const double phi = (((iy - 3) + kHalf) * (2.0 * kPi)) / (double)(float)a.ny;
-      if (phi <= a.phi_mom * kPi) {
-        const double vrout = a.cs[cell];
-        const double sigma_out = a.m_acc / vrout / a.phi_mom / kPi / a.box_l;
-        const double sq = sqrt(a.mstar / 100.0);
-        for (int v = s; v < NV; v += NS) {
-          const bool gas = v < 5;
-          const int f = gas ? v : (v - 5) / NDL;   // dust: plane f (density, x, y, z momentum)
-          const double rs = gas ? sigma_out : a.dust2gas * sigma_out;
-          if (f == 0) un[v][c] = rs;
-          else if (f == 1) un[v][c] = -(rs * vrout);
-          else if (f == 2) un[v][c] = rs * a.l_out * sq * 100.0 / a.box_l;
-        }
+      for (int v = s; v < NV; v += NS) {
+        double x;
+        if (accretion_inflow(a, v, iy, x)) un[v][c] = x;
       }
     }
   } else {
@@ -1150,11 +1158,14 @@ static void launch_planet(const StepArgs& a, hipStream_t s) {
 // Accretion through the outer boundary of the polar disk (disk_polar/boundaries.f90:63-94),
 // after the ghost fill: the two outer ghost columns and the last active column of every row with
 // phi <= phi_mom pi take the inflow state of the accretion rate at the last active cell's cs.
-__global__ void k_bc_accrete(StepArgs a) {
+// X is the state written (U at the start of the step); ncol columns from the right edge are
+// written, skipping the first col0 (FARGO's own boundary call before its remap,
+// fargo_scheme.f90:15, writes only the last active column of B: col0 2, ncol 1).
+__global__ void k_bc_accrete(StepArgs a, double* X, int col0, int ncol) {
   if (!a.sc->active) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 3 * a.ny) return;
-  const int ix = a.nxm - t / a.ny, iy = 3 + t % a.ny;
+  if (t >= ncol * a.ny) return;
+  const int ix = a.nxm - col0 - t / a.ny, iy = 3 + t % a.ny;
   const double phi = (((iy - 3) + kHalf) * (2.0 * kPi)) / (double)(float)a.ny;
   if (!(phi <= a.phi_mom * kPi)) return;
   const size_t nc = a.nc;
@@ -1162,14 +1173,14 @@ __global__ void k_bc_accrete(StepArgs a) {
   const double vrout = a.cs[cell_of(a, a.nx + 2, iy)];
   const double sigma_out = a.m_acc / vrout / a.phi_mom / kPi / a.box_l;
   const double sq = sqrt(a.mstar / 100.0);
-  a.U[br] = sigma_out;
-  a.U[nc + br] = -(sigma_out * vrout);
-  a.U[2 * nc + br] = sigma_out * a.l_out * sq * 100.0 / a.box_l;
+  X[br] = sigma_out;
+  X[nc + br] = -(sigma_out * vrout);
+  X[2 * nc + br] = sigma_out * a.l_out * sq * 100.0 / a.box_l;
   const int nd = a.nd;
   for (int k = 0; k < nd; ++k) {
-    a.U[(5 + k) * nc + br] = a.dust2gas * sigma_out;
-    a.U[(5 + nd + k) * nc + br] = -(a.dust2gas * sigma_out * vrout);
-    a.U[(5 + 2 * nd + k) * nc + br] = a.dust2gas * sigma_out * a.l_out * sq * 100.0 / a.box_l;
+    X[(5 + k) * nc + br] = a.dust2gas * sigma_out;
+    X[(5 + nd + k) * nc + br] = -(a.dust2gas * sigma_out * vrout);
+    X[(5 + 2 * nd + k) * nc + br] = a.dust2gas * sigma_out * a.l_out * sq * 100.0 / a.box_l;
   }
 }
 
@@ -1184,7 +1195,7 @@ void launch_bc(const StepArgs& a, hipStream_t s) {
   const int n = (a.ndim == 2 ? 4 * a.nx : 0) + 4 * a.nym;
   hipLaunchKernelGGL(k_bc, dim3((n + 255) / 256), dim3(256), 0, s, a);
   if (a.accretion && a.bc_kind == 2) {
-    hipLaunchKernelGGL(k_bc_accrete, dim3((3 * a.ny + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_bc_accrete, dim3((3 * a.ny + 255) / 256), dim3(256), 0, s, a, a.U, 0, 3);
     // the inflow rewrites the last active column, so this step's dt is taken here, from the
     // filled state, as courant does after apply_boundaries (solve.f90:29-56); the candidate the
     // previous post pass left is replaced
@@ -1276,7 +1287,11 @@ static void post_nd_tw(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL((k_post2<ND, false, 1, TW, PART>), grid, dim3(256), 0, s, a);
   } else {
     dim3 grid((a.nx + TW - 1) / TW, a.ny);
-    if (a.fargo && PART != 2) hipLaunchKernelGGL((k_post2<ND, true, 2, TW, PART>), grid, dim3(256), 0, s, a);
+    if (a.fargo && PART != 2) {
+      if (a.accretion && a.bc_kind == 2)
+        hipLaunchKernelGGL(k_bc_accrete, dim3((a.ny + 255) / 256), dim3(256), 0, s, a, a.B, 2, 1);
+      hipLaunchKernelGGL((k_post2<ND, true, 2, TW, PART>), grid, dim3(256), 0, s, a);
+    }
     else hipLaunchKernelGGL((k_post2<ND, false, 2, TW, PART>), grid, dim3(256), 0, s, a);
   }
 }

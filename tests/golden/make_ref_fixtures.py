@@ -342,9 +342,55 @@ CASES = {
                     CDISK.format(cfl=0.4, slope=3, visc=".false.", smooth="2.0d0", alpha="1d-2", mplanet="0.0d0"), 30),
     "cdisk_visc_gas": ("disk", 32, 24, 0, 0, 0, 1, 0,
                        CDISK.format(cfl=0.5, slope=2, visc=".true.", smooth="0.5d0", alpha="5d-2", mplanet="0.0d0"), 30),
+    # 100 steps: the shearing box, the polar disk and the PSI namelist's 8 species
+    "si_2d_100": ("SI", 32, 32, 2, 0, 0, 1, 0, SI.format(slope=1), 100),
+    "disk_2d_100": ("disk_polar", 32, 48, 4, 2, 0, 1, 0,
+                    DISK.format(slope=1, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 100),
+    "test_psi_100": ("SI", 32, 32, 8, 0, 0, 1, 0, test_nml("PSI/psi.nml", freq_out="100000", tend="1d10"), 100),
     "cdisk_planet_n1": ("disk", 32, 32, 1, 0, 0, 1, 1,
                         CDISK.format(cfl=0.5, slope=1, visc=".true.", smooth="1.0d0", alpha="1d-2",
                                      mplanet="3.0d-3"), 40),
+}
+
+# FARGO runs: the reference's fargo_scheme as written is undefined (`one` undeclared and never
+# assigned, fargo_scheme.f90:57,69; `eps` implicit REAL(4), :47), so these cases build it with
+# `one` = 1 and double-precision implicit reals (REF_FARGO_FLAGS, applied to that file alone).
+# "shifts": one signed integer per column (nx+4 of them); the harness then sets each column's
+# fargo_velocity before every step so that the ring shifts by exactly that many cells (eps = 0;
+# the velocities used are stored as fargo_v, one row per step).  Without "shifts" the setup's own
+# velocity -Omega r is used; the reference then leaves the slopes of the last two active columns
+# and of the last two rows unset (:20-31), so those fixtures are compared on the cells whose
+# remap reads only set slopes.
+FARGO_FLAGS = "-Done=1.0d0 -fdefault-real-8 -fdefault-double-8"
+DISK_FARGO = DISK.replace("force_kick =.true.\n", "force_kick =.true.\nfargo = .true.\n")
+assert DISK_FARGO != DISK
+
+
+def _shifts(nx, ny, seed):
+    rng = np.random.default_rng(seed)
+    m = [0, 0]
+    for _ in range(nx):
+        mag = int(rng.choice([0, 1, 2, 4, 8, 16, ny // 2, ny - 1]))
+        m.append(mag * int(rng.choice([-1, 1])))
+    return m + [0, 0]
+
+
+FARGO_CASES = {
+    "fargo_int_disk": ("disk_polar", 24, 32, 2, 2, 0, 1, 0,
+                       DISK_FARGO.format(slope=1, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 6,
+                       dict(shifts=_shifts(24, 32, 1))),
+    "fargo_int_disk_s3": ("disk_polar", 20, 48, 4, 2, 0, 2, 0,
+                          DISK_FARGO.format(slope=3, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 6,
+                          dict(shifts=_shifts(20, 48, 2))),
+    "fargo_frac_disk": ("disk_polar", 24, 32, 2, 2, 0, 1, 0,
+                        DISK_FARGO.format(slope=1, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 1,
+                        dict()),
+    "fargo_frac_disk_s2": ("disk_polar", 28, 40, 4, 2, 0, 1, 0,
+                           DISK_FARGO.format(slope=2, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 1,
+                           dict()),
+    "fargo_frac_disk_s3": ("disk_polar", 24, 36, 3, 2, 0, 2, 0,
+                           DISK_FARGO.format(slope=3, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 1,
+                           dict()),
 }
 
 # Runs of the reference that are not fixtures: its Collapse_1D step turns the first active cell into
@@ -372,21 +418,33 @@ def _meta(path):
     return m
 
 
-def run_case(name, spec, force_build=False):
+def run_case(name, spec, force_build=False, fargo=None):
     setup, nx, ny, nd, geom, gs, solver, sdust, nml, nsteps = spec[:10]
     extra = list(spec[10:])
     bname = f"{setup}_{nx}x{ny}_n{nd}_g{geom}{gs}_s{solver}{sdust}" + ("_grav" if extra else "")
+    benv = dict(os.environ)
+    if fargo is not None:
+        bname += "_fargofix"
+        benv["REF_FARGO_FLAGS"] = FARGO_FLAGS
+    benv["REF_NAME"] = bname
     exe = os.path.join(REPO, "oracle", "_ref", bname, "shark_ref")
     if force_build or not os.path.exists(exe):
         subprocess.run([BUILD, setup, str(nx), str(ny), str(nd), str(geom), str(gs), str(solver),
                         str(sdust)] + extra, check=True, stdout=subprocess.DEVNULL,
-                       stderr=subprocess.DEVNULL, env=dict(os.environ, REF_NAME=bname))
+                       stderr=subprocess.DEVNULL, env=benv)
+    fargo_v = None
     with tempfile.TemporaryDirectory() as td:
         nmlp = os.path.join(td, "run.nml")
         open(nmlp, "w").write(nml)
         env = dict(os.environ, OMP_NUM_THREADS="1")
+        if fargo and "shifts" in fargo:
+            sp = os.path.join(td, "shifts.txt")
+            open(sp, "w").write(" ".join(str(x) for x in fargo["shifts"]) + "\n")
+            env["REF_FARGO_SHIFTS"] = sp
         subprocess.run([exe, nmlp, str(nsteps), td], check=True, cwd=td, env=env,
                        stdout=subprocess.DEVNULL)
+        if fargo and "shifts" in fargo:
+            fargo_v = np.fromfile(os.path.join(td, "fargo_v.bin"))
         m = _meta(os.path.join(td, "meta.txt"))
         nc, nv, ndim = m["ncells"], m["nvar"], m["ndim"]
         ic = np.fromfile(os.path.join(td, "ic.bin")).reshape(nv, nc)
@@ -425,6 +483,9 @@ def run_case(name, spec, force_build=False):
     if setup == "disk_polar":
         arrs["condinit"] = take(nc * nv, (nv, nc))
     assert off == aux.size, (name, off, aux.size)
+    if fargo_v is not None:
+        arrs["fargo_v"] = fargo_v.reshape(nsteps, nc)
+        arrs["shifts"] = np.array(fargo["shifts"])
     m.update(setup=setup, geom=geom, gridspace=gs, solver=solver, solverdust=sdust,
              namelist=nml, case=name)
     np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), meta=json.dumps(m), **arrs)
@@ -458,9 +519,12 @@ def main(argv):
     if argv[1:] == ["--check-collapse"]:
         check_collapse()
         return
-    names = argv[1:] or list(CASES)
+    names = argv[1:] or list(CASES) + list(FARGO_CASES)
     for n in names:
-        m = run_case(n, CASES[n])
+        if n in FARGO_CASES:
+            m = run_case(n, FARGO_CASES[n][:10], fargo=FARGO_CASES[n][10])
+        else:
+            m = run_case(n, CASES[n])
         print(n, "steps", m["nsteps"], "ncells", m["ncells"], "nvar", m["nvar"])
 
 

@@ -10,7 +10,12 @@ from shark_amd import cfg as C
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 REF_CASES = sorted(f[4:-4] for f in os.listdir(GOLDEN)
                    if f.startswith("ref_") and f.endswith(".npz")
-                   and not f.startswith(("ref_growth_", "ref_charge", "ref_output_", "ref_marchand")))
+                   and not f.startswith(("ref_growth_", "ref_charge", "ref_output_", "ref_marchand",
+                                         "ref_fargo_")))
+# FARGO fixtures (make_ref_fixtures.py FARGO_CASES): integer shifts set before every step, or the
+# setup's own shift velocity for one step, compared where the reference's remap is defined
+FARGO_REF_CASES = sorted(f[4:-4] for f in os.listdir(GOLDEN)
+                         if f.startswith("ref_fargo_") and f.endswith(".npz"))
 
 
 def _K(nml):
@@ -91,3 +96,61 @@ def rel_l1(a, b, mask=None):
     num = np.abs(a - b).sum(axis=1)
     den = np.abs(b).sum(axis=1)
     return np.where(den > 0, num / np.where(den > 0, den, 1.0), num)
+
+
+def run_fargo_fixture(name, make, get_u):
+    """Run a FARGO fixture through a stepper: make(cfg, geom, state) builds it (oracle or GPU),
+    get_u(stepper) returns its u_prim.  Integer-shift fixtures restart from the previous state
+    with that step's fargo_velocity before every step (the reference harness sets it per step).
+    Returns (dts, u, final, mask): mask selects the cells to compare (all cells for integer
+    shifts; for the one-step fixtures with the setup's velocity, the active cells whose remap
+    reads only the slopes the reference sets, fargo_scheme.f90:20-31)."""
+    cfg, geom, state, final, dts, meta = load_ref(name)
+    z = np.load(os.path.join(GOLDEN, f"ref_{name}.npz"))
+    st = dict(state)
+    got = []
+    if "fargo_v" in z:
+        for s in range(len(dts)):
+            st["fargo_velocity"] = z["fargo_v"][s]
+            x = make(cfg, geom, st)
+            got.append(x.step())
+            st["u"] = get_u(x)
+        mask = np.ones(final.shape[1], bool)
+    else:
+        x = make(cfg, geom, st)
+        got = [x.step() for _ in range(len(dts))]
+        st["u"] = get_u(x)
+        mask = fargo_defined_cells(cfg, geom, state["fargo_velocity"], dts[0])
+    return np.array(got), st["u"], final, mask
+
+
+def fargo_defined_cells(cfg, geom, vk, dt):
+    """Active cells of columns 3..nx whose two source rows (fargo_scheme.f90:46-67, one wrap) lie
+    in rows 3..ny, where the reference has set the slopes."""
+    nxm, nym = cfg.nx + 4, cfg.ny + 4
+    mask = np.zeros((nym, nxm), bool)
+    dx2 = geom["dx"][1].reshape(nym, nxm)
+    rc = geom["radii_c"].reshape(nym, nxm)
+    v = np.asarray(vk).reshape(nym, nxm)
+    ny = cfg.ny
+    for iy in range(3, ny + 3):
+        for ix in range(3, cfg.nx + 1):
+            c = (iy - 1, ix - 1)
+            dy = float(dx2[c]) * float(rc[c])
+            nshft = int(abs(float(v[c]) * dt) / dy)
+            if v[c] > 0.0:
+                j = iy - nshft - 1
+                if j < 3:
+                    j += ny
+                jp = j + 1
+                if jp < 3:
+                    jp += ny
+            else:
+                j = iy + nshft
+                if j > ny + 2:
+                    j -= ny
+                jp = j + 1
+                if jp > ny + 2:
+                    jp -= ny
+            mask[c] = 3 <= j <= ny and 3 <= jp <= ny
+    return mask.reshape(-1)

@@ -55,7 +55,12 @@ def test_create_rejects_out_of_scope_configs_without_gpu():
                      (dict(nx=64, ndust=5), b"ndust=5"),
                      (dict(nx=64, ndust=1, slope_type=7), b"slope limiter"),
                      (dict(nx=64, ndust=1, viscosity=1), b"viscosity"),
-                     (dict(nx=64, ndust=1, fargo=1), b"FARGO")]:
+                     (dict(nx=64, ndust=1, fargo=1), b"FARGO"),
+                     # FARGO shifts rings of constant radius: polar grids only (the cartesian
+                     # disk's hooks would otherwise run in a kernel that leaves them out)
+                     (dict(nx=16, ny=16, ndust=1, fargo=1, force_kick=1, force_kind=C.FORCE_CDISK,
+                           inloop_kind=C.INLOOP_CDISK), b"FARGO"),
+                     (dict(nx=16, ny=16, ndust=1, fargo=1), b"polar")]:
         c = C.make_cfg(**kw)
         assert lib.shark_gpu_create(ctypes.byref(c), ctypes.byref(h)) != 0
         assert frag in lib.shark_gpu_last_error(), lib.shark_gpu_last_error()

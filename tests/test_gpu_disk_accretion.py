@@ -9,8 +9,11 @@ from shark_amd import setups
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("fargo,nd", [(False, 2), (True, 4)])
-def test_gpu_accreting_disk_matches_oracle(fargo, nd):
+@pytest.mark.parametrize("fargo,nd,phi_mom", [(False, 2, 2.0), (True, 4, 2.0), (True, 4, 1.0),
+                                             (False, 2, 1.0)])
+def test_gpu_accreting_disk_matches_oracle(fargo, nd, phi_mom):
+    """phi_mom < 2 feeds only part of the outer edge; with FARGO the rows left out are remapped
+    from the column as the boundaries before the remap left it (fargo_scheme.f90:15)."""
     import pyoracle
     from shark_amd import gpu
     cfg, g, st = setups.disk(nr=128, nphi=192, nd=nd, fargo=fargo)
@@ -18,7 +21,8 @@ def test_gpu_accreting_disk_matches_oracle(fargo, nd):
     cfg.m_acc = 1e-6
     cfg.box_l = 2.1
     cfg.dust2gas = 0.01
-    print(f"accreting polar disk 128x192 nd={nd} fargo={fargo}", flush=True)
+    cfg.phi_mom = phi_mom
+    print(f"accreting polar disk 128x192 nd={nd} fargo={fargo} phi_mom={phi_mom}", flush=True)
     o = pyoracle.Oracle(cfg, g, st)
     dts_o = [o.step() for _ in range(25)]
     s = gpu.Step(cfg, g, st)

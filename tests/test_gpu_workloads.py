@@ -43,6 +43,22 @@ def test_disk_fargo_reduced(frame):
     _compare(*setups.disk(nr=96, nphi=128, nd=16, frame=frame), nsteps=30)
 
 
+def test_si_box_100_steps():
+    """The 8-species shearing box (the 1024^2 workload's setup at 256^2) after 100 steps: every dt
+    and value bit for bit."""
+    from shark_amd import setups
+    print("si_box 256x256 nd=8, 100 steps", flush=True)
+    _compare(*setups.si_box(n=256, nd=8), nsteps=100)
+
+
+def test_disk_fargo_100_steps():
+    """The 16-species FARGO disk (the 1024 x 2048 workload's setup at 256 x 512) after 100 steps:
+    every dt and value bit for bit."""
+    from shark_amd import setups
+    print("disk 256x512 nd=16 fargo, 100 steps", flush=True)
+    _compare(*setups.disk(nr=256, nphi=512, nd=16), nsteps=100)
+
+
 def test_disk_relaxation_exp_tolerance():
     """The in-loop relaxation calls exp; ROCm's and glibc's exp may differ by an ulp, so this
     case is held to relative L1 <= 1e-12 instead of bitwise."""
