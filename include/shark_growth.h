@@ -99,6 +99,22 @@ int shark_charge_marchand(const shark_marchand_params* p, int64_t ncells, int nb
                           const double* mdust, double* psi_old, double* zd, double* ni, double* ne,
                           double* cond, double* gamma_d);
 
+/* shark_charge_marchand without any allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It has no
+ * counterpart in the reference.  Every array is a device pointer the caller owns (same layouts as
+ * shark_charge_marchand); the two kernels are enqueued on `stream` (a hipStream_t, NULL: the
+ * default stream) and the call returns without waiting: synchronise before reading the outputs.
+ * The per-(bin, cell) scratch is a static device array of SHARK_CHARGE_NOALLOC_MAX_PAIRS pairs
+ * (24 MiB, held by each GPU the library runs on until the process ends); ncells * nbins beyond it
+ * is rejected (SHARK_GROWTH_EINVAL: call on chunks of cells).  All calls on one GPU share that
+ * array: calls that overlap in time on one GPU (on different streams, or from different host
+ * threads) race.  cond and gamma_d are required when resistivities is set and unused otherwise. */
+#define SHARK_CHARGE_NOALLOC_MAX_PAIRS (1 << 20)
+int shark_charge_marchand_noalloc(const shark_marchand_params* p, int64_t ncells, int nbins,
+                                  const double* rho, const double* rhod, const double* T, const double* cs,
+                                  const double* sdust, const double* mdust, double* psi_old, double* zd,
+                                  double* ni, double* ne, double* cond, double* gamma_d, void* stream);
+
 int shark_growth_charge(shark_growth* g, double zeta, double mu_ions, int electrons, double* zmean,
                         double* xion);
 /* Per-cell growth intervals (ncells, >= 0) used instead of run's dt; NULL returns to dt. */

@@ -27,6 +27,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "../../include/shark_growth.h"
@@ -1729,16 +1730,9 @@ int shark_growth_download(shark_growth* g, double* rhod, int64_t* nsub) {
   return 0;
 }
 
-int shark_charge_marchand(const shark_marchand_params* p, int64_t nc, int nb, const double* rho,
-                          const double* rhod, const double* T, const double* cs, const double* sdust,
-                          const double* mdust, double* psi_old, double* zd, double* ni, double* ne,
-                          double* cond, double* gamma_d) {
-  if (!p || !rho || !rhod || !T || !cs || !sdust || !mdust || !psi_old || !zd || !ni || !ne)
-    return gfail(SHARK_GROWTH_EINVAL, "shark_charge_marchand: null argument");
-  if (nc < 1 || nb < 1 || nb > MAXB) return gfail(SHARK_GROWTH_EINVAL, "shark_charge_marchand: need ncells >= 1 and 1 <= nbins <= 64");
-  if (!(p->zeta > 0.0) || !(p->mu_ions > 0.0) || !(p->stickeff_el > 0.0) || p->nitermax_ionis < 1)
-    return gfail(SHARK_GROWTH_EINVAL, "shark_charge_marchand: zeta, mu_ions, stickeff_el and nitermax_ionis must be positive");
-  // the global initial guess (ionisation.f90:34-52), on the host
+// Scalar arguments of the Marchand kernels, with the global initial guess of psi
+// (ionisation.f90:34-52) computed on the host; the array pointers are left null.
+static void marchand_args(const shark_marchand_params* p, int64_t nc, int nb, MArgs* out) {
   const double epsone = 0.99999;
   const double thetai = p->stickeff_el * sqrt(p->mu_ions * kMH / kMel);
   double conv = 1e4, psi0 = -2.5;
@@ -1750,12 +1744,25 @@ int shark_charge_marchand(const shark_marchand_params* p, int64_t nc, int nb, co
     conv = fabs(fpsi);
     psi0 = psi;
   }
-  MArgs a;
+  MArgs& a = *out;
   memset(&a, 0, sizeof(a));
   a.nc = nc; a.nb = nb; a.nitermax = p->nitermax_ionis; a.res = p->resistivities;
   a.thetai = thetai; a.psi0 = psi0; a.x = p->zeta; a.mu_ions = p->mu_ions; a.eps_ionis = p->epsilon_ionis;
   a.B0 = p->B_0_lee; a.rhograin = p->rhograin; a.gamma = p->gamma;
   a.unit_l = p->unit_l; a.unit_d = p->unit_d; a.unit_m = p->unit_m; a.unit_v = p->unit_v; a.unit_nh = p->unit_nh;
+}
+
+int shark_charge_marchand(const shark_marchand_params* p, int64_t nc, int nb, const double* rho,
+                          const double* rhod, const double* T, const double* cs, const double* sdust,
+                          const double* mdust, double* psi_old, double* zd, double* ni, double* ne,
+                          double* cond, double* gamma_d) {
+  if (!p || !rho || !rhod || !T || !cs || !sdust || !mdust || !psi_old || !zd || !ni || !ne)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_charge_marchand: null argument");
+  if (nc < 1 || nb < 1 || nb > MAXB) return gfail(SHARK_GROWTH_EINVAL, "shark_charge_marchand: need ncells >= 1 and 1 <= nbins <= 64");
+  if (!(p->zeta > 0.0) || !(p->mu_ions > 0.0) || !(p->stickeff_el > 0.0) || p->nitermax_ionis < 1)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_charge_marchand: zeta, mu_ions, stickeff_el and nitermax_ionis must be positive");
+  MArgs a;
+  marchand_args(p, nc, nb, &a);
   const size_t n1 = (size_t)nc * 8, nn = (size_t)nc * nb * 8;
   // one device block: rho T cs psi ni ne | cond(6) | rhod sdust mdust zd gd tau alpha nk
   double* d = nullptr;
@@ -1792,6 +1799,82 @@ int shark_charge_marchand(const shark_marchand_params* p, int64_t nc, int nb, co
   }
   (void)hipFree(d);
   GCHK(e);
+  return 0;
+}
+
+// ---- Marchand charging that allocates nothing (shark_charge_marchand_noalloc)
+// The per-(bin, cell) scratch of k_mcharge_pre / k_mcharge (tau, alpha, n_k: 24 B per pair) lives
+// in this file-scope device array, 24 MiB, which every GPU the library runs on holds until the
+// process ends.  Host code never passes the array's name to a kernel: in host code it is a
+// stand-in, not a device address.  Each device's address is looked up once, with that device
+// current, and kept in g_scratch_dev.
+static __device__ __align__(256) double scratch[3 * SHARK_CHARGE_NOALLOC_MAX_PAIRS];
+static constexpr int kMaxDevices = 64;
+static double* g_scratch_dev[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// The scratch array's address on the current device (looked up on the device's first call).
+static int scratch_on_current_device(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return gfail(SHARK_GROWTH_EHIP, "shark_charge_marchand_noalloc: hipGetDevice failed");
+  if (dev < 0 || dev >= kMaxDevices)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_charge_marchand_noalloc: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      char msg[256];
+      snprintf(msg, sizeof(msg), "shark_charge_marchand_noalloc: hipGetSymbolAddress(scratch) failed: %s",
+               hipGetErrorString(e));
+      return gfail(SHARK_GROWTH_EHIP, msg);
+    }
+    g_scratch_dev[dev] = (double*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+int shark_charge_marchand_noalloc(const shark_marchand_params* p, int64_t nc, int nb, const double* rho,
+                                  const double* rhod, const double* T, const double* cs, const double* sdust,
+                                  const double* mdust, double* psi_old, double* zd, double* ni, double* ne,
+                                  double* cond, double* gamma_d, void* stream) {
+  if (!p || !rho || !rhod || !T || !cs || !sdust || !mdust || !psi_old || !zd || !ni || !ne)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_charge_marchand_noalloc: null argument");
+  if (nc < 1 || nb < 1 || nb > MAXB)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_charge_marchand_noalloc: need ncells >= 1 and 1 <= nbins <= 64");
+  if (nc * (int64_t)nb > SHARK_CHARGE_NOALLOC_MAX_PAIRS) {
+    char msg[256];
+    snprintf(msg, sizeof(msg),
+             "shark_charge_marchand_noalloc: ncells * nbins = %lld exceeds the static scratch of %lld pairs "
+             "(call it on chunks of cells, or use shark_charge_marchand)",
+             (long long)(nc * (int64_t)nb), (long long)SHARK_CHARGE_NOALLOC_MAX_PAIRS);
+    return gfail(SHARK_GROWTH_EINVAL, msg);
+  }
+  if (!(p->zeta > 0.0) || !(p->mu_ions > 0.0) || !(p->stickeff_el > 0.0) || p->nitermax_ionis < 1)
+    return gfail(SHARK_GROWTH_EINVAL, "shark_charge_marchand_noalloc: zeta, mu_ions, stickeff_el and nitermax_ionis must be positive");
+  if (p->resistivities && (!cond || !gamma_d))
+    return gfail(SHARK_GROWTH_EINVAL, "shark_charge_marchand_noalloc: resistivities need cond and gamma_d");
+  double* sp = nullptr;
+  int rc = scratch_on_current_device(&sp);
+  if (rc) return rc;
+  MArgs a;
+  marchand_args(p, nc, nb, &a);
+  const int64_t nn = nc * nb;
+  // k_mcharge_pre writes every pair's tau, alpha and n_k before k_mcharge reads them, so nothing
+  // in the scratch has to start at zero
+  a.tau = sp;
+  a.alpha = sp + nn;
+  a.nk = sp + 2 * nn;
+  a.rho = rho; a.T = T; a.cs = cs; a.rhod = rhod; a.sdust = sdust; a.mdust = mdust;
+  a.psi = psi_old; a.zd = zd; a.ni = ni; a.ne = ne;
+  a.cond = p->resistivities ? cond : nullptr;
+  a.gd = p->resistivities ? gamma_d : nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_mcharge_pre, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_mcharge, dim3((unsigned)((nc + 63) / 64)), dim3(64), 0, s, a);
+  GCHK(hipGetLastError());
   return 0;
 }
 

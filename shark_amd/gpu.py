@@ -29,6 +29,7 @@ EXPORTS = [
     "shark_growth_destroy", "shark_growth_kernel_time", "shark_growth_charge",
     "shark_growth_set_cell_dt", "shark_growth_features", "shark_growth_set_surrogate",
     "shark_growth_surrogate_stats", "shark_gpu_attach_growth", "shark_charge_marchand",
+    "shark_charge_marchand_noalloc",
     "shark_mlp_create", "shark_mlp_num_params", "shark_mlp_set_params", "shark_mlp_get_params",
     "shark_mlp_forward", "shark_mlp_forward_layered", "shark_mlp_grads", "shark_mlp_grads_weighted",
     "shark_mlp_set_data_weighted", "shark_mlp_set_data", "shark_mlp_train",

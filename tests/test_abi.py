@@ -79,3 +79,17 @@ def test_growth_create_rejects_out_of_scope_configs_without_gpu():
     p = G.disk_params(nbins=65)
     with pytest.raises(gpu.SharkGPUError, match="nbins"):
         G.Growth(p, 10)
+
+
+def test_noalloc_marchand_rejects_oversized_calls_without_gpu():
+    """The size limit of shark_charge_marchand_noalloc's static scratch is checked before any
+    device call."""
+    from shark_amd import growth as G
+    lib = gpu.load()
+    lib.shark_charge_marchand_noalloc.argtypes = ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_int]
+                                                  + [ctypes.c_void_p] * 13)
+    p = G.marchand_params()
+    dummy = ctypes.c_void_p(8)   # never dereferenced: the size check comes first
+    rc = lib.shark_charge_marchand_noalloc(ctypes.byref(p), (1 << 20) // 16 + 1, 16, *([dummy] * 12), None)
+    assert rc != 0
+    assert b"exceeds the static scratch" in lib.shark_gpu_last_error()
