@@ -52,15 +52,10 @@ DUST_SOLVERS = {0: "huang_bai", 1: "llf", 2: "hll", 3: "hllc"}  # SOLVERDUST
 
 
 def copy_probe(nbytes=1 << 30, reps=10):
-    """A plain device copy of known size, for calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE."""
-    import torch
-    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    a.fill_(1.0)
-    for _ in range(reps):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}))
+    """A plain device copy of known size (the library's streaming probe), for calibrating
+    rocprofv3 FETCH_SIZE/WRITE_SIZE."""
+    from shark_amd import gpu
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps, **gpu.hbm_probe(nbytes, reps)}))
 
 
 # the reference's step time over the CPU restatement's on the same state and step count, from
@@ -182,7 +177,7 @@ def growth_block(steps_probe_dt=2e-8, surrogate_cells=16384, seed=99):
         ns = st["sigma"].size
         g = G.Growth(p, ns)
         try:
-            g.set_surrogate(net, norm)
+            g.set_surrogate(net, norm, resid_max=S.resid_max(meta))
             g.set_cell_dt(st["dt"])
             g.upload(st["rhod"], st["sigma"], st["cs"], st["r"])
             g.run(G.SURROGATE, 0.0)
@@ -222,6 +217,8 @@ def main():
                     "instead of the hipGraph replay")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-growth", action="store_true", help="skip the growth timing block")
+    ap.add_argument("--growth-only", action="store_true",
+                    help="print only the growth timing block (for profiling its kernels)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the shearing-box and learned-growth sub-blocks of the default line")
@@ -233,6 +230,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.growth_only:
+        print(json.dumps({"growth": growth_block()}))
         return
 
     from shark_amd import setups
@@ -285,28 +285,13 @@ def main():
 
 
 def hbm_attainable(nbytes=1 << 30, reps=20):
-    """On-box HBM bandwidth a plain streaming kernel attains (torch's device copy, read and fill of
-    1 GiB FP64 buffers, timed with events): the figure the kernels' traffic is also rated against."""
-    import torch
-    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    a.fill_(1.0)
-    res = {}
-    for name, fn, moved in (("copy", lambda: b.copy_(a), 2 * nbytes), ("read", lambda: a.sum(), nbytes),
-                            ("write", lambda: b.fill_(2.0), nbytes)):
-        fn()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
-            fn()
-        e1.record()
-        torch.cuda.synchronize()
-        res[name + "_GBs"] = moved * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
-    del a, b
-    torch.cuda.empty_cache()
-    res["GBs"] = max(res["copy_GBs"], 0.0)
-    res["source"] = "torch device copy (read + write) of 1 GiB, 20 reps, HIP events"
+    """On-box HBM bandwidth a plain streaming kernel attains (shark_hbm_probe: write, copy and read
+    of 1 GiB FP64 buffers, 16-byte accesses, HIP events): the figure the kernels' traffic is also
+    rated against.  The best of the three is the attainable figure."""
+    from shark_amd import gpu
+    res = gpu.hbm_probe(nbytes, reps)
+    res["GBs"] = max(res["write_GBs"], res["copy_GBs"], res["read_GBs"])
+    res["source"] = "shark_hbm_probe, 1 GiB buffers, 20 reps, HIP events"
     return res
 
 
@@ -330,9 +315,9 @@ def time_workload(key, steps, warmup, tune=True, events=False, attainable=None):
     if learned:
         from shark_amd import growth as G
         from shark_amd import surrogate as S
-        net, norm, _ = S.load(os.path.join(ROOT, "shark_amd", "data", "surrogate_disk64.npz"))
+        net, norm, smeta = S.load(os.path.join(ROOT, "shark_amd", "data", "surrogate_disk64.npz"))
         gr = G.Growth(made[3], (cfg.nx + 4) * (cfg.ny + 4))
-        gr.set_surrogate(net, norm)
+        gr.set_surrogate(net, norm, resid_max=S.resid_max(smeta))
         g.attach_growth(gr, G.SURROGATE)
     g.run(warmup)
 

@@ -179,6 +179,13 @@ int shark_gpu_autotune(shark_ctx* ctx, int reps, shark_tune* chosen);
 
 int shark_gpu_destroy(shark_ctx* ctx);
 const char* shark_gpu_last_error(void);
+
+/* HBM bandwidth a plain streaming kernel attains on the current device (no counterpart in the
+ * reference: the measurement the step's achieved bandwidth is rated against beside the vendor
+ * peak).  Two buffers of nbytes (>= 1 MiB) are allocated and freed; each of write, copy and read
+ * runs `reps` times after one untimed pass, 16-byte accesses, grid-stride over 8 blocks per CU.
+ * gbs[0] write, gbs[1] copy (read + write bytes), gbs[2] read, in GB/s. */
+int shark_hbm_probe(int64_t nbytes, int reps, double* gbs);
 int shark_gpu_version(void);
 
 #ifdef __cplusplus

@@ -127,15 +127,26 @@ int shark_growth_features(shark_growth* g, double dt, float* feat);
  * in_mean[nbins+5], in_std[nbins+5], out_mean[nbins+1], out_std[nbins+1]; outputs are the change of
  * the natural log of each bin's share of the dust mass and log10(-<Z>).  New shares are the
  * softmax of old log-shares + change (positive, summing to one).  A cell falls back to mode 0 and
- * the charge solve when an output is not finite, when its mass residual after that normalisation
- * exceeds resid_max, or when an input lies beyond 8 standard deviations of the training set.
+ * the charge solve when the raw prediction's mass residual |sum_k share_k exp(change_k) - 1|
+ * (before that normalisation; coagulation conserves mass, so a consistent prediction has none)
+ * exceeds resid_max, when an output is not finite, or when an input lies beyond 8 standard
+ * deviations of the training set.
  * alpha (<= 0: turb_coef^2), zeta and mu_ions feed the features and the fallback charge solve.
  * The network must outlive its use here. */
 int shark_growth_set_surrogate(shark_growth* g, struct shark_mlp* m, const float* norm,
                                double alpha, double zeta, double mu_ions, double resid_max);
+/* A second network on the same (standardised) inputs whose outputs, de-standardised with
+ * out_norm2 = out_mean2[nbins+1], out_std2[nbins+1], are added to the first network's: it is
+ * trained on what the first network misses (a residual stage).  Call after
+ * shark_growth_set_surrogate (which detaches it); m2 == NULL detaches.  The network must outlive
+ * its use here. */
+int shark_growth_set_surrogate_residual(shark_growth* g, struct shark_mlp* m2, const float* out_norm2);
 /* Of the last SHARK_GROWTH_SURROGATE run: cells that fell back, and the mean mass residual of the
  * accepted raw predictions before normalisation, |sum_k share_k exp(change_k) - 1|. */
 int shark_growth_surrogate_stats(shark_growth* g, int64_t* n_fallback, double* raw_resid_mean);
+/* Per-cell mean grain charge <Z> (ncells) of the last SHARK_GROWTH_SURROGATE run (the network's
+ * charge output; the charge solve for cells that fell back) or of the last shark_growth_charge. */
+int shark_growth_zmean(shark_growth* g, double* zmean);
 /* Dust growth inside the step (src/solver/solve.f90:112): after the drag of every shark_gpu_step /
  * shark_gpu_run of ctx, a growth call of `mode` over the step's dt on the step's dust densities
  * (u_prim(:,irhod), active cells only; the gas density u_prim(:,irho) and cs of the step, the

@@ -938,6 +938,8 @@ struct SArgs {
   const float* y;        // network outputs, [k * fld + cc]
   int64_t fld;
   const float* norm;  // in_mean[nin], in_std[nin], out_mean[nout], out_std[nout]; null: raw
+  const float* y2;    // residual network's outputs (same layout as y), or null
+  const float* norm2; // out_mean2[nout], out_std2[nout] of the residual network
   double* zmean;
   int* fb;            // [ncells] 1 = plain fallback
   double* tg;
@@ -972,12 +974,26 @@ __global__ void k_surr_feat(SArgs a) {
 // New state from the network output.  The network predicts the change of the log of each bin's
 // share of the cell's dust mass; the new shares are the softmax of (old log-shares + change),
 // i.e. w_k = rho_k exp(change_k) normalised by sum_j w_j, so they are positive and sum to one by
-// construction and the cell's dust mass is kept in FP64.  A cell falls back to the plain
-// coagulation when an output is not finite, when an old density is not positive, when the mass
-// residual |sum_k share_k - 1| after that normalisation exceeds resid_max (only possible through a
-// non-finite or degenerate prediction), or when an input lies more than zmax standard deviations
-// from the training set's mean (outside the envelope the network was trained on).  The per-bin
-// values live in registers: every loop over bins is unrolled over MAXB with a bound check.
+// construction and the cell's dust mass is kept in FP64.  Coagulation and fragmentation conserve
+// the dust mass, so the raw prediction should too: its mass residual |sum_k share_k exp(change_k)
+// - 1| (before the normalisation) measures how far the network is from a consistent answer.  A
+// cell falls back to the plain coagulation when that residual exceeds resid_max, when an output is
+// not finite, when an old density is not positive, or when an input lies more than zmax standard
+// deviations from the training set's mean (outside the envelope the network was trained on).  The
+// per-bin values live in registers: every loop over bins is unrolled over MAXB with a bound check.
+// Output j of the learned step: the network's de-standardised output, plus the residual
+// network's when one is attached (it is trained on what the first network misses).
+__device__ __forceinline__ double out_value(const SArgs& a, const float* y, int j, int64_t ld, const float* om,
+                                            const float* os) {
+  double v = (double)(y[j * ld] * os[j] + om[j]);
+  if (a.y2) {
+    const float* m2 = a.norm2;
+    const float* s2 = a.norm2 + a.nout;
+    v += (double)(a.y2[j * ld + (y - a.y)] * s2[j] + m2[j]);
+  }
+  return v;
+}
+
 __global__ __launch_bounds__(256) void k_surr_post(SArgs a) {
   const int cc = blockIdx.x * blockDim.x + threadIdx.x;
   if (cc >= a.nch) return;
@@ -1005,20 +1021,14 @@ __global__ __launch_bounds__(256) void k_surr_post(SArgs a) {
       const double r = a.rhod[(int64_t)k * a.ld + c];
       tot += r;
       if (!(r > 0.0)) fallback = true;
-      w[k] = (float)(r * exp((double)(y[k * ld] * os[k] + om[k])));
+      w[k] = (float)(r * exp(out_value(a, y, k, ld, om, os)));
       W += (double)w[k];
     }
   }
   for (int j = 0; j < a.nin && !fallback; ++j)
     if (!(fabsf(f[j * ld]) <= a.zmax)) fallback = true;
   if (!(W > 0.0) || !(W < INFINITY)) fallback = true;
-  if (!fallback) {
-    double chk = 0.0;
-#pragma unroll
-    for (int k = 0; k < MAXB; ++k)
-      if (k < nb) chk += (double)w[k] / W;
-    if (!(fabs(chk - 1.0) <= a.resid_max)) fallback = true;
-  }
+  if (!fallback && !(fabs(W / tot - 1.0) <= a.resid_max)) fallback = true;   // raw mass residual
   const double dtl = cell_dt(a, c);
   a.nit[c] = 0;
   if (fallback) {
@@ -1041,7 +1051,7 @@ __global__ __launch_bounds__(256) void k_surr_post(SArgs a) {
 #pragma unroll
   for (int k = 0; k < MAXB; ++k)
     if (k < nb) a.rhod[(int64_t)k * a.ld + c] = fmax((double)w[k] * sc0, floor_v) * sc;
-  a.zmean[c] = -pow(10.0, (double)(y[nb * ld] * os[nb] + om[nb]));
+  a.zmean[c] = -pow(10.0, out_value(a, y, nb, ld, om, os));
 }
 
 }  // namespace
@@ -1080,6 +1090,8 @@ struct shark_growth {
   shark_mlp* mlp = nullptr;
   float* dnorm = nullptr;
   float *dfeat = nullptr, *dy = nullptr;
+  shark_mlp* mlp2 = nullptr;   // residual network (optional)
+  float *dnorm2 = nullptr, *dy2 = nullptr;
   int* dfb = nullptr;
   double* dresid = nullptr;
   // bound to a step context (shark_gpu_attach_growth): rhod, gas density and cs are the step's
@@ -1100,7 +1112,7 @@ void shark_growth_free(shark_growth* g) {
   void* ptrs[] = {g->dpairs, g->dredis, g->dmdust, g->dsdust, g->drhod, g->drho, g->dcs, g->drad,
                   g->denv, g->dts, g->dA, g->dF, g->dAh, g->dFh, g->dtg, g->dnit, g->dflags,
                   g->dGf32, g->dGbhi, g->dGblo, g->dzmean, g->dxion, g->ddtc, g->dnorm,
-                  g->dfeat, g->dy, g->dfb, g->dresid, g->dmask, g->dsT};
+                  g->dfeat, g->dy, g->dfb, g->dresid, g->dmask, g->dsT, g->dnorm2, g->dy2};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (g->hflags) (void)hipHostFree(g->hflags);
@@ -1464,6 +1476,8 @@ static int surrogate_run(shark_growth* g, double dt) {
   sa.norm = g->dnorm;
   sa.feat = g->dfeat;
   sa.y = g->dy;
+  sa.y2 = g->mlp2 ? g->dy2 : nullptr;
+  sa.norm2 = g->dnorm2;
   sa.env = g->dsT;
   sa.fld = g->sch;
   q.env = g->dsT;
@@ -1486,6 +1500,10 @@ static int surrogate_run(shark_growth* g, double dt) {
     GCHK(hipGetLastError());
     int rc = shark::mlp_infer_device(g->mlp, g->dfeat, nch, g->dy, s, B, B);
     if (rc) return rc;
+    if (g->mlp2) {
+      rc = shark::mlp_infer_device(g->mlp2, g->dfeat, nch, g->dy2, s, B, B);
+      if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_surr_post, dim3(nblk), dim3(256), 0, s, sa);
     GCHK(hipGetLastError());
   }
@@ -1628,11 +1646,13 @@ int shark_growth_set_surrogate(shark_growth* g, shark_mlp* m, const float* norm,
   if (!g) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate: null handle");
   if (!m) {
     g->mlp = nullptr;
+    g->mlp2 = nullptr;
     return 0;
   }
   const int nb = g->p.nbins;
   if (shark::mlp_n_in(m) != nb + 5 || shark::mlp_n_out(m) != nb + 1)
     return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate: the network needs nbins+5 inputs and nbins+1 outputs");
+  g->mlp2 = nullptr;   // a residual network belongs to the network it was trained on
   if (!shark::mlp_fused_ok(m))
     return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate: the network needs <= 8 layers, widths <= 256 and hidden widths that are multiples of 32");
   if (!norm) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate: null normalisation");
@@ -1708,6 +1728,37 @@ int shark_growth_features(shark_growth* g, double dt, float* feat) {
       for (int i = 0; i < nch; ++i) feat[(c0 + i) * nin + j] = tmp[(size_t)j * nch + i];
   }
   GCHK(hipFree(dfe));
+  return 0;
+}
+
+int shark_growth_set_surrogate_residual(shark_growth* g, shark_mlp* m2, const float* out_norm2) {
+  if (!g) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate_residual: null handle");
+  if (!m2) {
+    g->mlp2 = nullptr;
+    return 0;
+  }
+  if (!g->mlp) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate_residual: attach the first network first");
+  const int nb = g->p.nbins;
+  if (shark::mlp_n_in(m2) != nb + 5 || shark::mlp_n_out(m2) != nb + 1 || !shark::mlp_fused_ok(m2))
+    return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate_residual: the network needs nbins+5 inputs, nbins+1 outputs and the fused kernel's shape");
+  if (!out_norm2) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate_residual: null normalisation");
+  for (int i = 0; i < 2 * (nb + 1); ++i)
+    if (!(out_norm2[i] == out_norm2[i])) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_surrogate_residual: non-finite normalisation");
+  if (!g->dnorm2) GCHK(hipMalloc(&g->dnorm2, sizeof(float) * 2 * (MAXB + 1)));
+  if (g->dy2) (void)hipFree(g->dy2);
+  g->dy2 = nullptr;
+  GCHK(hipMalloc(&g->dy2, sizeof(float) * g->sch * (nb + 1)));
+  GCHK(hipMemcpy(g->dnorm2, out_norm2, sizeof(float) * 2 * (nb + 1), hipMemcpyHostToDevice));
+  g->mlp2 = m2;
+  return 0;
+}
+
+int shark_growth_zmean(shark_growth* g, double* zmean) {
+  if (!g || !zmean) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_zmean: null argument");
+  if (!g->dzmean) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_zmean: no learned step or charge solve has run");
+  GCHK(hipStreamSynchronize(g->run_stream));
+  GCHK(hipStreamSynchronize(g->stream));
+  GCHK(hipMemcpy(zmean, g->dzmean, sizeof(double) * g->ncells, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -19,13 +19,14 @@ __device__ __forceinline__ double fminf_(double a, double b) { return a < b ? a 
 // (one IEEE division shared by several quotients).  q0 = a*y is within one ulp of a/b, the
 // residual a - b*q0 is exact in an FMA, and q0 + (a - b*q0)*y rounds to the correctly rounded
 // quotient (Markstein, IBM J. Res. Dev. 34 (1990) 111-119; Muller et al., Handbook of
-// Floating-Point Arithmetic, division with FMA) for quotients in the normal range.  A zero
-// dividend takes the product, so the sign of zero is that of a/b.
+// Floating-Point Arithmetic, division with FMA) for quotients in the normal range.  The result
+// carries the sign of q0 = a*y, which is the quotient's sign: a zero dividend then gives the zero of
+// a/b's sign (the FMA chain alone can turn -0 into +0), at the cost of one bit insert.
 __device__ __forceinline__ double exdiv(double a, double b, double y) {
   const double q0 = a * y;
   const double r = fma(-b, q0, a);
   const double q = fma(r, y, q0);
-  return a == 0.0 ? q0 : q;
+  return copysign(q, q0);
 }
 
 // slope_limit, src/solver/slope_limiters.f90:5-37

@@ -101,7 +101,7 @@ static int validate(const shark_cfg* c) {
   if (c->nghost != 2) return fail(SHARK_EUNSUP, "nghost=%d: only NGHOST=2 is built", c->nghost);
   if (c->ndust < 0 || c->ndust > SHARK_MAX_DUST) return fail(SHARK_EINVAL, "ndust=%d out of range", c->ndust);
   if (!shark::nd_supported(c->ndust))
-    return fail(SHARK_EUNSUP, "ndust=%d: hydro step is instantiated for ndust in {0,1,2,3,4,8,16,64}", c->ndust);
+    return fail(SHARK_EUNSUP, "ndust=%d: the step is built for 0 to %d dust species", c->ndust, SHARK_MAX_DUST);
   if (c->iso_cs > 1)
     return fail(SHARK_EINVAL, "iso_cs=%d: 1 is isothermal, below 1 adiabatic", c->iso_cs);
   if (c->iso_cs != 1 && (c->geom != SHARK_GEOM_CARTESIAN || c->fargo))

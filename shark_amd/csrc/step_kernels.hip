@@ -593,8 +593,9 @@ DEV double start_cs(const StepArgs& a, int cell) {
 }
 
 // next: 1 when the candidate is for the step after the one being taken (cell_force)
+// ND: species capacity of the arrays; nd (<= ND): species in the run
 template <int ND>
-DEV double courant_cell(const StepArgs& a, const double* un, int ix, int iy, int next, int cell) {
+DEV double courant_cell(const StepArgs& a, const double* un, int ix, int iy, int next, int cell, int nd) {
   double dxx;
   if (a.ndim == 1) dxx = a.g.dx1[ix - 1];
   else if (a.geom == 2) dxx = fminf_(a.g.dx1[ix - 1], a.g.rc[ix - 1] * a.g.dx2[ix - 1]);
@@ -607,10 +608,11 @@ DEV double courant_cell(const StepArgs& a, const double* un, int ix, int iy, int
   double vdx[ND > 0 ? ND : 1], vdz[ND > 0 ? ND : 1];
 #pragma unroll
   for (int k = 0; k < ND; ++k) {
+    if (k >= nd) break;
     const double rd = un[5 + k];
-    vdx[k] = un[5 + ND + k] / rd;
-    const double vdy = un[5 + 2 * ND + k] / rd;
-    vdz[k] = un[5 + 3 * ND + k] / rd;
+    vdx[k] = un[5 + nd + k] / rd;
+    const double vdy = un[5 + 2 * nd + k] / rd;
+    vdz[k] = un[5 + 3 * nd + k] / rd;
     vv = (fabs(vdx[k]) + fabs(vdy)) + fabs(vdz[k]);
     vmax = fmaxf_(vmax, 0.0 + vv);
   }
@@ -621,6 +623,7 @@ DEV double courant_cell(const StepArgs& a, const double* un, int ix, int iy, int
     double fm = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
 #pragma unroll
     for (int k = 0; k < ND; ++k) {
+      if (k >= nd) break;
       cell_force(a, ix, iy, next, vdx[k], vdz[k], false, f1, f2, f3);
       fm = fmaxf_(fm, sqrt((f1 * f1 + f2 * f2) + f3 * f3));
     }
@@ -700,10 +703,14 @@ DEV double tstop_of(const StepArgs& a, int k, int cell, int ix, double rho_q, do
 // PART 0: the whole post pass.  With a growth step between drag and the in-loop hook
 // (solve.f90:110-152) the pass is split: PART 1 remaps and drags (writes U, no Courant candidate),
 // PART 2 reloads U after growth, applies the in-loop hook and computes the Courant candidate.
-template <int ND, bool FARGO, int NDIM, int TW, int PART>
+template <int ND, bool FARGO, int NDIM, int TW, int PART, bool CAP>
 __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
   if (!a.sc->active) return;
   constexpr int NV = 5 + 4 * ND;
+  // CAP: ND is a capacity and the run's species count nd (<= ND) is read at run time; otherwise
+  // nd is the compile-time ND
+  const int nd = CAP ? a.nd : ND;
+  const int nv = 5 + 4 * nd;
   constexpr int NS = 256 / TW;
   constexpr int NDL = ND > 0 ? ND : 1;
   // the Courant scratch of phase 5 reuses the drag arrays, which are dead by then, to keep the
@@ -740,15 +747,15 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
   // ---- phase 1: remap (FARGO) or copy every variable of the segment into LDS
   if (PART == 2) {
 #pragma unroll
-    for (int v = s; v < NV; v += NS) un[v][c] = a.U[v * nc + cell];
+    for (int v = s; v < nv; v += NS) un[v][c] = a.U[v * nc + cell];
     __syncthreads();   // the planes below were loaded by other slots
     // growth changed the dust densities only; each species keeps its velocity:
     // momentum = (momentum / density before growth) * density after growth
-    for (int k = s; k < ND; k += NS) {
+    for (int k = s; k < nd; k += NS) {
       const double r0 = a.Rold[(size_t)k * nc + cell], r1 = un[5 + k][c];
-      un[5 + ND + k][c] = (un[5 + ND + k][c] / r0) * r1;
-      un[5 + 2 * ND + k][c] = (un[5 + 2 * ND + k][c] / r0) * r1;
-      un[5 + 3 * ND + k][c] = (un[5 + 3 * ND + k][c] / r0) * r1;
+      un[5 + nd + k][c] = (un[5 + nd + k][c] / r0) * r1;
+      un[5 + 2 * nd + k][c] = (un[5 + 2 * nd + k][c] / r0) * r1;
+      un[5 + 3 * nd + k][c] = (un[5 + 3 * nd + k][c] / r0) * r1;
     }
     __syncthreads();
   } else if (FARGO) {
@@ -779,7 +786,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
 #pragma unroll
       for (int j = 0; j < CH; ++j) {
         const int v = s + (i0 + j) * NS;
-        if (i0 + j < NPS && v < NV) {
+        if (i0 + j < NPS && v < nv) {
           const double* Bv = a.B + v * nc;
           um[j] = Bv[cm]; u0[j] = Bv[c0]; u1[j] = Bv[c1]; up[j] = Bv[cp];
         }
@@ -787,7 +794,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
 #pragma unroll
       for (int j = 0; j < CH; ++j) {
         const int v = s + (i0 + j) * NS;
-        if (i0 + j < NPS && v < NV) {
+        if (i0 + j < NPS && v < nv) {
           const double d0 = slope_limit(a.slope, u0[j] - um[j], u1[j] - u0[j]);
           const double d1 = slope_limit(a.slope, u1[j] - u0[j], up[j] - u1[j]);
           double x = u0[j] * w0 + u1[j] * w1;
@@ -799,16 +806,16 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
     if (a.accretion && ix == a.nx + 2) {
       // fargo_scheme applies the boundaries again after the remap (fargo_scheme.f90:80), and
       // with accretion they reset the last active column to the inflow state (as k_bc_accrete)
-      for (int v = s; v < NV; v += NS) {
+      for (int v = s; v < nv; v += NS) {
         double x;
         if (accretion_inflow(a, v, iy, x)) un[v][c] = x;
       }
     }
   } else {
 #pragma unroll
-    for (int v = s; v < NV; v += NS) un[v][c] = a.B[v * nc + cell];
+    for (int v = s; v < nv; v += NS) un[v][c] = a.B[v * nc + cell];
   }
-  const bool do_drag = ND > 0 && a.drag && PART != 2;
+  const bool do_drag = nd > 0 && a.drag && PART != 2;
   if (do_drag) {
     // stopping times of the step (compute_tstop on the start-of-step primitives); alpha_k and
     // alpha_k / (1 + alpha_k) are shared by the drag sums and the momentum update
@@ -818,7 +825,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
       const double H = a.hoverr * a.g.rc[ixs - 1];
       const double den = (rho_q / (a.s2pi * H)) * a.cs[cell];
       const double yden = 1.0 / den;
-      for (int k = s; k < ND; k += NS) {
+      for (int k = s; k < nd; k += NS) {
         const double sk = a.sdust_uniform ? a.sdust_k[k] : a.sdust[(size_t)k * a.nc + cell];
         const double ak = (1.0 * dt) / exdiv(a.eps_k1 * sk, den, yden);
         akS[k][c] = ak;
@@ -826,7 +833,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
       }
     } else {
       const double cs_q = a.tstop_kind == 3 ? start_cs(a, cell) : 0.0;
-      for (int k = s; k < ND; k += NS) {
+      for (int k = s; k < nd; k += NS) {
         const double rq = (a.tstop_kind == 0) ? a.U[(5 + k) * nc + cell] : 0.0;
         const double ak = (1.0 * dt) / tstop_of<ND, !FARGO>(a, k, cell, ixs, rho_q, rq, cs_q);
         akS[k][c] = ak;
@@ -838,10 +845,10 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
   // ---- phase 2: drag sums in species order (dust_drag.f90:24-38)
   if (do_drag) {
     if (s < 4) {
-      const int base = (s == 3) ? 5 : 5 + (s + 1) * ND;   // rhod, vdx, vdy, vdz planes
+      const int base = (s == 3) ? 5 : 5 + (s + 1) * nd;   // rhod, vdx, vdy, vdz planes
       double acc = un[s == 3 ? 0 : s + 1][c];
 #pragma unroll 4
-      for (int k = 0; k < ND; ++k) {
+      for (int k = 0; k < nd; ++k) {
         const double ak = akS[k][c];
         acc = acc + exdiv(ak, 1.0 + ak, gS[k][c]) * un[base + k][c];
       }
@@ -851,13 +858,13 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
     // ---- phase 3: momentum update and floor (dust_drag.f90:40-73)
     const double pnx = sumS[0][c], pny = sumS[1][c], pnz = sumS[2][c], rhon = sumS[3][c];
     const double yrho = 1.0 / rhon;
-    for (int k = s; k < ND; k += NS) {
+    for (int k = s; k < nd; k += NS) {
       const double ak = akS[k][c], ap1 = 1.0 + ak, y1 = gS[k][c];
       const double f = exdiv(ak, ap1, y1);
       const double rd = un[5 + k][c];
-      un[5 + ND + k][c] = exdiv(un[5 + ND + k][c], ap1, y1) + exdiv(f * pnx, rhon, yrho) * rd;
-      un[5 + 2 * ND + k][c] = exdiv(un[5 + 2 * ND + k][c], ap1, y1) + exdiv(f * pny, rhon, yrho) * rd;
-      un[5 + 3 * ND + k][c] = exdiv(un[5 + 3 * ND + k][c], ap1, y1) + exdiv(f * pnz, rhon, yrho) * rd;
+      un[5 + nd + k][c] = exdiv(un[5 + nd + k][c], ap1, y1) + exdiv(f * pnx, rhon, yrho) * rd;
+      un[5 + 2 * nd + k][c] = exdiv(un[5 + 2 * nd + k][c], ap1, y1) + exdiv(f * pny, rhon, yrho) * rd;
+      un[5 + 3 * nd + k][c] = exdiv(un[5 + 3 * nd + k][c], ap1, y1) + exdiv(f * pnz, rhon, yrho) * rd;
       un[5 + k][c] = fmaxf_(un[0][c] * a.dust_ratio_min, rd);
     }
     if (s == NS - 1 && a.back_reaction) {
@@ -878,10 +885,10 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
                            a.n_rel * 2.0 * kPi / (sqrt(a.mstar / (a.smooth_r * a.smooth_r * a.smooth_r)));
       const double e = exp(-dt / t_rel);
       const double* ci = a.condinit;
-      for (int f = s; f <= ND; f += NS) {
+      for (int f = s; f <= nd; f += NS) {
         const int ir = f == 0 ? 0 : 5 + f - 1;
-        const int iu = f == 0 ? 1 : 5 + ND + f - 1;
-        const int iv = f == 0 ? 2 : 5 + 2 * ND + f - 1;
+        const int iu = f == 0 ? 1 : 5 + nd + f - 1;
+        const int iv = f == 0 ? 2 : 5 + 2 * nd + f - 1;
         const double rho_i = ci[ir * nc + cell], vx_i = ci[iu * nc + cell] / rho_i,
                      vy_i = ci[iv * nc + cell] / rho_i;
         const double rho_o = un[ir][c], vx_o = un[iu][c] / rho_o, vy_o = un[iv][c] / rho_o;
@@ -893,7 +900,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
     }
     __syncthreads();
     if (ix == 3)
-      for (int v = s; v < NV; v += NS) un[v][c] = a.condinit[v * nc + cell];
+      for (int v = s; v < nv; v += NS) un[v][c] = a.condinit[v * nc + cell];
     __syncthreads();
   } else if (!FARGO && a.inloop_kind == 3) {
     // cartesian disk buffers (disk/setup.f90:203-262): reset to the initial disk (rate = 0)
@@ -908,7 +915,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
       double rho_ana = a.sigma_r0 * (a.r0_disk / rr);
       if (rr > R_out_disk) rho_ana = a.sigma_r0 * (a.r0_disk / R_out_disk) / 100.0;
       const double v_ana = -(Omega * rho_ana);
-      for (int f = s; f <= ND; f += NS) {
+      for (int f = s; f <= nd; f += NS) {
         if (f == 0) {
           un[0][c] = rho_ana + (un[0][c] - rho_ana) * rate;
           un[1][c] = -(y_soft * v_ana) + (un[1][c] + yy * v_ana) * rate;
@@ -918,8 +925,8 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
           const int k = f - 1;
           const double dust_to = a.dust2gas;
           un[5 + k][c] = dust_to * rho_ana + (un[5 + k][c] - rho_ana * dust_to) * rate;
-          un[5 + ND + k][c] = -(yy * v_ana * dust_to) + (un[5 + ND + k][c] + dust_to * yy * v_ana) * rate;
-          un[5 + 2 * ND + k][c] = xx * v_ana * dust_to + (un[5 + 2 * ND + k][c] - dust_to * xx * v_ana) * rate;
+          un[5 + nd + k][c] = -(yy * v_ana * dust_to) + (un[5 + nd + k][c] + dust_to * yy * v_ana) * rate;
+          un[5 + 2 * nd + k][c] = xx * v_ana * dust_to + (un[5 + 2 * nd + k][c] - dust_to * xx * v_ana) * rate;
         }
       }
     }
@@ -927,10 +934,10 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
   } else if (a.inloop_kind == 1 && !a.force_kick) {
     const double Om = a.omega_shear, O = Om * dt;
     const double x0 = a.g.pos1[ixs - 1] - kHalf * a.box_l;
-    for (int f = s; f <= ND; f += NS) {
+    for (int f = s; f <= nd; f += NS) {
       const int ir = f == 0 ? 0 : 5 + f - 1;
-      const int ixv = f == 0 ? 1 : 5 + ND + f - 1;
-      const int iz = f == 0 ? 3 : 5 + 3 * ND + f - 1;
+      const int ixv = f == 0 ? 1 : 5 + nd + f - 1;
+      const int iz = f == 0 ? 3 : 5 + 3 * nd + f - 1;
       const double AA = (f == 0) ? (2.0 * a.q_shear) * Om * x0 - (2.0 * a.rad0) * a.eta_stream
                                  : (2.0 * a.q_shear) * Om * x0;
       const double u = un[iz][c] / un[ir][c], v = un[ixv][c] / un[ir][c], d = un[ir][c];
@@ -941,10 +948,10 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
   }
   // ---- write the final state
   if (valid)
-    for (int v = s; v < NV; v += NS) a.U[v * nc + cell] = un[v][c];
+    for (int v = s; v < nv; v += NS) a.U[v * nc + cell] = un[v][c];
   if (PART == 1) {
     if (valid)
-      for (int k = s; k < ND; k += NS) a.Rold[(size_t)k * nc + cell] = un[5 + k][c];
+      for (int k = s; k < nd; k += NS) a.Rold[(size_t)k * nc + cell] = un[5 + k][c];
     return;
   }
   // ---- phase 5: next step's Courant candidate (courant.f90:68-158), species split over slots
@@ -958,7 +965,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
                    vz = exdiv(un[3][c], rho, yr);
       const double vv = (fabs(vx) + fabs(vy)) + fabs(vz);
       vmax = cs_cell(a, rho, un[4][c], vx, vy, vz, cell) + vv;
-      if (ND == 0) vvlast[c] = vv;
+      if (nd == 0) vvlast[c] = vv;
       if (a.force_kick) {
         cell_force<!FARGO>(a, ixs, iy, 1, vx, vz, true, f1, f2, f3);
         fm = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
@@ -967,14 +974,14 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
     }
     // the disk's dust acceleration does not depend on the species' velocity: one evaluation
     double fk_disk = 0.0;
-    if (a.force_kick && a.force_kind == 2 && s < ND) {
+    if (a.force_kick && a.force_kind == 2 && s < nd) {
       cell_force<!FARGO>(a, ixs, iy, 1, 0.0, 0.0, false, f1, f2, f3);
       fk_disk = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
     }
-    for (int k = s; k < ND; k += NS) {
+    for (int k = s; k < nd; k += NS) {
       const double rd = un[5 + k][c], yd = 1.0 / rd;
-      const double vdx = exdiv(un[5 + ND + k][c], rd, yd), vdy = exdiv(un[5 + 2 * ND + k][c], rd, yd),
-                   vdz = exdiv(un[5 + 3 * ND + k][c], rd, yd);
+      const double vdx = exdiv(un[5 + nd + k][c], rd, yd), vdy = exdiv(un[5 + 2 * nd + k][c], rd, yd),
+                   vdz = exdiv(un[5 + 3 * nd + k][c], rd, yd);
       const double vv = (fabs(vdx) + fabs(vdy)) + fabs(vdz);
       vmax = any ? fmaxf_(vmax, 0.0 + vv) : 0.0 + vv;
       if (a.force_kick) {
@@ -986,7 +993,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
         fm = any ? fmaxf_(fm, fk) : fk;
       }
       any = true;
-      if (k == ND - 1) vvlast[c] = vv;
+      if (k == nd - 1) vvlast[c] = vv;
     }
     vmaxS[s][c] = any ? vmax : 0.0;
     fmS[s][c] = any ? fm : 0.0;
@@ -1013,9 +1020,10 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
   block_atomic_min(a.sc, dtc);
 }
 
-template <int ND, int NDIM>
+template <int ND, int NDIM, bool CAP>
 __global__ __launch_bounds__(256) void k_courant(StepArgs a) {
   constexpr int NV = 5 + 4 * ND;
+  const int nd = CAP ? a.nd : ND;   // CAP: ND is a capacity, the run has a.nd species
   int ix, iy;
   if (NDIM == 2) {
     ix = 3 + blockIdx.x * 64 + (threadIdx.x & 63);
@@ -1029,8 +1037,9 @@ __global__ __launch_bounds__(256) void k_courant(StepArgs a) {
     const int cell = cell_of(a, ix, iy);
     double un[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) un[v] = a.U[(size_t)v * a.nc + cell];
-    dtc = courant_cell<ND>(a, un, ix, iy, 0, cell);
+    for (int v = 0; v < NV; ++v)
+      if (v < 5 + 4 * nd) un[v] = a.U[(size_t)v * a.nc + cell];
+    dtc = courant_cell<ND>(a, un, ix, iy, 0, cell, nd);
   }
   block_atomic_min(a.sc, dtc);
 }
@@ -1247,72 +1256,80 @@ void launch_hydro_fluid(const StepArgs& a, bool gas, hipStream_t s) {
   else if (a.nd > 0) hydro_launch_geom<false>(a, s);
 }
 
-template <int ND, int PART>
+// CAP: the kernels take the species count at run time, with ND as the capacity (numbers of dust
+// species without an exact instantiation run in the next larger one)
+template <int ND, bool CAP, int PART>
 static void post_nd_part(const StepArgs& a, hipStream_t s);
-template <int ND>
-static void post_nd(const StepArgs& a, hipStream_t s) { post_nd_part<ND, 0>(a, s); }
-template <int ND>
-static void post1_nd(const StepArgs& a, hipStream_t s) { post_nd_part<ND, 1>(a, s); }
-template <int ND>
-static void post2_nd(const StepArgs& a, hipStream_t s) { post_nd_part<ND, 2>(a, s); }
+template <int ND, bool CAP>
+static void post_nd(const StepArgs& a, hipStream_t s) { post_nd_part<ND, CAP, 0>(a, s); }
+template <int ND, bool CAP>
+static void post1_nd(const StepArgs& a, hipStream_t s) { post_nd_part<ND, CAP, 1>(a, s); }
+template <int ND, bool CAP>
+static void post2_nd(const StepArgs& a, hipStream_t s) { post_nd_part<ND, CAP, 2>(a, s); }
 // Row-segment widths of the post pass.  Up to 16 species: 16 or 32 cells (32 cells with 8 slots
-// take 27 KB of LDS and <= 96 VGPRs at 16 species, so five blocks share a CU); 64 species
-// (261 variables): 8 or 16 cells, which keep the segment's LDS under 64 KB.
+// take 27 KB of LDS and <= 96 VGPRs at 16 species, so five blocks share a CU); more species (up to
+// 261 variables): 8 or 16 cells, which keep the segment's LDS under 64 KB.
 template <int ND>
 struct PostTW {
-  static constexpr int A = ND >= 32 ? 8 : 16;
-  static constexpr int B = ND >= 32 ? 16 : 32;
+  static constexpr int A = ND > 16 ? 8 : 16;
+  static constexpr int B = ND > 16 ? 16 : 32;
 };
 bool post_tw_ok(int nd, int tw) {
-  return nd >= 32 ? (tw == 8 || tw == 16) : (tw == 16 || tw == 32);
+  return nd > 16 ? (tw == 8 || tw == 16) : (tw == 16 || tw == 32);
 }
-int post_tw_default(int nd) { return nd >= 32 ? 16 : 32; }
+int post_tw_default(int nd) { return nd > 16 ? 16 : 32; }
 int post_tw_candidates(int nd, int* out) {
-  out[0] = nd >= 32 ? 8 : 16;
-  out[1] = nd >= 32 ? 16 : 32;
+  out[0] = nd > 16 ? 8 : 16;
+  out[1] = nd > 16 ? 16 : 32;
   return 2;
 }
-
-template <int ND, int PART, int TW>
+template <int ND, bool CAP, int PART, int TW>
 static void post_nd_tw(const StepArgs& a, hipStream_t s);
-template <int ND, int PART>
+template <int ND, bool CAP, int PART>
 static void post_nd_part(const StepArgs& a, hipStream_t s) {
-  if (a.post_tw == PostTW<ND>::A) post_nd_tw<ND, PART, PostTW<ND>::A>(a, s);
-  else post_nd_tw<ND, PART, PostTW<ND>::B>(a, s);
+  if (a.post_tw == PostTW<ND>::A) post_nd_tw<ND, CAP, PART, PostTW<ND>::A>(a, s);
+  else post_nd_tw<ND, CAP, PART, PostTW<ND>::B>(a, s);
 }
-template <int ND, int PART, int TW>
+template <int ND, bool CAP, int PART, int TW>
 static void post_nd_tw(const StepArgs& a, hipStream_t s) {
   if (a.ndim == 1) {
     dim3 grid((a.nx + TW - 1) / TW);
-    hipLaunchKernelGGL((k_post2<ND, false, 1, TW, PART>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_post2<ND, false, 1, TW, PART, CAP>), grid, dim3(256), 0, s, a);
   } else {
     dim3 grid((a.nx + TW - 1) / TW, a.ny);
     if (a.fargo && PART != 2) {
       if (a.accretion && a.bc_kind == 2)
         hipLaunchKernelGGL(k_bc_accrete, dim3((a.ny + 255) / 256), dim3(256), 0, s, a, a.B, 2, 1);
-      hipLaunchKernelGGL((k_post2<ND, true, 2, TW, PART>), grid, dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_post2<ND, true, 2, TW, PART, CAP>), grid, dim3(256), 0, s, a);
     }
-    else hipLaunchKernelGGL((k_post2<ND, false, 2, TW, PART>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_post2<ND, false, 2, TW, PART, CAP>), grid, dim3(256), 0, s, a);
   }
 }
 
-template <int ND>
+template <int ND, bool CAP>
 static void courant_nd(const StepArgs& a, hipStream_t s) {
-  if (a.ndim == 1) hipLaunchKernelGGL((k_courant<ND, 1>), dim3((a.nx + 255) / 256), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_courant<ND, 2>), dim3((a.nx + 63) / 64, (a.ny + 3) / 4), dim3(256), 0, s, a);
+  if (a.ndim == 1) hipLaunchKernelGGL((k_courant<ND, 1, CAP>), dim3((a.nx + 255) / 256), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_courant<ND, 2, CAP>), dim3((a.nx + 63) / 64, (a.ny + 3) / 4), dim3(256), 0, s, a);
 }
 
-#define SHARK_ND_SWITCH(FN)                          \
-  switch (a.nd) {                                    \
-    case 0: FN<0>(a, s); break;                      \
-    case 1: FN<1>(a, s); break;                      \
-    case 2: FN<2>(a, s); break;                      \
-    case 3: FN<3>(a, s); break;                      \
-    case 4: FN<4>(a, s); break;                      \
-    case 8: FN<8>(a, s); break;                      \
-    case 16: FN<16>(a, s); break;                    \
-    case 64: FN<64>(a, s); break;                    \
-    default: break;                                  \
+// Exact instantiations for the species counts of the reference's test and benchmark runs; any other
+// count up to 64 runs in the capacity instantiation of the next size up.
+#define SHARK_ND_SWITCH(FN)                                   \
+  switch (a.nd) {                                             \
+    case 0: FN<0, false>(a, s); break;                        \
+    case 1: FN<1, false>(a, s); break;                        \
+    case 2: FN<2, false>(a, s); break;                        \
+    case 3: FN<3, false>(a, s); break;                        \
+    case 4: FN<4, false>(a, s); break;                        \
+    case 8: FN<8, false>(a, s); break;                        \
+    case 16: FN<16, false>(a, s); break;                      \
+    case 64: FN<64, false>(a, s); break;                      \
+    default:                                                  \
+      if (a.nd <= 8) FN<8, true>(a, s);                       \
+      else if (a.nd <= 16) FN<16, true>(a, s);                \
+      else if (a.nd <= 32) FN<32, true>(a, s);                \
+      else FN<64, true>(a, s);                                \
+      break;                                                  \
   }
 
 void launch_post(const StepArgs& a, hipStream_t s) { SHARK_ND_SWITCH(post_nd) }
@@ -1342,8 +1359,6 @@ void launch_ctoprim(const StepArgs& a, double* q, hipStream_t s) {
   hipLaunchKernelGGL(k_ctoprim, dim3((a.nc + 255) / 256), dim3(256), 0, s, a, q);
 }
 
-bool nd_supported(int nd) {
-  return nd == 0 || nd == 1 || nd == 2 || nd == 3 || nd == 4 || nd == 8 || nd == 16 || nd == 64;
-}
+bool nd_supported(int nd) { return nd >= 0 && nd <= 64; }
 
 }  // namespace shark

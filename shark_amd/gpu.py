@@ -29,11 +29,11 @@ EXPORTS = [
     "shark_growth_destroy", "shark_growth_kernel_time", "shark_growth_charge",
     "shark_growth_set_cell_dt", "shark_growth_features", "shark_growth_set_surrogate",
     "shark_growth_surrogate_stats", "shark_gpu_attach_growth", "shark_charge_marchand",
-    "shark_charge_marchand_noalloc",
+    "shark_charge_marchand_noalloc", "shark_growth_zmean", "shark_growth_set_surrogate_residual",
     "shark_mlp_create", "shark_mlp_num_params", "shark_mlp_set_params", "shark_mlp_get_params",
     "shark_mlp_forward", "shark_mlp_forward_layered", "shark_mlp_grads", "shark_mlp_grads_weighted",
     "shark_mlp_set_data_weighted", "shark_mlp_set_data", "shark_mlp_train",
-    "shark_mlp_kernel_time", "shark_mlp_destroy",
+    "shark_mlp_kernel_time", "shark_mlp_destroy", "shark_hbm_probe",
 ]
 
 
@@ -91,8 +91,8 @@ def output_write(info, iout, cfg, q, sdust, position, radii_c):
 class ShARKTune(C.Structure):
     """struct shark_tune: launch shapes of the step kernels."""
     _fields_ = [("strip_h", C.c_int32), ("post_tw", C.c_int32), ("xcd_order", C.c_int32),
-                ("strip_gas", C.c_int32), ("reserved_i", C.c_int32 * 4), ("hydro_ms", C.c_double),
-                ("post_ms", C.c_double), ("reserved", C.c_double * 6)]
+                ("strip_gas", C.c_int32), ("reserved_i", C.c_int32 * 4),
+                ("hydro_ms", C.c_double), ("post_ms", C.c_double), ("reserved", C.c_double * 6)]
 
     def as_dict(self):
         return dict(strip_h=self.strip_h, strip_gas=self.strip_gas, post_tw=self.post_tw,
@@ -142,6 +142,17 @@ def load():
     lib.shark_gpu_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(ShARKTune)]
     _LIB = lib
     return lib
+
+
+def hbm_probe(nbytes=1 << 30, reps=20):
+    """shark_hbm_probe: GB/s of a plain write, copy (read + write bytes) and read kernel."""
+    lib = load()
+    lib.shark_hbm_probe.argtypes = [C.c_int64, C.c_int, _P]
+    out = (C.c_double * 3)()
+    rc = lib.shark_hbm_probe(int(nbytes), int(reps), out)
+    if rc != 0:
+        raise SharkGPUError(f"shark_hbm_probe failed ({rc})")
+    return {"write_GBs": out[0], "copy_GBs": out[1], "read_GBs": out[2]}
 
 
 def check(rc):

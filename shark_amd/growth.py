@@ -231,19 +231,39 @@ class Growth:
         gpu.check(self.L.shark_growth_features(self.h, float(dt), f.ctypes.data_as(C.POINTER(C.c_float))))
         return f
 
-    def set_surrogate(self, net, norm, alpha=0.0, zeta=5e-17, mu_ions=25.0, resid_max=1e-6):
-        """Attach a shark_amd.mlp.MLP (kept referenced here) with its normalisation vector."""
+    def set_surrogate(self, net, norm, alpha=0.0, zeta=5e-17, mu_ions=25.0, resid_max=3e-3):
+        """Attach a shark_amd.mlp.MLP (kept referenced here) with its normalisation vector.
+        resid_max: cells whose raw predicted mass residual exceeds it run the plain step (trained
+        weights carry their own calibrated value, surrogate.resid_max(meta))."""
         self._net = net
         self._norm = np.ascontiguousarray(norm, dtype=np.float32)
         gpu.check(self.L.shark_growth_set_surrogate(
             self.h, None if net is None else net.h, self._norm.ctypes.data_as(C.POINTER(C.c_float)),
             float(alpha), float(zeta), float(mu_ions), float(resid_max)))
 
+    def set_surrogate_residual(self, net2, out_norm2):
+        """Attach a residual network (its outputs, de-standardised with out_norm2 = out_mean2,
+        out_std2, are added to the first network's); None detaches."""
+        self.L.shark_growth_set_surrogate_residual.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]
+        self._net2 = net2
+        self._norm2 = None if net2 is None else np.ascontiguousarray(out_norm2, dtype=np.float32)
+        gpu.check(self.L.shark_growth_set_surrogate_residual(
+            self.h, None if net2 is None else net2.h,
+            None if net2 is None else self._norm2.ctypes.data_as(C.POINTER(C.c_float))))
+
     def surrogate_stats(self):
         n = C.c_int64()
         r = C.c_double()
         gpu.check(self.L.shark_growth_surrogate_stats(self.h, C.byref(n), C.byref(r)))
         return n.value, r.value
+
+    def zmean(self):
+        """Per-cell <Z> of the last learned step (network head; charge solve where it fell back)
+        or of the last charge()."""
+        self.L.shark_growth_zmean.argtypes = [C.c_void_p, _P]
+        z = np.empty(self.ncells)
+        gpu.check(self.L.shark_growth_zmean(self.h, z.ctypes.data_as(_P)))
+        return z
 
     def kernel_ms(self):
         ms = C.c_double()

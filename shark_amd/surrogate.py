@@ -99,10 +99,16 @@ def train(X, Y, norm, hidden=256, layers=4, epochs=(30, 20, 10), lrs=(2e-3, 5e-4
     return net, np.array(hist)
 
 
-def save(path, net, norm, meta):
+def save(path, net, norm, meta, net2=None, out_norm2=None):
+    """Weights, normalisation and metadata; with net2, the residual network and its output
+    normalisation (out_mean2, out_std2) as well."""
+    extra = {}
+    if net2 is not None:
+        extra = dict(params2=net2.get_params(), norm2=np.asarray(out_norm2, np.float32),
+                     dims2=np.array([net2.cfg.n_in, net2.cfg.n_hidden, net2.cfg.n_layers, net2.cfg.n_out]))
     np.savez_compressed(path, params=net.get_params(), norm=norm,
                         dims=np.array([net.cfg.n_in, net.cfg.n_hidden, net.cfg.n_layers, net.cfg.n_out]),
-                        meta=json.dumps(meta))
+                        meta=json.dumps(meta), **extra)
 
 
 def load(path, max_batch=65536):
@@ -111,6 +117,46 @@ def load(path, max_batch=65536):
     cfg = M.make_cfg(n_in=n_in, n_hidden=n_hidden, n_layers=n_layers, n_out=n_out, max_batch=max_batch)
     net = M.MLP(cfg, z["params"])
     return net, z["norm"], json.loads(str(z["meta"]))
+
+
+def load_residual(path, max_batch=65536):
+    """The residual network of a weights file and its output normalisation, or (None, None)."""
+    z = np.load(path, allow_pickle=False)
+    if "params2" not in z:
+        return None, None
+    n_in, n_hidden, n_layers, n_out = [int(v) for v in z["dims2"]]
+    cfg = M.make_cfg(n_in=n_in, n_hidden=n_hidden, n_layers=n_layers, n_out=n_out, max_batch=max_batch)
+    return M.MLP(cfg, z["params2"]), z["norm2"]
+
+
+class Learned:
+    """The learned step's networks from a weights file, attached to growth states with their
+    calibrated fallback threshold."""
+
+    def __init__(self, path=None, max_batch=65536):
+        path = path or os.path.join(DATA, "surrogate_disk64.npz")
+        self.net, self.norm, self.meta = load(path, max_batch)
+        self.net2, self.norm2 = load_residual(path, max_batch)
+
+    def attach(self, g, resid_max=None):
+        g.set_surrogate(self.net, self.norm, resid_max=resid_max if resid_max is not None else
+                        globals()["resid_max"](self.meta))
+        if self.net2 is not None:
+            g.set_surrogate_residual(self.net2, self.norm2)
+
+    def close(self):
+        for n in (self.net, self.net2):
+            if n is not None:
+                n.close()
+
+
+RESID_MAX_DEFAULT = 3e-3
+
+
+def resid_max(meta):
+    """The raw mass-residual threshold of the learned step's fallback calibrated for these weights
+    at training time (meta["resid_max"]), or the default."""
+    return float(meta.get("resid_max", RESID_MAX_DEFAULT))
 
 
 def mass_l1(a, ref):

@@ -1,6 +1,6 @@
 /* Sweep of the division used by the HIP kernels for shared divisors (exdiv in
- * shark_amd/csrc/physics.h): q = fma(fma(-b, a*y, a), y, a*y) with y = RN(1/b), a zero dividend
- * taking a*y.  Compared with IEEE a/b on random operands over a wide exponent range, on divisors
+ * shark_amd/csrc/physics.h): q = fma(fma(-b, a*y, a), y, a*y) with y = RN(1/b), carrying the
+ * sign of a*y (so a zero dividend gives the zero of a/b's sign).  Compared with IEEE a/b on random operands over a wide exponent range, on divisors
  * with all-ones significands, and on zeros of both signs.  Prints the mismatch count. */
 #include <math.h>
 #include <stdint.h>
@@ -22,7 +22,7 @@ static double exdiv(double a, double b, double y) {
   const double q0 = a * y;
   const double r = fma(-b, q0, a);
   const double q = fma(r, y, q0);
-  return a == 0.0 ? q0 : q;
+  return copysign(q, q0);
 }
 static int same(double x, double y) { return memcmp(&x, &y, 8) == 0; }
 

@@ -342,6 +342,13 @@ CASES = {
                     CDISK.format(cfl=0.4, slope=3, visc=".false.", smooth="2.0d0", alpha="1d-2", mplanet="0.0d0"), 30),
     "cdisk_visc_gas": ("disk", 32, 24, 0, 0, 0, 1, 0,
                        CDISK.format(cfl=0.5, slope=2, visc=".true.", smooth="0.5d0", alpha="5d-2", mplanet="0.0d0"), 30),
+    # species counts without an exact kernel instantiation (they run in the capacity kernels)
+    "disk_2d_n5": ("disk_polar", 24, 32, 5, 2, 0, 1, 0,
+                   DISK.format(slope=1, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 25),
+    "disk_2d_n12": ("disk_polar", 20, 24, 12, 2, 0, 1, 0,
+                    DISK.format(slope=3, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 25),
+    "disk_2d_n20": ("disk_polar", 16, 24, 20, 2, 0, 1, 0,
+                    DISK.format(slope=2, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 20),
     # 100 steps: the shearing box, the polar disk and the PSI namelist's 8 species
     "si_2d_100": ("SI", 32, 32, 2, 0, 0, 1, 0, SI.format(slope=1), 100),
     "disk_2d_100": ("disk_polar", 32, 48, 4, 2, 0, 1, 0,

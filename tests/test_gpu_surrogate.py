@@ -43,7 +43,7 @@ def test_surrogate_vs_plain_fp64(trained):
     n = st["sigma"].size
     g = G.Growth(p, n)
     try:
-        g.set_surrogate(net, norm)
+        g.set_surrogate(net, norm, resid_max=1e30)    # the network alone (no residual fallback)
         g.set_cell_dt(st["dt"])
         g.upload(st["rhod"], st["sigma"], st["cs"], st["r"])
         g.run(G.SURROGATE, 0.0)
@@ -92,3 +92,45 @@ def test_fallback_cells_are_exactly_plain(trained):
         g.close()
     assert nfb == sigma.size
     assert np.array_equal(out, ref)
+
+
+def raw_residuals(net, norm, X, rhod):
+    """|sum_k share_k exp(change_k) - 1| of the network's raw prediction (before normalisation)."""
+    ni, no = X.shape[1], NB + 1
+    Xs, _ = S.standardise(X, np.zeros((X.shape[0], no), np.float32), norm)
+    chg = net.forward(Xs) * norm[2 * ni + no:] + norm[2 * ni:2 * ni + no]
+    sh = (rhod / rhod.sum(0)).T
+    return np.abs((sh * np.exp(chg[:, :NB].astype(np.float64))).sum(1) - 1.0)
+
+
+def test_raw_mass_residual_fallback(trained):
+    """Coagulation conserves mass, so a consistent prediction has a zero raw mass residual; cells
+    above the threshold run the plain step and equal it exactly, the others keep the network's
+    answer.  With the median residual as threshold about half the cells fall back."""
+    p, sd, net, norm, _ = trained
+    X, _, st = S.sample_states(p, sd, nr=32, nphi=16, seed=11, dt_lo=1e-6, dt_hi=1e-5, age_max=1e-4,
+                               gen_mode=G.FP64)
+    n = st["sigma"].size
+    raw = raw_residuals(net, norm, X, st["rhod"])
+    th = float(np.median(raw))
+    g = G.Growth(p, n)
+    try:
+        g.set_cell_dt(st["dt"])
+        g.set_surrogate(net, norm, resid_max=th)
+        g.upload(st["rhod"], st["sigma"], st["cs"], st["r"])
+        g.run(G.SURROGATE, 0.0)
+        out, nsub = g.download()
+        nfb, _ = g.surrogate_stats()
+        g.set_surrogate(net, norm, resid_max=1e30)
+        g.upload(st["rhod"], st["sigma"], st["cs"], st["r"])
+        g.run(G.SURROGATE, 0.0)
+        net_only, _ = g.download()
+    finally:
+        g.close()
+    fb = nsub > 0                                   # fallback cells ran plain sub-steps
+    print(f"threshold {th:.2e}: {nfb}/{n} fell back, host count {(raw > th).sum()}", flush=True)
+    assert fb.sum() == nfb
+    assert abs(nfb - (raw > th).sum()) <= 0.02 * n  # the device's FP32 sum rounds near the threshold
+    assert 0.3 * n < nfb < 0.7 * n
+    assert np.array_equal(out[:, fb], st["rhod_next"][:, fb])
+    assert np.array_equal(out[:, ~fb], net_only[:, ~fb])
