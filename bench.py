@@ -170,14 +170,15 @@ def growth_block(steps_probe_dt=2e-8, surrogate_cells=16384, seed=99):
 
     wpath = os.path.join(ROOT, "shark_amd", "data", "surrogate_disk64.npz")
     if os.path.exists(wpath):
-        net, norm, meta = S.load(wpath)
+        L = S.Learned(wpath)
+        meta = L.meta
         nphi = max(1, surrogate_cells // 1024)
         _, _, st = S.sample_states(p, sd, 1024, nphi, seed=seed, age_max=meta["age_max"],
-                                   dt_lo=meta["dt_range"][0], dt_hi=meta["dt_range"][1])
+                                   dt_lo=meta["dt_range"][0], dt_hi=meta["dt_range"][1], gen_mode=G.FP64)
         ns = st["sigma"].size
         g = G.Growth(p, ns)
         try:
-            g.set_surrogate(net, norm, resid_max=S.resid_max(meta))
+            L.attach(g)
             g.set_cell_dt(st["dt"])
             g.upload(st["rhod"], st["sigma"], st["cs"], st["r"])
             g.run(G.SURROGATE, 0.0)
@@ -190,7 +191,7 @@ def growth_block(steps_probe_dt=2e-8, surrogate_cells=16384, seed=99):
             ms_f = g.kernel_ms()
         finally:
             g.close()
-            net.close()
+            L.close()
         err = S.mass_l1(so, fo)
         out["surrogate"] = {
             "cells": ns, "dt_range": meta["dt_range"], "age_max": meta["age_max"],
@@ -199,7 +200,10 @@ def growth_block(steps_probe_dt=2e-8, surrogate_cells=16384, seed=99):
             "l1_vs_fp64_mean": float(err.mean()), "l1_vs_fp64_p99": float(np.quantile(err, 0.99)),
             "no_change_l1_mean": float(S.mass_l1(st["rhod"], fo).mean()),
             "l1_per_bin": per_bin_l1(so, fo),
-            "fallback_cells": int(nfb), "raw_mass_residual_mean": raw,
+            "networks": 2 if L.net2 is not None else 1,
+            "activations": "bf16 hi+lo (first network), bf16 (residual network)",
+            "fallback_cells": int(nfb), "fallback_fraction": nfb / ns,
+            "fallback_threshold": S.resid_max(meta), "raw_mass_residual_mean": raw,
             "mass_residual_max": float(np.abs(so.sum(0) / st["rhod"].sum(0) - 1).max())}
     return out
 
@@ -259,7 +263,7 @@ def main():
     out.update({k: v for k, v in res.items() if k not in ("value", "ms_per_step")})
     out["hbm_attainable"] = attainable
     if learned:
-        out["dtype"] = "f64 step; learned growth bf16 x bf16 -> f32"
+        out["dtype"] = "f64 step; learned growth bf16 weights x bf16 hi+lo activations -> f32"
         # the oracle's FP64 coagulation needs up to ~1700 sub-steps per cell at this dt: a CPU
         # sample of the coupled step would not finish in a bounded time
         out["cpu_baseline"] = None
@@ -315,9 +319,9 @@ def time_workload(key, steps, warmup, tune=True, events=False, attainable=None):
     if learned:
         from shark_amd import growth as G
         from shark_amd import surrogate as S
-        net, norm, smeta = S.load(os.path.join(ROOT, "shark_amd", "data", "surrogate_disk64.npz"))
+        net = S.Learned()
         gr = G.Growth(made[3], (cfg.nx + 4) * (cfg.ny + 4))
-        gr.set_surrogate(net, norm, resid_max=S.resid_max(smeta))
+        net.attach(gr)
         g.attach_growth(gr, G.SURROGATE)
     g.run(warmup)
 
@@ -338,7 +342,8 @@ def time_workload(key, steps, warmup, tune=True, events=False, attainable=None):
     if learned:
         nfb, raw = gr.surrogate_stats()
         learned_info = {"growth_ms_per_step": kt["growth"], "fallback_cells_last_step": int(nfb),
-                        "raw_mass_residual_mean": raw}
+                        "fallback_fraction_last_step": nfb / cells, "raw_mass_residual_mean": raw,
+                        "networks": 2 if net.net2 is not None else 1}
 
     nv = 5 + 4 * cfg.ndust
     bytes_hydro = cells * 8 * (2 * nv + 1)            # read u_prim + cs, write u_prim
@@ -370,8 +375,8 @@ def time_workload(key, steps, warmup, tune=True, events=False, attainable=None):
         mlp = mlp_roofline(ncell_all)
         if kt["growth"] > dom_ms:
             # the learned step dominates: rate its network products against the dense BF16 peak
-            # (2 flops per multiply-add over every layer, every cell incl. ghosts)
-            flops = 2.0 * ncell_all * sum(a * b for a, b in zip(MLP_DIMS[:-1], MLP_DIMS[1:]))
+            # (2 flops per multiply-add over every layer of each network, every cell incl. ghosts)
+            flops = mlp["networks"] * 2.0 * ncell_all * sum(a * b for a, b in zip(MLP_DIMS[:-1], MLP_DIMS[1:]))
             tf = flops / (kt["growth"] * 1e-3) / 1e12
             roof = {"bound": "mfma", "kernel": "growth", "achieved": tf, "peak": 2500.0,
                     "unit": "TFLOP/s", "frac": tf / 2500.0, "traffic": None,
@@ -407,20 +412,33 @@ def time_workload(key, steps, warmup, tune=True, events=False, attainable=None):
 
 def mlp_roofline(ncells, seed=5):
     """k_mlp_infer (the fused network of the learned step) alone over `ncells` feature rows, one
-    launch, against the dense BF16 MFMA peak."""
+    launch per network, as the learned step runs it (BF16 hi + lo activations: two products per
+    weight fragment), against the dense BF16 MFMA peak; the single-product variant beside it."""
     import numpy as np
     from shark_amd import surrogate as S
-    net, _, _ = S.load(os.path.join(ROOT, "shark_amd", "data", "surrogate_disk64.npz"), max_batch=ncells)
+    L = S.Learned(max_batch=ncells)
+    nets = [n for n in (L.net, L.net2) if n is not None]
     try:
         x = np.random.default_rng(seed).standard_normal((ncells, MLP_DIMS[0])).astype(np.float32)
-        net.forward(x)
-        ms = net.kernel_ms()
+        ms = 0.0
+        for i, n in enumerate(nets):       # the residual network runs the single-product kernel
+            n.forward(x, hilo=i == 0)
+            ms += n.kernel_ms()
+        L.net.forward(x)
+        ms_single = L.net.kernel_ms()
     finally:
-        net.close()
-    flops = 2.0 * ncells * sum(a * b for a, b in zip(MLP_DIMS[:-1], MLP_DIMS[1:]))
+        L.close()
+    flops1 = 2.0 * ncells * sum(a * b for a, b in zip(MLP_DIMS[:-1], MLP_DIMS[1:]))
+    flops = flops1 * len(nets)
     tf = flops / (ms * 1e-3) / 1e12
-    return {"kernel": "k_mlp_infer", "rows": ncells, "ms": ms, "algorithmic_flops": flops,
-            "achieved_TFLOPs": tf, "peak_TFLOPs": 2500.0, "frac": tf / 2500.0}
+    issued = flops1 * (len(nets) + 1)      # two products per fragment in the first network
+    return {"kernel": "k_mlp_infer", "rows": ncells, "networks": len(nets),
+            "activations": "bf16 hi+lo (first network), bf16 (residual network)",
+            "ms": ms, "algorithmic_flops": flops, "achieved_TFLOPs": tf, "peak_TFLOPs": 2500.0,
+            "frac": tf / 2500.0, "mfma_flops_issued": issued,
+            "frac_issued": issued / (ms * 1e-3) / 1e12 / 2500.0,
+            "single_product_variant": {"ms_per_network": ms_single,
+                                       "achieved_TFLOPs": flops1 / (ms_single * 1e-3) / 1e12}}
 
 
 if __name__ == "__main__":

@@ -47,6 +47,11 @@ int shark_mlp_get_params(shark_mlp* m, float* params);
 /* y (n x n_out) = net(x (n x n_in)); n <= max_batch; host arrays.  Runs the fused inference
  * kernel (all layers in one launch, activations in LDS) when every width is <= 256. */
 int shark_mlp_forward(shark_mlp* m, const float* x, int64_t n, float* y);
+/* The fused forward with the activations carried as BF16 hi + lo pairs (two products per weight
+ * fragment): the weights stay BF16, the activations keep about 16 significant bits, so the network
+ * is evaluated without the BF16 rounding noise of its activations.  Needs <= 8 layers, widths <= 256
+ * and hidden widths that are multiples of 32.  The learned growth step uses it. */
+int shark_mlp_forward_hilo(shark_mlp* m, const float* x, int64_t n, float* y);
 /* The same forward as the training path computes it (one product kernel per layer).  Its outputs
  * equal shark_mlp_forward's bit for bit. */
 int shark_mlp_forward_layered(shark_mlp* m, const float* x, int64_t n, float* y);

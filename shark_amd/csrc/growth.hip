@@ -1498,10 +1498,14 @@ static int surrogate_run(shark_growth* g, double dt) {
     GCHK(hipGetLastError());
     hipLaunchKernelGGL(k_surr_feat, dim3(nblk), dim3(256), 0, s, sa);
     GCHK(hipGetLastError());
-    int rc = shark::mlp_infer_device(g->mlp, g->dfeat, nch, g->dy, s, B, B);
+    // the first network with hi + lo activations: the BF16 rounding of its activations would add
+    // about as much error as the network's own.  The residual network's outputs are a few 1e-4 of
+    // the first's, so its own BF16 rounding noise stays far below the target and it runs the
+    // single-product kernel.
+    int rc = shark::mlp_infer_device(g->mlp, g->dfeat, nch, g->dy, s, B, B, true);
     if (rc) return rc;
     if (g->mlp2) {
-      rc = shark::mlp_infer_device(g->mlp2, g->dfeat, nch, g->dy2, s, B, B);
+      rc = shark::mlp_infer_device(g->mlp2, g->dfeat, nch, g->dy2, s, B, B, false);
       if (rc) return rc;
     }
     hipLaunchKernelGGL(k_surr_post, dim3(nblk), dim3(256), 0, s, sa);

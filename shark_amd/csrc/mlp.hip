@@ -49,6 +49,7 @@ __device__ __forceinline__ unsigned short to_bf16(float x) {
   u += 0x7FFFu + ((u >> 16) & 1u);
   return (unsigned short)(u >> 16);
 }
+__device__ __forceinline__ float from_bf16(unsigned short u) { return __uint_as_float((unsigned int)u << 16); }
 
 // SiLU from the hardware exp2 and reciprocal (v_exp_f32, v_rcp_f32: about 1 ulp each), shared
 // by the training forward and the fused inference so the two stay bit-identical; the activations
@@ -234,31 +235,39 @@ __global__ void k_pack_frag(const float* W, int din, int dout, int nks, int nbn,
   out[i] = (n < dout && k < din) ? to_bf16(W[(int64_t)n * din + k]) : (unsigned short)0;
 }
 
+__device__ __forceinline__ f32x2 bf16x2_to_f32(unsigned int u) {
+  return f32x2{__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u)};
+}
+
 __device__ __forceinline__ unsigned int pack_bf16x2(float a, float b) {
   const f32x2 v = {a, b};
   return __builtin_bit_cast(unsigned int, __builtin_convertvector(v, bf16x2));
 }
 
-// One layer for all FM samples of the workgroup.  Hidden layers (every width a multiple of 32):
-// bias + SiLU, BF16 back into act once every wave has read the layer's input.  Last layer: bias,
-// FP32 out to y.
-template <bool LAST>
-__device__ __forceinline__ void infer_layer(const InferArgs& g, int l, unsigned short (*act)[FLD], int64_t r0,
-                                            int lane, int w) {
+// One layer for all FMT samples of the workgroup (FMT / 32 blocks of 32 samples per wave).  Hidden
+// layers (every width a multiple of 32): bias + SiLU, BF16 back into act once every wave has read
+// the layer's input.  Last layer: bias, FP32 out to y.  HL: activations carried as BF16 pairs
+// hi + lo (lo = BF16(v - hi)), two products per weight fragment, so the activations keep about 16
+// significant bits; the weights stay BF16.
+template <bool LAST, int FMT, bool HL>
+__device__ __forceinline__ void infer_layer(const InferArgs& g, int l, unsigned short (*act)[FLD],
+                                            unsigned short (*alo)[FLD], int64_t r0, int lane, int w) {
+  constexpr int MB = FMT / 32;
   const int nks = g.kpad[l] >> 4, nbn = g.nbn[l];
   const float* bias = g.p + g.boff[l];
   const int dout = g.dout[l];
   const u32x4* wf = reinterpret_cast<const u32x4*>(g.wf + g.foff[l]);
   // two passes, feature blocks w and w + 4; a hidden layer's outputs wait in registers as packed
   // BF16 until every wave has finished reading the layer's input
-  uint2 pk[2][4][4];
+  uint2 pk[2][MB][4];
+  uint2 pl[2][HL ? MB : 1][4];
 #pragma unroll
   for (int pass = 0; pass < 2; ++pass) {
     const int nb = w + 4 * pass;
     if (nb >= nbn) break;
-    f32x16 acc[4];
+    f32x16 acc[MB];
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
+    for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[mb][r] = 0.f;
     const u32x4* wp = wf + (int64_t)nb * nks * 64 + lane;
@@ -274,10 +283,16 @@ __device__ __forceinline__ void infer_layer(const InferArgs& g, int l, unsigned 
           const u32x4 c = b[u];
           if (ks + FPF < nks) b[u] = wp[(ks + FPF) * 64];
 #pragma unroll
-          for (int mb = 0; mb < 4; ++mb) {
-            const u32x4 a = *reinterpret_cast<const u32x4*>(&act[32 * mb + (lane & 31)][16 * ks + 8 * (lane >> 5)]);
+          for (int mb = 0; mb < MB; ++mb) {
+            const int row = 32 * mb + (lane & 31), col = 16 * ks + 8 * (lane >> 5);
+            const u32x4 a = *reinterpret_cast<const u32x4*>(&act[row][col]);
             acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, c),
                                                               __builtin_bit_cast(bf16x8, a), acc[mb], 0, 0, 0);
+            if constexpr (HL) {
+              const u32x4 al = *reinterpret_cast<const u32x4*>(&alo[row][col]);
+              acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, c),
+                                                                __builtin_bit_cast(bf16x8, al), acc[mb], 0, 0, 0);
+            }
           }
         }
       }
@@ -289,13 +304,21 @@ __device__ __forceinline__ void infer_layer(const InferArgs& g, int l, unsigned 
       if constexpr (!LAST) {
         const float4 bv = *reinterpret_cast<const float4*>(bias + n0);
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
+        for (int mb = 0; mb < MB; ++mb) {
           float v0 = acc[mb][4 * q], v1 = acc[mb][4 * q + 1], v2 = acc[mb][4 * q + 2], v3 = acc[mb][4 * q + 3];
           v0 += bv.x;
           v1 += bv.y;
           v2 += bv.z;
           v3 += bv.w;
-          pk[pass][mb][q] = uint2{pack_bf16x2(silu(v0), silu(v1)), pack_bf16x2(silu(v2), silu(v3))};
+          v0 = silu(v0);
+          v1 = silu(v1);
+          v2 = silu(v2);
+          v3 = silu(v3);
+          pk[pass][mb][q] = uint2{pack_bf16x2(v0, v1), pack_bf16x2(v2, v3)};
+          if constexpr (HL) {
+            const f32x2 h01 = bf16x2_to_f32(pk[pass][mb][q].x), h23 = bf16x2_to_f32(pk[pass][mb][q].y);
+            pl[pass][mb][q] = uint2{pack_bf16x2(v0 - h01.x, v1 - h01.y), pack_bf16x2(v2 - h23.x, v3 - h23.y)};
+          }
         }
       } else {
 #pragma unroll
@@ -304,7 +327,7 @@ __device__ __forceinline__ void infer_layer(const InferArgs& g, int l, unsigned 
           const float bv = bias[n0 + i];
           float* yc = g.y + (int64_t)(n0 + i) * g.yc;
 #pragma unroll
-          for (int mb = 0; mb < 4; ++mb) {
+          for (int mb = 0; mb < MB; ++mb) {
             const int64_t row = r0 + 32 * mb + (lane & 31);
             float v = acc[mb][4 * q + i];
             v += bv;
@@ -323,31 +346,37 @@ __device__ __forceinline__ void infer_layer(const InferArgs& g, int l, unsigned 
 #pragma unroll
       for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb)
-          *reinterpret_cast<uint2*>(&act[32 * mb + (lane & 31)][32 * nb + 8 * q + 4 * (lane >> 5)]) = pk[pass][mb][q];
+        for (int mb = 0; mb < MB; ++mb) {
+          const int row = 32 * mb + (lane & 31), col = 32 * nb + 8 * q + 4 * (lane >> 5);
+          *reinterpret_cast<uint2*>(&act[row][col]) = pk[pass][mb][q];
+          if constexpr (HL) *reinterpret_cast<uint2*>(&alo[row][col]) = pl[pass][mb][q];
+        }
     }
     __syncthreads();
   }
 }
 
+template <int FMT, bool HL>
 __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
-  __shared__ __attribute__((aligned(16))) unsigned short act[FM][FLD];
+  __shared__ __attribute__((aligned(16))) unsigned short act[FMT][FLD];
+  __shared__ __attribute__((aligned(16))) unsigned short alo[HL ? FMT : 1][FLD];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int64_t r0 = (int64_t)blockIdx.x * FM;
+  const int64_t r0 = (int64_t)blockIdx.x * FMT;
   {
     const int kp = g.kpad[0];
     const bool feat_major = g.xc != 1;   // consecutive lanes take consecutive samples
-    for (int e = tid; e < FM * kp; e += 256) {
+    for (int e = tid; e < FMT * kp; e += 256) {
       int m, k;
-      if (feat_major) { m = e % FM; k = e / FM; } else { m = e / kp; k = e - m * kp; }
+      if (feat_major) { m = e % FMT; k = e / FMT; } else { m = e / kp; k = e - m * kp; }
       const int64_t row = r0 + m;
       const float v = (row < g.n && k < g.n_in) ? g.x[row * g.xr + k * g.xc] : 0.f;
       act[m][k] = to_bf16(v);
+      if constexpr (HL) alo[m][k] = to_bf16(v - from_bf16(act[m][k]));
     }
   }
   __syncthreads();
-  for (int l = 0; l < g.L - 1; ++l) infer_layer<false>(g, l, act, r0, lane, w);
-  infer_layer<true>(g, g.L - 1, act, r0, lane, w);
+  for (int l = 0; l < g.L - 1; ++l) infer_layer<false, FMT, HL>(g, l, act, alo, r0, lane, w);
+  infer_layer<true, FMT, HL>(g, g.L - 1, act, alo, r0, lane, w);
 }
 
 }  // namespace
@@ -502,7 +531,7 @@ static int pack_weights(shark_mlp* m, hipStream_t s) {
 // else the layer-by-layer forward in chunks of max_batch rows (row-major data only).
 // ldx / ldy > 0: x / y are feature-major, element (row, feature) at [feature * ld + row].
 int mlp_infer_device(shark_mlp* m, const float* x, int64_t n, float* y, hipStream_t s, int64_t ldx,
-                     int64_t ldy) {
+                     int64_t ldy, bool hilo) {
   if (n <= 0) return 0;
   if (!fused_ok(m)) {
     if (ldx > 0 || ldy > 0) return mfail("mlp inference: feature-major data needs every width <= 256");
@@ -542,9 +571,13 @@ int mlp_infer_device(shark_mlp* m, const float* x, int64_t n, float* y, hipStrea
     g.boff[l] = m->boff[l];
     off += (int64_t)g.nbn[l] * (g.kpad[l] / 16) * 512;
   }
-  const int64_t nblk = (n + FM - 1) / FM;
+  // hi + lo activations take twice the LDS per sample: 64 samples per workgroup keep two
+  // workgroups per CU
+  const int fm = hilo ? FM / 2 : FM;
+  const int64_t nblk = (n + fm - 1) / fm;
   if (nblk > 0x7fffffff) return mfail("mlp inference: too many rows");
-  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)nblk), dim3(256), 0, s, g);
+  if (hilo) hipLaunchKernelGGL((k_mlp_infer<FM / 2, true>), dim3((unsigned)nblk), dim3(256), 0, s, g);
+  else hipLaunchKernelGGL((k_mlp_infer<FM, false>), dim3((unsigned)nblk), dim3(256), 0, s, g);
   MCHK(hipGetLastError());
   return 0;
 }
@@ -701,13 +734,14 @@ int shark_mlp_get_params(shark_mlp* m, float* params) {
   return 0;
 }
 
-static int forward_host(shark_mlp* m, const float* x, int64_t n, float* y, bool fused) {
+static int forward_host(shark_mlp* m, const float* x, int64_t n, float* y, bool fused, bool hilo = false) {
   if (!m || !x || !y) return mfail("shark_mlp_forward: null argument");
   if (n < 1 || n > m->cfg.max_batch) return mfail("shark_mlp_forward: n must lie in [1, max_batch]");
   hipStream_t s = m->stream;
   MCHK(hipMemcpyAsync(m->dx, x, sizeof(float) * n * m->cfg.n_in, hipMemcpyHostToDevice, s));
   MCHK(hipEventRecord(m->e0, s));
-  int rc = fused ? shark::mlp_infer_device(m, m->dx, n, m->dy, s) : shark::mlp_forward_device(m, m->dx, n, m->dy, s);
+  int rc = fused ? shark::mlp_infer_device(m, m->dx, n, m->dy, s, 0, 0, hilo)
+                 : shark::mlp_forward_device(m, m->dx, n, m->dy, s);
   if (rc) return rc;
   MCHK(hipEventRecord(m->e1, s));
   MCHK(hipMemcpyAsync(y, m->dy, sizeof(float) * n * m->cfg.n_out, hipMemcpyDeviceToHost, s));
@@ -719,6 +753,10 @@ static int forward_host(shark_mlp* m, const float* x, int64_t n, float* y, bool 
 }
 
 int shark_mlp_forward(shark_mlp* m, const float* x, int64_t n, float* y) { return forward_host(m, x, n, y, true); }
+int shark_mlp_forward_hilo(shark_mlp* m, const float* x, int64_t n, float* y) {
+  if (m && !shark::mlp_fused_ok(m)) return mfail("shark_mlp_forward_hilo: needs the fused kernel's shape");
+  return forward_host(m, x, n, y, true, true);
+}
 int shark_mlp_forward_layered(shark_mlp* m, const float* x, int64_t n, float* y) {
   return forward_host(m, x, n, y, false);
 }

@@ -31,7 +31,7 @@ EXPORTS = [
     "shark_growth_surrogate_stats", "shark_gpu_attach_growth", "shark_charge_marchand",
     "shark_charge_marchand_noalloc", "shark_growth_zmean", "shark_growth_set_surrogate_residual",
     "shark_mlp_create", "shark_mlp_num_params", "shark_mlp_set_params", "shark_mlp_get_params",
-    "shark_mlp_forward", "shark_mlp_forward_layered", "shark_mlp_grads", "shark_mlp_grads_weighted",
+    "shark_mlp_forward", "shark_mlp_forward_hilo", "shark_mlp_forward_layered", "shark_mlp_grads", "shark_mlp_grads_weighted",
     "shark_mlp_set_data_weighted", "shark_mlp_set_data", "shark_mlp_train",
     "shark_mlp_kernel_time", "shark_mlp_destroy", "shark_hbm_probe",
 ]

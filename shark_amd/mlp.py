@@ -113,14 +113,19 @@ class MLP:
         gpu.check(self.L.shark_mlp_get_params(self.h, p.ctypes.data_as(_P32)))
         return p
 
-    def forward(self, x, layered=False):
-        """Network outputs: the fused inference kernel (shark_mlp_forward), or with layered=True
-        the training path's one-product-per-layer forward (shark_mlp_forward_layered)."""
+    def forward(self, x, layered=False, hilo=False):
+        """Network outputs: the fused inference kernel (shark_mlp_forward), with hilo=True its
+        variant with BF16 hi + lo activations (shark_mlp_forward_hilo, as the learned growth step
+        runs it), or with layered=True the training path's one-product-per-layer forward
+        (shark_mlp_forward_layered)."""
         x = _f32(x)
         n = x.shape[0]
         y = np.empty((n, self.cfg.n_out), dtype=np.float32)
         mb = self.cfg.max_batch
-        fn = self.L.shark_mlp_forward_layered if layered else self.L.shark_mlp_forward
+        if hilo:
+            self.L.shark_mlp_forward_hilo.argtypes = [C.c_void_p, _P32, C.c_int64, _P32]
+        fn = (self.L.shark_mlp_forward_layered if layered else
+              self.L.shark_mlp_forward_hilo if hilo else self.L.shark_mlp_forward)
         for i in range(0, n, mb):
             xi = np.ascontiguousarray(x[i:i + mb])
             yi = np.empty((xi.shape[0], self.cfg.n_out), dtype=np.float32)

@@ -1,0 +1,121 @@
+"""The shipped learned growth step (shark_amd/data/surrogate_disk64.npz: a network and a residual
+network on the same features, BF16 weights with hi + lo activations, the fallback threshold
+calibrated at training time) on held-out cells of the 64-bin disk it was trained for, against the
+plain FP64 coagulation that is its reference and its fallback (bit-exact with the oracle in
+tests/test_gpu_growth.py).
+
+Held-out cells are drawn with a seed the training never used, aged and advanced by the FP64
+plain step over intervals in the training range.  Checked: the mass-weighted L1 of the size
+distribution, the dust mass, the fallback contract (fallback cells equal the plain step bit for
+bit and their <Z> is the charge solve's), the network's <Z> against the charge solve on the
+plain step's result, and the speed-up over the plain step."""
+import numpy as np
+import pytest
+
+from shark_amd import growth as G
+from shark_amd import surrogate as S
+from shark_amd.train_surrogate import tables_sdust
+
+pytestmark = pytest.mark.gpu
+
+NB = 64
+# 4,096 held-out cells: mass-weighted L1 mean measured 8.6e-5 (p99 3.8e-4; 2.6e-4 with the
+# activations rounded to BF16, 5.5e-4 for single-network weights); <Z> relative error mean 2.9e-4
+# (p99 9.8e-4).  The bounds hold the 1e-4 accuracy target with a little room for the sample.
+L1_MEAN_MAX = 1.0e-4
+L1_P99_MAX = 6.0e-4
+ZREL_MEAN_MAX = 5.0e-4
+
+
+@pytest.fixture(scope="module")
+def held_out():
+    p = G.disk_params(nbins=NB, max_substeps=1000000)
+    sd = tables_sdust(p)
+    L = S.Learned()
+    m = L.meta
+    _, _, st = S.sample_states(p, sd, 256, 16, seed=99, age_max=m["age_max"], dt_lo=m["dt_range"][0],
+                               dt_hi=m["dt_range"][1], gen_mode=G.FP64)
+    g = G.Growth(p, st["sigma"].size)
+    g.set_cell_dt(st["dt"])
+    yield p, L, st, g
+    g.close()
+    L.close()
+
+
+def learned(g, st, L, resid_max=None):
+    L.attach(g, resid_max=resid_max)
+    g.upload(st["rhod"], st["sigma"], st["cs"], st["r"])
+    g.run(G.SURROGATE, 0.0)
+    out, nsub = g.download()
+    nfb, raw = g.surrogate_stats()
+    return out, nsub, nfb, raw, g.kernel_ms(), g.zmean()
+
+
+def test_learned_step_accuracy_and_mass(held_out):
+    p, L, st, g = held_out
+    n = st["sigma"].size
+    out, nsub, nfb, raw, ms, _ = learned(g, st, L)
+    err = S.mass_l1(out, st["rhod_next"])
+    base = S.mass_l1(st["rhod"], st["rhod_next"])
+    mass = np.abs(out.sum(axis=0) / st["rhod"].sum(axis=0) - 1)
+    print(f"learned step on {n} held-out cells: L1 mean {err.mean():.2e} p99 {np.quantile(err, .99):.2e} "
+          f"max {err.max():.2e}; no-change {base.mean():.2e}; fallback {nfb}/{n} at resid_max "
+          f"{S.resid_max(L.meta):.1e}; raw residual mean {raw:.2e}; mass {mass.max():.1e}; {ms:.2f} ms", flush=True)
+    assert err.mean() <= L1_MEAN_MAX
+    assert np.quantile(err, 0.99) <= L1_P99_MAX
+    assert mass.max() <= 1e-12
+    assert nfb <= 0.01 * n                      # the calibrated threshold catches outliers only
+    assert (nsub > 0).sum() == nfb
+
+
+def test_learned_step_charge(held_out):
+    """The network's <Z> head against the charge solve on the plain step's result."""
+    p, L, st, g = held_out
+    _, nsub, nfb, _, _, zs = learned(g, st, L)
+    g.upload(st["rhod_next"], st["sigma"], st["cs"], st["r"])
+    zc, _ = g.charge()
+    ok = nsub == 0
+    zrel = np.abs(zs[ok] - zc[ok]) / np.abs(zc[ok])
+    print(f"<Z> relative error mean {zrel.mean():.2e} p99 {np.quantile(zrel, .99):.2e} max {zrel.max():.2e}",
+          flush=True)
+    assert np.all(zs < 0) and np.all(np.isfinite(zs))
+    assert zrel.mean() <= ZREL_MEAN_MAX
+    assert np.quantile(zrel, 0.99) <= 4 * ZREL_MEAN_MAX
+
+
+def test_fallback_contract(held_out):
+    """With the threshold at the median raw residual about half the cells fall back: those equal
+    the plain step bit for bit and report the charge solve's <Z>; the others keep the network's
+    answer unchanged."""
+    p, L, st, g = held_out
+    n = st["sigma"].size
+    free, nsub0, nfb0, _, _, zfree = learned(g, st, L, resid_max=1e30)
+    assert nfb0 == 0 and not nsub0.any()
+    # the device's per-cell raw residuals are not exported; bisect the threshold on its count
+    lo, hi = 0.0, S.resid_max(L.meta)
+    for _ in range(40):
+        th = 0.5 * (lo + hi)
+        out, nsub, nfb, _, _, zs = learned(g, st, L, resid_max=th)
+        if abs(nfb - n // 2) <= 0.05 * n:
+            break
+        lo, hi = (th, hi) if nfb > n // 2 else (lo, th)
+    fb = nsub > 0
+    assert fb.sum() == nfb and 0.3 * n < nfb < 0.7 * n
+    assert np.array_equal(out[:, fb], st["rhod_next"][:, fb])
+    assert np.array_equal(out[:, ~fb], free[:, ~fb])
+    assert np.array_equal(zs[~fb], zfree[~fb])
+    g.upload(st["rhod_next"], st["sigma"], st["cs"], st["r"])
+    zc, _ = g.charge()
+    assert np.array_equal(zs[fb], zc[fb])
+
+
+def test_learned_step_speedup(held_out):
+    p, L, st, g = held_out
+    _, _, _, _, ms_s, _ = learned(g, st, L, resid_max=1e30)
+    g.upload(st["rhod"], st["sigma"], st["cs"], st["r"])
+    g.run(G.FP64, 0.0)
+    ref, _ = g.download()
+    ms_p = g.kernel_ms()
+    print(f"learned {ms_s:.2f} ms vs plain FP64 {ms_p:.1f} ms: {ms_p / ms_s:.0f}x", flush=True)
+    assert np.array_equal(ref, st["rhod_next"])
+    assert ms_p >= 100 * ms_s

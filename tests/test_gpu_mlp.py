@@ -128,3 +128,27 @@ def test_weighted_gradients_match_oracle():
         for a, r in ((dW, rW), (db, rb)):
             err = np.linalg.norm(a - r) / np.linalg.norm(r)
             assert err <= 1e-3, (l, err)
+
+
+@pytest.mark.parametrize("dims,n", [((69, 256, 4, 65), 700), ((5, 64, 3, 7), 65)])
+def test_hilo_forward_matches_exact_activations(dims, n):
+    """shark_mlp_forward_hilo carries activations as BF16 hi + lo pairs: against the network with
+    BF16 weights and exact activations its error is the lo part's rounding plus FP32 round-off
+    (6e-6 of the output scale in an FP64 model of the split; bound 3e-5), a hundred times below
+    the plain fused kernel's (5.7e-3), whose activations are rounded to BF16."""
+    cfg = M.make_cfg(n_in=dims[0], n_hidden=dims[1], n_layers=dims[2], n_out=dims[3], max_batch=1024)
+    p = net(cfg, seed=3)
+    x = np.random.default_rng(4).standard_normal((n, dims[0])).astype(np.float32)
+    m = M.MLP(cfg, p)
+    try:
+        yh = m.forward(x, hilo=True)
+        yb = m.forward(x)
+    finally:
+        m.close()
+    layers = [(MO.bf16(W), b.astype(np.float64)) for W, b in M.unpack(cfg, p)]
+    ref = MO.forward(layers, x, bf16_ops=False)
+    scale = np.abs(ref).max()
+    eh = np.abs(yh - ref).max() / scale
+    eb = np.abs(yb - ref).max() / scale
+    assert eh <= 3e-5, eh
+    assert eh * 100 <= eb, (eh, eb)

@@ -45,7 +45,7 @@ def test_fp64_growth_in_step_bit_exact():
 def test_learned_growth_in_step():
     cfg, geom, state, p = setups.disk_growth(nr=1024, nphi=16, nbins=64)
     nc = (cfg.nx + 4) * (cfg.ny + 4)
-    net, norm, _ = S.load(S.DATA + "/surrogate_disk64.npz")
+    L = S.Learned()
     out = {}
     try:
         for mode in (G.FP64, G.SURROGATE):
@@ -53,7 +53,7 @@ def test_learned_growth_in_step():
             gr = G.Growth(p, nc)
             try:
                 if mode == G.SURROGATE:
-                    gr.set_surrogate(net, norm)
+                    L.attach(gr)
                 g.attach_growth(gr, mode)
                 g.set_timing(True)
                 g.run(1)
@@ -62,7 +62,7 @@ def test_learned_growth_in_step():
                 g.close()
                 gr.close()
     finally:
-        net.close()
+        L.close()
     (uf, tf, _), (us, ts, (nfb, _)) = out[G.FP64], out[G.SURROGATE]
     act = active(cfg)
     nd = cfg.ndust
@@ -99,11 +99,11 @@ def test_learned_growth_in_step_full_size():
         u0 = g.download()
     finally:
         g.close()
-    net, norm, _ = S.load(S.DATA + "/surrogate_disk64.npz")
+    L = S.Learned()
     g = gpu.Step(cfg, geom, state)
     gr = G.Growth(p, nc)
     try:
-        gr.set_surrogate(net, norm)
+        L.attach(gr)
         g.attach_growth(gr, G.SURROGATE)
         g.set_timing(True)
         dt1 = g.step()
@@ -113,7 +113,7 @@ def test_learned_growth_in_step_full_size():
     finally:
         g.close()
         gr.close()
-        net.close()
+        L.close()
     act = active(cfg)
     print(f"learned step {ms:.2f} ms for {act.sum()} cells, fallback {nfb}", flush=True)
     assert dt1 == dt0

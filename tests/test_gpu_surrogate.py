@@ -95,10 +95,11 @@ def test_fallback_cells_are_exactly_plain(trained):
 
 
 def raw_residuals(net, norm, X, rhod):
-    """|sum_k share_k exp(change_k) - 1| of the network's raw prediction (before normalisation)."""
+    """|sum_k share_k exp(change_k) - 1| of the network's raw prediction (before normalisation),
+    evaluated as the learned step evaluates it (hi + lo activations)."""
     ni, no = X.shape[1], NB + 1
     Xs, _ = S.standardise(X, np.zeros((X.shape[0], no), np.float32), norm)
-    chg = net.forward(Xs) * norm[2 * ni + no:] + norm[2 * ni:2 * ni + no]
+    chg = net.forward(Xs, hilo=True) * norm[2 * ni + no:] + norm[2 * ni:2 * ni + no]
     sh = (rhod / rhod.sum(0)).T
     return np.abs((sh * np.exp(chg[:, :NB].astype(np.float64))).sum(1) - 1.0)
 
