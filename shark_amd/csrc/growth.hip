@@ -41,7 +41,10 @@ constexpr double kKB = 1.38e-16, kMH = 1.67e-24;
 constexpr int MAXB = SHARK_GROWTH_MAX_BINS;
 constexpr int KC = 128;       // K values (2 per pair) per MFMA chunk
 constexpr int NCT = 64;       // cells per MFMA workgroup
-constexpr int SUB_PER_LAUNCH = 64;
+// sub-steps per launch of the sub-step kernels: enough that a growth call normally finishes in
+// one launch, so a tile that needs few sub-steps frees its slot for the next tile instead of
+// waiting at a launch boundary for the slowest tile of the chunk
+constexpr int SUB_PER_LAUNCH = 4096;
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -178,6 +181,7 @@ struct GArgs {
   double kts;   // sqrt(pi gamma/8) * (rhograin/unit_d): Epstein prefactor
   double s2pi;  // sqrt(2 pi)
   const PairC* pairs;
+  const int* pij;       // [kpad/2]: partner bins i | j << 8 of each pair (k_gmfma's X generation)
   const double* redis;  // [nb*nb]
   const double* mdust;
   const double* sdust;
@@ -453,17 +457,6 @@ struct GSmem {
   int any;
 };
 
-template <int MODE>
-__device__ __forceinline__ void put_x(GSmem<MODE>& sm, int kk, int cell, float v) {
-  if constexpr (MODE == 1) {
-    sm.u.xt.x[kk][cell] = v;
-  } else {
-    const unsigned short h = bf16_rne(v);
-    sm.u.xt.hi[kk >> 3][cell][kk & 7] = h;
-    if constexpr (MODE == 2) sm.u.xt.lo[kk >> 3][cell][kk & 7] = bf16_rne(v - bf16_to_f(h));
-  }
-}
-
 // Workgroup: 4 waves over a 64-bin x 64-cell rate tile; wave w owns bins 32*(w&1).. and cells
 // 32*(w>>1)..  The X chunk (KC values x 64 cells) is generated into LDS by all 256 lanes, then
 // each wave multiplies its 32 x KC slice of Gt (pre-laid-out in global memory, read through L2)
@@ -472,7 +465,7 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_gmfma(GArgs a) {
   __shared__ GSmem<MODE> sm;
   const int tid = threadIdx.x;
-  const int lane = tid & 63, w = tid >> 6;
+  const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   const int rb = w & 1, ch = w >> 1;
   const int cbase = blockIdx.x * NCT;
   const int nb = a.nb;
@@ -522,32 +515,84 @@ __global__ __launch_bounds__(256) void k_gmfma(GArgs a) {
     f32x16 sum, comp;
     for (int r = 0; r < 16; ++r) { sum[r] = 0.f; comp[r] = 0.f; }
     for (int q = 0; q < nchunks; ++q) {
-      // ---- X generation: pair pp = w + 4n of the chunk, cell = lane
+      // ---- X generation: wave w makes pairs pp = 16 w + n of the chunk (X entries 32 w ..
+      // 32 w + 31), cell = lane.  The partner bins come from one scalar load of the packed pair
+      // table, the 32 prefactor loads of a lane are issued together, and the BF16 modes write
+      // each group of 8 entries of a cell as one 16-byte LDS store.
       {
         const int cell = lane;
-        const int cc = cbase + cell;
         const bool cact = sm.act[cell] != 0;
         const unsigned long long mc = sm.okc[cell], mf = sm.okf[cell];
-        for (int n = 0; n < KC / 2 / 4; ++n) {
-          const int pp = w + 4 * n;
-          const int p = q * (KC / 2) + pp;
+        const int p0 = q * (KC / 2) + 16 * w;
+        const __attribute__((address_space(4))) int* pt =
+            (const __attribute__((address_space(4))) int*)(a.pij + p0);
+        int pvs[16];
+#pragma unroll
+        for (int n = 0; n < 16; ++n) pvs[n] = pt[n];   // the table is padded to kpad / 2
+        const bool full = p0 + 16 <= a.npairs;          // every chunk but the last
+        float ahv[16], fhv[16];
+#pragma unroll
+        for (int n = 0; n < 16; ++n) { ahv[n] = 0.f; fhv[n] = 0.f; }
+        if (cact) {
+          const float* ap = a.Ah + (int64_t)p0 * a.nch + cbase;   // wave-uniform
+          const float* fp = a.Fh + (int64_t)p0 * a.nch + cbase;
+          if (full) {
+#pragma unroll
+            for (int n = 0; n < 16; ++n) {
+              ahv[n] = ap[(int64_t)n * a.nch + cell];
+              fhv[n] = fp[(int64_t)n * a.nch + cell];
+            }
+          } else {
+#pragma unroll
+            for (int n = 0; n < 16; ++n)
+              if (p0 + n < a.npairs) {
+                ahv[n] = ap[(int64_t)n * a.nch + cell];
+                fhv[n] = fp[(int64_t)n * a.nch + cell];
+              }
+          }
+        }
+        float xv[32];
+#pragma unroll
+        for (int n = 0; n < 16; ++n) {
+          const int pv = pvs[n];
+          const int pi = pv & 0xff, pj = (pv >> 8) & 0xff;
           float xc = 0.f, xf = 0.f;
-          if (p < a.npairs && cact) {
-            const PairC P = a.pairs[p];
-            const int64_t o = (int64_t)p * a.nch + cc;
-            const float ah = a.Ah[o];
-            float f = a.Fh[o];
-            const bool pc = ((mc >> P.i) & (mc >> P.j) & 1ull) != 0;
-            if (!(((mf >> P.i) & (mf >> P.j) & 1ull) != 0)) f = 0.f;
-            float e = sm.epsf[P.i][cell] * sm.epsf[P.j][cell] * ah;
-            if (P.i == P.j) e *= 0.5f;
+          if ((full || p0 + n < a.npairs) && cact) {
+            const float ah = ahv[n];
+            float f = fhv[n];
+            const bool pc = ((mc >> pi) & (mc >> pj) & 1ull) != 0;
+            if (!(((mf >> pi) & (mf >> pj) & 1ull) != 0)) f = 0.f;
+            float e = sm.epsf[pi][cell] * sm.epsf[pj][cell] * ah;
+            if (pi == pj) e *= 0.5f;
             if (pc) {
               xc = (1.f - f) * e;
               xf = f * e;
             }
           }
-          put_x<MODE>(sm, 2 * pp, cell, xc);
-          put_x<MODE>(sm, 2 * pp + 1, cell, xf);
+          xv[2 * n] = xc;
+          xv[2 * n + 1] = xf;
+        }
+        if constexpr (MODE == 1) {
+#pragma unroll
+          for (int kk = 0; kk < 32; ++kk) sm.u.xt.x[32 * w + kk][cell] = xv[kk];
+        } else {
+#pragma unroll
+          for (int g8 = 0; g8 < 4; ++g8) {
+            unsigned short h[8], l[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+              h[e] = bf16_rne(xv[8 * g8 + e]);
+              if constexpr (MODE == 2) l[e] = bf16_rne(xv[8 * g8 + e] - bf16_to_f(h[e]));
+            }
+            u32x4 hv, lv;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              hv[e] = (unsigned int)h[2 * e] | ((unsigned int)h[2 * e + 1] << 16);
+              if constexpr (MODE == 2) lv[e] = (unsigned int)l[2 * e] | ((unsigned int)l[2 * e + 1] << 16);
+            }
+            *reinterpret_cast<u32x4*>(&sm.u.xt.hi[4 * w + g8][cell][0]) = hv;
+            if constexpr (MODE == 2) *reinterpret_cast<u32x4*>(&sm.u.xt.lo[4 * w + g8][cell][0]) = lv;
+          }
         }
       }
       __syncthreads();
@@ -1066,6 +1111,7 @@ struct shark_growth {
   int kpad = 0;
   hipStream_t stream = nullptr;
   PairC* dpairs = nullptr;
+  int* dpij = nullptr;
   double *dredis = nullptr, *dmdust = nullptr, *dsdust = nullptr;
   double *drhod = nullptr, *drho = nullptr, *dcs = nullptr, *drad = nullptr;
   double *denv = nullptr, *dts = nullptr, *dA = nullptr, *dF = nullptr;
@@ -1109,7 +1155,7 @@ struct shark_growth {
 void shark_growth_free(shark_growth* g) {
   if (!g) return;
   if (g->ext) g->drhod = g->drho = g->dcs = nullptr;  // the step context owns these
-  void* ptrs[] = {g->dpairs, g->dredis, g->dmdust, g->dsdust, g->drhod, g->drho, g->dcs, g->drad,
+  void* ptrs[] = {g->dpairs, g->dpij, g->dredis, g->dmdust, g->dsdust, g->drhod, g->drho, g->dcs, g->drad,
                   g->denv, g->dts, g->dA, g->dF, g->dAh, g->dFh, g->dtg, g->dnit, g->dflags,
                   g->dGf32, g->dGbhi, g->dGblo, g->dzmean, g->dxion, g->ddtc, g->dnorm,
                   g->dfeat, g->dy, g->dfb, g->dresid, g->dmask, g->dsT, g->dnorm2, g->dy2};
@@ -1211,8 +1257,9 @@ int shark_growth_create(const shark_growth_params* p, int64_t ncells, shark_grow
   const int np = (int)g->pairs.size();
   g->kpad = ((2 * np + KC - 1) / KC) * KC;
   // per-pair scratch (A, F in FP64 and their FP32 copies: 24 B per pair and cell) is bounded
-  // to 4 GiB by processing the cells in chunks
-  const int64_t cap = ((int64_t)4 << 30) / ((int64_t)np * 24);
+  // to 16 GiB by processing the cells in chunks (a 64-bin, 512 x 512 disk is one chunk; fewer,
+  // larger chunks keep the sub-step kernels' grids deep enough to hide the tiles' imbalance)
+  const int64_t cap = ((int64_t)16 << 30) / ((int64_t)np * 24);
   int64_t nch = ncells < cap ? ncells : cap;
   nch = nch >= NCT ? (nch / NCT) * NCT : nch;
   if (nch < 1) nch = 1;
@@ -1234,6 +1281,7 @@ int shark_growth_create(const shark_growth_params* p, int64_t ncells, shark_grow
   GC(hipEventCreate(&g->e0));
   GC(hipEventCreate(&g->e1));
   GC(hipMalloc(&g->dpairs, sizeof(PairC) * np));
+  GC(hipMalloc(&g->dpij, sizeof(int) * (g->kpad / 2)));
   GC(hipMalloc(&g->dredis, sizeof(double) * MAXB * MAXB));
   GC(hipMalloc(&g->dmdust, sizeof(double) * MAXB));
   GC(hipMalloc(&g->dsdust, sizeof(double) * MAXB));
@@ -1252,6 +1300,11 @@ int shark_growth_create(const shark_growth_params* p, int64_t ncells, shark_grow
   GC(hipMalloc(&g->dflags, sizeof(int) * 2));
   GC(hipHostMalloc(&g->hflags, sizeof(int) * 2, hipHostMallocDefault));
   GC(hipMemcpy(g->dpairs, g->pairs.data(), sizeof(PairC) * np, hipMemcpyHostToDevice));
+  {
+    std::vector<int> pij(g->kpad / 2, 0);   // pairs beyond np are never read (p < npairs)
+    for (int q = 0; q < np; ++q) pij[q] = g->pairs[q].i | (g->pairs[q].j << 8);
+    GC(hipMemcpy(g->dpij, pij.data(), sizeof(int) * pij.size(), hipMemcpyHostToDevice));
+  }
   GC(hipMemcpy(g->dredis, g->t.redis, sizeof(double) * MAXB * MAXB, hipMemcpyHostToDevice));
   GC(hipMemcpy(g->dmdust, g->t.mdust, sizeof(double) * MAXB, hipMemcpyHostToDevice));
   GC(hipMemcpy(g->dsdust, g->t.sdust, sizeof(double) * MAXB, hipMemcpyHostToDevice));
@@ -1343,6 +1396,7 @@ static GArgs make_args(shark_growth* g, int mode, double dt) {
   a.kts = sqrt(kPi * p.gamma / 8.0) * (p.rhograin / p.unit_d);
   a.s2pi = sqrt(2.0 * kPi);
   a.pairs = g->dpairs;
+  a.pij = g->dpij;
   a.redis = g->dredis;
   a.mdust = g->dmdust;
   a.sdust = g->dsdust;
