@@ -27,6 +27,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -202,6 +203,7 @@ struct GArgs {
   int* nit;     // [ld] sub-steps taken
   int* unfinished;
   int* bad;     // set when a rate or density is not finite
+  const int* tile_order;  // launch order of the 64-cell tiles (slowest first), or null: blockIdx
   const float* Gf32;     // MFMA A operand, FP32 layout
   const u32x4* Gbhi;     // MFMA A operand, BF16 hi
   const u32x4* Gblo;     // MFMA A operand, BF16 lo
@@ -323,7 +325,8 @@ __global__ void k_gpairs(GArgs a) {
 __global__ __launch_bounds__(64) void k_gfp64(GArgs a) {
   extern __shared__ double dd[];
   const int lane = threadIdx.x;
-  const int cc = blockIdx.x * 64 + lane;
+  const int tile = a.tile_order ? a.tile_order[blockIdx.x] : (int)blockIdx.x;
+  const int cc = tile * 64 + lane;
   const bool valid = cc < a.nch;
   const int64_t c = a.c0 + (valid ? cc : 0);
   const int nb = a.nb;
@@ -467,7 +470,7 @@ __global__ __launch_bounds__(256) void k_gmfma(GArgs a) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   const int rb = w & 1, ch = w >> 1;
-  const int cbase = blockIdx.x * NCT;
+  const int cbase = (a.tile_order ? a.tile_order[blockIdx.x] : (int)blockIdx.x) * NCT;
   const int nb = a.nb;
   const int kt = a.kernel_type;
   // per-cell state lives with thread `tid` for tid < 64
@@ -674,6 +677,23 @@ __global__ __launch_bounds__(256) void k_gmfma(GArgs a) {
     a.nit[c] = niter;
     if (tgrow < dtl && !(a.max_sub > 0 && niter >= a.max_sub)) atomicAdd(a.unfinished, 1);
   }
+}
+
+// Remaining sub-steps of each 64-cell tile after its first sub-steps, at the mean sub-step
+// length so far ((dt - t) n / t), for its slowest cell; -1 for a finished tile.  Orders the
+// tiles slowest first for the launches that follow, so the long tiles do not start last.
+__global__ __launch_bounds__(64) void k_tile_est(GArgs a, float* est) {
+  const int cc = blockIdx.x * 64 + threadIdx.x;
+  float v = -1.f;
+  if (cc < a.nch) {
+    const int64_t c = a.c0 + cc;
+    const double tgrow = a.tg[c], dtl = cell_dt(a, c);
+    const int niter = a.nit[c];
+    if (tgrow < dtl && !(a.max_sub > 0 && niter >= a.max_sub))
+      v = tgrow > 0.0 ? (float)((dtl - tgrow) / tgrow * (double)niter) : 3.0e38f;
+  }
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  if (threadIdx.x == 0) est[blockIdx.x] = v;
 }
 
 // ------------------------------------------------------------------ grain charging
@@ -1120,6 +1140,10 @@ struct shark_growth {
   int* dnit = nullptr;
   int* dflags = nullptr;  // [0] unfinished, [1] bad
   int* hflags = nullptr;  // pinned
+  float* dest = nullptr;  // per-tile remaining sub-step estimate [tiles of a chunk]
+  int* dorder = nullptr;  // tile launch order [tiles of a chunk]
+  float* hest = nullptr;  // pinned mirrors
+  int* horder = nullptr;
   int* dcfb = nullptr;    // learned step: fallback count per chunk
   int* hcfb = nullptr;    // pinned mirror
   int64_t sch = 0;        // learned step: cells per chunk (features, outputs, temperatures)
@@ -1162,6 +1186,10 @@ void shark_growth_free(shark_growth* g) {
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (g->hflags) (void)hipHostFree(g->hflags);
+  if (g->dest) (void)hipFree(g->dest);
+  if (g->dorder) (void)hipFree(g->dorder);
+  if (g->hest) (void)hipHostFree(g->hest);
+  if (g->horder) (void)hipHostFree(g->horder);
   if (g->dcfb) (void)hipFree(g->dcfb);
   if (g->hcfb) (void)hipHostFree(g->hcfb);
   if (g->e0) (void)hipEventDestroy(g->e0);
@@ -1299,6 +1327,13 @@ int shark_growth_create(const shark_growth_params* p, int64_t ncells, shark_grow
   GC(hipMalloc(&g->dFh, sizeof(float) * np * nch));
   GC(hipMalloc(&g->dflags, sizeof(int) * 2));
   GC(hipHostMalloc(&g->hflags, sizeof(int) * 2, hipHostMallocDefault));
+  {
+    const int64_t ntile = (nch + 63) / 64;
+    GC(hipMalloc(&g->dest, sizeof(float) * ntile));
+    GC(hipMalloc(&g->dorder, sizeof(int) * ntile));
+    GC(hipHostMalloc(&g->hest, sizeof(float) * ntile, hipHostMallocDefault));
+    GC(hipHostMalloc(&g->horder, sizeof(int) * ntile, hipHostMallocDefault));
+  }
   GC(hipMemcpy(g->dpairs, g->pairs.data(), sizeof(PairC) * np, hipMemcpyHostToDevice));
   {
     std::vector<int> pij(g->kpad / 2, 0);   // pairs beyond np are never read (p < npairs)
@@ -1458,30 +1493,53 @@ static int ensure_charge_buffers(shark_growth* g) {
   return 0;
 }
 
-// pair prefactors for the chunk, then the bounded sub-step kernel relaunched until every active
-// cell of the chunk reached its interval
+// pair prefactors for the chunk, then the sub-step kernel: one sub-step for every tile, then the
+// unfinished tiles, slowest first (their remaining sub-steps estimated from the first), relaunched
+// until every active cell of the chunk reached its interval.  The order changes which tiles run
+// side by side, not any cell's arithmetic.
 static int plain_chunk(shark_growth* g, GArgs& a, int mode) {
   hipStream_t s = g->run_stream;
   const int nblk = (a.nch + 255) / 256;
   hipLaunchKernelGGL(k_gpairs, dim3(nblk, a.npairs), dim3(256), 0, s, a);
   GCHK(hipGetLastError());
+  static_assert(NCT == 64, "both sub-step kernels take 64-cell tiles");
+  const int ntile = (a.nch + 63) / 64;
+  int grid = ntile;
+  a.tile_order = nullptr;
   for (int launch = 0;; ++launch) {
+    a.sub_per_launch = launch == 0 ? 1 : SUB_PER_LAUNCH;
     GCHK(hipMemsetAsync(g->dflags, 0, sizeof(int), s));
     if (mode == SHARK_GROWTH_FP64) {
-      hipLaunchKernelGGL(k_gfp64, dim3((a.nch + 63) / 64), dim3(64), sizeof(double) * 64 * a.nb, s, a);
+      hipLaunchKernelGGL(k_gfp64, dim3(grid), dim3(64), sizeof(double) * 64 * a.nb, s, a);
     } else if (mode == SHARK_GROWTH_MFMA_FP32) {
-      hipLaunchKernelGGL(k_gmfma<1>, dim3((a.nch + NCT - 1) / NCT), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(k_gmfma<1>, dim3(grid), dim3(256), 0, s, a);
     } else if (mode == SHARK_GROWTH_MFMA_BF16) {
-      hipLaunchKernelGGL(k_gmfma<2>, dim3((a.nch + NCT - 1) / NCT), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(k_gmfma<2>, dim3(grid), dim3(256), 0, s, a);
     } else {
-      hipLaunchKernelGGL(k_gmfma<3>, dim3((a.nch + NCT - 1) / NCT), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(k_gmfma<3>, dim3(grid), dim3(256), 0, s, a);
     }
     GCHK(hipGetLastError());
     GCHK(hipMemcpyAsync(g->hflags, g->dflags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
     GCHK(hipStreamSynchronize(s));
     if (g->hflags[1]) return gfail(SHARK_GROWTH_ENUM, "growth produced a non-finite density or a zero sub-step");
     if (g->hflags[0] == 0) break;
+    if (launch == 0) {
+      hipLaunchKernelGGL(k_tile_est, dim3(ntile), dim3(64), 0, s, a, g->dest);
+      GCHK(hipGetLastError());
+      GCHK(hipMemcpyAsync(g->hest, g->dest, sizeof(float) * ntile, hipMemcpyDeviceToHost, s));
+      GCHK(hipStreamSynchronize(s));
+      int n = 0;
+      for (int t = 0; t < ntile; ++t)
+        if (g->hest[t] >= 0.f) g->horder[n++] = t;
+      const float* est = g->hest;
+      std::stable_sort(g->horder, g->horder + n, [est](int x, int y) { return est[x] > est[y]; });
+      GCHK(hipMemcpyAsync(g->dorder, g->horder, sizeof(int) * n, hipMemcpyHostToDevice, s));
+      a.tile_order = g->dorder;
+      grid = n;
+    }
   }
+  a.tile_order = nullptr;
+  a.sub_per_launch = SUB_PER_LAUNCH;
   return 0;
 }
 
