@@ -418,8 +418,9 @@ __global__ __launch_bounds__(64) void k_gfp64(GArgs a) {
   if (valid) {
     a.tg[c] = tgrow;
     a.nit[c] = niter;
-    if (act) atomicAdd(a.unfinished, 1);
   }
+  // the host needs only whether any cell is unfinished: one atomic per wave
+  if (__any(valid && act) && lane == 0) atomicAdd(a.unfinished, 1);
 }
 
 // ------------------------------------------------------------------ modes 1-3: MFMA contraction
@@ -675,7 +676,11 @@ __global__ __launch_bounds__(256) void k_gmfma(GArgs a) {
   if (tid < NCT && valid) {
     a.tg[c] = tgrow;
     a.nit[c] = niter;
-    if (tgrow < dtl && !(a.max_sub > 0 && niter >= a.max_sub)) atomicAdd(a.unfinished, 1);
+  }
+  // the host needs only whether any cell is unfinished: one atomic per tile (wave 0 holds the cells)
+  if (w == 0) {
+    const bool unf = valid && tgrow < dtl && !(a.max_sub > 0 && niter >= a.max_sub);
+    if (__any(unf) && lane == 0) atomicAdd(a.unfinished, 1);
   }
 }
 
@@ -1059,9 +1064,9 @@ __device__ __forceinline__ double out_value(const SArgs& a, const float* y, int 
   return v;
 }
 
-__global__ __launch_bounds__(256) void k_surr_post(SArgs a) {
-  const int cc = blockIdx.x * blockDim.x + threadIdx.x;
-  if (cc >= a.nch) return;
+// one cell of the learned step's new state; returns the cell's raw mass residual (0 for a cell that
+// fell back or does not grow), which the kernel sums per wave before one atomic add
+__device__ __forceinline__ double surr_post_cell(const SArgs& a, int cc) {
   const int64_t c = a.c0 + cc;
   const float* y = a.y + cc;
   const float* f = a.feat + cc;
@@ -1073,7 +1078,7 @@ __global__ __launch_bounds__(256) void k_surr_post(SArgs a) {
     a.fb[c] = 0;
     a.tg[c] = 0.0;
     a.nit[c] = 0;
-    return;
+    return 0.0;
   }
   // w_k = rho_k exp(change_k), kept in FP32 registers: the network's prediction carries ~1e-3,
   // and the cell's mass is restored in FP64 from the final densities
@@ -1100,11 +1105,11 @@ __global__ __launch_bounds__(256) void k_surr_post(SArgs a) {
     a.fb[c] = 1;
     a.tg[c] = 0.0;
     atomicAdd(a.nfb, 1);
-    return;
+    return 0.0;
   }
   a.fb[c] = 0;
   a.tg[c] = dtl;
-  atomicAdd(a.resid_sum, fabs(W / tot - 1.0));   // sum_k share_k,old exp(change_k) - 1
+  const double resid = fabs(W / tot - 1.0);   // |sum_k share_k,old exp(change_k) - 1|
   // new densities share * rho_tot, floored as the plain step floors, rescaled to the cell's mass
   const double floor_v = a.rho_gas[c] * a.ratio_min;
   const double sc0 = tot / W;
@@ -1117,6 +1122,14 @@ __global__ __launch_bounds__(256) void k_surr_post(SArgs a) {
   for (int k = 0; k < MAXB; ++k)
     if (k < nb) a.rhod[(int64_t)k * a.ld + c] = fmax((double)w[k] * sc0, floor_v) * sc;
   a.zmean[c] = -pow(10.0, out_value(a, y, nb, ld, om, os));
+  return resid;
+}
+
+__global__ __launch_bounds__(256) void k_surr_post(SArgs a) {
+  const int cc = blockIdx.x * blockDim.x + threadIdx.x;
+  double r = cc < a.nch ? surr_post_cell(a, cc) : 0.0;
+  for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o);
+  if ((threadIdx.x & 63) == 0 && r != 0.0) atomicAdd(a.resid_sum, r);
 }
 
 }  // namespace
