@@ -40,7 +40,7 @@ enum {
 };
 
 /* geometry (reference GEOM macro, bin/Makefile:14) */
-enum { SHARK_GEOM_CARTESIAN = 0, SHARK_GEOM_POLAR = 2 };
+enum { SHARK_GEOM_CARTESIAN = 0, SHARK_GEOM_SPHERE = 1, SHARK_GEOM_POLAR = 2 };
 /* gas Riemann solver (SOLVER macro, src/solver/godunov.f90:783-791) */
 enum { SHARK_SOLVER_LLF = 0, SHARK_SOLVER_HLL = 1, SHARK_SOLVER_HLLC = 2 };
 /* dust Riemann solver (SOLVERDUST macro, src/solver/godunov.f90:797-813) */
@@ -50,13 +50,16 @@ enum { SHARK_BC_PERIODIC = 0,   /* src/grid/boundary_types.f90:149-186 (DUSTYWAV
        SHARK_BC_SHEAR_SI = 1,   /* src/setups/SI/boundaries.f90:10-85 */
        SHARK_BC_DISK = 2,       /* src/setups/disk_polar/boundaries.f90:10-62 */
        SHARK_BC_SOD = 3,        /* src/setups/SOD/boundaries.f90:10-80 */
-       SHARK_BC_ZERO_GRADIENT = 4 }; /* src/grid/boundary_types.f90:48-100 (cartesian disk) */
+       SHARK_BC_ZERO_GRADIENT = 4,   /* src/grid/boundary_types.f90:48-100 (cartesian disk) */
+       SHARK_BC_COLLAPSE = 5 };      /* src/grid/boundary_types.f90:4-44 (Collapse_1D) */
 enum { SHARK_TSTOP_DUSTYWAVE = 0,  /* src/setups/DUSTYWAVE/setup.f90:217 */
        SHARK_TSTOP_SI = 1,         /* src/setups/SI/setup.f90:392 */
        SHARK_TSTOP_EPSTEIN_DISK = 2, /* src/setups/disk_polar/setup.f90:436 */
        SHARK_TSTOP_EPSTEIN = 3,      /* src/setups/KH/setup.f90:238, SOD/setup.f90:182: uniform medium,
                                         sqrt(pi gamma/8) rhograin s / (rho cs), rhograin in code units */
-       SHARK_TSTOP_EPSTEIN_CDISK = 4 }; /* src/setups/disk/setup.f90:322-345 (cartesian disk) */
+       SHARK_TSTOP_EPSTEIN_CDISK = 4, /* src/setups/disk/setup.f90:322-345 (cartesian disk) */
+       SHARK_TSTOP_COLLAPSE = 5 };    /* src/setups/Collapse_1D/setup.f90:296-324: Epstein with the
+                                         barotropic cs */
 enum { SHARK_FORCE_NONE = 0,
        SHARK_FORCE_SI = 1,          /* src/setups/SI/setup.f90:335-368 */
        SHARK_FORCE_DISK = 2,        /* src/setups/disk_polar/setup.f90:360-403 */
@@ -99,9 +102,21 @@ typedef struct shark_cfg {
    * the two outer ghost columns and the last active column to the inflow state of the accretion
    * rate m_acc (code units: M_acc/(365.25*24*3600)*unit_t) at the local cs, with phi_mom
    * (only 2, every row) and L_out; box_l above is the disk's radial extent */
-  int32_t accretion, reserved_i2;
+  int32_t accretion;
+  /* 1D self-gravity (GRAVITY=1, src/gravity/gravity_utils.f90:1-51): the mass Mc enclosed within
+   * each cell centre acts as -Mc/(r^2 + l_soft^2) in the predictor, the sources and the Courant
+   * step (godunov.f90:243-246, 442-445; source_terms.f90:79-85; courant.f90:62-63) */
+  int32_t gravity;
   double m_acc, phi_mom, l_out;
-  double reserved2[8];
+  /* non_standard_eos = 1 (Collapse_1D, src/setups/Collapse_1D/eos.f90): with iso_cs < 1 the
+   * fluxes stay adiabatic but P = rho cs_eos(barotrop(rho))^2 in every cell and face */
+  int32_t eos_baro, reserved_i3;
+  double l_soft;            /* gravity softening in code units (l_soft/unit_l) */
+  double m_cent;            /* central point mass (M_cent) */
+  double t0_cloud;          /* barotrop's T0_cloud (K) */
+  double mu_gas;            /* cs_eos's mean molecular weight */
+  double unit_v, unit_nh;   /* the setup's units (src/setups/Collapse_1D/units.f90) */
+  double reserved3[1];
 } shark_cfg;
 
 typedef struct shark_ctx shark_ctx;
@@ -113,6 +128,13 @@ int shark_gpu_create(const shark_cfg* cfg, shark_ctx** ctx);
  * surf(ncells,ndim), vol(ncells), radii_c(ncells), position(ncells,ndim). */
 int shark_gpu_set_geometry(shark_ctx* ctx, const double* dx, const double* surf,
                            const double* vol, const double* radii_c, const double* position);
+
+/* Self-gravity on the spherical 1D grid: dvol(ncells) of gridinit_sphere1D
+ * (src/grid/grid.f90:343-346), the part of each cell between its centre and its outer face,
+ * which mtot (src/gravity/gravity_utils.f90:11-51) subtracts to get the mass within the cell
+ * centre.  Required before shark_gpu_upload when cfg.gravity = 1; replaces the reference's
+ * module array dvol. */
+int shark_gpu_set_dvol(shark_ctx* ctx, const double* dvol);
 
 /* State after setup: u_prim(ncells,nvar), cs(ncells), sdust(ncells,ndust) (may be NULL when
  * the stopping-time law does not read it), uprim_condinit(ncells,nvar) (disk setup only,

@@ -42,6 +42,7 @@ struct oracle {
   int irho, ivx, ivy, ivz, iP;       /* 0-based variable planes */
   double *u, *q, *qm, *qp, *flux, *dq, *cs, *force, *force_dust, *tstop, *sdust, *eps_dust;
   double *dx, *surf, *vol, *radii_c, *position, *condinit, *fargo_v, *S_U, *uold, *duprim;
+  double *dvol, *mc;                 /* 1D self-gravity: inner half-cell volume, enclosed mass */
   unsigned char *active, *active_pred;
   double dt, time;
   int growth_on;                     /* dust_growth after the drag (solve.f90:112), disk env */
@@ -98,6 +99,7 @@ oracle* oracle_create(const shark_cfg* cfg) {
   o->radii_c = calloc(nc, 8); o->position = calloc(nc * o->ndim, 8);
   o->condinit = calloc(nc * nv, 8); o->fargo_v = calloc(nc, 8); o->S_U = calloc(nc * nv, 8);
   o->uold = calloc(nc * nv, 8); o->duprim = calloc(nc * nv, 8);
+  o->dvol = calloc(nc, 8); o->mc = calloc(nc, 8);
   o->active = calloc(nc, 1); o->active_pred = calloc(nc, 1);
   /* get_active_cells (src/commons/allocate_init.f90:339-381) */
   for (int iy = 1; iy <= o->nym; ++iy)
@@ -118,7 +120,7 @@ void oracle_destroy(oracle* o) {
   if (!o) return;
   double* ps[] = {o->u, o->q, o->qm, o->qp, o->flux, o->dq, o->cs, o->force, o->force_dust,
                   o->tstop, o->sdust, o->eps_dust, o->dx, o->surf, o->vol, o->radii_c,
-                  o->position, o->condinit, o->fargo_v, o->S_U, o->uold, o->duprim};
+                  o->position, o->condinit, o->fargo_v, o->S_U, o->uold, o->duprim, o->dvol, o->mc};
   for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); ++i) free(ps[i]);
   free(o->active); free(o->active_pred); free(o);
 }
@@ -133,6 +135,10 @@ void oracle_set_geometry(oracle* o, const double* dx, const double* surf, const 
   memcpy(o->vol, vol, nc * 8); memcpy(o->radii_c, radii_c, nc * 8);
   if (position) memcpy(o->position, position, nc * o->ndim * 8);
 }
+
+/* dvol of gridinit_sphere1D (grid.f90:343-346): (radii^3 - radii_c^3)/3, the part of each cell
+ * between its centre and its outer face (the enclosed mass of mtot subtracts it) */
+void oracle_set_dvol(oracle* o, const double* dvol) { memcpy(o->dvol, dvol, (size_t)o->nc * 8); }
 
 void oracle_upload(oracle* o, const double* u, const double* cs, const double* sdust,
                    const double* condinit, const double* fargo_v) {
@@ -310,14 +316,85 @@ static void bc_zero_gradient(oracle* o) {
     }
 }
 
+/* boundary_collapse_1D, src/grid/boundary_types.f90:4-44: zero-gradient ghosts, no flow through
+ * the centre (the inner ghosts and the first active cell at rest), outflow-free outer edge (the
+ * outer ghosts and the last active cell keep only inward velocities); first_active/last_active
+ * are 1D cell indices */
+static void bc_collapse(oracle* o) {
+  const int al = o->fa - 1, ar = o->la - 1;
+  for (int ix = 1; ix <= 2; ++ix) {
+    const int bl = ix - 1, br = o->nxm - ix;
+    for (int v = 0; v < o->nvar; ++v) { U(v, bl) = U(v, al); U(v, br) = U(v, ar); }
+    U(o->ivx, bl) = 0.0;
+    U(o->ivx, al) = 0.0;
+    U(o->ivx, br) = fmin_f(U(o->ivx, br), 0.0);
+    U(o->ivx, ar) = fmin_f(U(o->ivx, ar), 0.0);
+    for (int k = 0; k < o->nd; ++k) {
+      U(IVDX(k), bl) = 0.0;
+      U(IVDX(k), al) = 0.0;
+      U(IVDX(k), br) = fmin_f(U(IVDX(k), br), 0.0);
+      U(IVDX(k), ar) = fmin_f(U(IVDX(k), ar), 0.0);
+    }
+  }
+}
+
 static void apply_boundaries(oracle* o) {
   switch (o->cfg.bc_kind) {
+    case SHARK_BC_COLLAPSE: bc_collapse(o); break;
     case SHARK_BC_ZERO_GRADIENT: bc_zero_gradient(o); break;
     case SHARK_BC_PERIODIC: bc_periodic(o); break;
     case SHARK_BC_SHEAR_SI: bc_shear_si(o); break;
     case SHARK_BC_DISK: bc_disk(o); break;
     case SHARK_BC_SOD: bc_sod(o); break;
   }
+}
+
+/* ---------------------------------------------------------------- barotropic EOS, gravity */
+#define KB_CGS 1.38e-16   /* phys_const.f90:9 */
+#define MH_CGS 1.67e-24   /* phys_const.f90:8 */
+/* barotrop, src/setups/Collapse_1D/eos.f90:1-11; its first exponent is the default-real
+ * literal 0.8 (single precision, widened) */
+double oracle_barotrop(const shark_cfg* p, double nH) {
+  const double e08 = (double)0.8f;
+  return ((p->t0_cloud * sqrt(1.0 + pow((nH * p->unit_nh) / 1e11, e08))) *
+          pow(1.0 + (nH * p->unit_nh) / 1e16, -0.3)) *
+         pow(1.0 + (nH * p->unit_nh) / 1e21, 0.56667);
+}
+/* cs_eos, eos.f90:13-20 */
+double oracle_cs_eos(const shark_cfg* p, double T) {
+  return sqrt(((p->gamma * KB_CGS) * T) / (p->mu_gas * MH_CGS)) / p->unit_v;
+}
+static double cs_baro(const shark_cfg* p, double rho) { return oracle_cs_eos(p, oracle_barotrop(p, rho)); }
+
+/* r^2 + l_soft^2 of the gravity terms, and the enclosed mass over it: zero where both vanish
+ * (the inner ghost cells, radii_c = 0 and Mc = M_cent = 0, where the reference divides 0 by 0) */
+static double grav_acc(const oracle* o, int c) {
+  const double r = o->radii_c[c], ls = o->cfg.l_soft;
+  const double d = r * r + ls * ls;
+  return d != 0.0 ? o->mc[c] / d : 0.0;
+}
+
+/* mtot, src/gravity/gravity_utils.f90:11-51 (NY = 1): enclosed mass at each cell centre,
+ * Mc(i) = M_cent + 4 pi sum_{j<=i} rho_j vol_j - 4 pi rho_i dvol_i (gas and dust), from the
+ * primitive densities; ghost cells other than the inner one next to the grid hold 0 (unset in
+ * the reference) */
+static double dmint_of(const oracle* o, int c) {
+  const double fourpi = 4.0 * PI;
+  double dmint = (Q(o->irho, c) * o->dvol[c]) * fourpi;
+  for (int k = 0; k < o->nd; ++k) dmint = dmint + (Q(IRHOD(k), c) * o->dvol[c]) * fourpi;
+  return dmint;
+}
+
+static void mtot(oracle* o) {
+  const double fourpi = 4.0 * PI;
+  memset(o->mc, 0, (size_t)o->nc * 8);
+  o->mc[o->fa - 2] = o->cfg.m_cent;
+  for (int c = o->fa - 1; c <= o->la - 1; ++c) {
+    double mint = (Q(o->irho, c) * o->vol[c]) * fourpi;
+    for (int k = 0; k < o->nd; ++k) mint = mint + (Q(IRHOD(k), c) * o->vol[c]) * fourpi;
+    o->mc[c] = o->mc[c - 1] + mint;
+  }
+  for (int c = o->fa - 1; c <= o->la - 1; ++c) o->mc[c] = o->mc[c] - dmint_of(o, c);
 }
 
 /* ---------------------------------------------------------------- ctoprim */
@@ -342,6 +419,10 @@ static void ctoprim(oracle* o) {
       Q(o->iP, c) = P;
       o->cs[c] = sqrt(gamma * P / Q(o->irho, c));
     }
+    if (o->cfg.eos_baro) {   /* non_standard_eos = 1 (:225-226) */
+      o->cs[c] = cs_baro(&o->cfg, Q(o->irho, c));
+      Q(o->iP, c) = rho * (o->cs[c] * o->cs[c]);
+    }
     for (int k = 0; k < o->nd; ++k) {
       double rd = U(IRHOD(k), c);
       Q(IRHOD(k), c) = rd;
@@ -350,6 +431,7 @@ static void ctoprim(oracle* o) {
       Q(IVDZ(k), c) = U(IVDZ(k), c) / rd;
     }
   }
+  if (o->cfg.gravity) mtot(o);   /* :249-251 */
 }
 
 /* ---------------------------------------------------------------- setup hooks */
@@ -446,6 +528,10 @@ static void compute_tstop(oracle* o) {
         ts = ((p->rhograin * SDUST(c, k)) / p->rho_init) / ((p->rad0 * p->omega_shear) * p->hoverr);
       } else if (p->tstop_kind == SHARK_TSTOP_EPSTEIN) {
         ts = (sqrt((PI * p->gamma) / 8.0) * p->rhograin) * SDUST(c, k) / (Q(o->irho, c) * o->cs[c]);
+      } else if (p->tstop_kind == SHARK_TSTOP_COLLAPSE) {
+        /* src/setups/Collapse_1D/setup.f90:310-320 */
+        ts = ((sqrt((PI * p->gamma) / 8.0) * p->rhograin) * SDUST(c, k)) /
+             (Q(o->irho, c) * cs_baro(p, Q(o->irho, c)));
       } else if (p->tstop_kind == SHARK_TSTOP_EPSTEIN_CDISK) {
         double xx, yy, rr;
         cdisk_r(o, c, &xx, &yy, &rr);
@@ -480,6 +566,10 @@ static void courant(oracle* o) {
       vmax = fmax_f(vmax, 0.0 + vv);
     }
     dt = fmin_f(dt, (p->cfl * dxx) / fabs(vmax));
+    if (p->gravity) {   /* :62-63 */
+      const double r = o->radii_c[c], ls = p->l_soft;
+      dt = fmin_f(dt, (p->cfl * dxx) / sqrt(o->mc[c] / sqrt(r * r + ls * ls)));
+    }
     if (p->force_kick) {
       double fm = sqrt((FORCE(c, 0) * FORCE(c, 0) + FORCE(c, 1) * FORCE(c, 1)) + FORCE(c, 2) * FORCE(c, 2));
       for (int k = 0; k < o->nd; ++k)
@@ -592,13 +682,15 @@ static void predictor(oracle* o) {
       su0 = ((-(u * dux)) - v * duy - dpx / r_rho) + (v * v) / rp;
       sv0 = ((-(u * dvx)) - v * dvy - dpy / r_rho) - (u * v) / rp;
       sw0 = (-(u * dwx)) - v * dwy;
-    } else {                                     /* :196-202 */
+    } else {                                     /* :196-202; GEOM 1 :205-211 */
       sr0 = (-(u * drx)) - v * dry - (dux + dvy) * r_rho;
+      if (p->geom == SHARK_GEOM_SPHERE && o->radii_c[c] != 0.0) sr0 = sr0 - (r_rho * u) / o->radii_c[c];
       sp0 = (-(u * dpx)) - v * dpy - (dux + dvy) * gamma * P;
       su0 = (-(u * dux)) - v * duy - dpx / r_rho;
       sv0 = (-(u * dvx)) - v * dvy - dpy / r_rho;
       sw0 = (-(u * dwx)) - v * dwy;
     }
+    if (p->gravity) su0 = su0 + (-grav_acc(o, c));   /* :243-246 */
     double dxl = DX(c, 0);                       /* :258-268 */
     QM(o->irho, c, 0) = r_rho + HALF * dt * sr0 + HALF * drx * dxl;
     QM(o->ivx, c, 0) = u + HALF * dt * su0 + HALF * dux * dxl;
@@ -647,6 +739,8 @@ static void predictor(oracle* o) {
         su0 = su0 + (v * v) / rp;
         sv0 = sv0 - (u * v) / rp;
       }
+      if (p->geom == SHARK_GEOM_SPHERE && o->radii_c[c] != 0.0) sr0 = sr0 - (r_rho * u) / o->radii_c[c];  /* :386-389 */
+      if (p->gravity) su0 = su0 + (-grav_acc(o, c));   /* :442-445 */
       QM(ir_, c, 0) = r_rho + HALF * dt * sr0 + HALF * drx * dxl;
       QM(iu, c, 0) = u + HALF * dt * su0 + HALF * dux * dxl;
       QM(iv, c, 0) = v + HALF * dt * sv0 + HALF * dvx * dxl;
@@ -672,6 +766,12 @@ static void predictor(oracle* o) {
       for (int d = 0; d < o->ndim; ++d) {
         QP(o->iP, c, d) = cs2 * QP(o->irho, c, d);
         QM(o->iP, c, d) = cs2 * QM(o->irho, c, d);
+      }
+    if (p->eos_baro)                             /* :612-617 */
+      for (int d = 0; d < o->ndim; ++d) {
+        const double cp = cs_baro(p, QP(o->irho, c, d)), cm = cs_baro(p, QM(o->irho, c, d));
+        QP(o->iP, c, d) = (cp * cp) * QP(o->irho, c, d);
+        QM(o->iP, c, d) = (cm * cm) * QM(o->irho, c, d);
       }
   }
 }
@@ -817,7 +917,10 @@ static void add_delta_u(oracle* o) {
       int il = (d == 0) ? cidx(o, ix - 1, iy) : cidx(o, ix, iy - 1);
       for (int v = 0; v < nv; ++v) { ql[v] = QM(v, il, d); qr[v] = QP(v, c, d); flx[v] = 0.0; }
       double csl = o->cs[il], csr = o->cs[c];
-      if (p->iso_cs != 1) {                      /* :679-680, :709-710 */
+      if (p->iso_cs != 1 && p->eos_baro) {       /* :675-677, :702-706 */
+        csl = cs_baro(p, ql[o->irho]);
+        csr = cs_baro(p, qr[o->irho]);
+      } else if (p->iso_cs != 1) {               /* :679-680, :709-710 */
         csl = sqrt(p->gamma * ql[o->iP] / ql[o->irho]);
         csr = sqrt(p->gamma * qr[o->iP] / qr[o->irho]);
       }
@@ -871,6 +974,25 @@ static void source_terms(oracle* o) {
       for (int k = 0; k < o->nd; ++k) {
         S[IVDX(k) * nc + c] = S[IVDX(k) * nc + c] + dt * (Q(IRHOD(k), c) * (Q(IVDY(k), c) * Q(IVDY(k), c)) / r);
         S[IVDY(k) * nc + c] = S[IVDY(k) * nc + c] - dt * (Q(IRHOD(k), c) * Q(IVDY(k), c) * Q(IVDX(k), c) / r);
+      }
+    }
+  }
+  if (p->geom == SHARK_GEOM_SPHERE || p->gravity) {
+    /* GEOM 1 :33-35 (the pressure term of the spherical momentum equation) and gravity
+     * :79-85, in the reference's order within the active-cell loop */
+    #pragma omp parallel for schedule(static)
+    for (int c = 0; c < o->nc; ++c) {
+      if (!o->active[c]) continue;
+      double* S = o->S_U;
+      size_t nc = o->nc;
+      const double r = o->radii_c[c];
+      if (p->geom == SHARK_GEOM_SPHERE)
+        S[o->ivx * nc + c] = S[o->ivx * nc + c] - dt * (-(((2.0 * Q(o->irho, c)) * (o->cs[c] * o->cs[c])) / r));
+      if (p->gravity) {
+        const double d = r * r + p->l_soft * p->l_soft;
+        S[o->ivx * nc + c] = S[o->ivx * nc + c] - dt * ((Q(o->irho, c) * o->mc[c]) / d);
+        for (int k = 0; k < o->nd; ++k)
+          S[IVDX(k) * nc + c] = S[IVDX(k) * nc + c] - dt * ((Q(IRHOD(k), c) * o->mc[c]) / d);
       }
     }
   }

@@ -13,6 +13,9 @@ int oracle_ncells(const oracle* o);
 int oracle_nvar(const oracle* o);
 void oracle_set_geometry(oracle* o, const double* dx, const double* surf, const double* vol,
                          const double* radii_c, const double* position);
+void oracle_set_dvol(oracle* o, const double* dvol);
+double oracle_barotrop(const shark_cfg* p, double nH);
+double oracle_cs_eos(const shark_cfg* p, double T);
 void oracle_upload(oracle* o, const double* u, const double* cs, const double* sdust,
                    const double* condinit, const double* fargo_v);
 double oracle_step(oracle* o);

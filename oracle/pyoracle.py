@@ -31,6 +31,11 @@ def load():
     lib.oracle_nvar.argtypes = [C.c_void_p]
     lib.oracle_set_geometry.argtypes = [C.c_void_p] + [_P] * 5
     lib.oracle_upload.argtypes = [C.c_void_p] + [_P] * 5
+    lib.oracle_set_dvol.argtypes = [C.c_void_p, _P]
+    lib.oracle_barotrop.restype = C.c_double
+    lib.oracle_barotrop.argtypes = [C.c_void_p, C.c_double]
+    lib.oracle_cs_eos.restype = C.c_double
+    lib.oracle_cs_eos.argtypes = [C.c_void_p, C.c_double]
     lib.oracle_step.restype = C.c_double
     lib.oracle_step.argtypes = [C.c_void_p]
     lib.oracle_run.restype = C.c_int64
@@ -92,6 +97,8 @@ class Oracle:
         g = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in geom.items()}
         self.L.oracle_set_geometry(self.h, _ptr(g["dx"]), _ptr(g["surf"]), _ptr(g["vol"]),
                                    _ptr(g["radii_c"]), _ptr(g.get("position")))
+        if g.get("dvol") is not None:
+            self.L.oracle_set_dvol(self.h, _ptr(g["dvol"]))
         s = {k: (None if v is None else np.ascontiguousarray(v, dtype=np.float64))
              for k, v in state.items()}
         self.L.oracle_upload(self.h, _ptr(s["u"]), _ptr(s["cs"]), _ptr(s.get("sdust")),

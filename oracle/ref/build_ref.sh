@@ -23,8 +23,11 @@ OUT=$HERE/../_ref/$NAME
 mkdir -p "$OUT/obj"
 if [ ! -d "$REF/src" ]; then echo "reference sources not found at $REF" >&2; exit 2; fi
 
+# GRAVITY defaults to 0; an extra -DGRAVITY=... replaces it (flang keeps the first definition)
+GRAV="-DGRAVITY=0"
+case " $EXTRA " in *" -DGRAVITY="*) GRAV="" ;; esac
 DEFINES="-DNDUST=$NDUST -DNX=$NX -DNY=$NY -DGRIDSPACE=$GRIDSPACE -DGEOM=$GEOM -DMHD=0 -DOPENMP=1 \
- -DSOLVER=$SOLVER -DSOLVERDUST=$SOLVERDUST -DSOLVERB=1 -DNGHOST=2 -DTURB=0 -DGRAVITY=0 -DNDUSTPSCAL=0 $EXTRA"
+ -DSOLVER=$SOLVER -DSOLVERDUST=$SOLVERDUST -DSOLVERB=1 -DNGHOST=2 -DTURB=0 $GRAV -DNDUSTPSCAL=0 $EXTRA"
 FC=${FC:-flang}
 FFLAGS="-cpp -O3 -fopenmp $DEFINES -module-dir $OUT/obj"
 

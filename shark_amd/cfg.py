@@ -10,11 +10,11 @@ from dataclasses import dataclass
 SHARK_CFG_VERSION = 2
 SHARK_MAX_DUST = 64
 
-GEOM_CARTESIAN, GEOM_POLAR = 0, 2
+GEOM_CARTESIAN, GEOM_SPHERE, GEOM_POLAR = 0, 1, 2
 SOLVER_LLF, SOLVER_HLL, SOLVER_HLLC = 0, 1, 2
 DUST_HUANG_BAI, DUST_LLF, DUST_HLL, DUST_HLLC = 0, 1, 2, 3
-BC_PERIODIC, BC_SHEAR_SI, BC_DISK, BC_SOD, BC_ZERO_GRADIENT = 0, 1, 2, 3, 4
-TSTOP_DUSTYWAVE, TSTOP_SI, TSTOP_EPSTEIN_DISK, TSTOP_EPSTEIN, TSTOP_EPSTEIN_CDISK = 0, 1, 2, 3, 4
+BC_PERIODIC, BC_SHEAR_SI, BC_DISK, BC_SOD, BC_ZERO_GRADIENT, BC_COLLAPSE = 0, 1, 2, 3, 4, 5
+TSTOP_DUSTYWAVE, TSTOP_SI, TSTOP_EPSTEIN_DISK, TSTOP_EPSTEIN, TSTOP_EPSTEIN_CDISK, TSTOP_COLLAPSE = 0, 1, 2, 3, 4, 5
 FORCE_NONE, FORCE_SI, FORCE_DISK, FORCE_CDISK = 0, 1, 2, 3
 INLOOP_NONE, INLOOP_SI, INLOOP_DISK, INLOOP_CDISK = 0, 1, 2, 3
 
@@ -41,8 +41,11 @@ class ShARKCfg(C.Structure):
         ("sigma_r0", C.c_double), ("r0_disk", C.c_double), ("frac_r_disk", C.c_double),
         ("buffer_in", C.c_double), ("buffer_out", C.c_double), ("alpha_visc", C.c_double),
         ("dust2gas", C.c_double), ("reserved", C.c_double * 2),
-        ("accretion", C.c_int32), ("reserved_i2", C.c_int32), ("m_acc", C.c_double),
-        ("phi_mom", C.c_double), ("l_out", C.c_double), ("reserved2", C.c_double * 8),
+        ("accretion", C.c_int32), ("gravity", C.c_int32), ("m_acc", C.c_double),
+        ("phi_mom", C.c_double), ("l_out", C.c_double), ("eos_baro", C.c_int32),
+        ("reserved_i3", C.c_int32), ("l_soft", C.c_double), ("m_cent", C.c_double),
+        ("t0_cloud", C.c_double), ("mu_gas", C.c_double), ("unit_v", C.c_double),
+        ("unit_nh", C.c_double), ("reserved3", C.c_double * 1),
     ]
 
 

@@ -70,7 +70,7 @@ struct shark_ctx {
   double* dfargo = nullptr;
   DevScalars* dsc = nullptr;
   DevScalars* hsc = nullptr;  // pinned mirror
-  bool geom_set = false, uploaded = false;
+  bool geom_set = false, uploaded = false, dvol_set = false;
   hipGraphExec_t gexec = nullptr;
   bool timing = false;
   double kt_ms[KT_N] = {0, 0, 0, 0, 0};
@@ -79,6 +79,8 @@ struct shark_ctx {
   shark_growth* growth = nullptr;   // attached growth state (dust_growth inside each step)
   double* dRold = nullptr;          // dust densities before growth, for the velocity-keeping momenta
   double* dfcs = nullptr;           // gas face sound speeds for the HLLC dust solver (adiabatic gas)
+  double* ddvol = nullptr;          // spherical grid: the cells' inner volumes (shark_gpu_set_dvol)
+  double* dmc = nullptr;            // 1D self-gravity: enclosed mass per cell
   int growth_mode = 0;
 };
 
@@ -104,13 +106,23 @@ static int validate(const shark_cfg* c) {
     return fail(SHARK_EUNSUP, "ndust=%d: the step is built for 0 to %d dust species", c->ndust, SHARK_MAX_DUST);
   if (c->iso_cs > 1)
     return fail(SHARK_EINVAL, "iso_cs=%d: 1 is isothermal, below 1 adiabatic", c->iso_cs);
-  if (c->iso_cs != 1 && (c->geom != SHARK_GEOM_CARTESIAN || c->fargo))
-    return fail(SHARK_EUNSUP, "iso_cs=%d: adiabatic gas is built on cartesian grids without FARGO", c->iso_cs);
+  if (c->iso_cs != 1 && ((c->geom != SHARK_GEOM_CARTESIAN && c->geom != SHARK_GEOM_SPHERE) || c->fargo))
+    return fail(SHARK_EUNSUP, "iso_cs=%d: adiabatic gas is built on cartesian and spherical 1D grids without FARGO",
+                c->iso_cs);
   if (c->bc_kind == SHARK_BC_SOD && (c->ndust > 0 || c->shock_dir < 0 || c->shock_dir > 1 ||
                                      (c->shock_dir == 1 && c->ny < 4)))
     return fail(SHARK_EUNSUP, "Sod boundaries need gas only and shock_dir 0 (x) or 1 (y, 2D)");
-  if (c->geom != SHARK_GEOM_CARTESIAN && c->geom != SHARK_GEOM_POLAR)
-    return fail(SHARK_EUNSUP, "geom=%d: GEOM 0 and 2 are built", c->geom);
+  if (c->geom != SHARK_GEOM_CARTESIAN && c->geom != SHARK_GEOM_POLAR && c->geom != SHARK_GEOM_SPHERE)
+    return fail(SHARK_EUNSUP, "geom=%d: GEOM 0, 1 and 2 are built", c->geom);
+  if (c->geom == SHARK_GEOM_SPHERE && c->ny != 1)
+    return fail(SHARK_EUNSUP, "the spherical geometry (GEOM 1) is built in 1D (Collapse_1D)");
+  if (c->gravity && c->geom != SHARK_GEOM_SPHERE)
+    return fail(SHARK_EUNSUP, "self-gravity is built for the spherical 1D grid (GRAVITY=1 with NY=1)");
+  if (c->eos_baro && c->iso_cs >= 1)
+    return fail(SHARK_EUNSUP, "the barotropic EOS (non_standard_eos) is built with iso_cs < 1 (Collapse_1D)");
+  if ((c->bc_kind == SHARK_BC_COLLAPSE || c->tstop_kind == SHARK_TSTOP_COLLAPSE) &&
+      (c->ny != 1 || !c->eos_baro))
+    return fail(SHARK_EUNSUP, "the Collapse_1D hooks need a 1D grid and the barotropic EOS");
   if (c->geom == SHARK_GEOM_POLAR && c->ny < 4) return fail(SHARK_EUNSUP, "polar geometry needs a 2D grid");
   if (c->solver < 0 || c->solver > 2) return fail(SHARK_EINVAL, "solver=%d", c->solver);
   if (c->solverdust < 0 || c->solverdust > 3) return fail(SHARK_EINVAL, "solverdust=%d", c->solverdust);
@@ -127,7 +139,7 @@ static int validate(const shark_cfg* c) {
     return fail(SHARK_EUNSUP, "FARGO is built for 2D polar grids (rings of constant radius, the disk_polar setup)");
   if (c->force_kick && !c->drag && c->ndust > 0)
     return fail(SHARK_EUNSUP, "explicit drag in the force (drag=.false. with force_kick) is not built");
-  if (c->bc_kind < 0 || c->bc_kind > 4 || c->tstop_kind < 0 || c->tstop_kind > 4 ||
+  if (c->bc_kind < 0 || c->bc_kind > 5 || c->tstop_kind < 0 || c->tstop_kind > 5 ||
       c->force_kind < 0 || c->force_kind > 3 || c->inloop_kind < 0 || c->inloop_kind > 3)
     return fail(SHARK_EINVAL, "unknown setup hook kind");
   if (c->bc_kind == SHARK_BC_DISK && c->ny < 4) return fail(SHARK_EUNSUP, "disk boundaries need a 2D grid");
@@ -180,6 +192,8 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   a.buffer_out = cfg->buffer_out; a.alpha_visc = cfg->alpha_visc; a.dust2gas = cfg->dust2gas;
   a.viscosity = cfg->viscosity;
   a.accretion = cfg->accretion; a.m_acc = cfg->m_acc; a.phi_mom = cfg->phi_mom; a.l_out = cfg->l_out;
+  a.gravity = cfg->gravity; a.eos_baro = cfg->eos_baro; a.l_soft = cfg->l_soft; a.m_cent = cfg->m_cent;
+  a.t0_cloud = cfg->t0_cloud; a.mu_gas = cfg->mu_gas; a.unit_v = cfg->unit_v; a.unit_nh = cfg->unit_nh;
   for (int k = 0; k < SHARK_MAX_DUST; ++k) a.K[k] = cfg->K[k];
   // launch shapes (shark_gpu_set_tune / shark_gpu_autotune change them)
   a.strip_h = 32;
@@ -212,6 +226,18 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
     (void)hipMemsetAsync(c->dfcs, 0, (size_t)4 * a.nc * 8, c->stream);
   }
   a.fcs = c->dfcs;
+  if (cfg->gravity) {
+    if ((e = hipMalloc(&c->dmc, (size_t)a.nc * 8)) != hipSuccess ||
+        (e = hipMalloc(&c->ddvol, (size_t)a.nc * 8)) != hipSuccess) {
+      rc = fail(e == hipErrorOutOfMemory ? SHARK_ENOMEM : SHARK_EHIP, "allocation failed: %s", hipGetErrorString(e));
+      shark_gpu_destroy(c);
+      return rc;
+    }
+    (void)hipMemsetAsync(c->dmc, 0, (size_t)a.nc * 8, c->stream);
+    (void)hipMemsetAsync(c->ddvol, 0, (size_t)a.nc * 8, c->stream);
+  }
+  a.mc = c->dmc;
+  a.dvol = c->ddvol;
   (void)hipMemsetAsync(c->dU, 0, nbytes, c->stream);
   (void)hipMemsetAsync(c->dB, 0, nbytes, c->stream);
   memset(c->hsc, 0, sizeof(DevScalars));
@@ -280,6 +306,16 @@ extern "C" int shark_gpu_set_geometry(shark_ctx* c, const double* dx, const doub
   return SHARK_OK;
 }
 
+extern "C" int shark_gpu_set_dvol(shark_ctx* c, const double* dvol) {
+  if (!c) return fail(SHARK_EINVAL, "null ctx");
+  if (!dvol) return fail(SHARK_EINVAL, "null dvol");
+  if (!c->ddvol) return fail(SHARK_ESTATE, "dvol is read only with self-gravity (cfg.gravity = 1)");
+  HIPCHK(hipMemcpyAsync(c->ddvol, dvol, (size_t)c->a.nc * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->dvol_set = true;
+  return SHARK_OK;
+}
+
 static void drop_graph(shark_ctx* c) {
   if (c->gexec) {
     (void)hipGraphExecDestroy(c->gexec);
@@ -291,6 +327,8 @@ extern "C" int shark_gpu_upload(shark_ctx* c, const double* u, const double* cs,
                                 const double* condinit, const double* fargo_velocity) {
   if (!c) return fail(SHARK_EINVAL, "null ctx");
   if (!c->geom_set) return fail(SHARK_ESTATE, "shark_gpu_set_geometry must precede upload");
+  if (c->cfg.gravity && !c->dvol_set)
+    return fail(SHARK_ESTATE, "self-gravity needs the cells' inner volumes: shark_gpu_set_dvol before upload");
   if (!u || !cs) return fail(SHARK_EINVAL, "u_prim and cs are required");
   StepArgs& a = c->a;
   const shark_cfg& cf = c->cfg;
@@ -841,6 +879,8 @@ extern "C" int shark_gpu_destroy(shark_ctx* c) {
   (void)hipFree(c->dU); (void)hipFree(c->dB); (void)hipFree(c->dcs); (void)hipFree(c->dgeom); (void)hipFree(c->dsc);
   if (c->dRold) (void)hipFree(c->dRold);
   if (c->dfcs) (void)hipFree(c->dfcs);
+  if (c->dmc) (void)hipFree(c->dmc);
+  if (c->ddvol) (void)hipFree(c->ddvol);
   if (c->dsdust) (void)hipFree(c->dsdust);
   if (c->dcond) (void)hipFree(c->dcond);
   if (c->dfargo) (void)hipFree(c->dfargo);

@@ -73,6 +73,11 @@ struct StepArgs {
   int viscosity;
   int accretion;        // polar disk inflow through the outer boundary (boundaries.f90:63-94)
   double m_acc, phi_mom, l_out;
+  // Collapse_1D (spherical 1D grid): 1D self-gravity and the barotropic EOS
+  int gravity, eos_baro;
+  double l_soft, m_cent, t0_cloud, mu_gas, unit_v, unit_nh;
+  const double* dvol;   // (ncells) part of each cell between its centre and its outer face
+  double* mc;           // (ncells) mass enclosed within each cell centre, this step (k_mtot)
 };
 
 // launch shapes: every shape gives the same results bit for bit, only the speed differs

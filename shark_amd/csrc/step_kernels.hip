@@ -148,6 +148,86 @@ __global__ void k_bc(StepArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ Collapse_1D
+constexpr double kKB = 1.38e-16, kMH = 1.67e-24;   // phys_const.f90:8-9
+
+// barotrop (src/setups/Collapse_1D/eos.f90:1-11; its first exponent is the default-real literal
+// 0.8, single precision widened) and cs_eos (:13-20): the barotropic sound speed of density rho
+DEV double cs_baro(const StepArgs& a, double rho) {
+  const double e08 = (double)0.8f;
+  const double x = rho * a.unit_nh;
+  const double T = ((a.t0_cloud * sqrt(1.0 + pow(x / 1e11, e08))) * pow(1.0 + x / 1e16, -0.3)) *
+                   pow(1.0 + x / 1e21, 0.56667);
+  return sqrt(((a.gamma * kKB) * T) / (a.mu_gas * kMH)) / a.unit_v;
+}
+
+// boundary_collapse_1D (src/grid/boundary_types.f90:4-44), 1D: ghosts copy the first / last
+// active cell; nothing flows through the centre (the inner ghosts and the first active cell at
+// rest) or out of the outer edge (the outer ghosts and the last active cell keep only inward
+// velocities).  One thread per variable plane.
+__global__ void k_bc_collapse(StepArgs a) {
+  if (!a.sc->active) return;
+  const size_t nc = a.nc;
+  const int fa = 2, la = a.nx + 1;   // 0-based first / last active cell
+  for (int v = threadIdx.x; v < a.nvar; v += blockDim.x) {
+    const bool mom = v == 1 || (v >= 5 + a.nd && v < 5 + 2 * a.nd);   // vx of the gas and dust
+    double* U = a.U + (size_t)v * nc;
+    const double l = U[fa], r = U[la];
+    const double lv = mom ? 0.0 : l, rv = mom ? fminf_(r, 0.0) : r;
+    U[0] = lv;
+    U[1] = lv;
+    U[nc - 2] = rv;
+    U[nc - 1] = rv;
+    if (mom) {
+      U[fa] = 0.0;
+      U[la] = rv;
+    }
+  }
+}
+
+// mtot (src/gravity/gravity_utils.f90:11-51, NY = 1): the mass within each cell centre,
+// Mc = M_cent + 4 pi sum_{j<=i} rho_j vol_j - 4 pi rho_i dvol_i over gas and dust, from the
+// start-of-step densities.  The running sum is one chain in the reference's order (one thread
+// over each block-wide tile of per-cell terms); the ghost cells hold 0, the inner one next to the
+// grid M_cent.
+__global__ __launch_bounds__(1024) void k_mtot(StepArgs a) {
+  __shared__ double mint[1024], dmint[1024];
+  __shared__ double carry;
+  const double fourpi = 4.0 * kPi;
+  const size_t nc = a.nc;
+  for (int c = threadIdx.x; c < a.nc; c += blockDim.x) a.mc[c] = 0.0;
+  if (threadIdx.x == 0) carry = a.m_cent;
+  __syncthreads();
+  if (threadIdx.x == 0) a.mc[1] = a.m_cent;
+  for (int base = 2; base <= a.nx + 1; base += 1024) {
+    const int c = base + (int)threadIdx.x;
+    if (c <= a.nx + 1) {
+      const double rho = fmaxf_(a.U[c], a.smallr), vol = a.g.vol[c], dv = a.dvol[c];
+      double mi = (rho * vol) * fourpi, dm = (rho * dv) * fourpi;
+      for (int k = 0; k < a.nd; ++k) {
+        const double rd = a.U[(5 + k) * nc + c];
+        mi = mi + (rd * vol) * fourpi;
+        dm = dm + (rd * dv) * fourpi;
+      }
+      mint[threadIdx.x] = mi;
+      dmint[threadIdx.x] = dm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int n = (a.nx + 2 - base) < 1024 ? (a.nx + 2 - base) : 1024;
+      double m = carry;
+      for (int i = 0; i < n; ++i) {
+        m = m + mint[i];
+        mint[i] = m;
+      }
+      carry = m;
+    }
+    __syncthreads();
+    if (c <= a.nx + 1) a.mc[c] = mint[threadIdx.x] - dmint[threadIdx.x];
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ hydro
 // Per-lane state of one loaded row of one fluid.
 struct RowF {
@@ -201,6 +281,15 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
   const double dsx = dx1 + dx1m, ysx = 1.0 / dsx;
   const double dsy = dx2 + dx2, ysy = (NDIM == 2) ? 1.0 / dsy : 0.0;
   const double yrp = 1.0 / rp, yvol = 1.0 / vol;
+  // spherical 1D grid (GEOM 1): the cell-centre radius, and the gravity of the enclosed mass
+  // Mc / (r^2 + l_soft^2), 0 where both vanish (the inner ghost cells: radii_c = 0, Mc = M_cent)
+  constexpr bool SPH = GEOM == 1;
+  const double rsph = SPH ? a.g.rc[ixc - 1] : 0.0;
+  double gacc = 0.0;
+  if (SPH && a.gravity) {
+    const double d = rsph * rsph + a.l_soft * a.l_soft;
+    gacc = d != 0.0 ? a.mc[ixc - 1] / d : 0.0;
+  }
 
   int y0 = 1, y1 = 1;
   if (NDIM == 2) {
@@ -229,6 +318,10 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
                           kHalf * r.u[0] * (r.q[3] * r.q[3]);
       r.q[4] = fmaxf_((a.gamma - 1.0) * (r.u[4] - ekin - 0.0), a.smallp);
       r.cs = sqrt(a.gamma * r.q[4] / r.q[0]);
+      if (a.eos_baro) {   // non_standard_eos = 1 (solve.f90:225-226)
+        r.cs = cs_baro(a, r.q[0]);
+        r.q[4] = r.u[0] * (r.cs * r.cs);
+      }
     }
   };
 
@@ -267,10 +360,12 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
         sw0 = (-(u * dwx)) - v * dwy;
       } else {
         sr0 = (-(u * drx)) - v * dry - (dux + dvy) * r_rho;
+        if (SPH && rsph != 0.0) sr0 = sr0 - (r_rho * u) / rsph;   // godunov.f90:205-207
         su0 = (-(u * dux)) - v * duy - dpx / r_rho;
         sv0 = (-(u * dvx)) - v * dvy - dpy / r_rho;
         sw0 = (-(u * dwx)) - v * dwy;
       }
+      if (SPH && a.gravity) su0 = su0 + (-gacc);   // :243-246
     } else {
       sr0 = (-(u * drx)) - v * dry - (dux + dvy) * r_rho;
       su0 = (-(u * dux)) - v * duy;
@@ -281,6 +376,8 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
         su0 = su0 + exdiv(v * v, rp, yrp);
         sv0 = sv0 - exdiv(u * v, rp, yrp);
       }
+      if (SPH && rsph != 0.0) sr0 = sr0 - (r_rho * u) / rsph;   // :386-389
+      if (SPH && a.gravity) su0 = su0 + (-gacc);               // :442-445
     }
     const double hdt = kHalf * dt;
     const double b0 = r_rho + hdt * sr0, b1 = u + hdt * su0, b2 = v + hdt * sv0, b3 = w + hdt * sw0;
@@ -292,6 +389,11 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
       const double b4 = P + hdt * sp0;
       fx.m[4] = fmaxf_(b4 + kHalf * dpx * dx1, a.smallp);
       fx.p[4] = fmaxf_(b4 - kHalf * dpx * dx1, a.smallp);
+      if (a.eos_baro) {   // :612-617
+        const double cm = cs_baro(a, fx.m[0]), cp = cs_baro(a, fx.p[0]);
+        fx.m[4] = (cm * cm) * fx.m[0];
+        fx.p[4] = (cp * cp) * fx.p[0];
+      }
     }
     if (NDIM == 2) {
       const double dyl = rp * dx2;
@@ -303,6 +405,11 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
         const double b4 = P + hdt * sp0;
         fy.m[4] = fmaxf_(b4 + kHalf * dpy * dyl, a.smallp);
         fy.p[4] = fmaxf_(b4 - kHalf * dpy * dyl, a.smallp);
+        if (a.eos_baro) {
+          const double cm = cs_baro(a, fy.m[0]), cp = cs_baro(a, fy.p[0]);
+          fy.m[4] = (cm * cm) * fy.m[0];
+          fy.p[4] = (cp * cp) * fy.p[0];
+        }
       }
     }
   };
@@ -311,9 +418,14 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
   // dir 1: y (normal vy). Returns conservative components in plane order.
   auto riemann = [&](const double L[5], double csl, const double R[5], double csr, int dir,
                      double F[5]) {
-    if constexpr (ADI) {   // sound speeds of the face states, godunov.f90:679-680, 709-710
-      csl = sqrt(a.gamma * L[4] / L[0]);
-      csr = sqrt(a.gamma * R[4] / R[0]);
+    if constexpr (ADI) {   // sound speeds of the face states, godunov.f90:675-680, 702-710
+      if (a.eos_baro) {
+        csl = cs_baro(a, L[0]);
+        csr = cs_baro(a, R[0]);
+      } else {
+        csl = sqrt(a.gamma * L[4] / L[0]);
+        csr = sqrt(a.gamma * R[4] / R[0]);
+      }
     }
     Face fl, fr;
     fl.r = L[0]; fr.r = R[0];
@@ -406,8 +518,8 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
         if constexpr (GAS && ADI) {
           // the face on the left of every owned column and of the ghost column nx+3
           if (a.fcs && (own_col || ix == a.nx + 3)) {
-            a.fcs[fcell] = sqrt(a.gamma * L[4] / L[0]);
-            a.fcs[nc + fcell] = sqrt(a.gamma * fx1.p[4] / fx1.p[0]);
+            a.fcs[fcell] = a.eos_baro ? cs_baro(a, L[0]) : sqrt(a.gamma * L[4] / L[0]);
+            a.fcs[nc + fcell] = a.eos_baro ? cs_baro(a, fx1.p[0]) : sqrt(a.gamma * fx1.p[4] / fx1.p[0]);
           }
         }
 #pragma unroll
@@ -423,8 +535,8 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
         }
         if constexpr (GAS && ADI) {
           if (a.fcs && own_col) {
-            a.fcs[2 * nc + fcell] = sqrt(a.gamma * fym_prev[4] / fym_prev[0]);
-            a.fcs[3 * nc + fcell] = sqrt(a.gamma * fy1.p[4] / fy1.p[0]);
+            a.fcs[2 * nc + fcell] = a.eos_baro ? cs_baro(a, fym_prev[0]) : sqrt(a.gamma * fym_prev[4] / fym_prev[0]);
+            a.fcs[3 * nc + fcell] = a.eos_baro ? cs_baro(a, fy1.p[0]) : sqrt(a.gamma * fy1.p[4] / fy1.p[0]);
           }
         }
       }
@@ -475,6 +587,15 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
         } else {
           S[1] = 0.0 + dt * exdiv(Rf.q[0] * (Rf.q[2] * Rf.q[2]), r, yrp);
           S[2] = 0.0 - dt * exdiv(Rf.q[0] * Rf.q[2] * Rf.q[1], r, yrp);
+        }
+      }
+      if constexpr (SPH) {
+        // source_terms.f90:33-35 (the pressure term of the spherical momentum equation) and the
+        // gravity of the enclosed mass :79-85, from the start-of-step primitives
+        if (GAS) S[1] = S[1] - dt * (-(((2.0 * Rf.q[0]) * (Rf.cs * Rf.cs)) / rsph));
+        if (a.gravity) {
+          const double d = rsph * rsph + a.l_soft * a.l_soft;
+          S[1] = S[1] - dt * ((Rf.q[0] * a.mc[cell]) / d);
         }
       }
       if constexpr (GAS && ADI) {
@@ -577,6 +698,7 @@ __global__ __launch_bounds__(256, (GAS ? 2 : 1)) void k_hydro(StepArgs a) {
 // ctoprim's sqrt(gamma P / max(rho, smallr)) with P = max((gamma-1)(E - ekin), smallp)
 // (solve.f90:214-221), from the velocities u/rho.
 DEV double cs_cell(const StepArgs& a, double rho, double E, double vx, double vy, double vz, int cell) {
+  if (a.eos_baro) return cs_baro(a, fmaxf_(rho, a.smallr));
   if (a.iso) return a.cs[cell];
   const double ekin = kHalf * rho * (vx * vx) + kHalf * rho * (vy * vy) + kHalf * rho * (vz * vz);
   const double P = fmaxf_((a.gamma - 1.0) * (E - ekin - 0.0), a.smallp);
@@ -585,6 +707,7 @@ DEV double cs_cell(const StepArgs& a, double rho, double E, double vx, double vy
 
 // ctoprim's sound speed of a cell's start-of-step state (a.U)
 DEV double start_cs(const StepArgs& a, int cell) {
+  if (a.eos_baro) return cs_baro(a, fmaxf_(a.U[cell], a.smallr));
   if (a.iso) return a.cs[cell];
   const size_t nc = a.nc;
   const double rho = a.U[cell];
@@ -617,6 +740,10 @@ DEV double courant_cell(const StepArgs& a, const double* un, int ix, int iy, int
     vmax = fmaxf_(vmax, 0.0 + vv);
   }
   double dt = (a.cfl * dxx) / fabs(vmax);
+  if (a.gravity) {   // courant.f90:62-63
+    const double r = a.g.rc[ix - 1];
+    dt = fminf_(dt, (a.cfl * dxx) / sqrt(a.mc[cell] / sqrt(r * r + a.l_soft * a.l_soft)));
+  }
   if (a.force_kick) {
     double f1, f2, f3;
     cell_force(a, ix, iy, next, vx, vz, true, f1, f2, f3);
@@ -684,7 +811,9 @@ DEV double tstop_of(const StepArgs& a, int k, int cell, int ix, double rho_q, do
   const double s = a.sdust_uniform ? a.sdust_k[k] : a.sdust[(size_t)k * a.nc + cell];
   if (a.tstop_kind == 1)  // SI setup.f90:392
     return ((a.rhograin * s) / a.rho_init) / ((a.rad0 * a.omega_shear) * a.hoverr);
-  if (a.tstop_kind == 3) return ((sqrt((kPi * a.gamma) / 8.0) * a.rhograin) * s) / (rho_q * cs_q);
+  // uniform Epstein (kind 3) and Collapse_1D's (kind 5, src/setups/Collapse_1D/setup.f90:310-320,
+  // with the barotropic cs_q)
+  if (a.tstop_kind == 3 || a.tstop_kind == 5) return ((sqrt((kPi * a.gamma) / 8.0) * a.rhograin) * s) / (rho_q * cs_q);
   if (CD && a.tstop_kind == 4) {   // cartesian disk, disk/setup.f90:333-341
     const int iy = cell / a.nxm + 1;
     const double xx = a.g.pos1[ix - 1] - kHalf * a.box_l, yy = a.g.pos2[iy - 1] - kHalf * a.box_l_y;
@@ -832,7 +961,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
         gS[k][c] = 1.0 / (1.0 + ak);
       }
     } else {
-      const double cs_q = a.tstop_kind == 3 ? start_cs(a, cell) : 0.0;
+      const double cs_q = (a.tstop_kind == 3 || a.tstop_kind == 5) ? start_cs(a, cell) : 0.0;
       for (int k = s; k < nd; k += NS) {
         const double rq = (a.tstop_kind == 0) ? a.U[(5 + k) * nc + cell] : 0.0;
         const double ak = (1.0 * dt) / tstop_of<ND, !FARGO>(a, k, cell, ixs, rho_q, rq, cs_q);
@@ -1197,6 +1326,14 @@ static void launch_courant_now(const StepArgs& a, hipStream_t s);
 
 void launch_bc(const StepArgs& a, hipStream_t s) {
   launch_planet(a, s);
+  if (a.bc_kind == 5) {
+    hipLaunchKernelGGL(k_bc_collapse, dim3(1), dim3(256), 0, s, a);
+    // the boundary rewrites the first and last active cells' velocities, so this step's dt is
+    // taken here from the filled state (with the enclosed mass of this state), as courant
+    // follows apply_boundaries and ctoprim in solve (solve.f90:29-56)
+    launch_courant_now(a, s);
+    return;
+  }
   if (a.bc_kind == 4) {
     hipLaunchKernelGGL(k_bc_zg, dim3(1), dim3(1024), 0, s, a);
     return;
@@ -1217,8 +1354,8 @@ static void hydro_launch1(const StepArgs& a, hipStream_t s) {
   const long items = (long)a.ngroups * (GAS ? a.nstrips_g : a.nstrips) * (GAS ? 1 : a.nd);
   const dim3 grid((unsigned)((items + 3) / 4));
   const bool fast = GAS ? a.solver == 1 : a.solverdust == 0;   // HLL gas / Huang-Bai dust
-  if constexpr (GAS && GEOM == 0) {
-    if (!a.iso) {   // adiabatic gas (cartesian grids only: shark_gpu_create rejects the others)
+  if constexpr (GAS && (GEOM == 0 || GEOM == 1)) {
+    if (!a.iso) {   // adiabatic gas (cartesian and spherical 1D grids: shark_gpu_create rejects the others)
       if (fast) hipLaunchKernelGGL((k_hydro<NDIM, GEOM, true, SLOPE, 1, true>), grid, dim3(256), 0, s, a);
       else hipLaunchKernelGGL((k_hydro<NDIM, GEOM, true, SLOPE, -1, true>), grid, dim3(256), 0, s, a);
       return;
@@ -1239,7 +1376,8 @@ static void hydro_launch_slope(const StepArgs& a, hipStream_t s) {
 
 template <bool GAS>
 static void hydro_launch_geom(const StepArgs& a, hipStream_t s) {
-  if (a.ndim == 1) hydro_launch_slope<1, 0, GAS>(a, s);
+  if (a.ndim == 1 && a.geom == 1) hydro_launch_slope<1, 1, GAS>(a, s);   // spherical (Collapse_1D)
+  else if (a.ndim == 1) hydro_launch_slope<1, 0, GAS>(a, s);
   else if (a.geom == 2 && a.viscosity) hydro_launch_slope<2, 4, GAS>(a, s);   // viscous polar disk
   else if (a.geom == 2) hydro_launch_slope<2, 2, GAS>(a, s);
   else if (a.force_kind == 3 || a.viscosity) hydro_launch_slope<2, 3, GAS>(a, s);   // cartesian disk
@@ -1338,6 +1476,7 @@ void launch_post_part(const StepArgs& a, int part, hipStream_t s) {
   else { SHARK_ND_SWITCH(post2_nd) }
 }
 static void launch_courant_now(const StepArgs& a, hipStream_t s) {
+  if (a.gravity) hipLaunchKernelGGL(k_mtot, dim3(1), dim3(1024), 0, s, a);
   SHARK_ND_SWITCH(courant_nd)
   hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, s, a.sc);
 }

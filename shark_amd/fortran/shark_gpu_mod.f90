@@ -35,9 +35,11 @@ module shark_gpu_mod
      real(c_double) :: box_l_y, mplanet, rplanet, sigma_r0, r0_disk, frac_r_disk, buffer_in, buffer_out
      real(c_double) :: alpha_visc, dust2gas
      real(c_double) :: reserved(2)
-     integer(c_int32_t) :: accretion, reserved_i2
+     integer(c_int32_t) :: accretion, gravity
      real(c_double) :: m_acc, phi_mom, l_out
-     real(c_double) :: reserved2(8)
+     integer(c_int32_t) :: eos_baro, reserved_i3
+     real(c_double) :: l_soft, m_cent, t0_cloud, mu_gas, unit_v, unit_nh
+     real(c_double) :: reserved3(1)
   end type shark_cfg
 
   ! same layout as struct shark_tune (launch shapes of the step kernels)
@@ -169,7 +171,8 @@ contains
     c%frac_r_disk = 0.0d0; c%buffer_in = 0.0d0; c%buffer_out = 0.0d0; c%alpha_visc = 0.0d0
     c%dust2gas = 0.0d0
     c%reserved = 0.0d0
-    c%accretion = 0; c%reserved_i2 = 0; c%m_acc = 0.0d0; c%phi_mom = 2.0d0; c%l_out = 1.0d0
-    c%reserved2 = 0.0d0
+    c%accretion = 0; c%gravity = 0; c%m_acc = 0.0d0; c%phi_mom = 2.0d0; c%l_out = 1.0d0
+    c%eos_baro = 0; c%reserved_i3 = 0; c%l_soft = 0.0d0; c%m_cent = 0.0d0; c%t0_cloud = 0.0d0
+    c%mu_gas = 0.0d0; c%unit_v = 0.0d0; c%unit_nh = 0.0d0; c%reserved3 = 0.0d0
   end subroutine shark_cfg_defaults
 end module shark_gpu_mod

@@ -19,7 +19,7 @@ _P = C.POINTER(C.c_double)
 
 # every entry point of include/shark_gpu.h
 EXPORTS = [
-    "shark_gpu_create", "shark_gpu_set_geometry", "shark_gpu_upload", "shark_gpu_step",
+    "shark_gpu_create", "shark_gpu_set_geometry", "shark_gpu_set_dvol", "shark_gpu_upload", "shark_gpu_step",
     "shark_gpu_run", "shark_gpu_download", "shark_gpu_get_time", "shark_gpu_set_time",
     "shark_gpu_kernel_time", "shark_gpu_set_timing", "shark_gpu_algorithmic_bytes",
     "shark_gpu_destroy", "shark_gpu_last_error", "shark_gpu_version",
@@ -178,6 +178,9 @@ class Step:
         check(self.L.shark_gpu_set_geometry(self.h, _ptr(g["dx"]), _ptr(g["surf"]),
                                             _ptr(g["vol"]), _ptr(g["radii_c"]),
                                             _ptr(g.get("position"))))
+        if g.get("dvol") is not None and cfg.gravity:
+            self.L.shark_gpu_set_dvol.argtypes = [C.c_void_p, _P]
+            check(self.L.shark_gpu_set_dvol(self.h, _ptr(g["dvol"])))
         self.upload(state)
 
     def upload(self, state):

@@ -228,6 +228,92 @@ def kh(nx=256, ny=256, ndust=1, cfl=0.5, slope_type=3, solver=C.SOLVER_HLLC, sol
 AU, MSUN = 1.5e13, 2e33   # src/commons/phys_const.f90:11,13
 
 
+AU, MSUN = 1.5e13, 2e33   # phys_const.f90:11, 13
+
+
+def barotrop(nH, t0_cloud=10.0, unit_nh=1.0):
+    """barotrop (src/setups/Collapse_1D/eos.f90:1-11); the first exponent is the default-real
+    literal 0.8 (single precision)."""
+    x = nH * unit_nh
+    return ((t0_cloud * math.sqrt(1.0 + math.pow(x / 1e11, float(np.float32(0.8))))) *
+            math.pow(1.0 + x / 1e16, -0.3)) * math.pow(1.0 + x / 1e21, 0.56667)
+
+
+def cs_eos(T, gamma, unit_v, mu_gas=2.31):
+    """cs_eos (eos.f90:13-20)."""
+    return math.sqrt(((gamma * KB) * T) / (mu_gas * MH)) / unit_v
+
+
+def grid_sphere_1d(nx, rmax, rin, unit_l):
+    """gridinit_sphere1D (src/grid/grid.f90:304-357): the faces radii(i) = inner_r zeta^(i-fa) of
+    the active cells (integer powers as flang lowers them), the first cell spanning [0, inner_r],
+    from rin (au) to rmax (cm).  Ghost cells as the reference leaves them: radii_c 0 in the inner
+    two and (0 + r_last)/2, 0 in the outer two; dx of the inner ghosts that of the first cell, 0
+    in the outer; vol, dvol and surf 0 (no flux through the centre or the outer edge).
+    dvol = (radii^3 - radii_c^3)/3 is the part of a cell between its centre and its outer face."""
+    nxm = nx + 4
+    fa, la = 3, nx + 2
+    zeta = math.pow(rmax / (rin * AU), 1.0 / (nx - 1))
+    inner = (rin * AU) / unit_l
+    radii = np.zeros(nxm)
+    for i in range(fa, la + 1):
+        radii[i - 1] = inner * _powi(zeta, i - fa)
+    dx = np.zeros(nxm)
+    for i in range(fa, la + 1):
+        dx[i - 1] = radii[i - 1] - radii[i - 2]
+    dx[:fa] = dx[fa - 1]
+    rc = np.zeros(nxm)
+    for i in range(fa, nxm + 1):
+        rc[i - 1] = (radii[i - 1] + radii[i - 2]) / 2.0
+    vol, dvol, surf = np.zeros(nxm), np.zeros(nxm), np.zeros(nxm)
+    for i in range(fa, la + 1):
+        vol[i - 1] = (_powi(radii[i - 1], 3) - _powi(radii[i - 2], 3)) / 3.0
+        dvol[i - 1] = (_powi(radii[i - 1], 3) - _powi(rc[i - 1], 3)) / 3.0
+        surf[i - 1] = radii[i - 2] * radii[i - 2]
+    return dict(dx=dx.reshape(1, nxm), surf=surf.reshape(1, nxm), vol=vol, radii_c=rc,
+                position=rc.reshape(1, nxm).copy(), dvol=dvol)
+
+
+def collapse(nx=1024, ndust=0, cfl=0.5, slope_type=1, solver=C.SOLVER_HLL, solverdust=0, gamma=1.66667,
+             m_cloud=1.0, alpha_cloud=0.25, t0_cloud=10.0, rin=0.1, mu_gas=2.31, rhograin=2.3,
+             dust2gas=0.01, smin=5e-7, smax=250e-7, scutmin=1e-7, scut=1.0, mrn=3.5, dust_ratio_min=1e-30):
+    """Collapse_1D (src/setups/Collapse_1D/setup.f90:1-60, units.f90, eos.f90): a uniform cloud of
+    M_cloud solar masses at rest on the spherical 1D grid, radius 2/5 G M/(kB T0) mu mH alpha_cloud,
+    barotropic EOS (P = rho cs_eos(barotrop(rho))^2, iso_cs < 1), 1D self-gravity, the collapse
+    boundaries; dust from the MRN distribution at dust2gas.  Defaults: the reference's test-suite
+    collapse runs (test/test-suite/collapse, dustycollapse)."""
+    u = units_kh(mu_gas)
+    unit_nh = u["unit_d"] / (mu_gas * MH)
+    mcloud = m_cloud * MSUN
+    r_cloud = 2.0 / 5.0 * GRAV * mcloud / (KB * t0_cloud) * mu_gas * MH * alpha_cloud
+    rho_cloud = mcloud / ((4.0 / 3.0 * PI) * _powi(r_cloud, 3)) / u["unit_d"]
+    g = grid_sphere_1d(nx, r_cloud, rin, u["unit_l"])
+    nc = nx + 4
+    nv = 5 + 4 * ndust
+    cs = cs_eos(barotrop(rho_cloud, t0_cloud, unit_nh), gamma, u["unit_v"], mu_gas)
+    q = np.zeros((nv, nc))
+    q[0] = rho_cloud
+    q[4] = rho_cloud * (cs * cs)
+    sdust = None
+    if ndust:
+        sizes, eps = mrn_sizes_and_ratios(ndust, u, smin=smin, smax=smax, scutmin=scutmin, scut=scut,
+                                          mrn=mrn, rhograin=rhograin, dust2gas=dust2gas,
+                                          dust_ratio_min=dust_ratio_min)
+        for k in range(ndust):
+            q[5 + k] = eps[k] * q[0]
+        sdust = np.repeat(np.asarray(sizes)[:, None], nc, axis=1)
+    cfg = C.make_cfg(nx=nx, ny=1, ndust=ndust, geom=C.GEOM_SPHERE, iso_cs=-1, cfl=cfl,
+                     slope_type=slope_type, solver=solver, solverdust=solverdust, gamma=gamma,
+                     smallr=1e-27 / u["unit_d"], smallc=1e-27 / u["unit_v"],
+                     smallp=(1e-27 / u["unit_d"]) * ((1e-27 / u["unit_v"]) * (1e-27 / u["unit_v"])),
+                     dust_ratio_min=dust_ratio_min, bc_kind=C.BC_COLLAPSE, tstop_kind=C.TSTOP_COLLAPSE,
+                     force_kind=C.FORCE_NONE, inloop_kind=C.INLOOP_NONE, gravity=1, eos_baro=1,
+                     l_soft=0.0, m_cent=0.0, t0_cloud=t0_cloud, mu_gas=mu_gas, unit_v=u["unit_v"],
+                     unit_nh=unit_nh, rhograin=rhograin / u["unit_d"])
+    return cfg, g, dict(u=primtoc(q, ndust, gamma), cs=np.zeros(nc), sdust=sdust, condinit=None,
+                        fargo_velocity=None)
+
+
 def units_cdisk():
     """The cartesian disk's units module (src/setups/disk/units.f90)."""
     unit_l = AU

@@ -400,14 +400,29 @@ FARGO_CASES = {
                            dict()),
 }
 
-# Runs of the reference that are not fixtures: its Collapse_1D step turns the first active cell into
-# NaN on the first step (gridinit_sphere1D, grid.f90:335-337, leaves radii_c = 0 in the inner
-# ghost cells, so the spherical predictor source -rho u / radii_c, godunov.f90:207, is 0/0 there and
-# the inner-face flux carries it in).  `python make_ref_fixtures.py --check-collapse` shows it.
-DEFECTIVE = {
-    "collapse_gas": ("Collapse_1D", 128, 1, 0, 1, 0, 1, 0, COLLAPSE.format(cfl=0.8, slope=1, dust=""), 1,
-                     "-DGRAVITY=1"),
+# Collapse_1D (built with GRAVITY=1): its step turns the first active cell into NaN on the first
+# step (gridinit_sphere1D, grid.f90:335-337, leaves radii_c = 0 in the inner ghost cells, so the
+# spherical predictor source -rho u / radii_c, godunov.f90:207, is 0/0 there and the inner-face
+# flux carries it in), and the NaN reaches every cell through the enclosed mass of the next step.
+# The one-step runs are fixtures for every other active cell and for dt.
+COLLAPSE_DUST = """drag = .true.
+dust_distribution = 'mrn'
+smin = 5d-7
+scutmin = 1d-7
+smax = 250d-7
+scut = 1.0d0
+rhograin = 2.3d0
+dust2gas = 0.01d0
+mrn = 3.5d0
+growth = .false.
+dust_ratio_min = 1d-30"""
+COLLAPSE_CASES = {
+    "collapse_gas_1": ("Collapse_1D", 128, 1, 0, 1, 0, 1, 0, COLLAPSE.format(cfl=0.8, slope=1, dust=""), 1,
+                       "-DGRAVITY=1"),
+    "collapse_dust_1": ("Collapse_1D", 96, 1, 4, 1, 0, 1, 0,
+                        COLLAPSE.format(cfl=0.5, slope=1, dust=COLLAPSE_DUST), 1, "-DGRAVITY=1"),
 }
+DEFECTIVE = {"collapse_gas": COLLAPSE_CASES["collapse_gas_1"]}
 
 
 def _meta(path):
@@ -526,10 +541,12 @@ def main(argv):
     if argv[1:] == ["--check-collapse"]:
         check_collapse()
         return
-    names = argv[1:] or list(CASES) + list(FARGO_CASES)
+    names = argv[1:] or list(CASES) + list(FARGO_CASES) + list(COLLAPSE_CASES)
     for n in names:
         if n in FARGO_CASES:
             m = run_case(n, FARGO_CASES[n][:10], fargo=FARGO_CASES[n][10])
+        elif n in COLLAPSE_CASES:
+            m = run_case(n, COLLAPSE_CASES[n])
         else:
             m = run_case(n, CASES[n])
         print(n, "steps", m["nsteps"], "ncells", m["ncells"], "nvar", m["nvar"])

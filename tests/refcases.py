@@ -11,7 +11,11 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 REF_CASES = sorted(f[4:-4] for f in os.listdir(GOLDEN)
                    if f.startswith("ref_") and f.endswith(".npz")
                    and not f.startswith(("ref_growth_", "ref_charge", "ref_output_", "ref_marchand",
-                                         "ref_fargo_")))
+                                         "ref_fargo_", "ref_collapse_")))
+# Collapse_1D fixtures (make_ref_fixtures.py COLLAPSE_CASES): one reference step, compared on the
+# active cells the reference leaves finite (tests/test_collapse_oracle.py)
+COLLAPSE_REF_CASES = sorted(f[4:-4] for f in os.listdir(GOLDEN)
+                            if f.startswith("ref_collapse_") and f.endswith(".npz"))
 # FARGO fixtures (make_ref_fixtures.py FARGO_CASES): integer shifts set before every step, or the
 # setup's own shift velocity for one step, compared where the reference's remap is defined
 FARGO_REF_CASES = sorted(f[4:-4] for f in os.listdir(GOLDEN)
@@ -68,8 +72,16 @@ def load_ref(name):
     elif setup == "KH":
         kw.update(bc_kind=C.BC_PERIODIC, tstop_kind=C.TSTOP_EPSTEIN, force_kind=C.FORCE_NONE,
                   inloop_kind=C.INLOOP_NONE, rhograin=m["rhograin"] / m["unit_d"])
+    elif setup == "Collapse_1D":
+        kw.update(bc_kind=C.BC_COLLAPSE, tstop_kind=C.TSTOP_COLLAPSE, force_kind=C.FORCE_NONE,
+                  inloop_kind=C.INLOOP_NONE, gravity=1, eos_baro=int(m["non_standard_eos"] == 1),
+                  l_soft=m["l_soft"] / m["unit_l"], m_cent=m["m_cent"], t0_cloud=m["t0_cloud"],
+                  mu_gas=m["mu_gas"], unit_v=m["unit_v"], unit_nh=m["unit_nh"],
+                  rhograin=m.get("rhograin", 0.0) / m["unit_d"])
     cfg = C.make_cfg(**kw)
     geom = dict(dx=z["dx"], surf=z["surf"], vol=z["vol"], radii_c=z["radii_c"], position=z["position"])
+    if "dvol" in z:
+        geom["dvol"] = z["dvol"]
     state = dict(u=z["ic"], cs=z["cs"], sdust=z["sdust"] if "sdust" in z else None,
                  condinit=z["condinit"] if "condinit" in z else None,
                  fargo_velocity=z["fargo_velocity"] if "fargo_velocity" in z else None)
