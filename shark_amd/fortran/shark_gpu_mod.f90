@@ -70,6 +70,11 @@ module shark_gpu_mod
        type(c_ptr), value :: ctx
        real(c_double), intent(in) :: dx(*), surf(*), vol(*), radii_c(*), position(*)
      end function shark_gpu_set_geometry
+     integer(c_int) function shark_gpu_set_dvol(ctx, dvol) bind(C, name="shark_gpu_set_dvol")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: dvol(*)
+     end function shark_gpu_set_dvol
      integer(c_int) function shark_gpu_upload(ctx, u_prim, cs, sdust, condinit, fargo_velocity) &
           bind(C, name="shark_gpu_upload")
        import :: c_int, c_ptr
