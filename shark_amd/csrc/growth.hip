@@ -204,6 +204,7 @@ struct GArgs {
   int* unfinished;
   int* bad;     // set when a rate or density is not finite
   const int* tile_order;  // launch order of the 64-cell tiles (slowest first), or null: blockIdx
+  const int* cells;       // [nch] cell of each chunk index (a list of cells), or null: c0 + index
   const float* Gf32;     // MFMA A operand, FP32 layout
   const u32x4* Gbhi;     // MFMA A operand, BF16 hi
   const u32x4* Gblo;     // MFMA A operand, BF16 lo
@@ -219,6 +220,11 @@ __device__ __forceinline__ double cell_dt(const Args& a, int64_t c) {
   return a.dt_dev ? *a.dt_dev : a.dt;
 }
 __device__ __forceinline__ double fmx(double a, double b) { return a > b ? a : b; }
+// cell of chunk index cc: from the chunk's cell list when it has one, else c0 + cc
+template <class Args>
+__device__ __forceinline__ int64_t chunk_cell(const Args& a, int cc) {
+  return a.cells ? (int64_t)a.cells[cc] : (int64_t)a.c0 + cc;
+}
 
 // Disk environment of one cell: T from the local isothermal sound speed, t_L = 1/Omega, the
 // Reynolds number of the midplane density (the Reynolds formula of dust_growth.f90:77), and the
@@ -226,7 +232,7 @@ __device__ __forceinline__ double fmx(double a, double b) { return a > b ? a : b
 __global__ void k_genv(GArgs a) {
   const int cc = blockIdx.x * blockDim.x + threadIdx.x;
   if (cc >= a.nch) return;
-  const int64_t c = a.c0 + cc;
+  const int64_t c = chunk_cell(a, cc);
   const double r = a.radius[c], sigma = a.rho_gas[c], cs = a.cs[c];
   const double H = a.hoverr * r;
   const double om = sqrt(a.mstar / (r * r * r));
@@ -290,7 +296,7 @@ __global__ void k_gpairs(GArgs a) {
   if (a.kernel_type == 0) {
     const double T = a.env[cc], t_L = a.env[a.nch + cc], t_eta = a.env[2 * a.nch + cc];
     const double Re = a.env[3 * a.nch + cc];
-    const double cs_T = a.cs[a.c0 + cc];
+    const double cs_T = a.cs[chunk_cell(a, cc)];
     const double dv = pair_dv(a, a.ts[(int64_t)P.i * a.nch + cc], a.ts[(int64_t)P.j * a.nch + cc],
                               m1, m2, T, cs_T, t_L, t_eta, Re);
     if (a.frag) {
@@ -421,6 +427,142 @@ __global__ __launch_bounds__(64) void k_gfp64(GArgs a) {
   }
   // the host needs only whether any cell is unfinished: one atomic per wave
   if (__any(valid && act) && lane == 0) atomicAdd(a.unfinished, 1);
+}
+
+// ------------------------------------------------------------------ mode 0, one wave per cell
+// The same pair loop with lane k holding bin k's density and rate (one cell per wave, four cells
+// per workgroup).  The loop is uniform across the wave, so the pair constants come through scalar
+// loads, the partner densities are read across lanes, and the cell's pair prefactors are loaded
+// 64 pairs at a time (one pair per lane) one block ahead.  Each lane applies, in the serial loop's
+// order, the updates whose target is its bin: every bin's rate is the same chain of FP64
+// operations as in k_gfp64, and the CFL minimum folds the bins in order, so the two kernels agree
+// bit for bit.  k_gfp64's one lane per cell waits on its serial chain of LDS updates every pair;
+// here a sub-step costs the pair loop's instructions once per cell, which suits few cells (small
+// chunks, the cells where the learned step falls back).  LDS: the redistribution table.
+__device__ __forceinline__ double rdlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__global__ __launch_bounds__(256) void k_gfp64w(GArgs a) {
+  extern __shared__ double rds[];   // redis [nb * nb]
+  const int nb = a.nb;
+  for (int t = threadIdx.x; t < nb * nb; t += 256) rds[t] = a.redis[t];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int cc = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (cc >= a.nch) return;   // wave-uniform, after the only barrier
+  const int64_t c = chunk_cell(a, cc);
+  const bool mine = lane < nb;
+  double* __restrict__ rh = a.rhod + c;
+  double r = mine ? rh[(int64_t)lane * a.ld] : 0.0;
+  const double rg = a.rho_gas[c];
+  double tgrow = a.tg[c];
+  int niter = a.nit[c];
+  const double dtl = cell_dt(a, c);
+  bool act = tgrow < dtl && !(a.max_sub > 0 && niter >= a.max_sub);
+  const int kt = a.kernel_type;
+  const double st = a.sticking;
+  const int np = a.npairs;
+  typedef const __attribute__((address_space(4))) PairC* PairPtr;
+  const PairPtr pairs = (PairPtr)a.pairs;
+  for (int s = 0; s < a.sub_per_launch && act; ++s) {
+    niter = niter + 1;
+    double dd = 0.0;
+    // prefactors of pairs p0 + lane, the next block loaded while this one is processed
+    double Ab = 0.0, Fb = 0.0;
+    {
+      const int q = lane;
+      if (kt == 0 && q < np) Ab = a.A[(int64_t)q * a.nch + cc];
+      if (a.frag && q < np) Fb = a.F[(int64_t)q * a.nch + cc];
+    }
+    for (int p0 = 0; p0 < np; p0 += 64) {
+      double An = 0.0, Fn = 0.0;
+      {
+        const int q = p0 + 64 + lane;
+        if (kt == 0 && q < np) An = a.A[(int64_t)q * a.nch + cc];
+        if (a.frag && q < np) Fn = a.F[(int64_t)q * a.nch + cc];
+      }
+      const int pe = min(np, p0 + 64);
+      for (int p = p0; p < pe; ++p) {
+        const int i = pairs[p].i, j = pairs[p].j, t1 = pairs[p].t1, t2 = pairs[p].t2;
+        const int nfrag = pairs[p].nfrag, rrow = pairs[p].rrow;
+        const double m1 = pairs[p].m1, m2 = pairs[p].m2, ym1 = pairs[p].ym1, ym2 = pairs[p].ym2;
+        const double eps = pairs[p].eps;
+        const double rr = (lane < nfrag) ? rds[rrow + lane] : 0.0;
+        const double ri = rdlane(r, i), rj = rdlane(r, j);
+        const double Ac = (kt == 0) ? rdlane(Ab, p - p0) : 0.0;
+        double f = a.frag ? rdlane(Fb, p - p0) : 0.0, pc = 1.0;
+        double dndt;
+        if (kt == 1) {
+          pc = 1.0; f = 0.0;
+          dndt = exdiv(exdiv(ri * rj, m1, ym1), m2, ym2);
+        } else if (kt == 2) {
+          pc = 1.0; f = 0.0;
+          dndt = exdiv(exdiv((m1 + m2) * ri * rj, m1, ym1), m2, ym2);
+        } else {
+          dndt = exdiv(exdiv(Ac * ri * rj, m1, ym1), m2, ym2);
+        }
+        if (i == j) dndt = dndt / 2.0;
+        if (kt == 0) {
+          if (a.eps_thr > 0.0 && (ri < a.eps_thr * rg)) pc = 0.0;
+          if (a.eps_thr > 0.0 && (rj < a.eps_thr * rg)) pc = 0.0;
+          if (a.eps_thr_frag > 0.0 && (ri < a.eps_thr_frag * rg)) f = 0.0;
+          if (a.eps_thr_frag > 0.0 && (rj < a.eps_thr_frag * rg)) f = 0.0;
+        }
+        if (lane == i) dd = dd - pc * (1.0 - f) * m1 * dndt * st - pc * f * m1 * dndt;
+        if (lane == j) dd = dd - pc * (1.0 - f) * m2 * dndt * st - pc * f * m2 * dndt;
+        if (f < 1.0) {
+          if (lane == t1) dd = dd + eps * pc * (1.0 - f) * (m1 + m2) * dndt * st;
+          if (lane == t2) dd = dd + (1. - eps) * pc * (1.0 - f) * (m1 + m2) * dndt * st;
+        }
+        if ((a.frag && f > 0.0) && pc > 0.) {
+          const double base = pc * f * (m1 + m2) * dndt;
+          if (lane < nfrag) dd = dd + base * rr;
+        }
+      }
+      Ab = An;
+      Fb = Fn;
+    }
+    // CFL-limited sub-step: the minimum over the bins with a non-zero rate, folded in bin order
+    const bool use = mine && fabs(dd) != 0.0;
+    const double v = use ? r / fabs(dd) * a.cfl : 0.0;
+    const unsigned long long m = __ballot(use);
+    double dtg = 1e32;
+    for (int k = 0; k < nb; ++k)
+      if ((m >> k) & 1ull) dtg = fmn(dtg, rdlane(v, k));
+    dtg = fmn(dtg, dtl - tgrow);
+    if (mine) r = fmx(r + dd * dtg, rg * a.ratio_min);
+    tgrow = tgrow + dtg;
+    if (!(tgrow == tgrow) || !(dtg > 0.0)) {  // NaN time or a sub-step that makes no progress
+      if (lane == 0) atomicOr(a.bad, 1);
+      act = false;
+    } else {
+      act = tgrow < dtl && !(a.max_sub > 0 && niter >= a.max_sub);
+    }
+  }
+  if (mine) rh[(int64_t)lane * a.ld] = r;
+  if (lane == 0) {
+    a.tg[c] = tgrow;
+    a.nit[c] = niter;
+    if (act) atomicAdd(a.unfinished, 1);
+  }
+}
+
+// Compacts the cells whose flag is set into a list (order unspecified; each cell's arithmetic
+// does not depend on it): one atomic per wave.
+__global__ __launch_bounds__(256) void k_flag_list(const int* __restrict__ flag, int64_t n, int* list, int* count) {
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool on = c < n && flag[c] != 0;
+  const unsigned long long m = __ballot(on);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, 0);
+  if (on) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int)c;
 }
 
 // ------------------------------------------------------------------ modes 1-3: MFMA contraction
@@ -721,6 +863,7 @@ struct QArgs {
   double* zmean;         // [ncells]
   double* xion;          // [ncells]
   const int* mask;       // [ncells] or null: only cells with mask != 0 are charged
+  const int* cells;      // [nch] cell of each chunk index (a list of cells), or null: c0 + index
 };
 
 constexpr double kEel = 4.8e-10, kMel = 9.109e-28;
@@ -752,7 +895,7 @@ __device__ double charge_fp(double y, double zeta, double rho, double m_d, doubl
 __global__ void k_gcharge(QArgs a) {
   const int cc = blockIdx.x * blockDim.x + threadIdx.x;
   if (cc >= a.nch) return;
-  const int64_t c = a.c0 + cc;
+  const int64_t c = chunk_cell(a, cc);
   if (a.mask && !a.mask[c]) return;
   const double T = a.env[cc];
   const double H = a.hoverr * a.radius[c];
@@ -1176,6 +1319,7 @@ struct shark_growth {
   shark_mlp* mlp2 = nullptr;   // residual network (optional)
   float *dnorm2 = nullptr, *dy2 = nullptr;
   int* dfb = nullptr;
+  int* dlist = nullptr;   // learned step: the cells that fell back, then their count
   double* dresid = nullptr;
   // bound to a step context (shark_gpu_attach_growth): rhod, gas density and cs are the step's
   // device arrays, dt is the step's device scalar, ghost cells are masked out
@@ -1185,6 +1329,9 @@ struct shark_growth {
   int* dmask = nullptr;
   double alpha = 0.0, zeta = 5e-17, mu_ions = 25.0, resid_max = 1e-6;
   float zmax = 8.0f;
+  // the plain FP64 step runs one wave per cell (k_gfp64w) for chunks of at most this many cells
+  // and for cell lists, else one lane per cell (k_gfp64)
+  int64_t wave_cells = 32768;
   int64_t last_nfb = 0;
   double last_resid = 0.0;
 };
@@ -1195,7 +1342,7 @@ void shark_growth_free(shark_growth* g) {
   void* ptrs[] = {g->dpairs, g->dpij, g->dredis, g->dmdust, g->dsdust, g->drhod, g->drho, g->dcs, g->drad,
                   g->denv, g->dts, g->dA, g->dF, g->dAh, g->dFh, g->dtg, g->dnit, g->dflags,
                   g->dGf32, g->dGbhi, g->dGblo, g->dzmean, g->dxion, g->ddtc, g->dnorm,
-                  g->dfeat, g->dy, g->dfb, g->dresid, g->dmask, g->dsT, g->dnorm2, g->dy2};
+                  g->dfeat, g->dy, g->dfb, g->dlist, g->dresid, g->dmask, g->dsT, g->dnorm2, g->dy2};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (g->hflags) (void)hipHostFree(g->hflags);
@@ -1515,6 +1662,21 @@ static int plain_chunk(shark_growth* g, GArgs& a, int mode) {
   const int nblk = (a.nch + 255) / 256;
   hipLaunchKernelGGL(k_gpairs, dim3(nblk, a.npairs), dim3(256), 0, s, a);
   GCHK(hipGetLastError());
+  if (mode == SHARK_GROWTH_FP64 && (a.cells || a.nch <= g->wave_cells)) {
+    // one wave per cell: every wave runs its cell to the end of the interval, so one launch
+    // normally finishes the chunk
+    a.tile_order = nullptr;
+    a.sub_per_launch = SUB_PER_LAUNCH;
+    for (;;) {
+      GCHK(hipMemsetAsync(g->dflags, 0, sizeof(int), s));
+      hipLaunchKernelGGL(k_gfp64w, dim3((a.nch + 3) / 4), dim3(256), sizeof(double) * a.nb * a.nb, s, a);
+      GCHK(hipGetLastError());
+      GCHK(hipMemcpyAsync(g->hflags, g->dflags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+      GCHK(hipStreamSynchronize(s));
+      if (g->hflags[1]) return gfail(SHARK_GROWTH_ENUM, "growth produced a non-finite density or a zero sub-step");
+      if (g->hflags[0] == 0) return 0;
+    }
+  }
   static_assert(NCT == 64, "both sub-step kernels take 64-cell tiles");
   const int ntile = (a.nch + 63) / 64;
   int grid = ntile;
@@ -1641,17 +1803,23 @@ static int surrogate_run(shark_growth* g, double dt) {
   GCHK(hipMemcpyAsync(&rs, g->dresid, sizeof(double), hipMemcpyDeviceToHost, s));
   GCHK(hipStreamSynchronize(s));
   int64_t nfb_total = 0;
-  QArgs qf = make_qargs(g);   // exact charge solve on the fallback cells (plain environment)
-  qf.mask = g->dfb;
-  for (int i = 0; i < nchunk; ++i) {
-    if (g->hcfb[i] == 0) continue;
-    nfb_total += g->hcfb[i];
-    const int64_t end = std::min<int64_t>(g->ncells, (int64_t)(i + 1) * B);
-    for (int64_t c0 = (int64_t)i * B; c0 < end; c0 += g->nch_max) {
-      const int nch = (int)std::min<int64_t>(g->nch_max, end - c0);
-      a.c0 = qf.c0 = (int)c0;
+  for (int i = 0; i < nchunk; ++i) nfb_total += g->hcfb[i];
+  if (nfb_total > 0) {
+    // the cells that fell back, as a list: the plain FP64 coagulation one wave per cell, then the
+    // exact charge solve (plain environment), over pieces of the list
+    int* cnt = g->dlist + g->ncells;
+    GCHK(hipMemsetAsync(cnt, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_flag_list, dim3((unsigned)((g->ncells + 255) / 256)), dim3(256), 0, s,
+                       (const int*)g->dfb, g->ncells, g->dlist, cnt);
+    GCHK(hipGetLastError());
+    QArgs qf = make_qargs(g);
+    for (int64_t o = 0; o < nfb_total; o += g->nch_max) {
+      const int nch = (int)std::min<int64_t>(g->nch_max, nfb_total - o);
+      a.c0 = qf.c0 = 0;
+      a.cells = qf.cells = g->dlist + o;
       a.nch = qf.nch = nch;
       a.env = g->denv;
+      qf.env = g->denv;
       const int nblk = (nch + 255) / 256;
       hipLaunchKernelGGL(k_genv, dim3(nblk), dim3(256), 0, s, a);
       GCHK(hipGetLastError());
@@ -1660,6 +1828,7 @@ static int surrogate_run(shark_growth* g, double dt) {
       hipLaunchKernelGGL(k_gcharge, dim3(nblk), dim3(256), 0, s, qf);
       GCHK(hipGetLastError());
     }
+    a.cells = nullptr;
   }
   g->last_nfb = nfb_total;
   const int64_t nacc = g->ncells - nfb_total;
@@ -1756,6 +1925,13 @@ int growth_nbins(const shark_growth* g) { return g->p.nbins; }
 
 extern "C" {
 
+int shark_growth_set_wave_cells(shark_growth* g, int64_t max_cells) {
+  if (!g) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_wave_cells: null handle");
+  if (max_cells < 0) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_wave_cells: max_cells must be >= 0");
+  g->wave_cells = max_cells;
+  return 0;
+}
+
 int shark_growth_set_cell_dt(shark_growth* g, const double* dt_cells) {
   if (!g) return gfail(SHARK_GROWTH_EINVAL, "shark_growth_set_cell_dt: null handle");
   if (!dt_cells) {
@@ -1806,6 +1982,7 @@ int shark_growth_set_surrogate(shark_growth* g, shark_mlp* m, const float* norm,
   GCHK(hipMalloc(&g->dfeat, sizeof(float) * B * (nb + 5)));
   GCHK(hipMalloc(&g->dy, sizeof(float) * B * (nb + 1)));
   if (!g->dfb) GCHK(hipMalloc(&g->dfb, sizeof(int) * g->ncells));
+  if (!g->dlist) GCHK(hipMalloc(&g->dlist, sizeof(int) * (g->ncells + 1)));
   if (!g->dresid) GCHK(hipMalloc(&g->dresid, sizeof(double)));
   GCHK(hipMemcpy(g->dnorm, norm, sizeof(float) * nn, hipMemcpyHostToDevice));
   int rc = ensure_charge_buffers(g);

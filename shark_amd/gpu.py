@@ -27,7 +27,7 @@ EXPORTS = [
     "shark_output_write", "shark_gpu_write_output", "shark_gpu_save_state", "shark_gpu_load_state",
     "shark_growth_create", "shark_growth_run", "shark_growth_download", "shark_growth_upload",
     "shark_growth_destroy", "shark_growth_kernel_time", "shark_growth_charge",
-    "shark_growth_set_cell_dt", "shark_growth_features", "shark_growth_set_surrogate",
+    "shark_growth_set_cell_dt", "shark_growth_set_wave_cells", "shark_growth_features", "shark_growth_set_surrogate",
     "shark_growth_surrogate_stats", "shark_gpu_attach_growth", "shark_charge_marchand",
     "shark_charge_marchand_noalloc", "shark_growth_zmean", "shark_growth_set_surrogate_residual",
     "shark_mlp_create", "shark_mlp_num_params", "shark_mlp_set_params", "shark_mlp_get_params",

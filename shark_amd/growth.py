@@ -174,6 +174,7 @@ def _lib():
         lib.shark_growth_kernel_time.argtypes = [C.c_void_p, _P]
         lib.shark_growth_charge.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _P, _P]
         lib.shark_growth_set_cell_dt.argtypes = [C.c_void_p, _P]
+        lib.shark_growth_set_wave_cells.argtypes = [C.c_void_p, C.c_int64]
         lib.shark_growth_features.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_float)]
         lib.shark_growth_set_surrogate.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_float),
                                                    C.c_double, C.c_double, C.c_double, C.c_double]
@@ -225,6 +226,11 @@ class Growth:
         assert d.size == self.ncells
         self._dtc = d
         gpu.check(self.L.shark_growth_set_cell_dt(self.h, d.ctypes.data_as(_P)))
+
+    def set_wave_cells(self, max_cells):
+        """Chunks of at most max_cells cells run mode 0 one wave per cell, larger ones one lane per
+        cell (shark_growth_set_wave_cells; 0: always one lane per cell)."""
+        gpu.check(self.L.shark_growth_set_wave_cells(self.h, int(max_cells)))
 
     def features(self, dt):
         f = np.empty((self.ncells, self.p.nbins + 5), dtype=np.float32)

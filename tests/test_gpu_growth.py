@@ -21,14 +21,20 @@ def cells(nb=64, nr=64, nphi=8, **kw):
     return p, t, rhod, sigma, cs, r
 
 
-def run_gpu(p, rhod, sigma, cs, r, mode, dt):
+def run_gpu(p, rhod, sigma, cs, r, mode, dt, wave_cells=None):
     g = G.Growth(p, sigma.size)
     try:
+        if wave_cells is not None:
+            g.set_wave_cells(wave_cells)
         g.upload(rhod, sigma, cs, r)
         g.run(mode, dt)
         return g.download()
     finally:
         g.close()
+
+
+# mode 0's two launch shapes: one lane per cell (k_gfp64) and one wave per cell (k_gfp64w)
+SHAPES = [pytest.param(0, id="lane-per-cell"), pytest.param(1 << 40, id="wave-per-cell")]
 
 
 def l1_mass(a, ref):
@@ -43,23 +49,26 @@ def l1_mass(a, ref):
     (16, dict(kernel_type=2, fragmentation=0), 1e-8),
     (24, dict(fragmentation=0, decomposed=0, eps_threshold=1e-3), 1e4),
 ])
-def test_fp64_bit_exact(nb, kw, dt):
+@pytest.mark.parametrize("wave_cells", SHAPES)
+def test_fp64_bit_exact(nb, kw, dt, wave_cells):
     p, t, rhod, sigma, cs, r = cells(nb=nb, **kw)
     ref, nref = pyoracle.growth_disk(p, t, sigma, cs, r, dt, rhod)
-    out, n = run_gpu(p, rhod, sigma, cs, r, G.FP64, dt)
-    print(f"nb={nb} {kw} dt={dt} substeps {nref.min()}..{nref.max()}")
+    out, n = run_gpu(p, rhod, sigma, cs, r, G.FP64, dt, wave_cells)
+    print(f"nb={nb} {kw} dt={dt} wave_cells={wave_cells} substeps {nref.min()}..{nref.max()}", flush=True)
     assert nref.max() >= 2
     assert np.array_equal(n, nref)
     assert np.array_equal(out, ref), np.abs(out / ref - 1).max()
 
 
-def test_fp64_chunked_relaunch_matches():
-    """More sub-steps than one launch runs (64): the relaunch loop resumes every cell."""
+@pytest.mark.parametrize("wave_cells", SHAPES)
+def test_fp64_chunked_relaunch_matches(wave_cells):
+    """Many sub-steps per cell (the first launch of the one-lane kernel runs one sub-step, the
+    relaunches resume every unfinished cell)."""
     p, t, rhod, sigma, cs, r = cells(nb=32, nr=16, nphi=4)
     dt = 4e-7
     ref, nref = pyoracle.growth_disk(p, t, sigma, cs, r, dt, rhod)
     assert nref.max() > 64
-    out, n = run_gpu(p, rhod, sigma, cs, r, G.FP64, dt)
+    out, n = run_gpu(p, rhod, sigma, cs, r, G.FP64, dt, wave_cells)
     assert np.array_equal(n, nref)
     assert np.array_equal(out, ref)
 
