@@ -25,9 +25,11 @@ polar disk with 64 mass bins (MRN initial distribution), one growth call per mod
 same cells and interval: the plain FP64 coagulation (reference order) and the FP32 / BF16
 MFMA contractions, each with its sub-step count, time per cell-sub-step, speedup over FP64,
 mass-weighted L1 error against it and the relative L1 error of every bin's density field
-(l1_per_bin).  With trained weights in shark_amd/data, the learned step is timed against the
-plain FP64 step on cells aged by the plain solver, at intervals in the disk's hydro-step range,
-with its L1 errors (per cell and per bin) and fallback count.
+(l1_per_bin); then the FP64 coagulation's two launch shapes (one lane per cell, one wave per
+cell) on 2,048 to 65,536 of the same cells, with their times and bit-equality.  With trained
+weights in shark_amd/data, the learned step is timed against the plain FP64 step on cells aged by
+the plain solver, at intervals in the disk's hydro-step range, with its L1 errors (per cell and
+per bin) and fallback count.  In the c5 workload every timed step's fallback count is summed.
 """
 import argparse
 import json
@@ -167,6 +169,25 @@ def growth_block(steps_probe_dt=2e-8, surrogate_cells=16384, seed=99):
     for v in modes.values():
         v["speedup_vs_fp64"] = base / v["ms"]
     out = {"workload": "disk_512x512_b64_mrn", "cells": n, "dt": steps_probe_dt, "modes": modes}
+
+    # mode 0's two launch shapes (one lane per cell, one wave per cell) on the first m cells of
+    # the same disk: their times, and that they give the same bits
+    shapes = []
+    for m in (2048, 16384, 65536):
+        res = {}
+        for wc in (0, 1 << 40):
+            g = G.Growth(p, m)
+            try:
+                g.set_wave_cells(wc)
+                g.upload(np.ascontiguousarray(rhod[:, :m]), sigma[:m].copy(), cs[:m].copy(), r[:m].copy())
+                g.run(G.FP64, steps_probe_dt)
+                res[wc] = g.download() + (g.kernel_ms(),)
+            finally:
+                g.close()
+        lane, wave = res[0], res[1 << 40]
+        shapes.append({"cells": m, "lane_per_cell_ms": lane[2], "wave_per_cell_ms": wave[2],
+                       "bit_identical": bool(np.array_equal(lane[0], wave[0]) and np.array_equal(lane[1], wave[1]))})
+    out["fp64_launch_shapes"] = shapes
 
     wpath = os.path.join(ROOT, "shark_amd", "data", "surrogate_disk64.npz")
     if os.path.exists(wpath):
@@ -331,8 +352,17 @@ def time_workload(key, steps, warmup, tune=True, events=False, attainable=None):
     # from a second, event-bracketed pass. A growth step inside the step is launched directly.
     use_graph = not learned and not events
     g.set_timing(not use_graph)
+    nfb_steps = []
     t0 = time.perf_counter()
-    n, _ = g.run(steps)
+    if learned:
+        # one step per call (each already synchronises the host once): the fallback count of
+        # every timed step is read between them
+        for _ in range(steps):
+            g.step()
+            nfb_steps.append(int(gr.surrogate_stats()[0]))
+        n = steps
+    else:
+        n, _ = g.run(steps)
     wall = time.perf_counter() - t0
     assert n == steps, (n, steps)
     if use_graph:
@@ -344,7 +374,10 @@ def time_workload(key, steps, warmup, tune=True, events=False, attainable=None):
     if learned:
         nfb, raw = gr.surrogate_stats()
         learned_info = {"growth_ms_per_step": kt["growth"], "fallback_cells_last_step": int(nfb),
-                        "fallback_fraction_last_step": nfb / cells, "raw_mass_residual_mean": raw,
+                        "fallback_fraction_last_step": nfb / cells,
+                        "fallback_cells_timed_steps": int(sum(nfb_steps)),
+                        "fallback_fraction_timed_steps": sum(nfb_steps) / (cells * steps),
+                        "steps_before_timing": warmup, "raw_mass_residual_mean": raw,
                         "networks": 2 if net.net2 is not None else 1}
 
     nv = 5 + 4 * cfg.ndust

@@ -120,9 +120,9 @@ int shark_growth_charge(shark_growth* g, double zeta, double mu_ions, int electr
 /* Per-cell growth intervals (ncells, >= 0) used instead of run's dt; NULL returns to dt. */
 int shark_growth_set_cell_dt(shark_growth* g, const double* dt_cells);
 /* Launch shape of mode 0 (the per-cell sub-step loop of dust_growth.f90:117-241): chunks of at
- * most max_cells cells, and the cells where the learned step falls back, run one wave per cell
- * (lane = bin); larger chunks run one lane per cell.  Both give the same bits.  Default 32768;
- * 0 puts every chunk on the one-lane-per-cell kernel (the fallback list stays one wave per cell). */
+ * most max_cells cells (the learned step's fallback cells are one chunk) run one wave per cell
+ * (lane = bin); larger chunks run one lane per cell.  Both give the same bits.  Default 24576;
+ * 0 puts every chunk on the one-lane-per-cell kernel. */
 int shark_growth_set_wave_cells(shark_growth* g, int64_t max_cells);
 /* Raw (unstandardised) network inputs of the current state, (ncells x (nbins+5)) row-major:
  * log10 midplane dust density per bin (g cm^-3), log10 T, log10 midplane gas density, log10 alpha,

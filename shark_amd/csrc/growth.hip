@@ -334,7 +334,7 @@ __global__ __launch_bounds__(64) void k_gfp64(GArgs a) {
   const int tile = a.tile_order ? a.tile_order[blockIdx.x] : (int)blockIdx.x;
   const int cc = tile * 64 + lane;
   const bool valid = cc < a.nch;
-  const int64_t c = a.c0 + (valid ? cc : 0);
+  const int64_t c = chunk_cell(a, valid ? cc : 0);
   const int nb = a.nb;
   double* __restrict__ rh = a.rhod + c;
   const double rg = valid ? a.rho_gas[c] : 1.0;
@@ -833,7 +833,7 @@ __global__ __launch_bounds__(64) void k_tile_est(GArgs a, float* est) {
   const int cc = blockIdx.x * 64 + threadIdx.x;
   float v = -1.f;
   if (cc < a.nch) {
-    const int64_t c = a.c0 + cc;
+    const int64_t c = chunk_cell(a, cc);
     const double tgrow = a.tg[c], dtl = cell_dt(a, c);
     const int niter = a.nit[c];
     if (tgrow < dtl && !(a.max_sub > 0 && niter >= a.max_sub))
@@ -1329,9 +1329,10 @@ struct shark_growth {
   int* dmask = nullptr;
   double alpha = 0.0, zeta = 5e-17, mu_ions = 25.0, resid_max = 1e-6;
   float zmax = 8.0f;
-  // the plain FP64 step runs one wave per cell (k_gfp64w) for chunks of at most this many cells
-  // and for cell lists, else one lane per cell (k_gfp64)
-  int64_t wave_cells = 32768;
+  // the plain FP64 step runs one wave per cell (k_gfp64w) for chunks (or cell lists) of at most
+  // this many cells, else one lane per cell (k_gfp64): the two take the same time near 24k cells
+  // of the 64-bin disk
+  int64_t wave_cells = 24576;
   int64_t last_nfb = 0;
   double last_resid = 0.0;
 };
@@ -1662,7 +1663,7 @@ static int plain_chunk(shark_growth* g, GArgs& a, int mode) {
   const int nblk = (a.nch + 255) / 256;
   hipLaunchKernelGGL(k_gpairs, dim3(nblk, a.npairs), dim3(256), 0, s, a);
   GCHK(hipGetLastError());
-  if (mode == SHARK_GROWTH_FP64 && (a.cells || a.nch <= g->wave_cells)) {
+  if (mode == SHARK_GROWTH_FP64 && a.nch <= g->wave_cells) {
     // one wave per cell: every wave runs its cell to the end of the interval, so one launch
     // normally finishes the chunk
     a.tile_order = nullptr;
@@ -1805,8 +1806,8 @@ static int surrogate_run(shark_growth* g, double dt) {
   int64_t nfb_total = 0;
   for (int i = 0; i < nchunk; ++i) nfb_total += g->hcfb[i];
   if (nfb_total > 0) {
-    // the cells that fell back, as a list: the plain FP64 coagulation one wave per cell, then the
-    // exact charge solve (plain environment), over pieces of the list
+    // the cells that fell back, as a list: the plain FP64 coagulation (one wave per cell unless
+    // the list is long), then the exact charge solve (plain environment), over pieces of the list
     int* cnt = g->dlist + g->ncells;
     GCHK(hipMemsetAsync(cnt, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_flag_list, dim3((unsigned)((g->ncells + 255) / 256)), dim3(256), 0, s,

@@ -21,12 +21,12 @@ DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
 
 def sample_states(params, tables_sdust, nr, nphi, seed, age_max=1e-2, dt_lo=1e-5, dt_hi=1e-3,
-                  gen_mode=G.MFMA_FP32, perturb=0.0, sigma=(1e-3, 0.05), sigma_small=(1e-2, 1.0),
+                  gen_mode=G.MFMA_FP32, perturb=0.0, scale=(1e-3, 0.05), scale_small=(1e-2, 1.0),
                   small_share=1e-5):
     """(X raw features, Y raw targets, state dict) for nr*nphi cells.  With perturb > 0 that
     fraction of the aged cells is moved off the coagulation's own states before the interval: each
-    bin's density times exp(s z) (z standard normal per bin; s per cell log-uniform in `sigma`, or
-    in `sigma_small` for bins holding less than small_share of the cell's dust mass), the cell's
+    bin's density times exp(s z) (z standard normal per bin; s per cell log-uniform in `scale`, or
+    in `scale_small` for bins holding less than small_share of the cell's dust mass), the cell's
     dust mass kept.  The targets of such cells are still the plain step's, so the network learns
     how the coagulation pulls a perturbed distribution back: the bins above the fragmentation
     barrier hold ~1e-8 of the mass in a fast gain-loss balance that a learned change of log-share
@@ -48,8 +48,8 @@ def sample_states(params, tables_sdust, nr, nphi, seed, age_max=1e-2, dt_lo=1e-5
         g.close()
     if perturb > 0.0:
         pick = rng.random(n) < perturb
-        sg = np.where(pick, 10.0 ** rng.uniform(np.log10(sigma[0]), np.log10(sigma[1]), n), 0.0)
-        ss = np.where(pick, 10.0 ** rng.uniform(np.log10(sigma_small[0]), np.log10(sigma_small[1]), n), 0.0)
+        sg = np.where(pick, 10.0 ** rng.uniform(np.log10(scale[0]), np.log10(scale[1]), n), 0.0)
+        ss = np.where(pick, 10.0 ** rng.uniform(np.log10(scale_small[0]), np.log10(scale_small[1]), n), 0.0)
         small = s1 < small_share * s1.sum(axis=0)[None, :]
         z = rng.standard_normal(s1.shape)
         s1p = s1 * np.exp(np.where(small, ss[None, :], sg[None, :]) * z)

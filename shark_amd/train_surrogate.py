@@ -99,7 +99,7 @@ def main(argv=None):
     for c in range(a.chunks):
         X, Y, st = S.sample_states(p, sd, a.nr, a.nphi, seed=a.seed + c, age_max=a.age_max,
                                    dt_lo=a.dt_lo, dt_hi=a.dt_hi, gen_mode=gen, perturb=a.perturb,
-                                   sigma=psig, sigma_small=psig_s)
+                                   scale=psig, scale_small=psig_s)
         Xs.append(X)
         Ys.append(Y)
         if a.weighted:
@@ -232,7 +232,7 @@ def main(argv=None):
                 raw_resid_heldout=dict(mean=float(raw.mean()), p99=float(np.quantile(raw, 0.99)),
                                        max=float(raw.max())),
                 init=os.path.basename(a.init) if a.init else None,
-                perturb=dict(fraction=a.perturb, sigma=psig, sigma_small=psig_s) if a.perturb else None,
+                perturb=dict(fraction=a.perturb, scale=psig, scale_small=psig_s) if a.perturb else None,
                 weight_floor=a.weight_floor,
                 trajectory=dict(steps=a.traj_steps, every=a.traj_every, cells=a.traj_cells,
                                 weights=os.path.basename(traj_w) if traj_w else "shipped") if a.traj_steps else None,
