@@ -121,7 +121,7 @@ int shark_growth_charge(shark_growth* g, double zeta, double mu_ions, int electr
 int shark_growth_set_cell_dt(shark_growth* g, const double* dt_cells);
 /* Launch shape of mode 0 (the per-cell sub-step loop of dust_growth.f90:117-241): chunks of at
  * most max_cells cells (the learned step's fallback cells are one chunk) run one wave per cell
- * (lane = bin); larger chunks run one lane per cell.  Both give the same bits.  Default 24576;
+ * (lane = bin); larger chunks run one lane per cell.  Both give the same bits.  Default 8192;
  * 0 puts every chunk on the one-lane-per-cell kernel. */
 int shark_growth_set_wave_cells(shark_growth* g, int64_t max_cells);
 /* Raw (unstandardised) network inputs of the current state, (ncells x (nbins+5)) row-major:

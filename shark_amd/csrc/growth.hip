@@ -1330,9 +1330,10 @@ struct shark_growth {
   double alpha = 0.0, zeta = 5e-17, mu_ions = 25.0, resid_max = 1e-6;
   float zmax = 8.0f;
   // the plain FP64 step runs one wave per cell (k_gfp64w) for chunks (or cell lists) of at most
-  // this many cells, else one lane per cell (k_gfp64): the two take the same time near 24k cells
-  // of the 64-bin disk
-  int64_t wave_cells = 24576;
+  // this many cells, else one lane per cell (k_gfp64).  On the 64-bin disk one wave per cell is 3x
+  // faster for 2,048 cells; for 16,384 cells it is faster at ~50 sub-steps per cell and slower at
+  // ~500 (the one-lane kernel's latency is then hidden by its many waves)
+  int64_t wave_cells = 8192;
   int64_t last_nfb = 0;
   double last_resid = 0.0;
 };

@@ -111,7 +111,7 @@ def test_fallback_contract(held_out, wave_cells):
     assert np.array_equal(zs[~fb], zfree[~fb])
     g.upload(st["rhod_next"], st["sigma"], st["cs"], st["r"])
     zc, _ = g.charge()
-    g.set_wave_cells(24576)
+    g.set_wave_cells(8192)
     assert np.array_equal(zs[fb], zc[fb])
 
 
