@@ -80,9 +80,59 @@ def test_learned_growth_in_step():
     mass = np.abs(rs.sum(0) / rf.sum(0) - 1)
     print(f"learned step in the step: {ts:.2f} ms vs FP64 {tf:.2f} ms; L1 mean {err.mean():.2e} "
           f"max {err.max():.2e}; fallback {nfb}; mass {mass.max():.1e}")
-    assert err.mean() < 5e-3
+    # measured 1.19e-4 (max 5.2e-4) on this unaged MRN state, the hardest start of the training
+    # range; tests/test_gpu_learned_step.py holds the held-out accuracy check
+    assert err.mean() < 1.5e-4
+    assert err.max() < 1e-3
     assert mass.max() < 1e-12
     assert tf > 10 * ts
+
+
+def test_learned_growth_rollout():
+    """60 coupled steps of a 1024 x 16 disk with 64 bins, the learned step against the plain step
+    (FP32 MFMA contraction, within 1e-8 of FP64) inside otherwise identical steps: no cell falls
+    back, every cell keeps its dust mass, and the dust distributions stay close to the plain
+    trajectory (the learned step's errors do not compound into drift)."""
+    cfg, geom, state, p = setups.disk_growth(nr=1024, nphi=16, nbins=64)
+    nc = (cfg.nx + 4) * (cfg.ny + 4)
+    nsteps = 60
+    L = S.Learned()
+    out = {}
+    try:
+        for mode in (G.MFMA_FP32, G.SURROGATE):
+            print(f"rollout: {nsteps} steps with growth mode {G.MODE_NAMES[mode]}", flush=True)
+            g = gpu.Step(cfg, geom, state)
+            gr = G.Growth(p, nc)
+            try:
+                if mode == G.SURROGATE:
+                    L.attach(gr)
+                g.attach_growth(gr, mode)
+                nfb = 0
+                for _ in range(nsteps):
+                    g.step()
+                    if mode == G.SURROGATE:
+                        nfb += gr.surrogate_stats()[0]
+                out[mode] = (g.download(), nfb)
+            finally:
+                g.close()
+                gr.close()
+    finally:
+        L.close()
+    (uf, _), (us, nfb) = out[G.MFMA_FP32], out[G.SURROGATE]
+    act = active(cfg)
+    nd = cfg.ndust
+    rf, rs = uf[5:5 + nd][:, act], us[5:5 + nd][:, act]
+    err = S.mass_l1(rs, rf)
+    mass = np.abs(rs.sum(0) / rf.sum(0) - 1)
+    print(f"after {nsteps} steps: L1 mean {err.mean():.2e} p99 {np.quantile(err, 0.99):.2e} max {err.max():.2e}; "
+          f"fallback cells {nfb}; dust-mass difference {mass.max():.1e}", flush=True)
+    # measured 3.1e-3 (p99 2.3e-2, max 2.9e-2): about 5e-5 per step, accumulating without growth
+    # of the rate (the earlier weights drifted, 4.8e-3 after 60 steps of a 1024 x 128 disk, with
+    # fallbacks from step 46 at full size)
+    assert nfb == 0
+    assert np.isfinite(us).all()
+    assert err.mean() < 5e-3
+    assert np.quantile(err, 0.99) < 4e-2
 
 
 def test_learned_growth_in_step_full_size():
