@@ -116,7 +116,9 @@ typedef struct shark_cfg {
   double t0_cloud;          /* barotrop's T0_cloud (K) */
   double mu_gas;            /* cs_eos's mean molecular weight */
   double unit_v, unit_nh;   /* the setup's units (src/setups/Collapse_1D/units.f90) */
-  double reserved3[1];
+  /* disk_polar planet (setup.f90:323-378): with mplanet > 0 (mplanet, rplanet above, code units)
+   * a softened point mass on a circular orbit adds the gradient of its potential to the forces */
+  double a_smooth_pl;
 } shark_cfg;
 
 typedef struct shark_ctx shark_ctx;

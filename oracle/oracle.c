@@ -43,6 +43,7 @@ struct oracle {
   double *u, *q, *qm, *qp, *flux, *dq, *cs, *force, *force_dust, *tstop, *sdust, *eps_dust;
   double *dx, *surf, *vol, *radii_c, *position, *condinit, *fargo_v, *S_U, *uold, *duprim;
   double *dvol, *mc;                 /* 1D self-gravity: inner half-cell volume, enclosed mass */
+  double *phi_sg;                    /* disk_polar planet: its potential per cell */
   unsigned char *active, *active_pred;
   double dt, time;
   int growth_on;                     /* dust_growth after the drag (solve.f90:112), disk env */
@@ -99,7 +100,7 @@ oracle* oracle_create(const shark_cfg* cfg) {
   o->radii_c = calloc(nc, 8); o->position = calloc(nc * o->ndim, 8);
   o->condinit = calloc(nc * nv, 8); o->fargo_v = calloc(nc, 8); o->S_U = calloc(nc * nv, 8);
   o->uold = calloc(nc * nv, 8); o->duprim = calloc(nc * nv, 8);
-  o->dvol = calloc(nc, 8); o->mc = calloc(nc, 8);
+  o->dvol = calloc(nc, 8); o->mc = calloc(nc, 8); o->phi_sg = calloc(nc, 8);
   o->active = calloc(nc, 1); o->active_pred = calloc(nc, 1);
   /* get_active_cells (src/commons/allocate_init.f90:339-381) */
   for (int iy = 1; iy <= o->nym; ++iy)
@@ -120,7 +121,8 @@ void oracle_destroy(oracle* o) {
   if (!o) return;
   double* ps[] = {o->u, o->q, o->qm, o->qp, o->flux, o->dq, o->cs, o->force, o->force_dust,
                   o->tstop, o->sdust, o->eps_dust, o->dx, o->surf, o->vol, o->radii_c,
-                  o->position, o->condinit, o->fargo_v, o->S_U, o->uold, o->duprim, o->dvol, o->mc};
+                  o->position, o->condinit, o->fargo_v, o->S_U, o->uold, o->duprim, o->dvol, o->mc,
+                  o->phi_sg};
   for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); ++i) free(ps[i]);
   free(o->active); free(o->active_pred); free(o);
 }
@@ -446,9 +448,40 @@ static void cdisk_r(const oracle* o, int c, double* xx, double* yy, double* rr) 
   *rr = sqrt((*xx * *xx + *yy * *yy) + p->smooth_r * p->smooth_r);
 }
 
+/* disk_polar planet (setup.f90:323-340): the softened potential of a planet on a circular orbit
+ * in every active cell, at the step's time; x = radii_c cos(phi), y = radii_c sin(phi) are the
+ * grid's position (grid.f90 evaluates the same products).  Then apply_boundaries_phi
+ * (boundaries.f90:98-140): y-ghost rows of the active columns are periodic copies, and every
+ * x-ghost cell takes phi_sg(first_active) / phi_sg(last_active), the reference using those column
+ * numbers as cell numbers: cells (3,1) and (nx+2,1) of the first ghost row, which the y copy set
+ * to rows ny+1 of columns 3 and nx+2. */
+static void planet_potential(oracle* o) {
+  const shark_cfg* p = &o->cfg;
+  const double theta_p = 2.0 * PI * fmod(o->time / (2.0 * PI / sqrt(p->mstar / cube_i(p->rplanet))), 1.0);
+  const double cx = cos(theta_p) * p->rplanet, cy = sin(theta_p) * p->rplanet;
+  for (int c = 0; c < o->nc; ++c) {
+    if (!o->active[c]) continue;
+    const double ex = POS(c, 0) - cx, ey = POS(c, 1) - cy;
+    const double rr_p = sqrt((ex * ex + ey * ey) + p->a_smooth_pl * p->a_smooth_pl);
+    o->phi_sg[c] = -(p->mplanet / rr_p);
+  }
+  for (int ix = o->fa; ix <= o->la; ++ix)
+    for (int g = 1; g <= 2; ++g) {
+      o->phi_sg[cidx(o, ix, g)] = o->phi_sg[cidx(o, ix, o->lay - 2 + g)];
+      o->phi_sg[cidx(o, ix, o->nym + 1 - g)] = o->phi_sg[cidx(o, ix, o->fay + 2 - g)];
+    }
+  for (int g = 1; g <= 2; ++g)
+    for (int iy = 1; iy <= o->nym; ++iy) {
+      o->phi_sg[cidx(o, g, iy)] = o->phi_sg[o->fa - 1];
+      o->phi_sg[cidx(o, o->nxm + 1 - g, iy)] = o->phi_sg[o->la - 1];
+    }
+}
+
 /* update_force_setup: SI src/setups/SI/setup.f90:335-368; disk :360-403 */
 static void update_force(oracle* o) {
   const shark_cfg* p = &o->cfg;
+  const int planet = p->force_kind == SHARK_FORCE_DISK && p->mplanet > 0.0;
+  if (planet) planet_potential(o);
   #pragma omp parallel for schedule(static)
   for (int c = 0; c < o->nc; ++c) {
     if (!o->active[c]) continue;
@@ -489,6 +522,20 @@ static void update_force(oracle* o) {
       for (int k = 0; k < o->nd; ++k) {
         FDUST(c, 0, k) = -(p->mstar / (r * r));
         FDUST(c, 1, k) = 0.0; FDUST(c, 2, k) = 0.0;
+      }
+      if (planet) {
+        /* centred gradient of the potential (setup.f90:343-356), subtracted from every fluid's
+         * force (:369-382) */
+        const int ix = ixx(o, c), iy = iyy(o, c);
+        const int xm = cidx(o, ix - 1, iy), xp = cidx(o, ix + 1, iy), ym = cidx(o, ix, iy - 1), yp = cidx(o, ix, iy + 1);
+        const double g1 = (o->phi_sg[xp] - o->phi_sg[xm]) / ((HALF * DX(xm, 0) + DX(c, 0)) + HALF * DX(xp, 0));
+        const double g2 = ((o->phi_sg[yp] - o->phi_sg[ym]) / ((HALF * DX(ym, 1) + DX(c, 1)) + HALF * DX(yp, 1))) / r;
+        FORCE(c, 0) = FORCE(c, 0) - g1;
+        FORCE(c, 1) = FORCE(c, 1) - g2;
+        for (int k = 0; k < o->nd; ++k) {
+          FDUST(c, 0, k) = FDUST(c, 0, k) - g1;
+          FDUST(c, 1, k) = FDUST(c, 1, k) - g2;
+        }
       }
     }
     if (!p->drag && p->force_kind != SHARK_FORCE_NONE) {

@@ -81,6 +81,7 @@ struct shark_ctx {
   double* dfcs = nullptr;           // gas face sound speeds for the HLLC dust solver (adiabatic gas)
   double* ddvol = nullptr;          // spherical grid: the cells' inner volumes (shark_gpu_set_dvol)
   double* dmc = nullptr;            // 1D self-gravity: enclosed mass per cell
+  double* dposxy = nullptr;         // polar-disk planet: the grid's position (2, ncells)
   int growth_mode = 0;
 };
 
@@ -143,8 +144,10 @@ static int validate(const shark_cfg* c) {
       c->force_kind < 0 || c->force_kind > 3 || c->inloop_kind < 0 || c->inloop_kind > 3)
     return fail(SHARK_EINVAL, "unknown setup hook kind");
   if (c->bc_kind == SHARK_BC_DISK && c->ny < 4) return fail(SHARK_EUNSUP, "disk boundaries need a 2D grid");
-  if (c->force_kind == SHARK_FORCE_DISK && c->mplanet > 0.0)
-    return fail(SHARK_EUNSUP, "the polar disk's planet (disk_polar/setup.f90:323-340) is not built");
+  if (c->force_kind == SHARK_FORCE_DISK && c->mplanet > 0.0 && (c->geom != SHARK_GEOM_POLAR || c->ny < 4))
+    return fail(SHARK_EUNSUP, "the polar disk's planet is built for the 2D polar grid");
+  if (c->force_kind == SHARK_FORCE_DISK && c->mplanet > 0.0 && !(c->rplanet > 0.0 && c->a_smooth_pl >= 0.0))
+    return fail(SHARK_EINVAL, "the polar disk's planet needs rplanet > 0 and a_smooth_pl >= 0");
   if (c->accretion && (c->bc_kind != SHARK_BC_DISK || c->iso_cs != 1))
     return fail(SHARK_EUNSUP, "accretion is built for the isothermal polar disk's boundaries");
   if (c->cfl <= 0) return fail(SHARK_EINVAL, "CFL must be positive");
@@ -238,6 +241,16 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   }
   a.mc = c->dmc;
   a.dvol = c->ddvol;
+  if (cfg->force_kind == SHARK_FORCE_DISK && cfg->mplanet > 0.0) {
+    if ((e = hipMalloc(&c->dposxy, (size_t)2 * a.nc * 8)) != hipSuccess) {
+      rc = fail(e == hipErrorOutOfMemory ? SHARK_ENOMEM : SHARK_EHIP, "allocation failed: %s", hipGetErrorString(e));
+      shark_gpu_destroy(c);
+      return rc;
+    }
+    (void)hipMemsetAsync(c->dposxy, 0, (size_t)2 * a.nc * 8, c->stream);
+  }
+  a.posxy = c->dposxy;
+  a.a_smooth_pl = cfg->a_smooth_pl;
   (void)hipMemsetAsync(c->dU, 0, nbytes, c->stream);
   (void)hipMemsetAsync(c->dB, 0, nbytes, c->stream);
   memset(c->hsc, 0, sizeof(DevScalars));
@@ -296,7 +309,9 @@ extern "C" int shark_gpu_set_geometry(shark_ctx* c, const double* dx, const doub
                         c->cfg.force_kind == SHARK_FORCE_CDISK || c->cfg.inloop_kind == SHARK_INLOOP_CDISK ||
                         c->cfg.tstop_kind == SHARK_TSTOP_EPSTEIN_CDISK ||
                         (c->cfg.viscosity && c->cfg.geom == SHARK_GEOM_CARTESIAN);
-  if (!position && need_pos) return fail(SHARK_EINVAL, "position is required by this setup's hooks");
+  if (!position && (need_pos || c->dposxy)) return fail(SHARK_EINVAL, "position is required by this setup's hooks");
+  if (c->dposxy)
+    HIPCHK(hipMemcpyAsync(c->dposxy, position, (size_t)2 * nc * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->dgeom, col.data(), col.size() * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->hrc.assign(radii_c, radii_c + nc);
@@ -880,6 +895,7 @@ extern "C" int shark_gpu_destroy(shark_ctx* c) {
   if (c->dRold) (void)hipFree(c->dRold);
   if (c->dfcs) (void)hipFree(c->dfcs);
   if (c->dmc) (void)hipFree(c->dmc);
+  if (c->dposxy) (void)hipFree(c->dposxy);
   if (c->ddvol) (void)hipFree(c->ddvol);
   if (c->dsdust) (void)hipFree(c->dsdust);
   if (c->dcond) (void)hipFree(c->dcond);

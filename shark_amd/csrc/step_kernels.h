@@ -15,8 +15,9 @@ struct DevScalars {
   long long nstep;      // completed steps
   int active;           // 0 once time >= tend: every kernel returns immediately
   int pad;
-  // cartesian disk planet (disk/setup.f90:292-301), Mplanet x_p / r_p^3 and Mplanet y_p / r_p^3
-  // at the step's time [0..1] and at time + dt [2..3] (the next step's Courant candidate)
+  // the planet at the step's time [0..1] and at time + dt [2..3] (the next step's Courant
+  // candidate): cartesian disk (disk/setup.f90:292-301) Mplanet x_p / r_p^3 and Mplanet y_p / r_p^3,
+  // polar disk (disk_polar/setup.f90:331-334) cos(theta_p) Rplanet and sin(theta_p) Rplanet
   double planet[4];
 };
 
@@ -78,6 +79,10 @@ struct StepArgs {
   double l_soft, m_cent, t0_cloud, mu_gas, unit_v, unit_nh;
   const double* dvol;   // (ncells) part of each cell between its centre and its outer face
   double* mc;           // (ncells) mass enclosed within each cell centre, this step (k_mtot)
+  // polar-disk planet (disk_polar/setup.f90:323-378): softening length, and the grid's position
+  // (2, ncells) for its potential; nullptr without a planet
+  double a_smooth_pl;
+  const double* posxy;
 };
 
 // launch shapes: every shape gives the same results bit for bit, only the speed differs

@@ -62,6 +62,29 @@ DEV void cell_force(const StepArgs& a, int ix, int iy, int next, double qx, doub
   } else if (a.force_kind == 2) {
     const double r = a.g.rc[ix - 1];
     f1 = -(a.mstar / (r * r));
+    if (a.mplanet > 0.0) {
+      // planet (disk_polar/setup.f90:323-378): centred gradient of its softened potential.  The
+      // potential's ghost values follow apply_boundaries_phi (boundaries.f90:98-140): y-ghost rows
+      // of active columns are periodic copies; every x-ghost cell takes the potential of cell
+      // number first_active / last_active, i.e. cells (3,1) and (nx+2,1) of the first ghost row,
+      // which hold rows ny+1 of columns 3 and nx+2
+      const double cx = a.sc->planet[2 * next], cy = a.sc->planet[2 * next + 1];
+      const double sm2 = a.a_smooth_pl * a.a_smooth_pl;
+      auto pot = [&](int jx, int jy) {
+        if (jx < 3) { jx = 3; jy = a.ny + 1; }
+        else if (jx > a.nx + 2) { jx = a.nx + 2; jy = a.ny + 1; }
+        else if (jy < 3) jy += a.ny;
+        else if (jy > a.ny + 2) jy -= a.ny;
+        const size_t c = (size_t)(jx - 1) + (size_t)(jy - 1) * a.nxm;
+        const double ex = a.posxy[c] - cx, ey = a.posxy[a.nc + c] - cy;
+        return -(a.mplanet / sqrt((ex * ex + ey * ey) + sm2));
+      };
+      const double d1 = a.g.dx1[ix - 1], d2 = a.g.dx2[ix - 1];
+      const double g1 = (pot(ix + 1, iy) - pot(ix - 1, iy)) / ((kHalf * a.g.dx1[ix - 2] + d1) + kHalf * a.g.dx1[ix]);
+      const double g2 = ((pot(ix, iy + 1) - pot(ix, iy - 1)) / ((kHalf * d2 + d2) + kHalf * d2)) / r;
+      f1 = f1 - g1;
+      f2 = f2 - g2;
+    }
   }
 }
 
@@ -1272,9 +1295,9 @@ __global__ __launch_bounds__(1024) void k_bc_zg(StepArgs a) {
   }
 }
 
-// The planet of the cartesian disk at the step's time and at time + dt (the time loop's
-// time = time + dt, shark.f90:107), once per step: theta_p = 2 pi mod(t / T_p, 1)
-// (disk/setup.f90:292-301).
+// The planet at the step's time and at time + dt (the time loop's time = time + dt,
+// shark.f90:107), once per step: theta_p = 2 pi mod(t / T_p, 1) (cartesian disk
+// disk/setup.f90:292-301, polar disk disk_polar/setup.f90:331).
 __global__ void k_planet(StepArgs a) {
   DevScalars* sc = a.sc;
   const double period = 2.0 * kPi / sqrt(a.mstar / (a.rplanet * (a.rplanet * a.rplanet)));
@@ -1282,6 +1305,11 @@ __global__ void k_planet(StepArgs a) {
     const double t = w == 0 ? sc->time : sc->time + sc->dt;
     const double theta_p = 2.0 * kPi * fmod(t / period, 1.0);
     const double xx_p = cos(theta_p) * a.rplanet, yy_p = sin(theta_p) * a.rplanet;
+    if (a.force_kind == 2) {   // polar disk: the planet's position (its potential per cell)
+      sc->planet[2 * w] = xx_p;
+      sc->planet[2 * w + 1] = yy_p;
+      continue;
+    }
     const double rr_p = sqrt((xx_p * xx_p + yy_p * yy_p) + a.smooth_r * a.smooth_r);
     const double rp3 = rr_p * (rr_p * rr_p);
     sc->planet[2 * w] = a.mplanet * xx_p / rp3;
@@ -1290,7 +1318,8 @@ __global__ void k_planet(StepArgs a) {
 }
 
 static void launch_planet(const StepArgs& a, hipStream_t s) {
-  if (a.force_kind == 3 && a.mplanet > 0.0) hipLaunchKernelGGL(k_planet, dim3(1), dim3(1), 0, s, a);
+  if ((a.force_kind == 3 || a.force_kind == 2) && a.mplanet > 0.0)
+    hipLaunchKernelGGL(k_planet, dim3(1), dim3(1), 0, s, a);
 }
 
 // Accretion through the outer boundary of the polar disk (disk_polar/boundaries.f90:63-94),

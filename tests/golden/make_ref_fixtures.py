@@ -114,6 +114,11 @@ charging=.false.
 /
 """
 
+# disk_polar with a planet on a circular orbit inside the disk (setup.f90:323-378)
+DISK_PLANET = DISK.replace("r_relax      = {r_relax}\n",
+                           "r_relax      = {r_relax}\nMplanet      = {mplanet}\nRplanet      = 1.3d0\n"
+                           "a_smooth_pl  = 0.05d0\n")
+
 SOD = """&GRID_PARAMS
 freq_out=100000
 CFL = {cfl}
@@ -351,6 +356,10 @@ CASES = {
                     DISK.format(slope=2, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 20),
     # 100 steps: the shearing box, the polar disk and the PSI namelist's 8 species
     "si_2d_100": ("SI", 32, 32, 2, 0, 0, 1, 0, SI.format(slope=1), 100),
+    "disk_2d_planet": ("disk_polar", 28, 40, 2, 2, 0, 1, 0,
+                       DISK_PLANET.format(slope=1, r_relax="0.3d0", visc=".false.", alpha="1d-2", mplanet="3d-3"), 40),
+    "disk_2d_planet_gas": ("disk_polar", 28, 40, 0, 2, 0, 2, 0,
+                           DISK_PLANET.format(slope=3, r_relax="0.3d0", visc=".false.", alpha="1d-2", mplanet="1d-2"), 40),
     "disk_2d_100": ("disk_polar", 32, 48, 4, 2, 0, 1, 0,
                     DISK.format(slope=1, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 100),
     "test_psi_100": ("SI", 32, 32, 8, 0, 0, 1, 0, test_nml("PSI/psi.nml", freq_out="100000", tend="1d10"), 100),
