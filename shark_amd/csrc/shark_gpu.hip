@@ -81,7 +81,8 @@ struct shark_ctx {
   double* dfcs = nullptr;           // gas face sound speeds for the HLLC dust solver (adiabatic gas)
   double* ddvol = nullptr;          // spherical grid: the cells' inner volumes (shark_gpu_set_dvol)
   double* dmc = nullptr;            // 1D self-gravity: enclosed mass per cell
-  double* dposxy = nullptr;         // polar-disk planet: the grid's position (2, ncells)
+  double* dposxy = nullptr;         // polar-disk planet: the grid's position (2, ncells), then the
+                                    // gradient of its potential (4, ncells)
   int growth_mode = 0;
 };
 
@@ -242,14 +243,15 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   a.mc = c->dmc;
   a.dvol = c->ddvol;
   if (cfg->force_kind == SHARK_FORCE_DISK && cfg->mplanet > 0.0) {
-    if ((e = hipMalloc(&c->dposxy, (size_t)2 * a.nc * 8)) != hipSuccess) {
+    if ((e = hipMalloc(&c->dposxy, (size_t)6 * a.nc * 8)) != hipSuccess) {
       rc = fail(e == hipErrorOutOfMemory ? SHARK_ENOMEM : SHARK_EHIP, "allocation failed: %s", hipGetErrorString(e));
       shark_gpu_destroy(c);
       return rc;
     }
-    (void)hipMemsetAsync(c->dposxy, 0, (size_t)2 * a.nc * 8, c->stream);
+    (void)hipMemsetAsync(c->dposxy, 0, (size_t)6 * a.nc * 8, c->stream);
   }
   a.posxy = c->dposxy;
+  a.pgrad = c->dposxy ? c->dposxy + (size_t)2 * a.nc : nullptr;
   a.a_smooth_pl = cfg->a_smooth_pl;
   (void)hipMemsetAsync(c->dU, 0, nbytes, c->stream);
   (void)hipMemsetAsync(c->dB, 0, nbytes, c->stream);

@@ -79,10 +79,12 @@ struct StepArgs {
   double l_soft, m_cent, t0_cloud, mu_gas, unit_v, unit_nh;
   const double* dvol;   // (ncells) part of each cell between its centre and its outer face
   double* mc;           // (ncells) mass enclosed within each cell centre, this step (k_mtot)
-  // polar-disk planet (disk_polar/setup.f90:323-378): softening length, and the grid's position
-  // (2, ncells) for its potential; nullptr without a planet
+  // polar-disk planet (disk_polar/setup.f90:323-378): softening length, the grid's position
+  // (2, ncells) for its potential, and the gradient of the potential at the step's time and at
+  // time + dt (4, ncells), written by k_planet_grad; nullptr without a planet
   double a_smooth_pl;
   const double* posxy;
+  double* pgrad;
 };
 
 // launch shapes: every shape gives the same results bit for bit, only the speed differs

@@ -62,28 +62,12 @@ DEV void cell_force(const StepArgs& a, int ix, int iy, int next, double qx, doub
   } else if (a.force_kind == 2) {
     const double r = a.g.rc[ix - 1];
     f1 = -(a.mstar / (r * r));
-    if (a.mplanet > 0.0) {
-      // planet (disk_polar/setup.f90:323-378): centred gradient of its softened potential.  The
-      // potential's ghost values follow apply_boundaries_phi (boundaries.f90:98-140): y-ghost rows
-      // of active columns are periodic copies; every x-ghost cell takes the potential of cell
-      // number first_active / last_active, i.e. cells (3,1) and (nx+2,1) of the first ghost row,
-      // which hold rows ny+1 of columns 3 and nx+2
-      const double cx = a.sc->planet[2 * next], cy = a.sc->planet[2 * next + 1];
-      const double sm2 = a.a_smooth_pl * a.a_smooth_pl;
-      auto pot = [&](int jx, int jy) {
-        if (jx < 3) { jx = 3; jy = a.ny + 1; }
-        else if (jx > a.nx + 2) { jx = a.nx + 2; jy = a.ny + 1; }
-        else if (jy < 3) jy += a.ny;
-        else if (jy > a.ny + 2) jy -= a.ny;
-        const size_t c = (size_t)(jx - 1) + (size_t)(jy - 1) * a.nxm;
-        const double ex = a.posxy[c] - cx, ey = a.posxy[a.nc + c] - cy;
-        return -(a.mplanet / sqrt((ex * ex + ey * ey) + sm2));
-      };
-      const double d1 = a.g.dx1[ix - 1], d2 = a.g.dx2[ix - 1];
-      const double g1 = (pot(ix + 1, iy) - pot(ix - 1, iy)) / ((kHalf * a.g.dx1[ix - 2] + d1) + kHalf * a.g.dx1[ix]);
-      const double g2 = ((pot(ix, iy + 1) - pot(ix, iy - 1)) / ((kHalf * d2 + d2) + kHalf * d2)) / r;
-      f1 = f1 - g1;
-      f2 = f2 - g2;
+    if (a.pgrad) {
+      // planet (disk_polar/setup.f90:323-378): the gradient of its potential, per step from
+      // k_planet_grad
+      const size_t cell = (size_t)(ix - 1) + (size_t)(iy - 1) * a.nxm;
+      f1 = f1 - a.pgrad[(size_t)(2 * next) * a.nc + cell];
+      f2 = f2 - a.pgrad[(size_t)(2 * next + 1) * a.nc + cell];
     }
   }
 }
@@ -1317,9 +1301,45 @@ __global__ void k_planet(StepArgs a) {
   }
 }
 
+// The polar disk's planet (disk_polar/setup.f90:323-358) at the step's time and at time + dt:
+// its softened potential in the four neighbours of every active cell, from the grid's position
+// and the planet's position (k_planet), and the centred gradient of it, written to
+// pgrad[(2 w + component) * nc + cell].  The potential's ghost values follow
+// apply_boundaries_phi (boundaries.f90:98-140): y-ghost rows of active columns are periodic
+// copies; every x-ghost cell takes the potential of cell number first_active / last_active, i.e.
+// cells (3,1) and (nx+2,1) of the first ghost row, which hold rows ny+1 of columns 3 and nx+2.
+__global__ void k_planet_grad(StepArgs a) {
+  if (!a.sc->active) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nx * a.ny) return;
+  const int ix = 3 + t % a.nx, iy = 3 + t / a.nx;
+  const size_t cell = (size_t)(ix - 1) + (size_t)(iy - 1) * a.nxm;
+  const double r = a.g.rc[ix - 1];
+  const double sm2 = a.a_smooth_pl * a.a_smooth_pl;
+  for (int w = 0; w < 2; ++w) {
+    const double cx = a.sc->planet[2 * w], cy = a.sc->planet[2 * w + 1];
+    auto pot = [&](int jx, int jy) {
+      if (jx < 3) { jx = 3; jy = a.ny + 1; }
+      else if (jx > a.nx + 2) { jx = a.nx + 2; jy = a.ny + 1; }
+      else if (jy < 3) jy += a.ny;
+      else if (jy > a.ny + 2) jy -= a.ny;
+      const size_t c = (size_t)(jx - 1) + (size_t)(jy - 1) * a.nxm;
+      const double ex = a.posxy[c] - cx, ey = a.posxy[a.nc + c] - cy;
+      return -(a.mplanet / sqrt((ex * ex + ey * ey) + sm2));
+    };
+    const double d1 = a.g.dx1[ix - 1], d2 = a.g.dx2[ix - 1];
+    a.pgrad[(size_t)(2 * w) * a.nc + cell] =
+        (pot(ix + 1, iy) - pot(ix - 1, iy)) / ((kHalf * a.g.dx1[ix - 2] + d1) + kHalf * a.g.dx1[ix]);
+    a.pgrad[(size_t)(2 * w + 1) * a.nc + cell] =
+        ((pot(ix, iy + 1) - pot(ix, iy - 1)) / ((kHalf * d2 + d2) + kHalf * d2)) / r;
+  }
+}
+
 static void launch_planet(const StepArgs& a, hipStream_t s) {
   if ((a.force_kind == 3 || a.force_kind == 2) && a.mplanet > 0.0)
     hipLaunchKernelGGL(k_planet, dim3(1), dim3(1), 0, s, a);
+  if (a.pgrad)
+    hipLaunchKernelGGL(k_planet_grad, dim3((a.nx * a.ny + 255) / 256), dim3(256), 0, s, a);
 }
 
 // Accretion through the outer boundary of the polar disk (disk_polar/boundaries.f90:63-94),
