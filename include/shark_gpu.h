@@ -110,7 +110,10 @@ typedef struct shark_cfg {
   double m_acc, phi_mom, l_out;
   /* non_standard_eos = 1 (Collapse_1D, src/setups/Collapse_1D/eos.f90): with iso_cs < 1 the
    * fluxes stay adiabatic but P = rho cs_eos(barotrop(rho))^2 in every cell and face */
-  int32_t eos_baro, reserved_i3;
+  int32_t eos_baro;
+  /* disk_polar self-gravity (&GRAVITY_PARAMS self_gravity, disk_polar/setup.f90:300-322): the
+   * potential of the disk's own active cells, a direct sum over all of them for every cell */
+  int32_t self_gravity;
   double l_soft;            /* gravity softening in code units (l_soft/unit_l) */
   double m_cent;            /* central point mass (M_cent) */
   double t0_cloud;          /* barotrop's T0_cloud (K) */

@@ -448,8 +448,8 @@ static void cdisk_r(const oracle* o, int c, double* xx, double* yy, double* rr) 
   *rr = sqrt((*xx * *xx + *yy * *yy) + p->smooth_r * p->smooth_r);
 }
 
-/* disk_polar planet (setup.f90:323-340): the softened potential of a planet on a circular orbit
- * in every active cell, at the step's time; x = radii_c cos(phi), y = radii_c sin(phi) are the
+/* disk_polar potential (setup.f90:300-340): the disk's own (self_gravity) and/or the softened
+ * potential of a planet on a circular orbit in every active cell, at the step's time; x = radii_c cos(phi), y = radii_c sin(phi) are the
  * grid's position (grid.f90 evaluates the same products).  Then apply_boundaries_phi
  * (boundaries.f90:98-140): y-ghost rows of the active columns are periodic copies, and every
  * x-ghost cell takes phi_sg(first_active) / phi_sg(last_active), the reference using those column
@@ -457,13 +457,37 @@ static void cdisk_r(const oracle* o, int c, double* xx, double* yy, double* rr) 
  * to rows ny+1 of columns 3 and nx+2. */
 static void planet_potential(oracle* o) {
   const shark_cfg* p = &o->cfg;
-  const double theta_p = 2.0 * PI * fmod(o->time / (2.0 * PI / sqrt(p->mstar / cube_i(p->rplanet))), 1.0);
-  const double cx = cos(theta_p) * p->rplanet, cy = sin(theta_p) * p->rplanet;
-  for (int c = 0; c < o->nc; ++c) {
-    if (!o->active[c]) continue;
-    const double ex = POS(c, 0) - cx, ey = POS(c, 1) - cy;
-    const double rr_p = sqrt((ex * ex + ey * ey) + p->a_smooth_pl * p->a_smooth_pl);
-    o->phi_sg[c] = -(p->mplanet / rr_p);
+  if (p->self_gravity) {
+    /* the disk's own potential (setup.f90:300-322): for every active cell a direct sum over
+     * the active cells, ix outer and iy inner, with the reference's terms: the density is the
+     * summing cell's own (q(i,irho)), the azimuths are float(iy) * dx(:,2) */
+    #pragma omp parallel for schedule(static)
+    for (int c = 0; c < o->nc; ++c) {
+      if (!o->active[c]) continue;
+      const double rr = o->radii_c[c], rho = Q(o->irho, c);
+      const double phi_loc = (double)iyy(o, c) * DX(c, 1);
+      double acc = 0.0;
+      for (int ix = o->fa; ix <= o->la; ++ix)
+        for (int iy = o->fay; iy <= o->lay; ++iy) {
+          const int s = cidx(o, ix, iy);
+          const double rr_p = o->radii_c[s], phi_p = (double)iy * DX(s, 1);
+          acc = acc - ((rho * DX(s, 0)) * DX(s, 1)) /
+                          sqrt(((p->a_smooth_pl * p->a_smooth_pl + rr * rr) + rr_p * rr_p) -
+                               ((2.0 * rr) * rr_p) * cos(phi_p - phi_loc));
+        }
+      o->phi_sg[c] = acc;
+    }
+  }
+  if (p->mplanet > 0.0) {
+    /* the planet's potential replaces the self-gravity one (:323-340 assign, not add) */
+    const double theta_p = 2.0 * PI * fmod(o->time / (2.0 * PI / sqrt(p->mstar / cube_i(p->rplanet))), 1.0);
+    const double cx = cos(theta_p) * p->rplanet, cy = sin(theta_p) * p->rplanet;
+    for (int c = 0; c < o->nc; ++c) {
+      if (!o->active[c]) continue;
+      const double ex = POS(c, 0) - cx, ey = POS(c, 1) - cy;
+      const double rr_p = sqrt((ex * ex + ey * ey) + p->a_smooth_pl * p->a_smooth_pl);
+      o->phi_sg[c] = -(p->mplanet / rr_p);
+    }
   }
   for (int ix = o->fa; ix <= o->la; ++ix)
     for (int g = 1; g <= 2; ++g) {
@@ -480,7 +504,7 @@ static void planet_potential(oracle* o) {
 /* update_force_setup: SI src/setups/SI/setup.f90:335-368; disk :360-403 */
 static void update_force(oracle* o) {
   const shark_cfg* p = &o->cfg;
-  const int planet = p->force_kind == SHARK_FORCE_DISK && p->mplanet > 0.0;
+  const int planet = p->force_kind == SHARK_FORCE_DISK && (p->mplanet > 0.0 || p->self_gravity);
   if (planet) planet_potential(o);
   #pragma omp parallel for schedule(static)
   for (int c = 0; c < o->nc; ++c) {

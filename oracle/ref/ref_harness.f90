@@ -24,6 +24,7 @@ program ref_harness
 #ifdef REF_DISK
   use setup_parameters, only: uprim_condinit, mstar, r_relax, n_rel, smooth_r, hoverr, alpha_visc, &
        box_l, accretion, M_acc, phi_mom, L_out, Mplanet, Rplanet, a_smooth_pl
+  use gravity_commons, only: self_gravity
 #endif
 #ifdef REF_SI
   use setup_parameters, only: box_l, rho_init, omega_shear, q_shear, hoverr, eta_stream, rad0
@@ -174,6 +175,7 @@ program ref_harness
   write(ilun, '(A,ES25.17)') 'mplanet ', Mplanet
   write(ilun, '(A,ES25.17)') 'rplanet ', Rplanet
   write(ilun, '(A,ES25.17)') 'a_smooth_pl ', a_smooth_pl
+  write(ilun, '(A,L2)') 'self_gravity ', self_gravity
   open(32, file=trim(outdir)//'/eta_visc.bin', form='unformatted', access='stream', status='replace')
   write(32) eta_visc
   close(32)

@@ -43,7 +43,7 @@ class ShARKCfg(C.Structure):
         ("dust2gas", C.c_double), ("reserved", C.c_double * 2),
         ("accretion", C.c_int32), ("gravity", C.c_int32), ("m_acc", C.c_double),
         ("phi_mom", C.c_double), ("l_out", C.c_double), ("eos_baro", C.c_int32),
-        ("reserved_i3", C.c_int32), ("l_soft", C.c_double), ("m_cent", C.c_double),
+        ("self_gravity", C.c_int32), ("l_soft", C.c_double), ("m_cent", C.c_double),
         ("t0_cloud", C.c_double), ("mu_gas", C.c_double), ("unit_v", C.c_double),
         ("unit_nh", C.c_double), ("a_smooth_pl", C.c_double),
     ]

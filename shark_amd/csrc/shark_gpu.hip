@@ -83,6 +83,8 @@ struct shark_ctx {
   double* dmc = nullptr;            // 1D self-gravity: enclosed mass per cell
   double* dposxy = nullptr;         // polar-disk planet: the grid's position (2, ncells), then the
                                     // gradient of its potential (4, ncells)
+  double* dphisg = nullptr;         // polar-disk self-gravity: potential (ncells), then the cosine
+                                    // table (ny, ny)
   int growth_mode = 0;
 };
 
@@ -149,6 +151,8 @@ static int validate(const shark_cfg* c) {
     return fail(SHARK_EUNSUP, "the polar disk's planet is built for the 2D polar grid");
   if (c->force_kind == SHARK_FORCE_DISK && c->mplanet > 0.0 && !(c->rplanet > 0.0 && c->a_smooth_pl >= 0.0))
     return fail(SHARK_EINVAL, "the polar disk's planet needs rplanet > 0 and a_smooth_pl >= 0");
+  if (c->self_gravity && (c->force_kind != SHARK_FORCE_DISK || c->geom != SHARK_GEOM_POLAR || c->ny < 4 || !c->force_kick))
+    return fail(SHARK_EUNSUP, "self_gravity is the polar disk's (disk_polar/setup.f90:300-322): 2D polar grid with its forces");
   if (c->accretion && (c->bc_kind != SHARK_BC_DISK || c->iso_cs != 1))
     return fail(SHARK_EUNSUP, "accretion is built for the isothermal polar disk's boundaries");
   if (c->cfl <= 0) return fail(SHARK_EINVAL, "CFL must be positive");
@@ -242,8 +246,10 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   }
   a.mc = c->dmc;
   a.dvol = c->ddvol;
-  if (cfg->force_kind == SHARK_FORCE_DISK && cfg->mplanet > 0.0) {
-    if ((e = hipMalloc(&c->dposxy, (size_t)6 * a.nc * 8)) != hipSuccess) {
+  if (cfg->force_kind == SHARK_FORCE_DISK && (cfg->mplanet > 0.0 || cfg->self_gravity)) {
+    if ((e = hipMalloc(&c->dposxy, (size_t)6 * a.nc * 8)) != hipSuccess ||
+        (cfg->self_gravity &&
+         (e = hipMalloc(&c->dphisg, ((size_t)a.nc + (size_t)a.ny * a.ny) * 8)) != hipSuccess)) {
       rc = fail(e == hipErrorOutOfMemory ? SHARK_ENOMEM : SHARK_EHIP, "allocation failed: %s", hipGetErrorString(e));
       shark_gpu_destroy(c);
       return rc;
@@ -252,6 +258,10 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   }
   a.posxy = c->dposxy;
   a.pgrad = c->dposxy ? c->dposxy + (size_t)2 * a.nc : nullptr;
+  a.self_gravity = cfg->self_gravity ? 1 : 0;
+  a.phi_sg = c->dphisg;
+  a.sgcos = c->dphisg ? c->dphisg + a.nc : nullptr;
+  if (c->dphisg) (void)hipMemsetAsync(c->dphisg, 0, (size_t)a.nc * 8, c->stream);
   a.a_smooth_pl = cfg->a_smooth_pl;
   (void)hipMemsetAsync(c->dU, 0, nbytes, c->stream);
   (void)hipMemsetAsync(c->dB, 0, nbytes, c->stream);
@@ -311,9 +321,24 @@ extern "C" int shark_gpu_set_geometry(shark_ctx* c, const double* dx, const doub
                         c->cfg.force_kind == SHARK_FORCE_CDISK || c->cfg.inloop_kind == SHARK_INLOOP_CDISK ||
                         c->cfg.tstop_kind == SHARK_TSTOP_EPSTEIN_CDISK ||
                         (c->cfg.viscosity && c->cfg.geom == SHARK_GEOM_CARTESIAN);
-  if (!position && (need_pos || c->dposxy)) return fail(SHARK_EINVAL, "position is required by this setup's hooks");
-  if (c->dposxy)
+  const bool planet = c->dposxy && c->cfg.mplanet > 0.0;
+  if (!position && (need_pos || planet)) return fail(SHARK_EINVAL, "position is required by this setup's hooks");
+  if (planet)
     HIPCHK(hipMemcpyAsync(c->dposxy, position, (size_t)2 * nc * 8, hipMemcpyHostToDevice, c->stream));
+  if (c->dphisg) {
+    // cos(phi_p - phi_loc) of the self-gravity sum (disk_polar/setup.f90:311-315) per (source
+    // row, target row): phi = float(iy) * dx(:,2), with the host's cos, as the reference calls it
+    const int ny = a.ny;
+    const double d2 = col[nxm + 2];
+    for (int ix = 3; ix <= a.nx + 2; ++ix)
+      if (col[nxm + ix - 1] != d2) return fail(SHARK_EUNSUP, "self-gravity needs dx(:,2) equal in every column");
+    std::vector<double> tab((size_t)ny * ny);
+    for (int sy = 0; sy < ny; ++sy)
+      for (int ty = 0; ty < ny; ++ty)
+        tab[(size_t)sy * ny + ty] = cos((double)(sy + 3) * d2 - (double)(ty + 3) * d2);
+    HIPCHK(hipMemcpyAsync(c->dphisg + nc, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));   // tab is released at the end of this block
+  }
   HIPCHK(hipMemcpyAsync(c->dgeom, col.data(), col.size() * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->hrc.assign(radii_c, radii_c + nc);
@@ -898,6 +923,7 @@ extern "C" int shark_gpu_destroy(shark_ctx* c) {
   if (c->dfcs) (void)hipFree(c->dfcs);
   if (c->dmc) (void)hipFree(c->dmc);
   if (c->dposxy) (void)hipFree(c->dposxy);
+  if (c->dphisg) (void)hipFree(c->dphisg);
   if (c->ddvol) (void)hipFree(c->ddvol);
   if (c->dsdust) (void)hipFree(c->dsdust);
   if (c->dcond) (void)hipFree(c->dcond);

@@ -85,6 +85,12 @@ struct StepArgs {
   double a_smooth_pl;
   const double* posxy;
   double* pgrad;
+  // polar-disk self-gravity (disk_polar/setup.f90:300-322; replaced by the planet's potential when
+  // there is a planet): the potential per cell, and cos(phi_p - phi_loc) per (source row, target
+  // row) of the active rows, computed on the host with the host's cos; nullptr without it
+  int self_gravity;
+  double* phi_sg;
+  const double* sgcos;
 };
 
 // launch shapes: every shape gives the same results bit for bit, only the speed differs

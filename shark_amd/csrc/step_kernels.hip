@@ -1308,6 +1308,8 @@ __global__ void k_planet(StepArgs a) {
 // apply_boundaries_phi (boundaries.f90:98-140): y-ghost rows of active columns are periodic
 // copies; every x-ghost cell takes the potential of cell number first_active / last_active, i.e.
 // cells (3,1) and (nx+2,1) of the first ghost row, which hold rows ny+1 of columns 3 and nx+2.
+// With self-gravity and no planet the potential is the one k_selfgrav left in phi_sg, the same
+// for both times (the state of the step's start; the next step's dt is taken again from its own).
 __global__ void k_planet_grad(StepArgs a) {
   if (!a.sc->active) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1316,6 +1318,7 @@ __global__ void k_planet_grad(StepArgs a) {
   const size_t cell = (size_t)(ix - 1) + (size_t)(iy - 1) * a.nxm;
   const double r = a.g.rc[ix - 1];
   const double sm2 = a.a_smooth_pl * a.a_smooth_pl;
+  const bool planet = a.mplanet > 0.0;
   for (int w = 0; w < 2; ++w) {
     const double cx = a.sc->planet[2 * w], cy = a.sc->planet[2 * w + 1];
     auto pot = [&](int jx, int jy) {
@@ -1324,6 +1327,7 @@ __global__ void k_planet_grad(StepArgs a) {
       else if (jy < 3) jy += a.ny;
       else if (jy > a.ny + 2) jy -= a.ny;
       const size_t c = (size_t)(jx - 1) + (size_t)(jy - 1) * a.nxm;
+      if (!planet) return a.phi_sg[c];
       const double ex = a.posxy[c] - cx, ey = a.posxy[a.nc + c] - cy;
       return -(a.mplanet / sqrt((ex * ex + ey * ey) + sm2));
     };
@@ -1335,11 +1339,43 @@ __global__ void k_planet_grad(StepArgs a) {
   }
 }
 
+// The disk's own potential (disk_polar/setup.f90:300-322) from the state at the step's start:
+// for every active cell a direct sum over the active cells, ix outer and iy inner as the
+// reference sums them, with its terms: the summing cell's own density (q(i,irho)), the source's
+// dx(:,1) dx(:,2), radii and the azimuth difference's cosine from the host's table.  A wave's
+// lanes hold consecutive rows of one column, so the sources' values are uniform and the table's
+// rows are read coalesced.
+__global__ __launch_bounds__(256) void k_selfgrav(StepArgs a) {
+  if (!a.sc->active) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nx * a.ny) return;
+  const int iy = 3 + t % a.ny, ix = 3 + t / a.ny;
+  const size_t cell = (size_t)(ix - 1) + (size_t)(iy - 1) * a.nxm;
+  const double rr = a.g.rc[ix - 1], rho = fmaxf_(a.U[cell], a.smallr);
+  const double base = a.a_smooth_pl * a.a_smooth_pl + rr * rr, two_rr = 2.0 * rr;
+  const double* __restrict__ ct = a.sgcos + (iy - 3);
+  double acc = 0.0;
+  for (int sx = 3; sx <= a.nx + 2; ++sx) {
+    const double rp = a.g.rc[sx - 1], w = rho * a.g.dx1[sx - 1] * a.g.dx2[sx - 1];
+    const double b2 = base + rp * rp, k2 = two_rr * rp;
+    for (int sy = 0; sy < a.ny; ++sy) acc = acc - w / sqrt(b2 - k2 * ct[(size_t)sy * a.ny]);
+  }
+  a.phi_sg[cell] = acc;
+}
+
 static void launch_planet(const StepArgs& a, hipStream_t s) {
   if ((a.force_kind == 3 || a.force_kind == 2) && a.mplanet > 0.0)
     hipLaunchKernelGGL(k_planet, dim3(1), dim3(1), 0, s, a);
-  if (a.pgrad)
+  if (a.pgrad && a.mplanet > 0.0)
     hipLaunchKernelGGL(k_planet_grad, dim3((a.nx * a.ny + 255) / 256), dim3(256), 0, s, a);
+}
+
+// self-gravity without a planet: the potential of this state and its gradient (with a planet,
+// the reference's planet potential replaces the disk's own, so k_planet_grad alone applies)
+static void launch_selfgrav(const StepArgs& a, hipStream_t s) {
+  if (!a.self_gravity || a.mplanet > 0.0) return;
+  hipLaunchKernelGGL(k_selfgrav, dim3((a.nx * a.ny + 255) / 256), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_planet_grad, dim3((a.nx * a.ny + 255) / 256), dim3(256), 0, s, a);
 }
 
 // Accretion through the outer boundary of the polar disk (disk_polar/boundaries.f90:63-94),
@@ -1389,11 +1425,15 @@ void launch_bc(const StepArgs& a, hipStream_t s) {
   }
   const int n = (a.ndim == 2 ? 4 * a.nx : 0) + 4 * a.nym;
   hipLaunchKernelGGL(k_bc, dim3((n + 255) / 256), dim3(256), 0, s, a);
-  if (a.accretion && a.bc_kind == 2) {
+  const bool sg = a.self_gravity && !(a.mplanet > 0.0);
+  if (a.accretion && a.bc_kind == 2)
     hipLaunchKernelGGL(k_bc_accrete, dim3((3 * a.ny + 255) / 256), dim3(256), 0, s, a, a.U, 0, 3);
-    // the inflow rewrites the last active column, so this step's dt is taken here, from the
-    // filled state, as courant does after apply_boundaries (solve.f90:29-56); the candidate the
-    // previous post pass left is replaced
+  if (sg) launch_selfgrav(a, s);
+  if ((a.accretion && a.bc_kind == 2) || sg) {
+    // the inflow rewrites the last active column, and the disk's potential is this state's, so
+    // this step's dt is taken here, from the filled state, as courant does after apply_boundaries
+    // and update_force_setup (solve.f90:29-74); the candidate the previous post pass left is
+    // replaced
     launch_courant_now(a, s);
   }
 }
@@ -1532,6 +1572,7 @@ static void launch_courant_now(const StepArgs& a, hipStream_t s) {
 
 void launch_courant_init(const StepArgs& a, hipStream_t s) {
   launch_planet(a, s);
+  launch_selfgrav(a, s);
   launch_courant_now(a, s);
 }
 void launch_set_dt(const StepArgs& a, hipStream_t s) {

@@ -37,7 +37,7 @@ module shark_gpu_mod
      real(c_double) :: reserved(2)
      integer(c_int32_t) :: accretion, gravity
      real(c_double) :: m_acc, phi_mom, l_out
-     integer(c_int32_t) :: eos_baro, reserved_i3
+     integer(c_int32_t) :: eos_baro, self_gravity
      real(c_double) :: l_soft, m_cent, t0_cloud, mu_gas, unit_v, unit_nh
      real(c_double) :: a_smooth_pl
   end type shark_cfg
@@ -177,7 +177,7 @@ contains
     c%dust2gas = 0.0d0
     c%reserved = 0.0d0
     c%accretion = 0; c%gravity = 0; c%m_acc = 0.0d0; c%phi_mom = 2.0d0; c%l_out = 1.0d0
-    c%eos_baro = 0; c%reserved_i3 = 0; c%l_soft = 0.0d0; c%m_cent = 0.0d0; c%t0_cloud = 0.0d0
+    c%eos_baro = 0; c%self_gravity = 0; c%l_soft = 0.0d0; c%m_cent = 0.0d0; c%t0_cloud = 0.0d0
     c%mu_gas = 0.0d0; c%unit_v = 0.0d0; c%unit_nh = 0.0d0; c%a_smooth_pl = 0.0d0
   end subroutine shark_cfg_defaults
 end module shark_gpu_mod

@@ -119,6 +119,9 @@ DISK_PLANET = DISK.replace("r_relax      = {r_relax}\n",
                            "r_relax      = {r_relax}\nMplanet      = {mplanet}\nRplanet      = 1.3d0\n"
                            "a_smooth_pl  = 0.05d0\n")
 
+# disk_polar with the disk's own potential (setup.f90:300-322)
+DISK_SG = DISK.replace("&GRAVITY_PARAMS\n/", "&GRAVITY_PARAMS\nself_gravity = .true.\n/")
+
 SOD = """&GRID_PARAMS
 freq_out=100000
 CFL = {cfl}
@@ -360,6 +363,10 @@ CASES = {
                        DISK_PLANET.format(slope=1, r_relax="0.3d0", visc=".false.", alpha="1d-2", mplanet="3d-3"), 40),
     "disk_2d_planet_gas": ("disk_polar", 28, 40, 0, 2, 0, 2, 0,
                            DISK_PLANET.format(slope=3, r_relax="0.3d0", visc=".false.", alpha="1d-2", mplanet="1d-2"), 40),
+    "disk_2d_selfgrav": ("disk_polar", 16, 24, 2, 2, 0, 1, 0,
+                         DISK_SG.format(slope=1, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 30),
+    "disk_2d_selfgrav_gas": ("disk_polar", 20, 28, 0, 2, 0, 2, 0,
+                             DISK_SG.format(slope=2, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 30),
     "disk_2d_100": ("disk_polar", 32, 48, 4, 2, 0, 1, 0,
                     DISK.format(slope=1, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 100),
     "test_psi_100": ("SI", 32, 32, 8, 0, 0, 1, 0, test_nml("PSI/psi.nml", freq_out="100000", tend="1d10"), 100),

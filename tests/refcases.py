@@ -55,7 +55,7 @@ def load_ref(name):
                   m_acc=m.get("M_acc", 0.0) / (365.25 * 24.0 * 3600.0) * m["unit_t"],
                   phi_mom=m.get("phi_mom", 2.0), l_out=m.get("L_out", 1.0), dust2gas=m.get("dust2gas", 0.0),
                   mplanet=m.get("mplanet", 0.0), rplanet=m.get("rplanet", 0.0),
-                  a_smooth_pl=m.get("a_smooth_pl", 0.0))
+                  a_smooth_pl=m.get("a_smooth_pl", 0.0), self_gravity=int(m.get("self_gravity", False)))
     elif setup == "SOD":
         kw.update(bc_kind=C.BC_SOD, tstop_kind=C.TSTOP_DUSTYWAVE, force_kind=C.FORCE_NONE,
                   inloop_kind=C.INLOOP_NONE, rho_l=m["rho_l"], rho_r=m["rho_r"], p_l=m["p_l"],

@@ -54,6 +54,8 @@ def test_create_rejects_out_of_scope_configs_without_gpu():
                       b"rplanet > 0"),
                      (dict(nx=16, ny=16, ndust=1, force_kind=C.FORCE_DISK, mplanet=1e-3, rplanet=1.0),
                       b"2D polar grid"),
+                     (dict(nx=16, ny=16, ndust=1, force_kick=1, force_kind=C.FORCE_SI, self_gravity=1),
+                      b"self_gravity"),
                      (dict(nx=64, ndust=65), b"ndust=65"),
                      (dict(nx=64, ndust=1, slope_type=7), b"slope limiter"),
                      (dict(nx=64, ndust=1, viscosity=1), b"viscosity"),
