@@ -142,6 +142,10 @@ void oracle_set_geometry(oracle* o, const double* dx, const double* surf, const 
  * between its centre and its outer face (the enclosed mass of mtot subtracts it) */
 void oracle_set_dvol(oracle* o, const double* dvol) { memcpy(o->dvol, dvol, (size_t)o->nc * 8); }
 
+static void ctoprim(oracle* o);
+static void distribution_dust_refresh(oracle* o);
+static void compute_tstop(oracle* o);
+
 void oracle_upload(oracle* o, const double* u, const double* cs, const double* sdust,
                    const double* condinit, const double* fargo_v) {
   size_t nc = o->nc;
@@ -150,6 +154,14 @@ void oracle_upload(oracle* o, const double* u, const double* cs, const double* s
   if (condinit) memcpy(o->condinit, condinit, nc * o->nvar * 8);
   if (fargo_v) memcpy(o->fargo_v, fargo_v, nc * 8);
   o->time = 0.0;
+  if (!o->cfg.drag && o->cfg.force_kick && o->nd > 0) {
+    /* explicit drag reads tstop in update_force_setup, which runs before the step's
+     * compute_tstop (solve.f90:29-35): the first step reads the stopping times the setup
+     * computed at initialisation (SI setup.f90:160, disk_polar :77) */
+    ctoprim(o);
+    distribution_dust_refresh(o);
+    compute_tstop(o);
+  }
 }
 
 void oracle_download(const oracle* o, double* u, double* q) {

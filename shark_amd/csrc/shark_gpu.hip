@@ -85,6 +85,7 @@ struct shark_ctx {
                                     // gradient of its potential (4, ncells)
   double* dphisg = nullptr;         // polar-disk self-gravity: potential (ncells), then the cosine
                                     // table (ny, ny)
+  double* dfexp = nullptr;          // explicit drag: every fluid's force of the step
   int growth_mode = 0;
 };
 
@@ -141,8 +142,9 @@ static int validate(const shark_cfg* c) {
     return fail(SHARK_EUNSUP, "viscosity is built for the isothermal cartesian and polar disks (eta_visc = alpha_visc H cs)");
   if (c->fargo && (c->ny < 4 || c->geom != SHARK_GEOM_POLAR))
     return fail(SHARK_EUNSUP, "FARGO is built for 2D polar grids (rings of constant radius, the disk_polar setup)");
-  if (c->force_kick && !c->drag && c->ndust > 0)
-    return fail(SHARK_EUNSUP, "explicit drag in the force (drag=.false. with force_kick) is not built");
+  if (c->force_kick && !c->drag && c->ndust > 0 && (c->force_kind != SHARK_FORCE_SI || c->tstop_kind != SHARK_TSTOP_SI))
+    return fail(SHARK_EUNSUP, "explicit drag in the force (drag=.false. with force_kick) is built for the shearing "
+                              "box, whose stopping times are constant");
   if (c->bc_kind < 0 || c->bc_kind > 5 || c->tstop_kind < 0 || c->tstop_kind > 5 ||
       c->force_kind < 0 || c->force_kind > 3 || c->inloop_kind < 0 || c->inloop_kind > 3)
     return fail(SHARK_EINVAL, "unknown setup hook kind");
@@ -258,6 +260,15 @@ extern "C" int shark_gpu_create(const shark_cfg* cfg, shark_ctx** out) {
   }
   a.posxy = c->dposxy;
   a.pgrad = c->dposxy ? c->dposxy + (size_t)2 * a.nc : nullptr;
+  if (cfg->force_kick && !cfg->drag && cfg->ndust > 0) {
+    if ((e = hipMalloc(&c->dfexp, (size_t)3 * (1 + cfg->ndust) * a.nc * 8)) != hipSuccess) {
+      rc = fail(e == hipErrorOutOfMemory ? SHARK_ENOMEM : SHARK_EHIP, "allocation failed: %s", hipGetErrorString(e));
+      shark_gpu_destroy(c);
+      return rc;
+    }
+    (void)hipMemsetAsync(c->dfexp, 0, (size_t)3 * (1 + cfg->ndust) * a.nc * 8, c->stream);
+  }
+  a.fexp = c->dfexp;
   a.self_gravity = cfg->self_gravity ? 1 : 0;
   a.phi_sg = c->dphisg;
   a.sgcos = c->dphisg ? c->dphisg + a.nc : nullptr;
@@ -924,6 +935,7 @@ extern "C" int shark_gpu_destroy(shark_ctx* c) {
   if (c->dmc) (void)hipFree(c->dmc);
   if (c->dposxy) (void)hipFree(c->dposxy);
   if (c->dphisg) (void)hipFree(c->dphisg);
+  if (c->dfexp) (void)hipFree(c->dfexp);
   if (c->ddvol) (void)hipFree(c->ddvol);
   if (c->dsdust) (void)hipFree(c->dsdust);
   if (c->dcond) (void)hipFree(c->dcond);

@@ -91,6 +91,9 @@ struct StepArgs {
   int self_gravity;
   double* phi_sg;
   const double* sgcos;
+  // explicit drag in the force (drag = .false. with force_kick; SI setup.f90:355-362): every
+  // fluid's force of the step, (3 (1 + ndust), ncells), from k_explicit_force; nullptr otherwise
+  double* fexp;
 };
 
 // launch shapes: every shape gives the same results bit for bit, only the speed differs

@@ -36,8 +36,8 @@ DEV bool is_active(const StepArgs& a, int ix, int iy) {
 // the planet's position in time.  CD: the cartesian-disk force is compiled in (the hydro
 // kernels of other grids leave it out, so it costs them no registers)
 template <bool CD = true>
-DEV void cell_force(const StepArgs& a, int ix, int iy, int next, double qx, double qz, bool gas,
-                    double& f1, double& f2, double& f3) {
+DEV void cell_force_setup(const StepArgs& a, int ix, int iy, int next, double qx, double qz, bool gas,
+                          double& f1, double& f2, double& f3) {
   f1 = 0.0; f2 = 0.0; f3 = 0.0;
   if (CD && a.force_kind == 3) {
     // cartesian disk (disk/setup.f90:274-301): softened star, plus a planet on a circular orbit
@@ -70,6 +70,22 @@ DEV void cell_force(const StepArgs& a, int ix, int iy, int next, double qx, doub
       f2 = f2 - a.pgrad[(size_t)(2 * next + 1) * a.nc + cell];
     }
   }
+}
+
+// The force of fluid k (-1: gas) in an active cell: with explicit drag, the step's forces from
+// k_explicit_force (setup forces plus the drag terms, all fluids of the cell coupled); else the
+// setup's forces of this cell and fluid alone.
+template <bool CD = true>
+DEV void cell_force(const StepArgs& a, int ix, int iy, int next, double qx, double qz, bool gas, int k,
+                    double& f1, double& f2, double& f3) {
+  if (a.fexp) {
+    const size_t cell = (size_t)(ix - 1) + (size_t)(iy - 1) * a.nxm, fl = (size_t)(k + 1);
+    f1 = a.fexp[(3 * fl) * a.nc + cell];
+    f2 = a.fexp[(3 * fl + 1) * a.nc + cell];
+    f3 = a.fexp[(3 * fl + 2) * a.nc + cell];
+    return;
+  }
+  cell_force_setup<CD>(a, ix, iy, next, qx, qz, gas, f1, f2, f3);
 }
 
 // ------------------------------------------------------------------ ghost fill
@@ -337,7 +353,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
                    FaceSet& fy) {
     double f1 = 0.0, f2 = 0.0, f3 = 0.0;
     if (a.force_kick && col_active && (NDIM == 1 || (iy >= 3 && iy <= a.ny + 2)))
-      cell_force<GEOM == 3>(a, ix, iy, 0, r.q[1], r.q[3], GAS, f1, f2, f3);
+      cell_force<GEOM == 3>(a, ix, iy, 0, r.q[1], r.q[3], GAS, fluid - 1, f1, f2, f3);
     const double r_rho = r.q[0];
     double u = r.q[1], v = r.q[2], w = r.q[3];
     const double drx = dqx[0], dux = dqx[1], dvx = dqx[2], dwx = dqx[3];
@@ -566,7 +582,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
       for (int c = 0; c < NU; ++c) uk[c] = Rf.u[c];
       if (a.force_kick) {  // kick(1.0d0), force_kicks.f90:12-39
         double f1, f2, f3;
-        cell_force<GEOM == 3>(a, ix, fr, 0, Rf.q[1], Rf.q[3], GAS, f1, f2, f3);
+        cell_force<GEOM == 3>(a, ix, fr, 0, Rf.q[1], Rf.q[3], GAS, fluid - 1, f1, f2, f3);
         const double rho = Rf.u[0];
         if (GAS) {
           const double eold = ((kHalf * (uk[1] * uk[1])) / rho + (kHalf * (uk[2] * uk[2])) / rho) +
@@ -753,12 +769,12 @@ DEV double courant_cell(const StepArgs& a, const double* un, int ix, int iy, int
   }
   if (a.force_kick) {
     double f1, f2, f3;
-    cell_force(a, ix, iy, next, vx, vz, true, f1, f2, f3);
+    cell_force(a, ix, iy, next, vx, vz, true, -1, f1, f2, f3);
     double fm = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
 #pragma unroll
     for (int k = 0; k < ND; ++k) {
       if (k >= nd) break;
-      cell_force(a, ix, iy, next, vdx[k], vdz[k], false, f1, f2, f3);
+      cell_force(a, ix, iy, next, vdx[k], vdz[k], false, k, f1, f2, f3);
       fm = fmaxf_(fm, sqrt((f1 * f1 + f2 * f2) + f3 * f3));
     }
     if (vv != 0.0) {
@@ -1103,7 +1119,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
       vmax = cs_cell(a, rho, un[4][c], vx, vy, vz, cell) + vv;
       if (nd == 0) vvlast[c] = vv;
       if (a.force_kick) {
-        cell_force<!FARGO>(a, ixs, iy, 1, vx, vz, true, f1, f2, f3);
+        cell_force<!FARGO>(a, ixs, iy, 1, vx, vz, true, -1, f1, f2, f3);
         fm = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
       }
       any = true;
@@ -1111,7 +1127,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
     // the disk's dust acceleration does not depend on the species' velocity: one evaluation
     double fk_disk = 0.0;
     if (a.force_kick && a.force_kind == 2 && s < nd) {
-      cell_force<!FARGO>(a, ixs, iy, 1, 0.0, 0.0, false, f1, f2, f3);
+      cell_force<!FARGO>(a, ixs, iy, 1, 0.0, 0.0, false, 0, f1, f2, f3);
       fk_disk = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
     }
     for (int k = s; k < nd; k += NS) {
@@ -1123,7 +1139,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
       if (a.force_kick) {
         double fk = fk_disk;
         if (a.force_kind != 2) {
-          cell_force<!FARGO>(a, ixs, iy, 1, vdx, vdz, false, f1, f2, f3);
+          cell_force<!FARGO>(a, ixs, iy, 1, vdx, vdz, false, k, f1, f2, f3);
           fk = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
         }
         fm = any ? fmaxf_(fm, fk) : fk;
@@ -1363,6 +1379,43 @@ __global__ __launch_bounds__(256) void k_selfgrav(StepArgs a) {
   a.phi_sg[cell] = acc;
 }
 
+// update_force_setup with explicit drag (SI setup.f90:340-365, drag = .false.): per active cell,
+// from ctoprim's primitives of the step's start (solve.f90:29), the setup's force of every fluid,
+// then for each species in order its drag term subtracted from the gas force and added to its own
+// force.  The stopping time is the shearing box's, constant in time, so the reference reading the
+// previous step's value (update_force_setup runs before compute_tstop) changes nothing.
+__global__ void k_explicit_force(StepArgs a) {
+  if (!a.sc->active) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nx * a.ny) return;
+  const int ix = 3 + t % a.nx, iy = 3 + t / a.nx;
+  const size_t nc = a.nc, cell = (size_t)(ix - 1) + (size_t)(iy - 1) * a.nxm;
+  const int nd = a.nd;
+  const double rho = a.U[cell], rho_q = fmaxf_(rho, a.smallr);
+  const double vx = a.U[nc + cell] / rho, vy = a.U[2 * nc + cell] / rho, vz = a.U[3 * nc + cell] / rho;
+  double f1, f2, f3;
+  cell_force_setup(a, ix, iy, 0, vx, vz, true, f1, f2, f3);
+  for (int k = 0; k < nd; ++k) {
+    const double rd = a.U[(5 + k) * nc + cell];
+    const double vdx = a.U[(5 + nd + k) * nc + cell] / rd, vdy = a.U[(5 + 2 * nd + k) * nc + cell] / rd,
+                 vdz = a.U[(5 + 3 * nd + k) * nc + cell] / rd;
+    double g1, g2, g3;
+    cell_force_setup(a, ix, iy, 0, vdx, vdz, false, g1, g2, g3);
+    const double ts = tstop_of<1, false>(a, k, (int)cell, ix, rho_q, rd, a.cs[cell]);
+    const double eps = rd / rho_q;
+    f1 = f1 - (eps * (vx - vdx)) / ts;
+    f2 = f2 - (eps * (vy - vdy)) / ts;
+    f3 = f3 - (eps * (vz - vdz)) / ts;
+    const size_t fl = (size_t)(k + 1);
+    a.fexp[(3 * fl) * nc + cell] = g1 + (vx - vdx) / ts;
+    a.fexp[(3 * fl + 1) * nc + cell] = g2 + (vy - vdy) / ts;
+    a.fexp[(3 * fl + 2) * nc + cell] = g3 + (vz - vdz) / ts;
+  }
+  a.fexp[cell] = f1;
+  a.fexp[nc + cell] = f2;
+  a.fexp[2 * nc + cell] = f3;
+}
+
 static void launch_planet(const StepArgs& a, hipStream_t s) {
   if ((a.force_kind == 3 || a.force_kind == 2) && a.mplanet > 0.0)
     hipLaunchKernelGGL(k_planet, dim3(1), dim3(1), 0, s, a);
@@ -1429,11 +1482,12 @@ void launch_bc(const StepArgs& a, hipStream_t s) {
   if (a.accretion && a.bc_kind == 2)
     hipLaunchKernelGGL(k_bc_accrete, dim3((3 * a.ny + 255) / 256), dim3(256), 0, s, a, a.U, 0, 3);
   if (sg) launch_selfgrav(a, s);
-  if ((a.accretion && a.bc_kind == 2) || sg) {
-    // the inflow rewrites the last active column, and the disk's potential is this state's, so
-    // this step's dt is taken here, from the filled state, as courant does after apply_boundaries
-    // and update_force_setup (solve.f90:29-74); the candidate the previous post pass left is
-    // replaced
+  if (a.fexp) hipLaunchKernelGGL(k_explicit_force, dim3((a.nx * a.ny + 255) / 256), dim3(256), 0, s, a);
+  if ((a.accretion && a.bc_kind == 2) || sg || a.fexp) {
+    // the inflow rewrites the last active column, and the disk's potential and the explicit drag
+    // are this state's, so this step's dt is taken here, from the filled state, as courant does
+    // after apply_boundaries and update_force_setup (solve.f90:29-74); the candidate the previous
+    // post pass left is replaced
     launch_courant_now(a, s);
   }
 }
@@ -1573,6 +1627,7 @@ static void launch_courant_now(const StepArgs& a, hipStream_t s) {
 void launch_courant_init(const StepArgs& a, hipStream_t s) {
   launch_planet(a, s);
   launch_selfgrav(a, s);
+  if (a.fexp) hipLaunchKernelGGL(k_explicit_force, dim3((a.nx * a.ny + 255) / 256), dim3(256), 0, s, a);
   launch_courant_now(a, s);
 }
 void launch_set_dt(const StepArgs& a, hipStream_t s) {

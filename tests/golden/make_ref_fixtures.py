@@ -301,6 +301,11 @@ CASES = {
                         WAVE.format(cfl=0.5, tend="1d10", slope=3, vx0="3.0d-1", drho="1.0d-1", K="5d0"), 60),
     "si_2d": ("SI", 24, 24, 2, 0, 0, 1, 0, SI.format(slope=1), 25),
     "si_2d_s3_hllc": ("SI", 24, 24, 2, 0, 0, 2, 0, SI.format(slope=3), 25),
+    # explicit drag in the force instead of the implicit drag step (setup.f90:355-362)
+    "si_2d_explicit_drag": ("SI", 24, 24, 2, 0, 0, 1, 0,
+                            SI.format(slope=1).replace("drag               = .true.", "drag               = .false."), 25),
+    "si_2d_explicit_drag_s2": ("SI", 24, 24, 2, 0, 0, 2, 1,
+                               SI.format(slope=2).replace("drag               = .true.", "drag               = .false."), 25),
     "disk_2d": ("disk_polar", 24, 32, 2, 2, 0, 1, 0, DISK.format(slope=1, r_relax="0.3d0", visc=".false.", alpha="1d-2"), 25),
     "disk_2d_relax_s2": ("disk_polar", 24, 32, 2, 2, 0, 1, 0, DISK.format(slope=2, r_relax="3.0d0", visc=".false.", alpha="1d-2"), 25),
     "sedov_hllc_s3": ("sedov", 32, 32, 0, 0, 0, 2, 0, SEDOV.format(cfl=0.5, slope=3), 40),

@@ -56,6 +56,9 @@ def test_create_rejects_out_of_scope_configs_without_gpu():
                       b"2D polar grid"),
                      (dict(nx=16, ny=16, ndust=1, force_kick=1, force_kind=C.FORCE_SI, self_gravity=1),
                       b"self_gravity"),
+                     # explicit drag: built for the shearing box (constant stopping times) only
+                     (dict(nx=16, ny=16, ndust=1, geom=C.GEOM_POLAR, force_kick=1, drag=0,
+                           force_kind=C.FORCE_DISK, tstop_kind=C.TSTOP_EPSTEIN_DISK), b"shearing box"),
                      (dict(nx=64, ndust=65), b"ndust=65"),
                      (dict(nx=64, ndust=1, slope_type=7), b"slope limiter"),
                      (dict(nx=64, ndust=1, viscosity=1), b"viscosity"),
