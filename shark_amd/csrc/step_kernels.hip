@@ -74,11 +74,13 @@ DEV void cell_force_setup(const StepArgs& a, int ix, int iy, int next, double qx
 
 // The force of fluid k (-1: gas) in an active cell: with explicit drag, the step's forces from
 // k_explicit_force (setup forces plus the drag terms, all fluids of the cell coupled); else the
-// setup's forces of this cell and fluid alone.
-template <bool CD = true>
+// setup's forces of this cell and fluid alone.  EXPL compiles the explicit-drag lookup in: the
+// shearing box's (2D cartesian) hydro kernels and the standalone Courant kernel (the post pass's candidate is
+// replaced by the latter whenever explicit drag is on), so the other kernels keep their registers.
+template <bool CD = true, bool EXPL = true>
 DEV void cell_force(const StepArgs& a, int ix, int iy, int next, double qx, double qz, bool gas, int k,
                     double& f1, double& f2, double& f3) {
-  if (a.fexp) {
+  if (EXPL && a.fexp) {
     const size_t cell = (size_t)(ix - 1) + (size_t)(iy - 1) * a.nxm, fl = (size_t)(k + 1);
     f1 = a.fexp[(3 * fl) * a.nc + cell];
     f2 = a.fexp[(3 * fl + 1) * a.nc + cell];
@@ -353,7 +355,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
                    FaceSet& fy) {
     double f1 = 0.0, f2 = 0.0, f3 = 0.0;
     if (a.force_kick && col_active && (NDIM == 1 || (iy >= 3 && iy <= a.ny + 2)))
-      cell_force<GEOM == 3>(a, ix, iy, 0, r.q[1], r.q[3], GAS, fluid - 1, f1, f2, f3);
+      cell_force<GEOM == 3, GEOM == 0 && NDIM == 2>(a, ix, iy, 0, r.q[1], r.q[3], GAS, fluid - 1, f1, f2, f3);
     const double r_rho = r.q[0];
     double u = r.q[1], v = r.q[2], w = r.q[3];
     const double drx = dqx[0], dux = dqx[1], dvx = dqx[2], dwx = dqx[3];
@@ -582,7 +584,7 @@ __device__ void hydro_fluid(const StepArgs& a, int fluid, int g, int strip, int 
       for (int c = 0; c < NU; ++c) uk[c] = Rf.u[c];
       if (a.force_kick) {  // kick(1.0d0), force_kicks.f90:12-39
         double f1, f2, f3;
-        cell_force<GEOM == 3>(a, ix, fr, 0, Rf.q[1], Rf.q[3], GAS, fluid - 1, f1, f2, f3);
+        cell_force<GEOM == 3, GEOM == 0 && NDIM == 2>(a, ix, fr, 0, Rf.q[1], Rf.q[3], GAS, fluid - 1, f1, f2, f3);
         const double rho = Rf.u[0];
         if (GAS) {
           const double eold = ((kHalf * (uk[1] * uk[1])) / rho + (kHalf * (uk[2] * uk[2])) / rho) +
@@ -1119,7 +1121,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
       vmax = cs_cell(a, rho, un[4][c], vx, vy, vz, cell) + vv;
       if (nd == 0) vvlast[c] = vv;
       if (a.force_kick) {
-        cell_force<!FARGO>(a, ixs, iy, 1, vx, vz, true, -1, f1, f2, f3);
+        cell_force<!FARGO, false>(a, ixs, iy, 1, vx, vz, true, -1, f1, f2, f3);
         fm = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
       }
       any = true;
@@ -1127,7 +1129,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
     // the disk's dust acceleration does not depend on the species' velocity: one evaluation
     double fk_disk = 0.0;
     if (a.force_kick && a.force_kind == 2 && s < nd) {
-      cell_force<!FARGO>(a, ixs, iy, 1, 0.0, 0.0, false, 0, f1, f2, f3);
+      cell_force<!FARGO, false>(a, ixs, iy, 1, 0.0, 0.0, false, 0, f1, f2, f3);
       fk_disk = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
     }
     for (int k = s; k < nd; k += NS) {
@@ -1139,7 +1141,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
       if (a.force_kick) {
         double fk = fk_disk;
         if (a.force_kind != 2) {
-          cell_force<!FARGO>(a, ixs, iy, 1, vdx, vdz, false, k, f1, f2, f3);
+          cell_force<!FARGO, false>(a, ixs, iy, 1, vdx, vdz, false, k, f1, f2, f3);
           fk = sqrt((f1 * f1 + f2 * f2) + f3 * f3);
         }
         fm = any ? fmaxf_(fm, fk) : fk;
