@@ -1208,7 +1208,18 @@ __device__ __forceinline__ double out_value(const SArgs& a, const float* y, int 
 }
 
 // one cell of the learned step's new state; returns the cell's raw mass residual (0 for a cell that
-// fell back or does not grow), which the kernel sums per wave before one atomic add
+// fell back or does not grow), which the kernel sums per wave before one atomic add.  The per-bin
+// weights w_k = rho_k exp(change_k) (rounded to FP32: the network's prediction carries ~1e-3, and
+// the cell's mass is restored in FP64 from the final densities) are recomputed in each of the
+// three passes (sums, floored sum, new densities) rather than held in 64 registers per lane: the
+// same operations in the same order, at four times the occupancy.
+__device__ __forceinline__ float surr_w(const SArgs& a, const float* y, int k, int64_t ld, const float* om,
+                                        const float* os, int64_t c, double* r_out) {
+  const double r = a.rhod[(int64_t)k * a.ld + c];
+  *r_out = r;
+  return (float)(r * exp(out_value(a, y, k, ld, om, os)));
+}
+
 __device__ __forceinline__ double surr_post_cell(const SArgs& a, int cc) {
   const int64_t c = a.c0 + cc;
   const float* y = a.y + cc;
@@ -1223,20 +1234,14 @@ __device__ __forceinline__ double surr_post_cell(const SArgs& a, int cc) {
     a.nit[c] = 0;
     return 0.0;
   }
-  // w_k = rho_k exp(change_k), kept in FP32 registers: the network's prediction carries ~1e-3,
-  // and the cell's mass is restored in FP64 from the final densities
-  float w[MAXB];
   double tot = 0.0, W = 0.0;
   bool fallback = false;
-#pragma unroll
-  for (int k = 0; k < MAXB; ++k) {
-    if (k < nb) {
-      const double r = a.rhod[(int64_t)k * a.ld + c];
-      tot += r;
-      if (!(r > 0.0)) fallback = true;
-      w[k] = (float)(r * exp(out_value(a, y, k, ld, om, os)));
-      W += (double)w[k];
-    }
+  for (int k = 0; k < nb; ++k) {
+    double r;
+    const float w = surr_w(a, y, k, ld, om, os, c, &r);
+    tot += r;
+    if (!(r > 0.0)) fallback = true;
+    W += (double)w;
   }
   for (int j = 0; j < a.nin && !fallback; ++j)
     if (!(fabsf(f[j * ld]) <= a.zmax)) fallback = true;
@@ -1257,13 +1262,16 @@ __device__ __forceinline__ double surr_post_cell(const SArgs& a, int cc) {
   const double floor_v = a.rho_gas[c] * a.ratio_min;
   const double sc0 = tot / W;
   double snew = 0.0;
-#pragma unroll
-  for (int k = 0; k < MAXB; ++k)
-    if (k < nb) snew += fmax((double)w[k] * sc0, floor_v);
+  for (int k = 0; k < nb; ++k) {
+    double r;
+    snew += fmax((double)surr_w(a, y, k, ld, om, os, c, &r) * sc0, floor_v);
+  }
   const double sc = tot / snew;
-#pragma unroll
-  for (int k = 0; k < MAXB; ++k)
-    if (k < nb) a.rhod[(int64_t)k * a.ld + c] = fmax((double)w[k] * sc0, floor_v) * sc;
+  for (int k = 0; k < nb; ++k) {
+    double r;
+    const double v = fmax((double)surr_w(a, y, k, ld, om, os, c, &r) * sc0, floor_v) * sc;
+    a.rhod[(int64_t)k * a.ld + c] = v;
+  }
   a.zmean[c] = -pow(10.0, out_value(a, y, nb, ld, om, os));
   return resid;
 }
