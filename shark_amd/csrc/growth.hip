@@ -1276,11 +1276,19 @@ __device__ __forceinline__ double surr_post_cell(const SArgs& a, int cc) {
   return resid;
 }
 
+// launched with 256 threads: the residuals are summed per block before one atomic add, since the
+// adds of all blocks land on one address and serialise there
 __global__ __launch_bounds__(256) void k_surr_post(SArgs a) {
+  __shared__ double part[4];
   const int cc = blockIdx.x * blockDim.x + threadIdx.x;
   double r = cc < a.nch ? surr_post_cell(a, cc) : 0.0;
   for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o);
-  if ((threadIdx.x & 63) == 0 && r != 0.0) atomicAdd(a.resid_sum, r);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double s = (part[0] + part[1]) + (part[2] + part[3]);
+    if (s != 0.0) atomicAdd(a.resid_sum, s);
+  }
 }
 
 }  // namespace

@@ -814,6 +814,11 @@ DEV unsigned long long wave_min_u64(unsigned long long v) {
   return v;
 }
 
+// The block's minimum goes to the device-wide running minimum.  Every block's atomic lands on the
+// same address, where they serialise (24% of the shearing-box post pass when each block issued
+// one), so a block first reads the stored minimum and skips the atomic when it cannot lower it.
+// The stored value only decreases within a launch, so any value read is >= the final minimum and
+// the skip never drops the smallest candidate: the result is the same bit pattern.
 DEV void block_atomic_min(DevScalars* sc, double d) {
   __shared__ unsigned long long red[4];
   unsigned long long v = wave_min_u64(dbits(d));
@@ -823,7 +828,8 @@ DEV void block_atomic_min(DevScalars* sc, double d) {
   if (threadIdx.x == 0) {
     unsigned long long m = red[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = red[w] < m ? red[w] : m;
-    atomicMin(&sc->dtbits, m);
+    const unsigned long long cur = __hip_atomic_load(&sc->dtbits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (m < cur) atomicMin(&sc->dtbits, m);
   }
 }
 
@@ -871,7 +877,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
   // block's LDS small enough for several blocks per CU
   __shared__ double un[NV][TW];
   __shared__ double akS[NDL > 2 * NS ? NDL : 2 * NS][TW];   // alpha_k; phase 5: vmax, |F| per slot
-  __shared__ double gS[NDL][TW];   // 1 / (1 + alpha_k)
+  __shared__ double gS[NDL][TW];   // f_k = alpha_k / (1 + alpha_k)
   __shared__ double sumS[4][TW];   // drag sums; phase 5: vv of the last species
   double (*vmaxS)[TW] = akS;
   double (*fmS)[TW] = akS + NS;
@@ -897,6 +903,11 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
   const size_t nc = a.nc;
   const int cell = cell_of(a, ixs, iy);
   const double dt = a.sc->dt;
+  // the start-of-step density and sound speed the stopping times read, loaded ahead of phase 1's
+  // loads so that their latency overlaps those
+  const bool drag_pre = nd > 0 && a.drag && PART != 2;
+  const double u0_pre = drag_pre ? a.U[cell] : 0.0;
+  const double cs_pre = (drag_pre && a.tstop_kind == 2) ? a.cs[cell] : 0.0;
 
   // ---- phase 1: remap (FARGO) or copy every variable of the segment into LDS
   if (PART == 2) {
@@ -972,18 +983,19 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
   const bool do_drag = nd > 0 && a.drag && PART != 2;
   if (do_drag) {
     // stopping times of the step (compute_tstop on the start-of-step primitives); alpha_k and
-    // alpha_k / (1 + alpha_k) are shared by the drag sums and the momentum update
-    const double rho_q = fmaxf_(a.U[cell], a.smallr);
+    // f_k = alpha_k / (1 + alpha_k) are shared by the drag sums and the momentum update, f_k
+    // computed once per species here rather than in each of the four sums and the update
+    const double rho_q = fmaxf_(u0_pre, a.smallr);
     if (a.tstop_kind == 2) {
       // Epstein (disk_polar setup.f90:436): the gas factor is the same for every species
       const double H = a.hoverr * a.g.rc[ixs - 1];
-      const double den = (rho_q / (a.s2pi * H)) * a.cs[cell];
+      const double den = (rho_q / (a.s2pi * H)) * cs_pre;
       const double yden = 1.0 / den;
       for (int k = s; k < nd; k += NS) {
         const double sk = a.sdust_uniform ? a.sdust_k[k] : a.sdust[(size_t)k * a.nc + cell];
         const double ak = (1.0 * dt) / exdiv(a.eps_k1 * sk, den, yden);
         akS[k][c] = ak;
-        gS[k][c] = 1.0 / (1.0 + ak);
+        gS[k][c] = exdiv(ak, 1.0 + ak, 1.0 / (1.0 + ak));
       }
     } else {
       const double cs_q = (a.tstop_kind == 3 || a.tstop_kind == 5) ? start_cs(a, cell) : 0.0;
@@ -991,7 +1003,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
         const double rq = (a.tstop_kind == 0) ? a.U[(5 + k) * nc + cell] : 0.0;
         const double ak = (1.0 * dt) / tstop_of<ND, !FARGO>(a, k, cell, ixs, rho_q, rq, cs_q);
         akS[k][c] = ak;
-        gS[k][c] = 1.0 / (1.0 + ak);
+        gS[k][c] = exdiv(ak, 1.0 + ak, 1.0 / (1.0 + ak));
       }
     }
   }
@@ -1003,8 +1015,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
       double acc = un[s == 3 ? 0 : s + 1][c];
 #pragma unroll 4
       for (int k = 0; k < nd; ++k) {
-        const double ak = akS[k][c];
-        acc = acc + exdiv(ak, 1.0 + ak, gS[k][c]) * un[base + k][c];
+        acc = acc + gS[k][c] * un[base + k][c];
       }
       sumS[s][c] = acc;
     }
@@ -1013,8 +1024,8 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
     const double pnx = sumS[0][c], pny = sumS[1][c], pnz = sumS[2][c], rhon = sumS[3][c];
     const double yrho = 1.0 / rhon;
     for (int k = s; k < nd; k += NS) {
-      const double ak = akS[k][c], ap1 = 1.0 + ak, y1 = gS[k][c];
-      const double f = exdiv(ak, ap1, y1);
+      const double ak = akS[k][c], ap1 = 1.0 + ak, y1 = 1.0 / ap1;
+      const double f = gS[k][c];
       const double rd = un[5 + k][c];
       un[5 + nd + k][c] = exdiv(un[5 + nd + k][c], ap1, y1) + exdiv(f * pnx, rhon, yrho) * rd;
       un[5 + 2 * nd + k][c] = exdiv(un[5 + 2 * nd + k][c], ap1, y1) + exdiv(f * pny, rhon, yrho) * rd;
