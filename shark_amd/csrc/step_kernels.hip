@@ -818,8 +818,10 @@ DEV unsigned long long wave_min_u64(unsigned long long v) {
 // same address, where they serialise (24% of the shearing-box post pass when each block issued
 // one), so a block first reads the stored minimum and skips the atomic when it cannot lower it.
 // The stored value only decreases within a launch, so any value read is >= the final minimum and
-// the skip never drops the smallest candidate: the result is the same bit pattern.
-DEV void block_atomic_min(DevScalars* sc, double d) {
+// the skip never drops the smallest candidate: the result is the same bit pattern.  cur0, a value of
+// the stored minimum read when the block started (with that read's latency hidden behind the
+// block's work), settles most blocks without waiting on a second read.
+DEV void block_atomic_min(DevScalars* sc, double d, unsigned long long cur0 = ~0ull) {
   __shared__ unsigned long long red[4];
   unsigned long long v = wave_min_u64(dbits(d));
   const int wave = threadIdx.x >> 6;
@@ -828,8 +830,10 @@ DEV void block_atomic_min(DevScalars* sc, double d) {
   if (threadIdx.x == 0) {
     unsigned long long m = red[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = red[w] < m ? red[w] : m;
-    const unsigned long long cur = __hip_atomic_load(&sc->dtbits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (m < cur) atomicMin(&sc->dtbits, m);
+    if (m < cur0) {
+      const unsigned long long cur = __hip_atomic_load(&sc->dtbits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (m < cur) atomicMin(&sc->dtbits, m);
+    }
   }
 }
 
@@ -866,6 +870,8 @@ DEV double tstop_of(const StepArgs& a, int k, int cell, int ix, double rho_q, do
 template <int ND, bool FARGO, int NDIM, int TW, int PART, bool CAP>
 __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
   if (!a.sc->active) return;
+  const unsigned long long cur0 =
+      (PART != 1 && threadIdx.x == 0) ? __hip_atomic_load(&a.sc->dtbits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ~0ull;
   constexpr int NV = 5 + 4 * ND;
   // CAP: ND is a capacity and the run's species count nd (<= ND) is read at run time; otherwise
   // nd is the compile-time ND
@@ -903,11 +909,13 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
   const size_t nc = a.nc;
   const int cell = cell_of(a, ixs, iy);
   const double dt = a.sc->dt;
-  // the start-of-step density and sound speed the stopping times read, loaded ahead of phase 1's
-  // loads so that their latency overlaps those
+  // the start-of-step density and sound speed the stopping times read, and the isothermal sound
+  // speed of the Courant candidate (a field no kernel writes), loaded ahead of phase 1's loads so
+  // that their latency overlaps those
   const bool drag_pre = nd > 0 && a.drag && PART != 2;
   const double u0_pre = drag_pre ? a.U[cell] : 0.0;
-  const double cs_pre = (drag_pre && a.tstop_kind == 2) ? a.cs[cell] : 0.0;
+  const bool cs_iso = a.iso && !a.eos_baro && PART != 1;
+  const double cs_pre = ((drag_pre && a.tstop_kind == 2) || (cs_iso && s == 0)) ? a.cs[cell] : 0.0;
 
   // ---- phase 1: remap (FARGO) or copy every variable of the segment into LDS
   if (PART == 2) {
@@ -1129,7 +1137,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
       const double vx = exdiv(un[1][c], rho, yr), vy = exdiv(un[2][c], rho, yr),
                    vz = exdiv(un[3][c], rho, yr);
       const double vv = (fabs(vx) + fabs(vy)) + fabs(vz);
-      vmax = cs_cell(a, rho, un[4][c], vx, vy, vz, cell) + vv;
+      vmax = (cs_iso ? cs_pre : cs_cell(a, rho, un[4][c], vx, vy, vz, cell)) + vv;
       if (nd == 0) vvlast[c] = vv;
       if (a.force_kick) {
         cell_force<!FARGO, false>(a, ixs, iy, 1, vx, vz, true, -1, f1, f2, f3);
@@ -1182,7 +1190,7 @@ __global__ __launch_bounds__(256, (TW == 32 ? 6 : 3)) void k_post2(StepArgs a) {
       dtc = fminf_(dtc, (((a.cfl * dxx) / vv) * (sqrt(1.0 + (2.0 * a.cfl) * fratio) - 1.0)) / fratio);
     }
   }
-  block_atomic_min(a.sc, dtc);
+  block_atomic_min(a.sc, dtc, cur0);
 }
 
 template <int ND, int NDIM, bool CAP>
