@@ -5,7 +5,7 @@ Development container only (needs /root/reference and flang).  For each configur
 reference is built with oracle/ref/build_ref.sh and run by its harness (which times the step loop
 alone, `loop_seconds` in meta.txt) with OMP_NUM_THREADS=1 and with every core; the oracle then
 steps the reference's own initial state the same number of steps, timed the same way, in a child
-process with the same thread count.  Writes oracle/ref/calibration.json:
+process with the same thread count.  Writes oracle/calibration.json:
   {config: {threads: {"reference_s", "oracle_s", "ratio"}}}  with ratio = reference_s / oracle_s,
 so a CPU figure of the oracle reads as that many times the reference's speed.
 
@@ -84,7 +84,7 @@ def main():
             if os.path.exists(npz):
                 os.remove(npz)
     out["host"] = {"cores": ncores, "cpu": _cpu_model()}
-    with open(os.path.join(HERE, "calibration.json"), "w") as f:
+    with open(os.path.join(os.path.dirname(HERE), "calibration.json"), "w") as f:
         json.dump(out, f, indent=1)
 
 
