@@ -1,5 +1,5 @@
 """State files read by the Fortran driver (shark_amd/fortran/shark_gpu_driver.f90, `state` mode)
-and the driver's result files.
+and the Fortran driver's result files.
 
 State file (little-endian stream): int32 magic 0x53484B31, int32 version 1; the shark_cfg
 struct; int64 ncells, int32 ndim, int32 nvar; int32 has_sdust, has_condinit, has_fargo, pad;

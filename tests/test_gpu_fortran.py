@@ -39,7 +39,7 @@ def test_fortran_dustywave_matches_python_host_and_oracle(tmp_path):
     assert np.array_equal(u, up)
     assert np.array_equal(u, o.u())
     assert time == pytest.approx(o.time, rel=0, abs=0)
-    # the driver's output(1) through the Fortran binding: rho and the dust velocity over active cells
+    # the Fortran driver's output(1) through the binding: rho and the dust velocity over active cells
     act = np.zeros(u.shape[1], bool)
     act[2:-2] = True
     rho = np.fromfile(str(tmp_path / "output_00001" / "rho"))
