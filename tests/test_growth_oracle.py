@@ -6,8 +6,9 @@ compute_tstop, and the dust densities after one dust_growth call
 (src/dust/dust_growth.f90:3-256).  The collapse branch must agree bit for bit.  Beyond the
 reference's branch, the invariants that pin the disk environment and the decomposed bin
 targets are checked here too: per-pair mass conservation of the rates, the constant- and
-additive-kernel closed forms of a bin's loss rate, the midplane Epstein Stokes number, and
-decomposed == reference when no pair fragments.
+additive-kernel closed forms of a bin's loss rate, the midplane Epstein Stokes number,
+decomposed == reference when no pair fragments, and the velocity-threshold fragmentation law's
+onset, saturation and slope.
 """
 import json
 import os
@@ -127,6 +128,45 @@ def test_rates_conserve_mass(kernel_type, frag):
     if frag:
         # fragments of the collisions refill the bins below the smaller grain
         assert (d1[:8] > 0).all()
+
+
+def test_velocity_threshold_law_onset_saturation_and_slope():
+    """The disk's velocity-threshold law f = clamp((|dv| - 0.1 v_frag) / v_frag, 0, 1) (the
+    reference's commented alternative, src/dust/dust_growth.f90:145), pinned without knowing |dv|.
+    One populated bin, so the self-collision pair carries the rates; with decomposed targets its
+    coagulation lands above the bin and its fragments below.  Fragments appear once
+    v_frag < 10 |dv| and coagulation stops once v_frag <= |dv| / 1.1, so the two thresholds are
+    exactly 11 apart (an onset coefficient c would give (1 + c) / c), and at v_frag = 10 |dv| / 6
+    the law gives f = 1/2: the mass going below equals the mass going above."""
+    p, t, sigma, cs, r, rhod, _ = disk_cells(nc=1)
+    p.eps_threshold = p.eps_threshold_frag = 0.0
+    p.sticking = 1.0
+    ib = 16
+    rhod[:] = 1e-32 * sigma
+    rhod[ib] = 0.01 * sigma
+
+    def split(vfrag):
+        p.vfrag = vfrag
+        d1 = first_rates(p, t, sigma, cs, r, rhod)[:, 0]
+        return d1[:ib].sum(), d1[ib + 1:].sum(), abs(d1[ib])
+
+    def threshold(pred, lo, hi):
+        # largest v_frag (between lo, where pred holds, and hi, where it fails) with pred true
+        for _ in range(80):
+            mid = np.sqrt(lo * hi)
+            lo, hi = (mid, hi) if pred(*split(mid)) else (lo, mid)
+        return lo
+
+    frag = lambda below, above, loss: below > 1e-20 * loss
+    no_coag = lambda below, above, loss: above < 1e-20 * loss
+    lo, hi = 1e-6, 1e12
+    assert frag(*split(lo)) and no_coag(*split(lo)) and not frag(*split(hi))
+    v_on = threshold(frag, lo, hi)
+    v_full = threshold(no_coag, lo, hi)
+    assert abs(v_on / v_full / 11.0 - 1.0) < 1e-9
+    below, above, loss = split(v_on / 6.0)
+    assert abs(below / above - 1.0) < 1e-9
+    assert abs((below + above) / loss - 1.0) < 1e-12
 
 
 def test_constant_kernel_self_collision_loss_closed_form():
