@@ -169,6 +169,27 @@ def test_velocity_threshold_law_onset_saturation_and_slope():
     assert abs((below + above) / loss - 1.0) < 1e-12
 
 
+def test_decomposed_targets_against_reference_order_in_the_disk():
+    """Decomposed bin targets (coagulated mass binned by m1 + m2, fragments below 0.1 (m1 + m2))
+    against the reference's (1 - f)(m1 + m2) and 0.1 f (m1 + m2) (dust_growth.f90:183-187, 220),
+    the latter pinned bit for bit on the collapse branch.  Without fragmentation (v_frag far above
+    every |dv|) the two agree bit for bit.  With every pair fully fragmenting (v_frag far below) the
+    reference's coagulated mass is 0, its target bin clamps to the first and so does the range
+    its fragments are spread over: all fragment mass lands in the smallest bin, which gains while
+    every other bin only loses; the decomposed targets spread the fragments instead."""
+    rates = {}
+    for vfrag in (1e12, 1e-9):
+        for dec in (1, 0):
+            p, t, sigma, cs, r, rhod, _ = disk_cells()
+            p.vfrag, p.decomposed = vfrag, dec
+            t = pyoracle.growth_tables(p)
+            rates[vfrag, dec] = first_rates(p, t, sigma, cs, r, rhod)
+    assert np.array_equal(rates[1e12, 1], rates[1e12, 0])
+    ref_order, dec = rates[1e-9, 0], rates[1e-9, 1]
+    assert (ref_order[0] > 0).all() and (ref_order[1:] <= 0).all()
+    assert (dec[1:8] > 0).all()
+
+
 def test_constant_kernel_self_collision_loss_closed_form():
     """Constant kernel K=1 with all mass in one bin k: the only non-negligible pair is (k,k),
     whose collision rate is n_k^2/2 (n = rho/m); it removes two grains of mass m_k per
